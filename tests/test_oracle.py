@@ -200,16 +200,16 @@ def test_esp_poisson_equation():
 
 
 def test_esp_far_field_monopole():
+    """r * ESP -> Z_tot - N_e; Richardson over r and 2r removes the dipole term."""
     w = synthetic.water(4)
     flat = w.basis.flat()
-    pts = np.array([[300.0, 0, 0], [0, 250.0, 10], [-10, 0, -400.0]])
-    esp = oracle.esp(flat, w.P, w.charges, pts)
-    S = oracle.overlap(flat)
-    n_el = np.trace(w.P @ S)
-    r = np.linalg.norm(pts, axis=1)
-    mono = (w.charges.sum() - n_el) / r
-    # the dipole term is below |mu| / r^2 < 1e-4 at these distances
-    assert np.allclose(esp, mono, rtol=0, atol=1e-4)
+    u = np.array([[1.0, 0, 0], [0, 0.6, 0.8], [-0.48, 0.64, -0.6]])
+    r1, r2 = 400.0, 800.0
+    e1 = oracle.esp(flat, w.P, w.charges, r1 * u)
+    e2 = oracle.esp(flat, w.P, w.charges, r2 * u)
+    q = 2.0 * r2 * e2 - r1 * e1
+    n_el = np.trace(w.P @ oracle.overlap(flat))
+    assert np.allclose(q, w.charges.sum() - n_el, rtol=0, atol=1e-3)
 
 
 def test_adjoints_against_finite_differences():
