@@ -1,0 +1,42 @@
+"""MI355X-native evaluator of Gaussian-basis electron-density descriptors.
+
+The product path is ``Molecule`` (pybind11 over the C ABI in
+``include/chemtools_amd.h``), whose methods run hand-written HIP kernels for
+gfx950.  Importing this package fails loudly when the native library has not
+been built: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+
+try:
+    from . import _chemtools_amd as _native  # noqa: F401
+except ImportError as exc:  # pragma: no cover - exercised only on an unbuilt tree
+    raise ImportError(
+        "chemtoolscuda_amd: native extension not built; run `python -m chemtoolscuda_amd.build_ext` "
+        f"(or __graft_entry__.build()) first ({exc})") from exc
+
+from .basis import BasisSet, Shell, n_functions  # noqa: E402
+
+Molecule = _native.Molecule
+
+LIBRARY_PATH = os.path.join(_HERE, "libchemtools_amd.so")
+
+
+def molecule_from_basis(basis: BasisSet, P, charges=None, numbers=None) -> "Molecule":
+    """Builds a ``Molecule`` from an in-memory ``BasisSet`` and density matrix."""
+    import numpy as np
+
+    f = basis.flat()
+    if charges is None:
+        charges = np.asarray(basis.atom_numbers, dtype=np.float64)
+    if numbers is None:
+        numbers = basis.atom_numbers
+    return Molecule(f["shell_atom"], f["shell_l"], f["shell_nprim"], f["shell_prim_off"], f["exps"], f["coefs"],
+                    f["atom_xyz"], np.asarray(charges, dtype=np.float64), np.asarray(P, dtype=np.float64),
+                    np.asarray(numbers, dtype=np.int32))
+
+
+__all__ = ["Molecule", "BasisSet", "Shell", "n_functions", "molecule_from_basis", "LIBRARY_PATH"]

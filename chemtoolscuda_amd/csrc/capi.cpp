@@ -1,0 +1,233 @@
+// extern "C" boundary (include/chemtools_amd.h).  Every entry point converts
+// exceptions into a status code plus a thread-local message, so errors reach
+// Python as readable RuntimeErrors instead of the reference's raw
+// cudaError_t throw (include/cuda_utils.cuh:55-60, seen in Python as
+// "Caught an unknown exception!").
+
+#include <cstring>
+#include <string>
+
+#include "mol.h"
+
+struct ct_mol {
+  ct::Mol* impl;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <typename F>
+int guard(F&& f) {
+  try {
+    f();
+    g_last_error.clear();
+    return CT_OK;
+  } catch (const ct::Error& e) {
+    g_last_error = e.what();
+    return e.status;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "host allocation failed";
+    return CT_E_NOMEM;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return CT_E_INVALID;
+  }
+}
+
+ct::HostBasis to_host_basis(const ct_basis_desc* b) {
+  CT_REQUIRE(b != nullptr && b->nshell > 0 && b->shell_atom && b->shell_l && b->shell_nprim && b->shell_prim_off &&
+                 b->exps && b->coefs,
+             "incomplete basis description");
+  ct::HostBasis h;
+  h.shell_atom.assign(b->shell_atom, b->shell_atom + b->nshell);
+  h.shell_l.assign(b->shell_l, b->shell_l + b->nshell);
+  h.shell_nprim.assign(b->shell_nprim, b->shell_nprim + b->nshell);
+  h.shell_prim_off.assign(b->shell_prim_off, b->shell_prim_off + b->nshell);
+  size_t nprim = 0;
+  for (int s = 0; s < b->nshell; ++s) {
+    CT_REQUIRE(b->shell_nprim[s] > 0 && b->shell_prim_off[s] >= 0, "bad primitive counts");
+    nprim = std::max(nprim, (size_t)b->shell_prim_off[s] + b->shell_nprim[s]);
+  }
+  h.exps.assign(b->exps, b->exps + nprim);
+  h.coefs.assign(b->coefs, b->coefs + nprim);
+  int M = 0;
+  for (int l : h.shell_l) M += ct::shell_nfunc(l);
+  h.nbasis = M;
+  return h;
+}
+
+ct::Mol& M(ct_mol* m) {
+  CT_REQUIRE(m != nullptr && m->impl != nullptr, "null molecule handle");
+  return *m->impl;
+}
+const ct::Mol& M(const ct_mol* m) {
+  CT_REQUIRE(m != nullptr && m->impl != nullptr, "null molecule handle");
+  return *m->impl;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ct_mol_create(const ct_basis_desc* basis, const double* atom_xyz, const double* charges, int32_t natom,
+                  const double* P, int32_t nbasis, ct_mol** out) {
+  return guard([&] {
+    CT_REQUIRE(out != nullptr && atom_xyz != nullptr && P != nullptr, "null argument");
+    *out = nullptr;
+    ct::HostBasis hb = to_host_basis(basis);
+    CT_REQUIRE(hb.nbasis == nbasis, "nbasis does not match the shell table (" + std::to_string(hb.nbasis) + ")");
+    auto* h = new ct_mol{nullptr};
+    try {
+      h->impl = new ct::Mol(hb, atom_xyz, charges, nullptr, natom, P);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+int ct_mol_create_from_fchk(const char* path, ct_mol** out) {
+  return guard([&] {
+    CT_REQUIRE(out != nullptr && path != nullptr, "null argument");
+    *out = nullptr;
+    ct::FchkData d = ct::load_fchk(path);
+    auto* h = new ct_mol{nullptr};
+    try {
+      h->impl = new ct::Mol(d.basis, d.atom_xyz.data(), d.charges.data(), d.numbers.data(),
+                            (int)d.numbers.size(), d.P.data());
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+int ct_mol_destroy(ct_mol* mol) {
+  return guard([&] {
+    if (!mol) return;
+    delete mol->impl;
+    delete mol;
+  });
+}
+
+int ct_mol_nbasis(const ct_mol* mol) { return mol && mol->impl ? mol->impl->nbasis() : CT_E_INVALID; }
+int ct_mol_natom(const ct_mol* mol) { return mol && mol->impl ? mol->impl->natom() : CT_E_INVALID; }
+
+int ct_mol_get_coordinates(const ct_mol* mol, double* xyz) {
+  return guard([&] {
+    CT_REQUIRE(xyz, "null output");
+    const auto& v = M(mol).coordinates();
+    std::memcpy(xyz, v.data(), sizeof(double) * v.size());
+  });
+}
+
+int ct_mol_get_numbers(const ct_mol* mol, int32_t* numbers) {
+  return guard([&] {
+    CT_REQUIRE(numbers, "null output");
+    const auto& v = M(mol).numbers();
+    std::memcpy(numbers, v.data(), sizeof(int32_t) * v.size());
+  });
+}
+
+int ct_mol_get_charges(const ct_mol* mol, double* charges) {
+  return guard([&] {
+    CT_REQUIRE(charges, "null output");
+    const auto& v = M(mol).charges();
+    std::memcpy(charges, v.data(), sizeof(double) * v.size());
+  });
+}
+
+int ct_mol_get_density_matrix(const ct_mol* mol, double* P) {
+  return guard([&] {
+    CT_REQUIRE(P, "null output");
+    const auto& v = M(mol).density_matrix();
+    std::memcpy(P, v.data(), sizeof(double) * v.size());
+  });
+}
+
+int ct_mol_set_coordinates(ct_mol* mol, const double* atom_xyz) {
+  return guard([&] { M(mol).set_coordinates(atom_xyz); });
+}
+
+int ct_mol_set_density_matrix(ct_mol* mol, const double* P) {
+  return guard([&] { M(mol).set_density_matrix(P); });
+}
+
+int ct_mol_set_workspace_limit(ct_mol* mol, int64_t bytes) {
+  return guard([&] {
+    CT_REQUIRE(bytes > 0, "workspace limit must be positive");
+    M(mol).set_workspace_limit(bytes);
+  });
+}
+
+int ct_eval(ct_mol* mol, const double* points, int64_t n, uint32_t quantities, int32_t precision,
+            const ct_eval_params* params, const ct_outputs* out, ct_eval_report* report) {
+  return guard([&] {
+    CT_REQUIRE(out != nullptr, "null outputs");
+    ct_eval_params p{};
+    if (params) p = *params;
+    if (p.mixed_tol <= 0) p.mixed_tol = 1e-7;
+    if (p.rho_floor <= 0) p.rho_floor = 1e-10;
+    M(mol).eval(points, n, quantities, precision, p, *out, report);
+  });
+}
+
+int ct_norm_rows(const double* v, int64_t n, double* out) {
+  return guard([&] {
+    CT_REQUIRE(n >= 0, "negative size");
+    if (n == 0) return;
+    CT_REQUIRE(v && out, "null pointer");
+    const bool vd = ct::is_device_pointer(v), od = ct::is_device_pointer(out);
+    ct::DeviceBuffer tmp;
+    if ((!vd || !od) && !tmp.reserve(sizeof(double) * 4 * (size_t)n))
+      throw ct::Error(CT_E_NOMEM, "cannot allocate");
+    const double* dv = v;
+    double* dout = out;
+    if (!vd) {
+      CT_HIP_CHECK(hipMemcpy(tmp.get(), v, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+      dv = static_cast<const double*>(tmp.get());
+    }
+    if (!od) dout = static_cast<double*>(tmp.get()) + 3 * n;
+    ct::launch_norm_rows(dv, n, dout, nullptr);
+    if (!od) CT_HIP_CHECK(hipMemcpy(out, dout, sizeof(double) * n, hipMemcpyDeviceToHost));
+    CT_HIP_CHECK(hipDeviceSynchronize());
+  });
+}
+
+int ct_esp(ct_mol* mol, const double* points, int64_t n, int32_t precision, double* out) {
+  return guard([&] {
+    (void)M(mol);
+    (void)points;
+    (void)n;
+    (void)precision;
+    (void)out;
+    throw ct::Error(CT_E_UNSUPPORTED, "ESP kernel not built yet");
+  });
+}
+
+int ct_backward(ct_mol* mol, const double* points, int64_t n, const double* dL_drho, int32_t precision,
+                double* dL_dP, double* dL_dR) {
+  return guard([&] {
+    (void)M(mol);
+    (void)points;
+    (void)n;
+    (void)dL_drho;
+    (void)precision;
+    (void)dL_dP;
+    (void)dL_dR;
+    throw ct::Error(CT_E_UNSUPPORTED, "adjoint kernels not built yet");
+  });
+}
+
+int ct_mol_synchronize(ct_mol* mol) {
+  return guard([&] { M(mol).synchronize(); });
+}
+
+const char* ct_last_error(void) { return g_last_error.c_str(); }
+
+const char* ct_version(void) { return "chemtools-amd 0.1 (gfx950)"; }
+
+}  // extern "C"

@@ -1,0 +1,182 @@
+// FP64 contraction on the matrix cores with fused per-point epilogues.
+//
+// The reference forms rho = sum_i phi_i (P phi)_i with a cuBLAS DGEMM into an
+// (N x M) scratch matrix, a Hadamard kernel and a DGEMV against a vector of
+// ones, once per derivative column -- 8 DGEMMs and 6 AO passes for
+// rho + grad + lap (src/evaluate_density.cu:780-823,
+// src/evaluate_gradient.cu:1318-1375, src/evaluate_laplacian.cu:479-821,
+// src/evaluate_hessian.cu:1906-2230).  Here one kernel computes
+// Y = P [phi | d_x phi | d_y phi | d_z phi] tile by tile with
+// v_mfma_f64_16x16x4f64 and immediately dots each Y tile with the matching
+// AO rows, so Y never leaves the registers and every quantity of a mode comes
+// out of a single pass.
+//
+// Work decomposition: one wavefront owns a set of points and all rows of P.
+//   * 1 GEMM column type (MODE_RHO, MODE_GRAD): 4 column blocks x 16 points
+//     = 64 points per wave.
+//   * 4 column types (MODE_LAP, MODE_HESS): the 4 column blocks are phi,
+//     d_x phi, d_y phi, d_z phi of the same 16 points.
+// For each 64-row block of P (4 MFMA row blocks) the wave streams all of K,
+// accumulating a 64 x 64 FP64 tile (16 MFMA accumulators), then runs the
+// epilogue.  Operands are read straight from global memory with coalesced
+// 128-byte row segments: A[i][k] = P[k][i] (P is symmetric), B[k][j] =
+// AO[k][p0 + j]; P (<= a few MB) and the chunk's AO rows stay L2-resident.
+// Waves are independent (no LDS, no barriers).
+//
+// MFMA 16x16x4 f64 lane layout (lane l, r = l % 16, q = l / 16):
+//   A operand: A[r][q];  B operand: B[q][r];  D: D[4 q + j][r], j = 0..3.
+
+#include "ct_internal.h"
+
+namespace ct {
+
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+template <int MODE>
+__global__ __launch_bounds__(256) void contract_f64_kernel(const double* __restrict__ P, int mp,
+                                                           const double* __restrict__ ao, int64_t ld,
+                                                           double* __restrict__ dots) {
+  constexpr int NT = (MODE == MODE_RHO || MODE == MODE_GRAD) ? 1 : 4;
+  constexpr int PPW = NT == 1 ? 64 : 16;
+  constexpr int ND = mode_ndot(MODE);
+  constexpr int NACC = NT == 1 ? 4 * ND : ND;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int r = lane & 15, q = lane >> 4;
+  const int64_t p0 = ((int64_t)blockIdx.x * 4 + wave) * PPW;
+  const int64_t sd = (int64_t)mp * ld;
+
+  const double* bptr[4];
+#pragma unroll
+  for (int cb = 0; cb < 4; ++cb) {
+    const int t = NT == 1 ? 0 : cb;
+    const int poff = NT == 1 ? 16 * cb : 0;
+    bptr[cb] = ao + t * sd + (int64_t)q * ld + p0 + poff + r;
+  }
+  double dacc[NACC];
+#pragma unroll
+  for (int k = 0; k < NACC; ++k) dacc[k] = 0.0;
+
+  for (int i0 = 0; i0 < mp; i0 += kRowBlock) {
+    d4 acc[4][4];
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+      for (int cb = 0; cb < 4; ++cb) acc[rb][cb] = d4{0.0, 0.0, 0.0, 0.0};
+
+    const double* aptr = P + (int64_t)q * mp + i0 + r;
+    double a[4], b[4];
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb) a[rb] = aptr[16 * rb];
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) b[cb] = bptr[cb][0];
+
+    for (int k0 = 0; k0 < mp; k0 += 4) {
+      // prefetch the next k-step (clamped on the last step; the value is unused)
+      const int kn = (k0 + 4 < mp) ? k0 + 4 : k0;
+      double an[4], bn[4];
+#pragma unroll
+      for (int rb = 0; rb < 4; ++rb) an[rb] = aptr[(int64_t)kn * mp + 16 * rb];
+#pragma unroll
+      for (int cb = 0; cb < 4; ++cb) bn[cb] = bptr[cb][(int64_t)kn * ld];
+#pragma unroll
+      for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb)
+          acc[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb], b[cb], acc[rb][cb], 0, 0, 0);
+#pragma unroll
+      for (int rb = 0; rb < 4; ++rb) a[rb] = an[rb];
+#pragma unroll
+      for (int cb = 0; cb < 4; ++cb) b[cb] = bn[cb];
+    }
+
+    // epilogue: dot the Y tile with the AO rows i0..i0+63 of this wave's points
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb) {
+#pragma unroll
+      for (int cb = 0; cb < 4; ++cb) {
+        const int64_t p = p0 + (NT == 1 ? 16 * cb : 0) + r;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int64_t i = i0 + 16 * rb + 4 * q + j;
+          const double y = acc[rb][cb][j];
+          const double* ai = ao + i * ld + p;
+          if constexpr (MODE == MODE_RHO) {
+            dacc[cb] += ai[0] * y;
+          } else if constexpr (MODE == MODE_GRAD) {
+#pragma unroll
+            for (int d = 0; d < 4; ++d) dacc[cb * 4 + d] += ai[d * sd] * y;
+          } else if constexpr (MODE == MODE_LAP) {
+            if (cb == 0) {
+#pragma unroll
+              for (int d = 0; d < 5; ++d) dacc[d] += ai[d * sd] * y;
+            } else {
+              dacc[4 + cb] += ai[cb * sd] * y;
+            }
+          } else {  // MODE_HESS
+            if (cb == 0) {
+#pragma unroll
+              for (int d = 0; d < 10; ++d) dacc[d] += ai[d * sd] * y;
+            } else if (cb == 1) {
+              dacc[10] += ai[1 * sd] * y;  // t_xx
+            } else if (cb == 2) {
+              dacc[11] += ai[1 * sd] * y;  // t_xy
+              dacc[12] += ai[2 * sd] * y;  // t_yy
+            } else {
+              dacc[13] += ai[1 * sd] * y;  // t_xz
+              dacc[14] += ai[2 * sd] * y;  // t_yz
+              dacc[15] += ai[3 * sd] * y;  // t_zz
+            }
+          }
+        }
+      }
+    }
+  }
+
+  // reduce the four lane groups that hold the same point
+#pragma unroll
+  for (int k = 0; k < NACC; ++k) {
+    double v = dacc[k];
+    v += __shfl_xor(v, 16);
+    v += __shfl_xor(v, 32);
+    dacc[k] = v;
+  }
+  if (q == 0) {
+    if constexpr (NT == 1) {
+#pragma unroll
+      for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+        for (int d = 0; d < ND; ++d) dots[(int64_t)d * ld + p0 + 16 * cb + r] = dacc[cb * ND + d];
+    } else {
+#pragma unroll
+      for (int d = 0; d < ND; ++d) dots[(int64_t)d * ld + p0 + r] = dacc[d];
+    }
+  }
+}
+
+}  // namespace
+
+void launch_contract_f64(int mode, const double* P, int nbasis_pad, const double* ao, int64_t ld, double* dots,
+                         hipStream_t s) {
+  const int ppw = (mode == MODE_RHO || mode == MODE_GRAD) ? 64 : 16;
+  const dim3 block(256), grid((unsigned)(ld / (4 * ppw)));
+  switch (mode) {
+    case MODE_RHO:
+      hipLaunchKernelGGL(contract_f64_kernel<MODE_RHO>, grid, block, 0, s, P, nbasis_pad, ao, ld, dots);
+      break;
+    case MODE_GRAD:
+      hipLaunchKernelGGL(contract_f64_kernel<MODE_GRAD>, grid, block, 0, s, P, nbasis_pad, ao, ld, dots);
+      break;
+    case MODE_LAP:
+      hipLaunchKernelGGL(contract_f64_kernel<MODE_LAP>, grid, block, 0, s, P, nbasis_pad, ao, ld, dots);
+      break;
+    default:
+      hipLaunchKernelGGL(contract_f64_kernel<MODE_HESS>, grid, block, 0, s, P, nbasis_pad, ao, ld, dots);
+      break;
+  }
+  CT_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace ct

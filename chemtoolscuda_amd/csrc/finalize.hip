@@ -1,0 +1,96 @@
+// FP64 per-point epilogue: turns the contraction dots into the requested
+// descriptors in one pass (the reference runs separate host round trips and
+// elementwise kernels per descriptor, src/evaluate_densbased.cu:44-223,
+// src/cuda_utils.cu:11-136).  The arithmetic follows the reference exactly,
+// including its masking conventions:
+//   * masked divide: num/den when den > 1e-20, else 1e-20 (src/cuda_utils.cu:71-84)
+//   * RDG  = 1/2 (3 pi^2)^{-1/3} |grad| / rho^{4/3} (src/evaluate_densbased.cu:44-89)
+//   * W    = |grad|^2 / (8 rho), masked (src/evaluate_densbased.cu:91-140)
+//   * TF   = 0.3 (3 pi^2)^{2/3} rho^{5/3} (src/evaluate_densbased.cu:142-173)
+//   * GGE  = TF + a W + b lap (src/evaluate_densbased.cu:175-193)
+//   * general(ish) KED = tau_PD + (a-1)/4 lap (src/evaluate_densbased.cu:195-208)
+//   * t_alpha = tau_PD + alpha lap (src/evaluate_kinetic_dens.cu:195-214)
+//   * Shannon = rho ln rho, unguarded (src/evaluate_densbased.cu:210-223)
+
+#include "ct_internal.h"
+
+namespace ct {
+
+namespace {
+
+__device__ __forceinline__ double masked_div(double num, double den) { return den > 1e-20 ? num / den : 1e-20; }
+
+__global__ __launch_bounds__(256) void finalize_kernel(FinalizeArgs a) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= a.n) return;
+  const double* D = a.dots;
+  const int64_t ld = a.ld;
+  auto dot = [&](int d) { return D[(int64_t)d * ld + p]; };
+  const int64_t o = a.out_off + p;
+
+  const double rho = dot(0);
+  double g[3] = {0.0, 0.0, 0.0};
+  if (a.mode != MODE_RHO)
+    for (int k = 0; k < 3; ++k) g[k] = 2.0 * dot(1 + k);
+  double lap = 0.0, tau = 0.0;
+  if (a.mode == MODE_LAP) {
+    const double t = dot(5) + dot(6) + dot(7);
+    lap = 2.0 * (dot(4) + t);
+    tau = 0.5 * t;
+  } else if (a.mode == MODE_HESS) {
+    const double t = dot(10) + dot(12) + dot(15);
+    lap = 2.0 * (dot(4) + dot(7) + dot(9) + t);
+    tau = 0.5 * t;
+  }
+  const uint32_t q = a.q;
+  if (q & CT_Q_RHO) a.out.rho[o] = rho;
+  if (q & CT_Q_GRAD)
+    for (int k = 0; k < 3; ++k) a.out.grad[3 * o + k] = g[k];
+  if (q & CT_Q_HESS) {
+    // h_kl slots 4..9 = xx,xy,xz,yy,yz,zz ; t_kl slots 10..15 = xx,xy,yy,xz,yz,zz
+    const int hs[3][3] = {{4, 5, 6}, {5, 7, 8}, {6, 8, 9}};
+    const int ts[3][3] = {{10, 11, 13}, {11, 12, 14}, {13, 14, 15}};
+    for (int k = 0; k < 3; ++k)
+      for (int l = 0; l < 3; ++l) a.out.hess[9 * o + 3 * k + l] = 2.0 * (dot(hs[k][l]) + dot(ts[k][l]));
+  }
+  if (q & CT_Q_LAP) a.out.lap[o] = lap;
+  if (q & CT_Q_TAU_PD) a.out.tau_pd[o] = tau;
+  const double gnorm2 = g[0] * g[0] + g[1] * g[1] + g[2] * g[2];
+  const double pi2x3 = 3.0 * M_PI * M_PI;
+  if (q & CT_Q_RDG) {
+    const double pref = 0.5 / pow(pi2x3, 1.0 / 3.0);
+    a.out.rdg[o] = pref * masked_div(sqrt(gnorm2), pow(rho, 4.0 / 3.0));
+  }
+  if (q & CT_Q_SHANNON) a.out.shannon[o] = rho * log(rho);
+  const double tf = 0.3 * pow(pi2x3, 2.0 / 3.0) * pow(rho, 5.0 / 3.0);
+  const double w = masked_div(gnorm2, 8.0 * rho);
+  if (q & CT_Q_WEIZSACKER) a.out.weizsacker[o] = w;
+  if (q & CT_Q_THOMAS_FERMI) a.out.thomas_fermi[o] = tf;
+  if (q & CT_Q_GGE) a.out.gge[o] = tf + a.par.gge_a * w + a.par.gge_b * lap;
+  if (q & CT_Q_GENERAL_KED) a.out.general_ked[o] = tau + (a.par.ked_a - 1.0) / 4.0 * lap;
+  if (q & CT_Q_KED_ALPHA) a.out.ked_alpha[o] = tau + a.par.ked_alpha * lap;
+}
+
+__global__ __launch_bounds__(256) void norm_rows_kernel(const double* __restrict__ v, int64_t n,
+                                                        double* __restrict__ out) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const double x = v[3 * p], y = v[3 * p + 1], z = v[3 * p + 2];
+  out[p] = sqrt(x * x + y * y + z * z);
+}
+
+}  // namespace
+
+void launch_finalize(const FinalizeArgs& a, hipStream_t s) {
+  if (a.n <= 0) return;
+  hipLaunchKernelGGL(finalize_kernel, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, s, a);
+  CT_HIP_CHECK(hipGetLastError());
+}
+
+void launch_norm_rows(const double* v, int64_t n, double* out, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(norm_rows_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, v, n, out);
+  CT_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace ct

@@ -1,0 +1,315 @@
+// Host orchestration of the density-descriptor path (replaces the reference's
+// chunked evaluate_* host functions, e.g. src/evaluate_density.cu:663-840,
+// and the constant-memory staging of src/basis_to_gpu.cu:16-220).
+//
+// Per call: points are split into chunks sized from the workspace budget
+// (the reference sizes chunks from free memory minus 0.5 GB,
+// src/evaluate_density.cu:690-705); per chunk the AO kernel, the fused
+// contraction and the FP64 epilogue run back to back on the handle's stream.
+// P and the shell table are uploaded once per handle (not per call).
+
+#include "mol.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace ct {
+
+namespace {
+
+const double kPi = 3.14159265358979323846264338327950288;
+
+int64_t round_up(int64_t v, int64_t a) { return (v + a - 1) / a * a; }
+
+int select_mode(uint32_t q) {
+  if (q & CT_Q_HESS) return MODE_HESS;
+  if (q & (CT_Q_LAP | CT_Q_TAU_PD | CT_Q_GGE | CT_Q_GENERAL_KED | CT_Q_KED_ALPHA)) return MODE_LAP;
+  if (q & (CT_Q_GRAD | CT_Q_RDG | CT_Q_WEIZSACKER)) return MODE_GRAD;
+  return MODE_RHO;
+}
+
+struct OutSlot {
+  uint32_t bit;
+  double* ct_outputs::*field;
+  int width;
+};
+
+const OutSlot kSlots[] = {
+    {CT_Q_RHO, &ct_outputs::rho, 1},
+    {CT_Q_GRAD, &ct_outputs::grad, 3},
+    {CT_Q_HESS, &ct_outputs::hess, 9},
+    {CT_Q_LAP, &ct_outputs::lap, 1},
+    {CT_Q_TAU_PD, &ct_outputs::tau_pd, 1},
+    {CT_Q_RDG, &ct_outputs::rdg, 1},
+    {CT_Q_SHANNON, &ct_outputs::shannon, 1},
+    {CT_Q_WEIZSACKER, &ct_outputs::weizsacker, 1},
+    {CT_Q_THOMAS_FERMI, &ct_outputs::thomas_fermi, 1},
+    {CT_Q_GGE, &ct_outputs::gge, 1},
+    {CT_Q_GENERAL_KED, &ct_outputs::general_ked, 1},
+    {CT_Q_KED_ALPHA, &ct_outputs::ked_alpha, 1},
+};
+
+}  // namespace
+
+int shell_code(int l) {
+  switch (l) {
+    case 0: return 0;
+    case 1: return 1;
+    case 2: return 2;
+    case 3: return 3;
+    case 4: return 4;
+    case -2: return 5;
+    case -3: return 6;
+    case -4: return 7;
+  }
+  throw Error(CT_E_INVALID, "unsupported shell angular momentum " + std::to_string(l));
+}
+
+int shell_nfunc(int l) {
+  switch (l) {
+    case 0: return 1;
+    case 1: return 3;
+    case 2: return 6;
+    case -2: return 5;
+    case 3: return 10;
+    case -3: return 7;
+    case 4: return 15;
+    case -4: return 9;
+  }
+  throw Error(CT_E_INVALID, "unsupported shell angular momentum " + std::to_string(l));
+}
+
+bool is_device_pointer(const void* ptr) {
+  if (ptr == nullptr) return false;
+  hipPointerAttribute_t attr;
+  std::memset(&attr, 0, sizeof(attr));
+  hipError_t e = hipPointerGetAttributes(&attr, ptr);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
+}
+
+bool DeviceBuffer::reserve(size_t bytes) {
+  if (bytes <= bytes_) return true;
+  release();
+  if (hipMalloc(&ptr_, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    ptr_ = nullptr;
+    return false;
+  }
+  bytes_ = bytes;
+  return true;
+}
+
+void DeviceBuffer::release() {
+  if (ptr_) (void)hipFree(ptr_);
+  ptr_ = nullptr;
+  bytes_ = 0;
+}
+
+void PinnedBuffer::reserve(size_t bytes) {
+  if (bytes <= bytes_) return;
+  release();
+  CT_HIP_CHECK(hipHostMalloc(&ptr_, bytes, hipHostMallocDefault));
+  bytes_ = bytes;
+}
+
+void PinnedBuffer::release() {
+  if (ptr_) (void)hipHostFree(ptr_);
+  ptr_ = nullptr;
+  bytes_ = 0;
+}
+
+Mol::Mol(const HostBasis& basis, const double* atom_xyz, const double* charges, const int32_t* numbers, int natom,
+         const double* P)
+    : basis_(basis), natom_(natom) {
+  CT_REQUIRE(natom > 0, "molecule needs at least one atom");
+  nshell_ = (int)basis.shell_l.size();
+  CT_REQUIRE(nshell_ > 0, "basis has no shells");
+  CT_REQUIRE(basis.shell_atom.size() == (size_t)nshell_ && basis.shell_nprim.size() == (size_t)nshell_ &&
+                 basis.shell_prim_off.size() == (size_t)nshell_,
+             "inconsistent shell arrays");
+  M_ = 0;
+  meta_.resize(nshell_);
+  for (int s = 0; s < nshell_; ++s) {
+    const int l = basis.shell_l[s];
+    CT_REQUIRE(basis.shell_atom[s] >= 0 && basis.shell_atom[s] < natom, "shell centre index out of range");
+    CT_REQUIRE(basis.shell_nprim[s] > 0, "shell without primitives");
+    CT_REQUIRE(basis.shell_prim_off[s] >= 0 &&
+                   (size_t)basis.shell_prim_off[s] + basis.shell_nprim[s] <= basis.exps.size() &&
+                   basis.exps.size() == basis.coefs.size(),
+               "primitive offsets out of range");
+    meta_[s] = ShellMeta{shell_code(l), basis.shell_nprim[s], basis.shell_prim_off[s], M_};
+    M_ += shell_nfunc(l);
+  }
+  basis_.nbasis = M_;
+  Mp_ = (int)round_up(M_, kRowBlock);
+  prim_.resize(basis.exps.size());
+  for (int s = 0; s < nshell_; ++s) {
+    const int L = std::abs(basis.shell_l[s]);
+    for (int k = 0; k < basis.shell_nprim[s]; ++k) {
+      const int j = basis.shell_prim_off[s] + k;
+      const double a = basis.exps[j];
+      CT_REQUIRE(a > 0.0 && std::isfinite(a), "exponents must be positive and finite");
+      prim_[j] = PrimData{a, basis.coefs[j] * std::sqrt(std::pow(2.0 * a / kPi, 1.5) * std::pow(4.0 * a, L))};
+    }
+  }
+  xyz_.assign(atom_xyz, atom_xyz + 3 * (size_t)natom);
+  charges_.assign(natom, 0.0);
+  if (charges) charges_.assign(charges, charges + natom);
+  numbers_.assign(natom, 0);
+  if (numbers) numbers_.assign(numbers, numbers + natom);
+  P_.assign(P, P + (size_t)M_ * M_);
+
+  CT_HIP_CHECK(hipGetDevice(&device_));
+  CT_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  if (!d_meta_.reserve(sizeof(ShellMeta) * nshell_) || !d_prim_.reserve(sizeof(PrimData) * prim_.size()) ||
+      !d_geom_.reserve(sizeof(ShellGeom) * nshell_) || !d_P_.reserve(sizeof(double) * (size_t)Mp_ * Mp_))
+    throw Error(CT_E_NOMEM, "cannot allocate the device shell table / density matrix");
+  CT_HIP_CHECK(hipMemcpy(d_meta_.get(), meta_.data(), sizeof(ShellMeta) * nshell_, hipMemcpyHostToDevice));
+  CT_HIP_CHECK(hipMemcpy(d_prim_.get(), prim_.data(), sizeof(PrimData) * prim_.size(), hipMemcpyHostToDevice));
+  upload_geometry();
+  upload_P();
+}
+
+Mol::~Mol() {
+  if (stream_) {
+    (void)hipStreamSynchronize(stream_);
+    (void)hipStreamDestroy(stream_);
+  }
+}
+
+void Mol::upload_geometry() {
+  std::vector<ShellGeom> geom(nshell_);
+  for (int s = 0; s < nshell_; ++s) {
+    const int a = basis_.shell_atom[s];
+    double amin = 1e300;
+    for (int k = 0; k < basis_.shell_nprim[s]; ++k) amin = std::min(amin, basis_.exps[basis_.shell_prim_off[s] + k]);
+    geom[s] = ShellGeom{xyz_[3 * a], xyz_[3 * a + 1], xyz_[3 * a + 2], amin};
+  }
+  CT_HIP_CHECK(hipMemcpy(d_geom_.get(), geom.data(), sizeof(ShellGeom) * nshell_, hipMemcpyHostToDevice));
+}
+
+void Mol::upload_P() {
+  // symmetrise: the descriptors depend on the symmetric part of P only
+  std::vector<double> Pp((size_t)Mp_ * Mp_, 0.0);
+  for (int i = 0; i < M_; ++i)
+    for (int j = 0; j < M_; ++j)
+      Pp[(size_t)i * Mp_ + j] = 0.5 * (P_[(size_t)i * M_ + j] + P_[(size_t)j * M_ + i]);
+  CT_HIP_CHECK(hipMemcpy(d_P_.get(), Pp.data(), sizeof(double) * Pp.size(), hipMemcpyHostToDevice));
+}
+
+void Mol::set_coordinates(const double* xyz) {
+  CT_REQUIRE(xyz != nullptr, "null coordinates");
+  CT_HIP_CHECK(hipStreamSynchronize(stream_));
+  if (is_device_pointer(xyz))
+    CT_HIP_CHECK(hipMemcpy(xyz_.data(), xyz, sizeof(double) * 3 * natom_, hipMemcpyDeviceToHost));
+  else
+    std::memcpy(xyz_.data(), xyz, sizeof(double) * 3 * natom_);
+  upload_geometry();
+}
+
+void Mol::set_density_matrix(const double* P) {
+  CT_REQUIRE(P != nullptr, "null density matrix");
+  CT_HIP_CHECK(hipStreamSynchronize(stream_));
+  if (is_device_pointer(P))
+    CT_HIP_CHECK(hipMemcpy(P_.data(), P, sizeof(double) * (size_t)M_ * M_, hipMemcpyDeviceToHost));
+  else
+    std::memcpy(P_.data(), P, sizeof(double) * (size_t)M_ * M_);
+  upload_P();
+}
+
+void Mol::synchronize() { CT_HIP_CHECK(hipStreamSynchronize(stream_)); }
+
+int64_t Mol::chunk_points(int mode, int64_t n) const {
+  const int64_t per_point = (int64_t)mode_nder(mode) * Mp_ * 8 + mode_ndot(mode) * 8 + 24 + 20 * 8;
+  int64_t nc = ws_limit_ / per_point / kPointAlign * kPointAlign;
+  nc = std::max<int64_t>(nc, kPointAlign);
+  return std::min<int64_t>(nc, round_up(n, kPointAlign));
+}
+
+void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const ct_eval_params& par,
+               const ct_outputs& out, ct_eval_report* rep) {
+  CT_REQUIRE(n >= 0, "negative number of points");
+  CT_REQUIRE(q != 0, "no quantity requested");
+  CT_REQUIRE(precision == CT_PREC_FP64 || precision == CT_PREC_MIXED, "unknown precision");
+  for (const OutSlot& s : kSlots)
+    if ((q & s.bit) && out.*(s.field) == nullptr) throw Error(CT_E_INVALID, "output array missing for a requested quantity");
+  if (rep) *rep = ct_eval_report{n, 0, 0, 0.0};
+  if (n == 0) return;
+  CT_REQUIRE(points != nullptr, "null points");
+  (void)precision;  // the mixed path is dispatched below once it exists
+
+  const int mode = select_mode(q);
+  const int nder = mode_nder(mode), ndot = mode_ndot(mode);
+  const bool pts_dev = is_device_pointer(points);
+  bool out_dev[sizeof(kSlots) / sizeof(kSlots[0])];
+  int stage_width = 0;
+  for (size_t k = 0; k < sizeof(kSlots) / sizeof(kSlots[0]); ++k) {
+    out_dev[k] = (q & kSlots[k].bit) ? is_device_pointer(out.*(kSlots[k].field)) : true;
+    if ((q & kSlots[k].bit) && !out_dev[k]) stage_width += kSlots[k].width;
+  }
+
+  int64_t Nc = chunk_points(mode, n);
+  auto bytes_for = [&](int64_t nc) {
+    return (size_t)nc * ((size_t)nder * Mp_ * 8 + (size_t)ndot * 8 + 24 + (size_t)stage_width * 8);
+  };
+  while (!d_ws_.reserve(bytes_for(Nc))) {
+    if (Nc <= kPointAlign) throw Error(CT_E_NOMEM, "cannot allocate the evaluation workspace");
+    Nc = std::max<int64_t>(kPointAlign, Nc / 2 / kPointAlign * kPointAlign);
+  }
+  char* ws = static_cast<char*>(d_ws_.get());
+  double* ao = reinterpret_cast<double*>(ws);
+  double* dots = ao + (size_t)nder * Mp_ * Nc;
+  double* pts_stage = dots + (size_t)ndot * Nc;
+  double* out_stage = pts_stage + 3 * (size_t)Nc;
+
+  int64_t chunks = 0;
+  for (int64_t start = 0; start < n; start += Nc) {
+    const int64_t nc = std::min(Nc, n - start);
+    const int64_t ld = round_up(nc, kPointAlign);
+    const double* pts = points + 3 * start;
+    if (!pts_dev) {
+      CT_HIP_CHECK(hipMemcpyAsync(pts_stage, pts, sizeof(double) * 3 * nc, hipMemcpyHostToDevice, stream_));
+      pts = pts_stage;
+    }
+    launch_ao_f64(mode, static_cast<const ShellMeta*>(d_meta_.get()), static_cast<const ShellGeom*>(d_geom_.get()),
+                  static_cast<const PrimData*>(d_prim_.get()), nshell_, M_, Mp_, pts, nc, ld, ao, 700.0, stream_);
+    launch_contract_f64(mode, static_cast<const double*>(d_P_.get()), Mp_, ao, ld, dots, stream_);
+
+    FinalizeArgs fa;
+    fa.dots = dots;
+    fa.ld = ld;
+    fa.n = nc;
+    fa.out_off = 0;
+    fa.mode = mode;
+    fa.q = q;
+    fa.par = par;
+    std::memset(&fa.out, 0, sizeof(fa.out));
+    double* stage_cursor = out_stage;
+    for (size_t k = 0; k < sizeof(kSlots) / sizeof(kSlots[0]); ++k) {
+      if (!(q & kSlots[k].bit)) continue;
+      double* user = out.*(kSlots[k].field);
+      if (out_dev[k]) {
+        fa.out.*(kSlots[k].field) = user + start * kSlots[k].width;
+      } else {
+        fa.out.*(kSlots[k].field) = stage_cursor;
+        stage_cursor += (size_t)kSlots[k].width * nc;
+      }
+    }
+    launch_finalize(fa, stream_);
+    for (size_t k = 0; k < sizeof(kSlots) / sizeof(kSlots[0]); ++k) {
+      if (!(q & kSlots[k].bit) || out_dev[k]) continue;
+      CT_HIP_CHECK(hipMemcpyAsync(out.*(kSlots[k].field) + start * kSlots[k].width, fa.out.*(kSlots[k].field),
+                                  sizeof(double) * kSlots[k].width * nc, hipMemcpyDeviceToHost, stream_));
+    }
+    ++chunks;
+  }
+  CT_HIP_CHECK(hipStreamSynchronize(stream_));
+  if (rep) rep->n_chunks = chunks;
+}
+
+}  // namespace ct

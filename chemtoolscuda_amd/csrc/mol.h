@@ -1,0 +1,106 @@
+// Host-side molecule handle: owns the device shell table, P, a HIP stream and
+// the chunk workspace.
+#pragma once
+
+#include <memory>
+
+#include "ct_internal.h"
+
+namespace ct {
+
+struct HostBasis {
+  std::vector<int32_t> shell_atom, shell_l, shell_nprim, shell_prim_off;
+  std::vector<double> exps, coefs;
+  int nbasis = 0;
+};
+
+struct FchkData {
+  HostBasis basis;
+  std::vector<double> atom_xyz;   // natom x 3 (bohr)
+  std::vector<double> charges;    // nuclear (core) charges
+  std::vector<int32_t> numbers;   // atomic numbers
+  std::vector<double> P;          // nbasis x nbasis, row-major
+};
+
+// Native reader for Gaussian formatted checkpoint files (fchk.cpp).
+FchkData load_fchk(const std::string& path);
+
+class DeviceBuffer {
+ public:
+  DeviceBuffer() = default;
+  ~DeviceBuffer() { release(); }
+  DeviceBuffer(const DeviceBuffer&) = delete;
+  DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+  void* get() const { return ptr_; }
+  size_t size() const { return bytes_; }
+  // grows (never shrinks) the allocation; returns false if hipMalloc fails
+  bool reserve(size_t bytes);
+  void release();
+
+ private:
+  void* ptr_ = nullptr;
+  size_t bytes_ = 0;
+};
+
+class PinnedBuffer {
+ public:
+  PinnedBuffer() = default;
+  ~PinnedBuffer() { release(); }
+  PinnedBuffer(const PinnedBuffer&) = delete;
+  PinnedBuffer& operator=(const PinnedBuffer&) = delete;
+  void* get() const { return ptr_; }
+  void reserve(size_t bytes);
+  void release();
+
+ private:
+  void* ptr_ = nullptr;
+  size_t bytes_ = 0;
+};
+
+class Mol {
+ public:
+  Mol(const HostBasis& basis, const double* atom_xyz, const double* charges, const int32_t* numbers, int natom,
+      const double* P);
+  ~Mol();
+
+  int nbasis() const { return M_; }
+  int nbasis_pad() const { return Mp_; }
+  int natom() const { return natom_; }
+  const std::vector<double>& coordinates() const { return xyz_; }
+  const std::vector<double>& charges() const { return charges_; }
+  const std::vector<int32_t>& numbers() const { return numbers_; }
+  const std::vector<double>& density_matrix() const { return P_; }
+  const HostBasis& basis() const { return basis_; }
+
+  void set_coordinates(const double* xyz);  // host or device
+  void set_density_matrix(const double* P);  // host or device
+  void set_workspace_limit(int64_t bytes) { ws_limit_ = bytes; }
+  hipStream_t stream() const { return stream_; }
+  void synchronize();
+
+  void eval(const double* points, int64_t n, uint32_t q, int precision, const ct_eval_params& par,
+            const ct_outputs& out, ct_eval_report* rep);
+
+ private:
+  void upload_geometry();
+  void upload_P();
+  int64_t chunk_points(int mode, int64_t n) const;
+
+  HostBasis basis_;
+  int M_ = 0, Mp_ = 0, natom_ = 0, nshell_ = 0;
+  std::vector<double> xyz_, charges_, P_;
+  std::vector<int32_t> numbers_;
+  std::vector<ShellMeta> meta_;
+  std::vector<PrimData> prim_;
+  int device_ = 0;
+  hipStream_t stream_ = nullptr;
+  DeviceBuffer d_meta_, d_geom_, d_prim_, d_P_;
+  DeviceBuffer d_ws_;      // AO + dots + staged points / outputs
+  PinnedBuffer h_stage_;   // host staging for pageable inputs/outputs
+  int64_t ws_limit_ = int64_t(8) << 30;
+};
+
+// true when ptr is device (HBM) memory usable directly by kernels
+bool is_device_pointer(const void* ptr);
+
+}  // namespace ct

@@ -1,0 +1,164 @@
+/*
+ * chemtools_amd.h -- C ABI of the MI355X density-descriptor evaluator.
+ *
+ * Plain pointers and sizes only.  Every function returns CT_OK (0) on success
+ * or a negative CT_E* status; the message of the last failure on the calling
+ * thread is available from ct_last_error().  Pointer arguments for points and
+ * outputs may be host memory or device (HBM) memory: the library detects
+ * which and copies through pinned staging buffers only when needed.
+ *
+ * Each ct_mol owns its device copies of the shell table and of P, its own HIP
+ * stream and workspace, so handles are independent and reentrant (the
+ * reference keeps the basis in one global __constant__ bank,
+ * src/basis_to_gpu.cu:13, which makes every call non-reentrant).  Calls on
+ * one handle from several threads must be serialised by the caller.
+ *
+ * Reference interfaces replaced (paths relative to qtchem/ChemToolsCUDA):
+ *   ct_mol_create            chemtools::Molecule::Molecule + IOData
+ *                            (src/pymolecule.cu:18-23, include/iodata.h:18-101)
+ *   ct_mol_create_from_fchk  get_molecular_basis_from_fchk (src/iodata.cpp:15-211)
+ *   ct_eval                  evaluate_electron_density_on_any_grid (src/evaluate_density.cu:663-840),
+ *                            evaluate_electron_density_gradient (src/evaluate_gradient.cu:1184-1371),
+ *                            evaluate_electron_density_hessian (src/evaluate_hessian.cu:1893-2307),
+ *                            evaluate_laplacian (src/evaluate_laplacian.cu:843-853),
+ *                            evaluate_positive_definite_kinetic_density and
+ *                            evaluate_general_kinetic_energy_density (src/evaluate_kinetic_dens.cu:182-214),
+ *                            compute_reduced_density_gradient, compute_weizsacker_ked,
+ *                            compute_thomas_fermi_energy_density, compute_ked_gradient_expansion_general,
+ *                            compute_general_ked, compute_shannon_information_density
+ *                            (src/evaluate_densbased.cu:44-223)
+ *   ct_norm_rows             compute_norm_of_3d_vector (src/evaluate_densbased.cu:10-41)
+ *   ct_esp                   compute_electrostatic_potential_over_points (src/evaluate_electrostatic.cu:10161-10372)
+ *   ct_backward              (no reference counterpart: adjoint of rho w.r.t. P and the centres)
+ */
+#ifndef CHEMTOOLS_AMD_H_
+#define CHEMTOOLS_AMD_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CT_OK 0
+#define CT_E_INVALID -1    /* bad argument (shape, null pointer, unsupported shell) */
+#define CT_E_HIP -2        /* HIP runtime error */
+#define CT_E_NOMEM -3      /* device allocation failed even at the smallest chunk */
+#define CT_E_IO -4         /* file could not be read or parsed */
+#define CT_E_UNSUPPORTED -5
+
+/* Shell table.  shell_l: 0..4 Cartesian, -2..-4 pure (reference convention,
+ * src/iodata.cpp:76-87); generalised / SP contractions are passed as separate
+ * shells sharing exponents.  coefs are contraction coefficients of
+ * L2-normalised primitives.  Basis functions are numbered shell by shell, fchk
+ * component order within a shell. */
+typedef struct {
+  int32_t nshell;
+  const int32_t* shell_atom;
+  const int32_t* shell_l;
+  const int32_t* shell_nprim;
+  const int32_t* shell_prim_off;
+  const double* exps;
+  const double* coefs;
+} ct_basis_desc;
+
+typedef struct ct_mol ct_mol;
+
+/* Quantity bits for ct_eval. */
+#define CT_Q_RHO (1u << 0)           /* rho                           out->rho      (N)      */
+#define CT_Q_GRAD (1u << 1)          /* grad rho                      out->grad     (N,3)    */
+#define CT_Q_HESS (1u << 2)          /* Hessian of rho                out->hess     (N,3,3)  */
+#define CT_Q_LAP (1u << 3)           /* Laplacian of rho              out->lap      (N)      */
+#define CT_Q_TAU_PD (1u << 4)        /* 1/2 sum_k d_k phi P d_k phi   out->tau_pd   (N)      */
+#define CT_Q_RDG (1u << 5)           /* reduced density gradient      out->rdg      (N)      */
+#define CT_Q_SHANNON (1u << 6)       /* rho ln rho                    out->shannon  (N)      */
+#define CT_Q_WEIZSACKER (1u << 7)    /* |grad rho|^2 / (8 rho)        out->weizsacker        */
+#define CT_Q_THOMAS_FERMI (1u << 8)  /* 0.3 (3 pi^2)^{2/3} rho^{5/3}  out->thomas_fermi      */
+#define CT_Q_GGE (1u << 9)           /* TF + a W + b lap              out->gge  (params a,b) */
+#define CT_Q_GENERAL_KED (1u << 10)  /* tau_PD + (a-1)/4 lap          out->general_ked (a)   */
+#define CT_Q_KED_ALPHA (1u << 11)    /* tau_PD + alpha lap            out->ked_alpha (alpha) */
+
+/* Precision. */
+#define CT_PREC_FP64 0        /* everything in FP64 (parity mode) */
+#define CT_PREC_MIXED 1       /* FP32 AOs, split-BF16 MFMA contraction with FP32 accumulation,
+                                 FP64 per-point sums/epilogues, FP64 re-evaluation of points
+                                 whose error estimate exceeds params.mixed_tol */
+
+typedef struct {
+  double gge_a, gge_b;     /* CT_Q_GGE */
+  double ked_a;            /* CT_Q_GENERAL_KED */
+  double ked_alpha;        /* CT_Q_KED_ALPHA */
+  double mixed_tol;        /* relative error target of the mixed path (0 => 1e-7) */
+  double rho_floor;        /* points with rho below this are not re-evaluated (0 => 1e-10) */
+} ct_eval_params;
+
+typedef struct {
+  double* rho;
+  double* grad;
+  double* hess;
+  double* lap;
+  double* tau_pd;
+  double* rdg;
+  double* shannon;
+  double* weizsacker;
+  double* thomas_fermi;
+  double* gge;
+  double* general_ked;
+  double* ked_alpha;
+} ct_outputs;
+
+typedef struct {
+  int64_t n_points;
+  int64_t n_reevaluated;   /* mixed path: points recomputed in FP64 */
+  int64_t n_chunks;
+  double max_error_estimate;
+} ct_eval_report;
+
+/* Lifetime ---------------------------------------------------------------- */
+int ct_mol_create(const ct_basis_desc* basis, const double* atom_xyz, const double* charges, int32_t natom,
+                  const double* P, int32_t nbasis, ct_mol** out);
+int ct_mol_create_from_fchk(const char* path, ct_mol** out);
+int ct_mol_destroy(ct_mol* mol);
+
+int ct_mol_nbasis(const ct_mol* mol);
+int ct_mol_natom(const ct_mol* mol);
+/* Copies of the molecule's data (host pointers). */
+int ct_mol_get_coordinates(const ct_mol* mol, double* xyz /* natom*3 */);
+int ct_mol_get_numbers(const ct_mol* mol, int32_t* numbers /* natom */);
+int ct_mol_get_charges(const ct_mol* mol, double* charges /* natom */);
+int ct_mol_get_density_matrix(const ct_mol* mol, double* P /* nbasis^2 */);
+/* Replace centres / P (host or device pointers). */
+int ct_mol_set_coordinates(ct_mol* mol, const double* atom_xyz);
+int ct_mol_set_density_matrix(ct_mol* mol, const double* P);
+/* Device-memory budget for the per-chunk workspace (bytes; default 8 GiB). */
+int ct_mol_set_workspace_limit(ct_mol* mol, int64_t bytes);
+
+/* Evaluation --------------------------------------------------------------- */
+/* points: (n,3) row-major.  Output arrays named by the quantity bits must be
+ * non-null; others are ignored.  report may be NULL. */
+int ct_eval(ct_mol* mol, const double* points, int64_t n, uint32_t quantities, int32_t precision,
+            const ct_eval_params* params, const ct_outputs* out, ct_eval_report* report);
+
+/* |v| of each row of an (n,3) array. */
+int ct_norm_rows(const double* v, int64_t n, double* out);
+
+/* Molecular electrostatic potential at (n,3) row-major points. */
+int ct_esp(ct_mol* mol, const double* points, int64_t n, int32_t precision, double* out);
+
+/* Adjoint of rho: given dL/drho at the points, dL/dP (nbasis x nbasis,
+ * row-major, accumulated as sum_g w_g phi_i phi_j) and dL/dR (natom x 3).
+ * Either output may be NULL. */
+int ct_backward(ct_mol* mol, const double* points, int64_t n, const double* dL_drho, int32_t precision,
+                double* dL_dP, double* dL_dR);
+
+/* Synchronise the handle's stream. */
+int ct_mol_synchronize(ct_mol* mol);
+
+const char* ct_last_error(void);
+const char* ct_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CHEMTOOLS_AMD_H_ */
