@@ -23,8 +23,11 @@
 // AO[k][p0 + j]; P (<= a few MB) and the chunk's AO rows stay L2-resident.
 // Waves are independent (no LDS, no barriers).
 //
-// MFMA 16x16x4 f64 lane layout (lane l, r = l % 16, q = l / 16):
-//   A operand: A[r][q];  B operand: B[q][r];  D: D[4 q + j][r], j = 0..3.
+// MFMA 16x16x4 f64 lane layout on gfx950 (lane l, r = l % 16, q = l / 16),
+// measured with an in-bounds probe:
+//   A operand: A[r][q];  B operand: B[q][r];  D: D[q + 4 j][r], j = 0..3
+// (rows interleaved -- unlike the f32/bf16 16x16 variants, whose D rows are
+// 4 q + j).
 
 #include "ct_internal.h"
 
@@ -100,7 +103,7 @@ __global__ __launch_bounds__(256) void contract_f64_kernel(const double* __restr
         const int64_t p = p0 + (NT == 1 ? 16 * cb : 0) + r;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-          const int64_t i = i0 + 16 * rb + 4 * q + j;
+          const int64_t i = i0 + 16 * rb + q + 4 * j;
           const double y = acc[rb][cb][j];
           const double* ai = ao + i * ld + p;
           if constexpr (MODE == MODE_RHO) {
