@@ -1,0 +1,170 @@
+"""Benchmark: grid points/s for rho, grad rho, lap rho (+ tau_PD) on one MI355X.
+
+Workload (default): the 60-atom organic-like system -- C30 N3 O3 H24,
+cc-pVDZ-like pure shells, M = 624 basis functions -- on its cubic grid
+(0.2 bohr spacing, 5 bohr padding, ~2.8M points), synthetic data from a fixed
+seed (no wavefunction files or network).  A step is one pass of the hot path
+over the whole grid: AO values + first derivatives + Laplacians, the fused
+P-contraction and the FP64 epilogue producing rho, grad rho, lap rho and
+tau_PD.  Points, outputs and P are resident in HBM before timing starts.
+
+Prints one JSON line (see --help for options).  `roofline` is the dominant
+kernel (the contraction): algorithmic flops 2 M^2 n_col per point (n_col = 4
+GEMM columns: phi and its three first derivatives) divided by its average
+launch time measured with HIP events on the library's stream; `cpu_baseline`
+is the FP64 host oracle (OpenMP) on a bounded seeded subsample of the same
+grid.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16": 2.5e15}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="organic60")
+    ap.add_argument("--precision", default="auto", choices=["auto", "fp64", "mixed"])
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU-baseline work (0 disables)")
+    ap.add_argument("--no-compare", action="store_true", help="skip the FP64-vs-mixed comparison run")
+    return ap.parse_args()
+
+
+def cpu_baseline(w, target_s):
+    from oracle import oracle
+
+    flat = w.basis.flat()
+    rng = np.random.default_rng(12345)
+    n_cal = 256
+    sample = w.points[rng.choice(w.points.shape[0], size=n_cal, replace=False)]
+    t0 = time.perf_counter()
+    oracle.density(flat, w.P, sample, rho=True, grad=True, lap=True, tau=True)
+    t_cal = time.perf_counter() - t0
+    n = int(min(w.points.shape[0], max(n_cal, n_cal * target_s / max(t_cal, 1e-6))))
+    sample = w.points[rng.choice(w.points.shape[0], size=n, replace=False)]
+    t0 = time.perf_counter()
+    oracle.density(flat, w.P, sample, rho=True, grad=True, lap=True, tau=True)
+    dt = time.perf_counter() - t0
+    return {"value": n / dt, "unit": "points/s", "cores": oracle.max_threads(), "kind": "port",
+            "sample": f"{n} seeded random points of the {w.name} grid, rho+grad+lap+tau_PD in FP64 "
+                      f"({dt:.1f} s, OpenMP, {oracle.max_threads()} threads)"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(points shard across GPUs with no exchange in the forward pass; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from chemtoolscuda_amd import capi, synthetic
+
+    meta = json.load(open(os.path.join(ROOT, "BASELINE.json")))
+    w = synthetic.WORKLOADS[args.workload]()
+    order = synthetic.morton_order(w.points)
+    pts_host = np.ascontiguousarray(w.points[order])
+    n = pts_host.shape[0]
+    M = w.basis.nbasis
+
+    dev = torch.device("cuda:0")
+    pts = torch.from_numpy(pts_host).to(dev)
+    quantities = ["rho", "grad", "lap", "tau_pd"]
+
+    def alloc():
+        return {q: torch.empty((n, capi.WIDTH.get(q, 1)), dtype=torch.float64, device=dev) for q in quantities}
+
+    h = capi.Handle(w.basis.flat(), w.P, w.charges)
+    precision = args.precision
+    if precision == "auto":
+        precision = "mixed" if os.environ.get("CT_BENCH_MIXED", "0") == "1" else "fp64"
+
+    def run(outs, prec):
+        return h.eval(pts.data_ptr(), n, {q: t.data_ptr() for q, t in outs.items()}, prec)
+
+    outs = alloc()
+    for _ in range(args.warmup):
+        run(outs, precision)
+    torch.cuda.synchronize()
+    h.enable_profiling(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rep = None
+    for _ in range(args.steps):
+        rep = run(outs, precision)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    stats = h.kernel_stats()
+    h.enable_profiling(False)
+    value = n * args.steps / dt
+
+    ncol = 4
+    flops_pt = 2.0 * M * M * ncol + 2.0 * M * 8
+    contract_ms = stats["contract_ms"] / max(stats["contract_calls"], 1)
+    pts_per_launch = n * args.steps / max(stats["contract_calls"], 1)
+    achieved = flops_pt * pts_per_launch / (contract_ms * 1e-3)
+    peak_key = "fp64" if precision == "fp64" else "bf16"
+    roofline = {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK[peak_key] / 1e12, "unit": "TFLOP/s",
+                "frac": achieved / PEAK[peak_key], "traffic": None,
+                "kernel": "contract_" + ("f64" if precision == "fp64" else "mixed"),
+                "avg_launch_ms": contract_ms, "points_per_launch": pts_per_launch,
+                "flops_per_point": flops_pt}
+
+    extra = {}
+    if precision == "mixed" and not args.no_compare:
+        ref = alloc()
+        t1 = time.perf_counter()
+        run(ref, "fp64")
+        torch.cuda.synchronize()
+        extra["fp64_points_per_s"] = n / (time.perf_counter() - t1)
+        extra["mixed_over_fp64"] = value / extra["fp64_points_per_s"]
+        rho = ref["rho"][:, 0]
+        mask = rho >= 1e-8
+        for q in quantities:
+            a, b = outs[q], ref[q]
+            scale = torch.clamp(torch.linalg.vector_norm(b, dim=1), min=1e-300)
+            rel = torch.linalg.vector_norm(a - b, dim=1) / scale
+            extra[f"max_rel_err_{q}"] = float(rel[mask].max())
+        extra["n_reevaluated"] = rep.get("n_reevaluated") if rep else None
+
+    kernel_ms = {k: stats[f"{k}_ms"] / args.steps for k in ("ao", "contract", "finalize", "reeval")}
+    line = {
+        "metric": meta["metric"],
+        "value": value,
+        "unit": "points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if precision == "fp64" else "fp32+bf16x3",
+        "data": "synthetic (seeded generator, chemtoolscuda_amd/synthetic.py)",
+        "config": {"workload": f"{w.name}: {w.basis.natom} atoms, M={M}, {n} grid points, rho+grad+lap+tau_PD",
+                   "nbasis": M, "n_points": n, "precision": precision, "quantities": quantities},
+        "roofline": roofline,
+        "kernel_ms_per_step": kernel_ms,
+        **extra,
+    }
+    if args.cpu_seconds > 0:
+        line["cpu_baseline"] = cpu_baseline(w, args.cpu_seconds)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,151 @@
+"""ctypes binding of the C ABI (include/chemtools_amd.h).
+
+Used where device pointers cross the boundary (torch tensors' data_ptr() in
+the autograd wrapper and the benchmark); the pybind11 ``Molecule`` covers the
+host-array API.  Raises on any non-zero status with the library's message.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libchemtools_amd.so")
+
+Q = {
+    "rho": 1 << 0, "grad": 1 << 1, "hess": 1 << 2, "lap": 1 << 3, "tau_pd": 1 << 4, "rdg": 1 << 5,
+    "shannon": 1 << 6, "weizsacker": 1 << 7, "thomas_fermi": 1 << 8, "gge": 1 << 9, "general_ked": 1 << 10,
+    "ked_alpha": 1 << 11,
+}
+WIDTH = {"grad": 3, "hess": 9}
+PREC = {"fp64": 0, "mixed": 1}
+
+
+class BasisDesc(ctypes.Structure):
+    _fields_ = [("nshell", ctypes.c_int32), ("shell_atom", ctypes.c_void_p), ("shell_l", ctypes.c_void_p),
+                ("shell_nprim", ctypes.c_void_p), ("shell_prim_off", ctypes.c_void_p), ("exps", ctypes.c_void_p),
+                ("coefs", ctypes.c_void_p)]
+
+
+class EvalParams(ctypes.Structure):
+    _fields_ = [("gge_a", ctypes.c_double), ("gge_b", ctypes.c_double), ("ked_a", ctypes.c_double),
+                ("ked_alpha", ctypes.c_double), ("mixed_tol", ctypes.c_double), ("rho_floor", ctypes.c_double)]
+
+
+class Outputs(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_void_p) for k in Q]
+
+
+class EvalReport(ctypes.Structure):
+    _fields_ = [("n_points", ctypes.c_int64), ("n_reevaluated", ctypes.c_int64), ("n_chunks", ctypes.c_int64),
+                ("max_error_estimate", ctypes.c_double)]
+
+
+class KernelStats(ctypes.Structure):
+    _fields_ = [(f"{k}_ms", ctypes.c_double) for k in ("ao", "contract", "finalize", "esp", "backward", "reeval")] + \
+               [(f"{k}_calls", ctypes.c_int64) for k in ("ao", "contract", "finalize", "esp", "backward", "reeval")]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} missing: build with `python -m chemtoolscuda_amd.build_ext`")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i32, i64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64
+        L.ct_mol_create.argtypes = [ctypes.POINTER(BasisDesc), vp, vp, i32, vp, i32, ctypes.POINTER(vp)]
+        L.ct_mol_create_from_fchk.argtypes = [ctypes.c_char_p, ctypes.POINTER(vp)]
+        L.ct_mol_destroy.argtypes = [vp]
+        L.ct_eval.argtypes = [vp, vp, i64, ctypes.c_uint32, i32, ctypes.POINTER(EvalParams), ctypes.POINTER(Outputs),
+                              ctypes.POINTER(EvalReport)]
+        L.ct_esp.argtypes = [vp, vp, i64, i32, vp]
+        L.ct_backward.argtypes = [vp, vp, i64, vp, i32, vp, vp]
+        L.ct_mol_set_coordinates.argtypes = [vp, vp]
+        L.ct_mol_set_density_matrix.argtypes = [vp, vp]
+        L.ct_mol_set_workspace_limit.argtypes = [vp, i64]
+        L.ct_mol_synchronize.argtypes = [vp]
+        L.ct_mol_enable_profiling.argtypes = [vp, ctypes.c_int]
+        L.ct_mol_kernel_stats.argtypes = [vp, ctypes.POINTER(KernelStats)]
+        L.ct_mol_nbasis.argtypes = [vp]
+        L.ct_mol_natom.argtypes = [vp]
+        L.ct_last_error.restype = ctypes.c_char_p
+        L.ct_version.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+def check(status: int) -> None:
+    if status != 0:
+        raise RuntimeError(f"chemtools_amd status {status}: {lib().ct_last_error().decode(errors='replace')}")
+
+
+class Handle:
+    """Owns a ct_mol*; built from a flat basis (``BasisSet.flat()``) and host arrays."""
+
+    def __init__(self, flat, P, charges):
+        import numpy as np
+
+        self._keep = {k: np.ascontiguousarray(flat[k], dtype=np.int32) for k in
+                      ("shell_atom", "shell_l", "shell_nprim", "shell_prim_off")}
+        self._keep["exps"] = np.ascontiguousarray(flat["exps"], dtype=np.float64)
+        self._keep["coefs"] = np.ascontiguousarray(flat["coefs"], dtype=np.float64)
+        xyz = np.ascontiguousarray(flat["atom_xyz"], dtype=np.float64)
+        Z = np.ascontiguousarray(charges, dtype=np.float64)
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        k = self._keep
+        desc = BasisDesc(len(k["shell_l"]), k["shell_atom"].ctypes.data, k["shell_l"].ctypes.data,
+                         k["shell_nprim"].ctypes.data, k["shell_prim_off"].ctypes.data, k["exps"].ctypes.data,
+                         k["coefs"].ctypes.data)
+        h = ctypes.c_void_p()
+        check(lib().ct_mol_create(ctypes.byref(desc), xyz.ctypes.data, Z.ctypes.data, xyz.shape[0], P.ctypes.data,
+                                  P.shape[0], ctypes.byref(h)))
+        self.ptr = h
+        self.nbasis = P.shape[0]
+        self.natom = xyz.shape[0]
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            lib().ct_mol_destroy(self.ptr)
+            self.ptr = None
+
+    def eval(self, points_ptr: int, n: int, outputs: dict, precision: str = "fp64", params: dict | None = None):
+        """outputs: {quantity: pointer}; pointers may be host or device."""
+        q = 0
+        o = Outputs()
+        for name, ptr in outputs.items():
+            q |= Q[name]
+            setattr(o, name, ptr)
+        par = EvalParams(**(params or {}))
+        rep = EvalReport()
+        check(lib().ct_eval(self.ptr, points_ptr, n, q, PREC[precision], ctypes.byref(par), ctypes.byref(o),
+                            ctypes.byref(rep)))
+        return {k: getattr(rep, k) for k, _ in EvalReport._fields_}
+
+    def esp(self, points_ptr: int, n: int, out_ptr: int, precision: str = "fp64"):
+        check(lib().ct_esp(self.ptr, points_ptr, n, PREC[precision], out_ptr))
+
+    def backward(self, points_ptr: int, n: int, w_ptr: int, dP_ptr, dR_ptr, precision: str = "fp64"):
+        check(lib().ct_backward(self.ptr, points_ptr, n, w_ptr, PREC[precision], dP_ptr, dR_ptr))
+
+    def set_coordinates(self, ptr: int):
+        check(lib().ct_mol_set_coordinates(self.ptr, ptr))
+
+    def set_density_matrix(self, ptr: int):
+        check(lib().ct_mol_set_density_matrix(self.ptr, ptr))
+
+    def set_workspace_limit(self, nbytes: int):
+        check(lib().ct_mol_set_workspace_limit(self.ptr, int(nbytes)))
+
+    def synchronize(self):
+        check(lib().ct_mol_synchronize(self.ptr))
+
+    def enable_profiling(self, on: bool = True):
+        check(lib().ct_mol_enable_profiling(self.ptr, 1 if on else 0))
+
+    def kernel_stats(self) -> dict:
+        st = KernelStats()
+        check(lib().ct_mol_kernel_stats(self.ptr, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in KernelStats._fields_}

@@ -226,6 +226,23 @@ int ct_mol_synchronize(ct_mol* mol) {
   return guard([&] { M(mol).synchronize(); });
 }
 
+int ct_mol_enable_profiling(ct_mol* mol, int enable) {
+  return guard([&] {
+    M(mol).synchronize();
+    M(mol).timer().collect();
+    M(mol).timer().enable(enable != 0);
+  });
+}
+
+int ct_mol_kernel_stats(ct_mol* mol, ct_kernel_stats* out) {
+  return guard([&] {
+    CT_REQUIRE(out, "null output");
+    M(mol).synchronize();
+    M(mol).timer().collect();
+    *out = M(mol).timer().stats();
+  });
+}
+
 const char* ct_last_error(void) { return g_last_error.c_str(); }
 
 const char* ct_version(void) { return "chemtools-amd 0.1 (gfx950)"; }

@@ -123,6 +123,74 @@ void PinnedBuffer::release() {
   bytes_ = 0;
 }
 
+KernelTimer::~KernelTimer() {
+  for (auto& p : pending_) {
+    pool_.push_back(p.a);
+    pool_.push_back(p.b);
+  }
+  for (hipEvent_t e : pool_) (void)hipEventDestroy(e);
+}
+
+hipEvent_t KernelTimer::take() {
+  if (!pool_.empty()) {
+    hipEvent_t e = pool_.back();
+    pool_.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  CT_HIP_CHECK(hipEventCreate(&e));
+  return e;
+}
+
+void KernelTimer::enable(bool on) {
+  on_ = on;
+  for (int k = 0; k < NKIND; ++k) {
+    ms_[k] = 0.0;
+    calls_[k] = 0;
+  }
+}
+
+void KernelTimer::begin(Kind k, hipStream_t s) {
+  if (!on_) return;
+  Pending p{k, take(), take()};
+  CT_HIP_CHECK(hipEventRecord(p.a, s));
+  pending_.push_back(p);
+}
+
+void KernelTimer::end(hipStream_t s) {
+  if (!on_ || pending_.empty()) return;
+  CT_HIP_CHECK(hipEventRecord(pending_.back().b, s));
+}
+
+void KernelTimer::collect() {
+  for (auto& p : pending_) {
+    float ms = 0.f;
+    CT_HIP_CHECK(hipEventElapsedTime(&ms, p.a, p.b));
+    ms_[p.kind] += ms;
+    calls_[p.kind] += 1;
+    pool_.push_back(p.a);
+    pool_.push_back(p.b);
+  }
+  pending_.clear();
+}
+
+ct_kernel_stats KernelTimer::stats() const {
+  ct_kernel_stats st;
+  st.ao_ms = ms_[AO];
+  st.contract_ms = ms_[CONTRACT];
+  st.finalize_ms = ms_[FINALIZE];
+  st.esp_ms = ms_[ESP];
+  st.backward_ms = ms_[BACKWARD];
+  st.reeval_ms = ms_[REEVAL];
+  st.ao_calls = calls_[AO];
+  st.contract_calls = calls_[CONTRACT];
+  st.finalize_calls = calls_[FINALIZE];
+  st.esp_calls = calls_[ESP];
+  st.backward_calls = calls_[BACKWARD];
+  st.reeval_calls = calls_[REEVAL];
+  return st;
+}
+
 Mol::Mol(const HostBasis& basis, const double* atom_xyz, const double* charges, const int32_t* numbers, int natom,
          const double* P)
     : basis_(basis), natom_(natom) {
@@ -276,9 +344,13 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
       CT_HIP_CHECK(hipMemcpyAsync(pts_stage, pts, sizeof(double) * 3 * nc, hipMemcpyHostToDevice, stream_));
       pts = pts_stage;
     }
+    timer_.begin(KernelTimer::AO, stream_);
     launch_ao_f64(mode, static_cast<const ShellMeta*>(d_meta_.get()), static_cast<const ShellGeom*>(d_geom_.get()),
                   static_cast<const PrimData*>(d_prim_.get()), nshell_, M_, Mp_, pts, nc, ld, ao, 700.0, stream_);
+    timer_.end(stream_);
+    timer_.begin(KernelTimer::CONTRACT, stream_);
     launch_contract_f64(mode, static_cast<const double*>(d_P_.get()), Mp_, ao, ld, dots, stream_);
+    timer_.end(stream_);
 
     FinalizeArgs fa;
     fa.dots = dots;
@@ -300,7 +372,9 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
         stage_cursor += (size_t)kSlots[k].width * nc;
       }
     }
+    timer_.begin(KernelTimer::FINALIZE, stream_);
     launch_finalize(fa, stream_);
+    timer_.end(stream_);
     for (size_t k = 0; k < sizeof(kSlots) / sizeof(kSlots[0]); ++k) {
       if (!(q & kSlots[k].bit) || out_dev[k]) continue;
       CT_HIP_CHECK(hipMemcpyAsync(out.*(kSlots[k].field) + start * kSlots[k].width, fa.out.*(kSlots[k].field),
@@ -309,6 +383,7 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
     ++chunks;
   }
   CT_HIP_CHECK(hipStreamSynchronize(stream_));
+  timer_.collect();
   if (rep) rep->n_chunks = chunks;
 }
 

@@ -57,6 +57,32 @@ class PinnedBuffer {
   size_t bytes_ = 0;
 };
 
+// Records HIP events around kernel launches on one stream and accumulates
+// their elapsed times when collected.
+class KernelTimer {
+ public:
+  enum Kind { AO = 0, CONTRACT, FINALIZE, ESP, BACKWARD, REEVAL, NKIND };
+  ~KernelTimer();
+  void enable(bool on);
+  bool enabled() const { return on_; }
+  void begin(Kind k, hipStream_t s);
+  void end(hipStream_t s);
+  void collect();  // after a stream synchronisation
+  ct_kernel_stats stats() const;
+
+ private:
+  struct Pending {
+    Kind kind;
+    hipEvent_t a, b;
+  };
+  bool on_ = false;
+  std::vector<Pending> pending_;
+  std::vector<hipEvent_t> pool_;
+  double ms_[NKIND] = {0};
+  int64_t calls_[NKIND] = {0};
+  hipEvent_t take();
+};
+
 class Mol {
  public:
   Mol(const HostBasis& basis, const double* atom_xyz, const double* charges, const int32_t* numbers, int natom,
@@ -77,6 +103,7 @@ class Mol {
   void set_workspace_limit(int64_t bytes) { ws_limit_ = bytes; }
   hipStream_t stream() const { return stream_; }
   void synchronize();
+  KernelTimer& timer() { return timer_; }
 
   void eval(const double* points, int64_t n, uint32_t q, int precision, const ct_eval_params& par,
             const ct_outputs& out, ct_eval_report* rep);
@@ -98,6 +125,7 @@ class Mol {
   DeviceBuffer d_ws_;      // AO + dots + staged points / outputs
   PinnedBuffer h_stage_;   // host staging for pageable inputs/outputs
   int64_t ws_limit_ = int64_t(8) << 30;
+  KernelTimer timer_;
 };
 
 // true when ptr is device (HBM) memory usable directly by kernels

@@ -154,6 +154,15 @@ int ct_backward(ct_mol* mol, const double* points, int64_t n, const double* dL_d
 /* Synchronise the handle's stream. */
 int ct_mol_synchronize(ct_mol* mol);
 
+/* Kernel timing with HIP events recorded on the handle's own stream around
+ * every launch (enable resets the counters). */
+typedef struct {
+  double ao_ms, contract_ms, finalize_ms, esp_ms, backward_ms, reeval_ms;
+  int64_t ao_calls, contract_calls, finalize_calls, esp_calls, backward_calls, reeval_calls;
+} ct_kernel_stats;
+int ct_mol_enable_profiling(ct_mol* mol, int enable);
+int ct_mol_kernel_stats(ct_mol* mol, ct_kernel_stats* out);
+
 const char* ct_last_error(void);
 const char* ct_version(void);
 
