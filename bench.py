@@ -41,6 +41,7 @@ def parse():
     ap.add_argument("--precision", default="auto", choices=["auto", "fp64", "mixed"])
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU-baseline work (0 disables)")
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64-vs-mixed comparison run")
+    ap.add_argument("--workspace-mb", type=int, default=0, help="per-chunk workspace budget (0: library default)")
     return ap.parse_args()
 
 
@@ -89,9 +90,11 @@ def main():
         return {q: torch.empty((n, capi.WIDTH.get(q, 1)), dtype=torch.float64, device=dev) for q in quantities}
 
     h = capi.Handle(w.basis.flat(), w.P, w.charges)
+    if args.workspace_mb > 0:
+        h.set_workspace_limit(args.workspace_mb << 20)
     precision = args.precision
     if precision == "auto":
-        precision = "mixed" if os.environ.get("CT_BENCH_MIXED", "0") == "1" else "fp64"
+        precision = "mixed"
 
     def run(outs, prec):
         return h.eval(pts.data_ptr(), n, {q: t.data_ptr() for q, t in outs.items()}, prec)
@@ -134,11 +137,19 @@ def main():
         extra["mixed_over_fp64"] = value / extra["fp64_points_per_s"]
         rho = ref["rho"][:, 0]
         mask = rho >= 1e-8
+        # rho, tau: pointwise relative.  grad: against max(|grad|, sqrt(8 rho tau)) (the
+        # Cauchy-Schwarz scale of d_k phi.P phi for a PSD P: grad vanishes at critical
+        # points).  lap: against |lap - 4 tau| + 4 tau, the size of its two summands.
+        tau = ref["tau_pd"][:, 0]
+        scales = {
+            "rho": rho.abs(),
+            "grad": torch.maximum(torch.linalg.vector_norm(ref["grad"], dim=1), torch.sqrt(8 * rho.abs() * tau.abs())),
+            "lap": (ref["lap"][:, 0] - 4 * tau).abs() + 4 * tau.abs(),
+            "tau_pd": tau.abs(),
+        }
         for q in quantities:
-            a, b = outs[q], ref[q]
-            scale = torch.clamp(torch.linalg.vector_norm(b, dim=1), min=1e-300)
-            rel = torch.linalg.vector_norm(a - b, dim=1) / scale
-            extra[f"max_rel_err_{q}"] = float(rel[mask].max())
+            err = torch.linalg.vector_norm(outs[q] - ref[q], dim=1) / torch.clamp(scales[q], min=1e-300)
+            extra[f"max_rel_err_{q}"] = float(err[mask].max())
         extra["n_reevaluated"] = rep.get("n_reevaluated") if rep else None
 
     kernel_ms = {k: stats[f"{k}_ms"] / args.steps for k in ("ao", "contract", "finalize", "reeval")}

@@ -28,6 +28,25 @@ namespace ct {
 
 namespace {
 
+// exp(-a r^2) with the argument formed in FP64.  For the FP32 path only the
+// reduced argument goes through the FP32 exponential: with t = a r^2 / ln 2,
+// n = rint(t), exp(-a r^2) = 2^-(t - n) * 2^-n, so the FP32 rounding of the
+// argument (|t - n| <= 1/2) costs at most ~2^-25 relative instead of
+// |a r^2| * 2^-24 (which reaches 5e-6 near the exp underflow limit).
+template <typename T>
+__device__ __forceinline__ T gauss_exp(double a, double r2d);
+template <>
+__device__ __forceinline__ double gauss_exp<double>(double a, double r2d) {
+  return exp(-a * r2d);
+}
+template <>
+__device__ __forceinline__ float gauss_exp<float>(double a, double r2d) {
+  const double t = a * r2d * 1.4426950408889634074;
+  const double n = rint(t);
+  const float f = (float)(t - n);
+  return ldexpf(exp2f(-f), -(int)n);
+}
+
 template <typename T, int MODE>
 __global__ __launch_bounds__(256) void ao_eval_kernel(const ShellMeta* __restrict__ meta,
                                                       const ShellGeom* __restrict__ geom,
@@ -42,15 +61,16 @@ __global__ __launch_bounds__(256) void ao_eval_kernel(const ShellMeta* __restric
   for (int s = 0; s < nshell; ++s) {
     const ShellMeta m = meta[s];
     const ShellGeom g = geom[s];
-    const T x = (T)(px - g.cx), y = (T)(py - g.cy), z = (T)(pz - g.cz);
-    const T r2 = x * x + y * y + z * z;
+    const double dx = px - g.cx, dy = py - g.cy, dz = pz - g.cz;
+    const double r2d = dx * dx + dy * dy + dz * dz;
+    const T x = (T)dx, y = (T)dy, z = (T)dz;
     T s0 = 0, s1 = 0, s2 = 0;
-    const bool far = (T)g.amin * r2 > screen;
+    const bool far = g.amin * r2d > (double)screen;
     if (!__all(far)) {
       for (int k = 0; k < m.nprim; ++k) {
         const PrimData pd = prim[m.prim_off + k];
         const T a = (T)pd.a;
-        const T e = (T)pd.c * exp(-a * r2);
+        const T e = (T)pd.c * gauss_exp<T>(pd.a, r2d);
         s0 += e;
         if constexpr (MODE != MODE_RHO) {
           const T m2a = T(-2) * a;
@@ -92,6 +112,31 @@ void launch_ao_f64(int mode, const ShellMeta* meta, const ShellGeom* geom, const
       break;
     default:
       hipLaunchKernelGGL((ao_eval_kernel<double, MODE_HESS>), grid, block, 0, s, meta, geom, prim, nshell, nbasis,
+                         nbasis_pad, pts, npts, ld, ao, screen);
+      break;
+  }
+  CT_HIP_CHECK(hipGetLastError());
+}
+
+void launch_ao_f32(int mode, const ShellMeta* meta, const ShellGeom* geom, const PrimData* prim, int nshell,
+                   int nbasis, int nbasis_pad, const double* pts, int64_t npts, int64_t ld, float* ao, float screen,
+                   hipStream_t s) {
+  const dim3 block(256), grid((unsigned)(ld / 256));
+  switch (mode) {
+    case MODE_RHO:
+      hipLaunchKernelGGL((ao_eval_kernel<float, MODE_RHO>), grid, block, 0, s, meta, geom, prim, nshell, nbasis,
+                         nbasis_pad, pts, npts, ld, ao, screen);
+      break;
+    case MODE_GRAD:
+      hipLaunchKernelGGL((ao_eval_kernel<float, MODE_GRAD>), grid, block, 0, s, meta, geom, prim, nshell, nbasis,
+                         nbasis_pad, pts, npts, ld, ao, screen);
+      break;
+    case MODE_LAP:
+      hipLaunchKernelGGL((ao_eval_kernel<float, MODE_LAP>), grid, block, 0, s, meta, geom, prim, nshell, nbasis,
+                         nbasis_pad, pts, npts, ld, ao, screen);
+      break;
+    default:
+      hipLaunchKernelGGL((ao_eval_kernel<float, MODE_HESS>), grid, block, 0, s, meta, geom, prim, nshell, nbasis,
                          nbasis_pad, pts, npts, ld, ao, screen);
       break;
   }

@@ -169,7 +169,7 @@ int ct_eval(ct_mol* mol, const double* points, int64_t n, uint32_t quantities, i
     CT_REQUIRE(out != nullptr, "null outputs");
     ct_eval_params p{};
     if (params) p = *params;
-    if (p.mixed_tol <= 0) p.mixed_tol = 1e-7;
+    if (p.mixed_tol <= 0) p.mixed_tol = 1e-6;
     if (p.rho_floor <= 0) p.rho_floor = 1e-10;
     M(mol).eval(points, n, quantities, precision, p, *out, report);
   });

@@ -37,6 +37,8 @@ namespace {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
+constexpr int kIBlock = 64;  // rows of P per wave pass (4 MFMA row blocks); the basis padding is a multiple
+
 template <int MODE>
 __global__ __launch_bounds__(256) void contract_f64_kernel(const double* __restrict__ P, int mp,
                                                            const double* __restrict__ ao, int64_t ld,
@@ -62,7 +64,7 @@ __global__ __launch_bounds__(256) void contract_f64_kernel(const double* __restr
 #pragma unroll
   for (int k = 0; k < NACC; ++k) dacc[k] = 0.0;
 
-  for (int i0 = 0; i0 < mp; i0 += kRowBlock) {
+  for (int i0 = 0; i0 < mp; i0 += kIBlock) {
     d4 acc[4][4];
 #pragma unroll
     for (int rb = 0; rb < 4; ++rb)

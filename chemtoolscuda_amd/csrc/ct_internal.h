@@ -59,8 +59,14 @@ struct PrimData {
   double c;
 };
 
+// Error model of the mixed path: |rho_mixed - rho| <= gamma * |phi|^T |P| |phi|.
+// FP32 AO rounding contributes ~2u|phi|^T|P||phi| (u = 2^-24) and the
+// three-piece bf16 products with FP32 accumulation a few u more; 2^-21 leaves
+// margin over the largest ratio observed.
+constexpr double kMixedGamma = 4.76837158203125e-07;  // 2^-21
+
 // Granularities.
-constexpr int kRowBlock = 64;      // contraction I-block (rows of P per pass)
+constexpr int kRowBlock = 128;     // basis padding: rows of P per contraction block
 constexpr int kPointAlign = 256;   // chunk stride alignment (points)
 
 // Evaluation modes of the fused contraction (which GEMM columns and dots).
@@ -78,7 +84,9 @@ constexpr int mode_ndot(int m) { return m == MODE_RHO ? 1 : m == MODE_GRAD ? 4 :
 // ---------------------------------------------------------------------------
 // Kernel launchers (defined in the .hip files).
 struct FinalizeArgs {
-  const double* dots;
+  const double* dots;     // FP64 path: [ndot][ld];  mixed path: partials [nrt][ndot+1][ld]
+  const int32_t* index;   // optional: output index of point p is out_off + index[p]
+  int nrt;                // mixed path: number of 128-row blocks to sum
   int64_t ld;
   int64_t n;        // points in this chunk
   int64_t out_off;  // index of the chunk's first point in the output arrays
@@ -94,6 +102,24 @@ void launch_ao_f64(int mode, const ShellMeta* meta, const ShellGeom* geom, const
 void launch_contract_f64(int mode, const double* P, int nbasis_pad, const double* ao, int64_t ld, double* dots,
                          hipStream_t s);
 void launch_finalize(const FinalizeArgs& a, hipStream_t s);
+
+// Mixed path.
+void launch_ao_f32(int mode, const ShellMeta* meta, const ShellGeom* geom, const PrimData* prim, int nshell,
+                   int nbasis, int nbasis_pad, const double* pts, int64_t npts, int64_t ld, float* ao, float screen,
+                   hipStream_t s);
+int mixed_points_per_block(int mode);
+void launch_contract_mixed(int mode, const uint16_t* Pp, int nbasis_pad, const float* ao, int64_t ld, double* part,
+                           hipStream_t s);
+struct FlagArgs {
+  double gamma;          // error estimate = gamma * B (B = |phi|^T |P| |phi|)
+  double tol;            // relative tolerance on rho
+  double rho_floor;      // points with |rho| below this are not re-evaluated
+  int32_t* count;        // number of flagged points
+  int32_t* index;        // flagged point indices (chunk-relative)
+  unsigned long long* max_est;  // max estimate / |rho| over accepted points (bit pattern of a double)
+};
+void launch_finalize_mixed(const FinalizeArgs& a, const FlagArgs& f, hipStream_t s);
+void launch_gather_points(const double* pts, const int32_t* index, int64_t n, double* out, hipStream_t s);
 void launch_norm_rows(const double* v, int64_t n, double* out, hipStream_t s);
 
 }  // namespace ct

@@ -11,6 +11,10 @@
 //   * general(ish) KED = tau_PD + (a-1)/4 lap (src/evaluate_densbased.cu:195-208)
 //   * t_alpha = tau_PD + alpha lap (src/evaluate_kinetic_dens.cu:195-214)
 //   * Shannon = rho ln rho, unguarded (src/evaluate_densbased.cu:210-223)
+//
+// The mixed path's finaliser also sums the per-row-block partials, forms the
+// error estimate gamma * |phi|^T |P| |phi| and appends the points whose
+// estimate exceeds tol * |rho| to a list for FP64 re-evaluation.
 
 #include "ct_internal.h"
 
@@ -20,26 +24,20 @@ namespace {
 
 __device__ __forceinline__ double masked_div(double num, double den) { return den > 1e-20 ? num / den : 1e-20; }
 
-__global__ __launch_bounds__(256) void finalize_kernel(FinalizeArgs a) {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= a.n) return;
-  const double* D = a.dots;
-  const int64_t ld = a.ld;
-  auto dot = [&](int d) { return D[(int64_t)d * ld + p]; };
-  const int64_t o = a.out_off + p;
-
-  const double rho = dot(0);
+// dot[] layout: see mode_ndot() in ct_internal.h
+__device__ void emit_outputs(const FinalizeArgs& a, int64_t o, const double* dot) {
+  const double rho = dot[0];
   double g[3] = {0.0, 0.0, 0.0};
   if (a.mode != MODE_RHO)
-    for (int k = 0; k < 3; ++k) g[k] = 2.0 * dot(1 + k);
+    for (int k = 0; k < 3; ++k) g[k] = 2.0 * dot[1 + k];
   double lap = 0.0, tau = 0.0;
   if (a.mode == MODE_LAP) {
-    const double t = dot(5) + dot(6) + dot(7);
-    lap = 2.0 * (dot(4) + t);
+    const double t = dot[5] + dot[6] + dot[7];
+    lap = 2.0 * (dot[4] + t);
     tau = 0.5 * t;
   } else if (a.mode == MODE_HESS) {
-    const double t = dot(10) + dot(12) + dot(15);
-    lap = 2.0 * (dot(4) + dot(7) + dot(9) + t);
+    const double t = dot[10] + dot[12] + dot[15];
+    lap = 2.0 * (dot[4] + dot[7] + dot[9] + t);
     tau = 0.5 * t;
   }
   const uint32_t q = a.q;
@@ -51,7 +49,7 @@ __global__ __launch_bounds__(256) void finalize_kernel(FinalizeArgs a) {
     const int hs[3][3] = {{4, 5, 6}, {5, 7, 8}, {6, 8, 9}};
     const int ts[3][3] = {{10, 11, 13}, {11, 12, 14}, {13, 14, 15}};
     for (int k = 0; k < 3; ++k)
-      for (int l = 0; l < 3; ++l) a.out.hess[9 * o + 3 * k + l] = 2.0 * (dot(hs[k][l]) + dot(ts[k][l]));
+      for (int l = 0; l < 3; ++l) a.out.hess[9 * o + 3 * k + l] = 2.0 * (dot[hs[k][l]] + dot[ts[k][l]]);
   }
   if (q & CT_Q_LAP) a.out.lap[o] = lap;
   if (q & CT_Q_TAU_PD) a.out.tau_pd[o] = tau;
@@ -62,13 +60,63 @@ __global__ __launch_bounds__(256) void finalize_kernel(FinalizeArgs a) {
     a.out.rdg[o] = pref * masked_div(sqrt(gnorm2), pow(rho, 4.0 / 3.0));
   }
   if (q & CT_Q_SHANNON) a.out.shannon[o] = rho * log(rho);
-  const double tf = 0.3 * pow(pi2x3, 2.0 / 3.0) * pow(rho, 5.0 / 3.0);
-  const double w = masked_div(gnorm2, 8.0 * rho);
-  if (q & CT_Q_WEIZSACKER) a.out.weizsacker[o] = w;
-  if (q & CT_Q_THOMAS_FERMI) a.out.thomas_fermi[o] = tf;
-  if (q & CT_Q_GGE) a.out.gge[o] = tf + a.par.gge_a * w + a.par.gge_b * lap;
+  if (q & (CT_Q_WEIZSACKER | CT_Q_THOMAS_FERMI | CT_Q_GGE)) {
+    const double tf = 0.3 * pow(pi2x3, 2.0 / 3.0) * pow(rho, 5.0 / 3.0);
+    const double w = masked_div(gnorm2, 8.0 * rho);
+    if (q & CT_Q_WEIZSACKER) a.out.weizsacker[o] = w;
+    if (q & CT_Q_THOMAS_FERMI) a.out.thomas_fermi[o] = tf;
+    if (q & CT_Q_GGE) a.out.gge[o] = tf + a.par.gge_a * w + a.par.gge_b * lap;
+  }
   if (q & CT_Q_GENERAL_KED) a.out.general_ked[o] = tau + (a.par.ked_a - 1.0) / 4.0 * lap;
   if (q & CT_Q_KED_ALPHA) a.out.ked_alpha[o] = tau + a.par.ked_alpha * lap;
+}
+
+__global__ __launch_bounds__(256) void finalize_kernel(FinalizeArgs a) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= a.n) return;
+  const int nd = mode_ndot(a.mode);
+  double dot[16];
+  for (int d = 0; d < 16; ++d) dot[d] = d < nd ? a.dots[(int64_t)d * a.ld + p] : 0.0;
+  const int64_t o = a.out_off + (a.index ? (int64_t)a.index[p] : p);
+  emit_outputs(a, o, dot);
+}
+
+__global__ __launch_bounds__(256) void finalize_mixed_kernel(FinalizeArgs a, FlagArgs f) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= a.n) return;
+  const int nd = mode_ndot(a.mode);
+  const int nd1 = nd + 1;
+  double dot[16];
+  for (int d = 0; d < 16; ++d) dot[d] = 0.0;
+  double bound = 0.0;
+  for (int rt = 0; rt < a.nrt; ++rt) {
+    const double* base = a.dots + (int64_t)rt * nd1 * a.ld + p;
+    for (int d = 0; d < 16; ++d)
+      if (d < nd) dot[d] += base[(int64_t)d * a.ld];
+    bound += base[(int64_t)nd * a.ld];
+  }
+  emit_outputs(a, a.out_off + p, dot);
+  const double arho = fabs(dot[0]);
+  if (arho < f.rho_floor) return;
+  const double est = f.gamma * bound;
+  if (!(est <= f.tol * arho)) {  // also catches NaN
+    const int k = atomicAdd(f.count, 1);
+    f.index[k] = (int32_t)p;
+  } else {
+    const double rel = est / arho;
+    atomicMax(f.max_est, (unsigned long long)__double_as_longlong(rel));
+  }
+}
+
+__global__ __launch_bounds__(256) void gather_points_kernel(const double* __restrict__ pts,
+                                                            const int32_t* __restrict__ index, int64_t n,
+                                                            double* __restrict__ out) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int64_t p = index[k];
+  out[3 * k] = pts[3 * p];
+  out[3 * k + 1] = pts[3 * p + 1];
+  out[3 * k + 2] = pts[3 * p + 2];
 }
 
 __global__ __launch_bounds__(256) void norm_rows_kernel(const double* __restrict__ v, int64_t n,
@@ -84,6 +132,18 @@ __global__ __launch_bounds__(256) void norm_rows_kernel(const double* __restrict
 void launch_finalize(const FinalizeArgs& a, hipStream_t s) {
   if (a.n <= 0) return;
   hipLaunchKernelGGL(finalize_kernel, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, s, a);
+  CT_HIP_CHECK(hipGetLastError());
+}
+
+void launch_finalize_mixed(const FinalizeArgs& a, const FlagArgs& f, hipStream_t s) {
+  if (a.n <= 0) return;
+  hipLaunchKernelGGL(finalize_mixed_kernel, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, s, a, f);
+  CT_HIP_CHECK(hipGetLastError());
+}
+
+void launch_gather_points(const double* pts, const int32_t* index, int64_t n, double* out, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(gather_points_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, pts, index, n, out);
   CT_HIP_CHECK(hipGetLastError());
 }
 

@@ -261,6 +261,21 @@ void Mol::upload_geometry() {
   CT_HIP_CHECK(hipMemcpy(d_geom_.get(), geom.data(), sizeof(ShellGeom) * nshell_, hipMemcpyHostToDevice));
 }
 
+namespace {
+uint16_t bf16_bits(float x) {
+  uint32_t u;
+  std::memcpy(&u, &x, 4);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+float bf16_float(uint16_t h) {
+  uint32_t u = (uint32_t)h << 16;
+  float x;
+  std::memcpy(&x, &u, 4);
+  return x;
+}
+}  // namespace
+
 void Mol::upload_P() {
   // symmetrise: the descriptors depend on the symmetric part of P only
   std::vector<double> Pp((size_t)Mp_ * Mp_, 0.0);
@@ -268,6 +283,21 @@ void Mol::upload_P() {
     for (int j = 0; j < M_; ++j)
       Pp[(size_t)i * Mp_ + j] = 0.5 * (P_[(size_t)i * M_ + j] + P_[(size_t)j * M_ + i]);
   CT_HIP_CHECK(hipMemcpy(d_P_.get(), Pp.data(), sizeof(double) * Pp.size(), hipMemcpyHostToDevice));
+  // mixed path: FP32 P split exactly into three bf16 pieces
+  const size_t plane = (size_t)Mp_ * Mp_;
+  std::vector<uint16_t> Pb(3 * plane);
+  for (size_t k = 0; k < plane; ++k) {
+    const float x = (float)Pp[k];
+    const uint16_t h = bf16_bits(x);
+    const float r1 = x - bf16_float(h);
+    const uint16_t m = bf16_bits(r1);
+    const uint16_t l = bf16_bits(r1 - bf16_float(m));
+    Pb[k] = h;
+    Pb[plane + k] = m;
+    Pb[2 * plane + k] = l;
+  }
+  if (!d_Pb_.reserve(sizeof(uint16_t) * Pb.size())) throw Error(CT_E_NOMEM, "cannot allocate the bf16 density matrix");
+  CT_HIP_CHECK(hipMemcpy(d_Pb_.get(), Pb.data(), sizeof(uint16_t) * Pb.size(), hipMemcpyHostToDevice));
 }
 
 void Mol::set_coordinates(const double* xyz) {
@@ -292,11 +322,33 @@ void Mol::set_density_matrix(const double* P) {
 
 void Mol::synchronize() { CT_HIP_CHECK(hipStreamSynchronize(stream_)); }
 
-int64_t Mol::chunk_points(int mode, int64_t n) const {
-  const int64_t per_point = (int64_t)mode_nder(mode) * Mp_ * 8 + mode_ndot(mode) * 8 + 24 + 20 * 8;
+int64_t Mol::chunk_points(int64_t per_point, int64_t n) const {
   int64_t nc = ws_limit_ / per_point / kPointAlign * kPointAlign;
   nc = std::max<int64_t>(nc, kPointAlign);
   return std::min<int64_t>(nc, round_up(n, kPointAlign));
+}
+
+// FP64 evaluation of `count` points of the current chunk (index = chunk-
+// relative positions), written into the chunk's outputs through the index map.
+void Mol::reevaluate(int mode, const double* pts, int64_t count, const int32_t* index, const FinalizeArgs& fin) {
+  const int nder = mode_nder(mode), ndot = mode_ndot(mode);
+  const int64_t ld = round_up(count, kPointAlign);
+  const size_t bytes = (size_t)ld * ((size_t)nder * Mp_ * 8 + (size_t)ndot * 8 + 24);
+  if (!d_ws64_.reserve(bytes)) throw Error(CT_E_NOMEM, "cannot allocate the FP64 re-evaluation workspace");
+  double* ao = static_cast<double*>(d_ws64_.get());
+  double* dots = ao + (size_t)nder * Mp_ * ld;
+  double* gp = dots + (size_t)ndot * ld;
+  launch_gather_points(pts, index, count, gp, stream_);
+  launch_ao_f64(mode, static_cast<const ShellMeta*>(d_meta_.get()), static_cast<const ShellGeom*>(d_geom_.get()),
+                static_cast<const PrimData*>(d_prim_.get()), nshell_, M_, Mp_, gp, count, ld, ao, 700.0, stream_);
+  launch_contract_f64(mode, static_cast<const double*>(d_P_.get()), Mp_, ao, ld, dots, stream_);
+  FinalizeArgs f = fin;
+  f.dots = dots;
+  f.index = index;
+  f.nrt = 0;
+  f.ld = ld;
+  f.n = count;
+  launch_finalize(f, stream_);
 }
 
 void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const ct_eval_params& par,
@@ -309,33 +361,50 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
   if (rep) *rep = ct_eval_report{n, 0, 0, 0.0};
   if (n == 0) return;
   CT_REQUIRE(points != nullptr, "null points");
-  (void)precision;  // the mixed path is dispatched below once it exists
 
+  const bool mixed = precision == CT_PREC_MIXED;
   const int mode = select_mode(q);
   const int nder = mode_nder(mode), ndot = mode_ndot(mode);
+  const int nrt = Mp_ / kRowBlock;
   const bool pts_dev = is_device_pointer(points);
-  bool out_dev[sizeof(kSlots) / sizeof(kSlots[0])];
+  constexpr size_t NS = sizeof(kSlots) / sizeof(kSlots[0]);
+  bool out_dev[NS];
   int stage_width = 0;
-  for (size_t k = 0; k < sizeof(kSlots) / sizeof(kSlots[0]); ++k) {
+  for (size_t k = 0; k < NS; ++k) {
     out_dev[k] = (q & kSlots[k].bit) ? is_device_pointer(out.*(kSlots[k].field)) : true;
     if ((q & kSlots[k].bit) && !out_dev[k]) stage_width += kSlots[k].width;
   }
 
-  int64_t Nc = chunk_points(mode, n);
-  auto bytes_for = [&](int64_t nc) {
-    return (size_t)nc * ((size_t)nder * Mp_ * 8 + (size_t)ndot * 8 + 24 + (size_t)stage_width * 8);
-  };
-  while (!d_ws_.reserve(bytes_for(Nc))) {
+  const int64_t per_point = mixed ? (int64_t)nder * Mp_ * 4 + (int64_t)nrt * (ndot + 1) * 8 + 4 + 24 + stage_width * 8
+                                  : (int64_t)nder * Mp_ * 8 + (int64_t)ndot * 8 + 24 + stage_width * 8;
+  int64_t Nc = chunk_points(per_point, n);
+  const size_t head = 256;  // flag counter + max estimate
+  while (!d_ws_.reserve(head + (size_t)Nc * per_point)) {
     if (Nc <= kPointAlign) throw Error(CT_E_NOMEM, "cannot allocate the evaluation workspace");
     Nc = std::max<int64_t>(kPointAlign, Nc / 2 / kPointAlign * kPointAlign);
   }
   char* ws = static_cast<char*>(d_ws_.get());
-  double* ao = reinterpret_cast<double*>(ws);
-  double* dots = ao + (size_t)nder * Mp_ * Nc;
-  double* pts_stage = dots + (size_t)ndot * Nc;
-  double* out_stage = pts_stage + 3 * (size_t)Nc;
+  int32_t* d_count = reinterpret_cast<int32_t*>(ws);
+  unsigned long long* d_maxest = reinterpret_cast<unsigned long long*>(ws + 8);
+  char* cur = ws + head;
+  void* ao = cur;
+  cur += (size_t)Nc * nder * Mp_ * (mixed ? 4 : 8);
+  double* dots = reinterpret_cast<double*>(cur);
+  cur += (size_t)Nc * (mixed ? nrt * (ndot + 1) : ndot) * 8;
+  double* pts_stage = reinterpret_cast<double*>(cur);
+  cur += (size_t)Nc * 24;
+  double* out_stage = reinterpret_cast<double*>(cur);
+  cur += (size_t)Nc * stage_width * 8;
+  int32_t* flag_idx = reinterpret_cast<int32_t*>(cur);
+  h_small_.reserve(64);
+  auto* h_small = static_cast<int64_t*>(h_small_.get());
 
-  int64_t chunks = 0;
+  const auto* meta = static_cast<const ShellMeta*>(d_meta_.get());
+  const auto* geom = static_cast<const ShellGeom*>(d_geom_.get());
+  const auto* prim = static_cast<const PrimData*>(d_prim_.get());
+  if (mixed) CT_HIP_CHECK(hipMemsetAsync(d_maxest, 0, 8, stream_));
+
+  int64_t chunks = 0, reevaluated = 0;
   for (int64_t start = 0; start < n; start += Nc) {
     const int64_t nc = std::min(Nc, n - start);
     const int64_t ld = round_up(nc, kPointAlign);
@@ -344,25 +413,19 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
       CT_HIP_CHECK(hipMemcpyAsync(pts_stage, pts, sizeof(double) * 3 * nc, hipMemcpyHostToDevice, stream_));
       pts = pts_stage;
     }
-    timer_.begin(KernelTimer::AO, stream_);
-    launch_ao_f64(mode, static_cast<const ShellMeta*>(d_meta_.get()), static_cast<const ShellGeom*>(d_geom_.get()),
-                  static_cast<const PrimData*>(d_prim_.get()), nshell_, M_, Mp_, pts, nc, ld, ao, 700.0, stream_);
-    timer_.end(stream_);
-    timer_.begin(KernelTimer::CONTRACT, stream_);
-    launch_contract_f64(mode, static_cast<const double*>(d_P_.get()), Mp_, ao, ld, dots, stream_);
-    timer_.end(stream_);
-
     FinalizeArgs fa;
+    std::memset(&fa, 0, sizeof(fa));
     fa.dots = dots;
+    fa.index = nullptr;
+    fa.nrt = nrt;
     fa.ld = ld;
     fa.n = nc;
     fa.out_off = 0;
     fa.mode = mode;
     fa.q = q;
     fa.par = par;
-    std::memset(&fa.out, 0, sizeof(fa.out));
     double* stage_cursor = out_stage;
-    for (size_t k = 0; k < sizeof(kSlots) / sizeof(kSlots[0]); ++k) {
+    for (size_t k = 0; k < NS; ++k) {
       if (!(q & kSlots[k].bit)) continue;
       double* user = out.*(kSlots[k].field);
       if (out_dev[k]) {
@@ -372,19 +435,62 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
         stage_cursor += (size_t)kSlots[k].width * nc;
       }
     }
-    timer_.begin(KernelTimer::FINALIZE, stream_);
-    launch_finalize(fa, stream_);
-    timer_.end(stream_);
-    for (size_t k = 0; k < sizeof(kSlots) / sizeof(kSlots[0]); ++k) {
+
+    if (!mixed) {
+      timer_.begin(KernelTimer::AO, stream_);
+      launch_ao_f64(mode, meta, geom, prim, nshell_, M_, Mp_, pts, nc, ld, static_cast<double*>(ao), 700.0, stream_);
+      timer_.end(stream_);
+      timer_.begin(KernelTimer::CONTRACT, stream_);
+      launch_contract_f64(mode, static_cast<const double*>(d_P_.get()), Mp_, static_cast<const double*>(ao), ld,
+                          dots, stream_);
+      timer_.end(stream_);
+      timer_.begin(KernelTimer::FINALIZE, stream_);
+      launch_finalize(fa, stream_);
+      timer_.end(stream_);
+    } else {
+      CT_HIP_CHECK(hipMemsetAsync(d_count, 0, 4, stream_));
+      timer_.begin(KernelTimer::AO, stream_);
+      launch_ao_f32(mode, meta, geom, prim, nshell_, M_, Mp_, pts, nc, ld, static_cast<float*>(ao), 50.0f, stream_);
+      timer_.end(stream_);
+      timer_.begin(KernelTimer::CONTRACT, stream_);
+      launch_contract_mixed(mode, static_cast<const uint16_t*>(d_Pb_.get()), Mp_, static_cast<const float*>(ao), ld,
+                            dots, stream_);
+      timer_.end(stream_);
+      FlagArgs fl;
+      fl.gamma = kMixedGamma;
+      fl.tol = par.mixed_tol;
+      fl.rho_floor = par.rho_floor;
+      fl.count = d_count;
+      fl.index = flag_idx;
+      fl.max_est = d_maxest;
+      timer_.begin(KernelTimer::FINALIZE, stream_);
+      launch_finalize_mixed(fa, fl, stream_);
+      timer_.end(stream_);
+      CT_HIP_CHECK(hipMemcpyAsync(h_small, d_count, 4, hipMemcpyDeviceToHost, stream_));
+      CT_HIP_CHECK(hipStreamSynchronize(stream_));
+      const int64_t count = (int64_t) * reinterpret_cast<int32_t*>(h_small);
+      if (count > 0) {
+        timer_.begin(KernelTimer::REEVAL, stream_);
+        reevaluate(mode, pts, count, flag_idx, fa);
+        timer_.end(stream_);
+        reevaluated += count;
+      }
+    }
+    for (size_t k = 0; k < NS; ++k) {
       if (!(q & kSlots[k].bit) || out_dev[k]) continue;
       CT_HIP_CHECK(hipMemcpyAsync(out.*(kSlots[k].field) + start * kSlots[k].width, fa.out.*(kSlots[k].field),
                                   sizeof(double) * kSlots[k].width * nc, hipMemcpyDeviceToHost, stream_));
     }
     ++chunks;
   }
+  if (mixed) CT_HIP_CHECK(hipMemcpyAsync(h_small + 1, d_maxest, 8, hipMemcpyDeviceToHost, stream_));
   CT_HIP_CHECK(hipStreamSynchronize(stream_));
   timer_.collect();
-  if (rep) rep->n_chunks = chunks;
+  if (rep) {
+    rep->n_chunks = chunks;
+    rep->n_reevaluated = reevaluated;
+    if (mixed) std::memcpy(&rep->max_error_estimate, h_small + 1, 8);
+  }
 }
 
 }  // namespace ct

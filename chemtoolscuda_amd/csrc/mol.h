@@ -111,7 +111,8 @@ class Mol {
  private:
   void upload_geometry();
   void upload_P();
-  int64_t chunk_points(int mode, int64_t n) const;
+  int64_t chunk_points(int64_t per_point, int64_t n) const;
+  void reevaluate(int mode, const double* pts, int64_t count, const int32_t* index, const FinalizeArgs& fin);
 
   HostBasis basis_;
   int M_ = 0, Mp_ = 0, natom_ = 0, nshell_ = 0;
@@ -122,7 +123,10 @@ class Mol {
   int device_ = 0;
   hipStream_t stream_ = nullptr;
   DeviceBuffer d_meta_, d_geom_, d_prim_, d_P_;
+  DeviceBuffer d_Pb_;      // mixed path: bf16 pieces of P (hi, mid, lo), [3][Mp][Mp]
   DeviceBuffer d_ws_;      // AO + dots + staged points / outputs
+  DeviceBuffer d_ws64_;    // mixed path: FP64 re-evaluation of flagged points
+  PinnedBuffer h_small_;   // pinned readback of the flag counter / error estimate
   PinnedBuffer h_stage_;   // host staging for pageable inputs/outputs
   int64_t ws_limit_ = int64_t(8) << 30;
   KernelTimer timer_;

@@ -88,7 +88,7 @@ typedef struct {
   double gge_a, gge_b;     /* CT_Q_GGE */
   double ked_a;            /* CT_Q_GENERAL_KED */
   double ked_alpha;        /* CT_Q_KED_ALPHA */
-  double mixed_tol;        /* relative error target of the mixed path (0 => 1e-7) */
+  double mixed_tol;        /* relative error target of the mixed path (0 => 1e-6) */
   double rho_floor;        /* points with rho below this are not re-evaluated (0 => 1e-10) */
 } ct_eval_params;
 

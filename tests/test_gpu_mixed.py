@@ -1,0 +1,109 @@
+"""Mixed-precision path (FP32 AOs, three-piece bf16 split MFMA contraction,
+FP64 epilogue, FP64 re-evaluation of flagged points) against the FP64 path
+and the oracle.
+
+Tolerances (the target of the mixed path): rho to <= 1e-6 relative where
+rho >= 1e-8.  Gradient, Laplacian, tau_PD and Hessian are compared against
+the magnitude of the terms they are summed from (the per-point condition
+scale), also at 1e-6: |d rho| against 2 sum_i |d phi_i (P phi)_i| and so on,
+because the gradient vanishes at critical points and the Laplacian changes
+sign, where a pointwise relative error has no meaning.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ct():
+    import chemtoolscuda_amd
+
+    return chemtoolscuda_amd
+
+
+def _condition_scales(flat, P, pts):
+    ao = oracle.ao(flat, pts, 2)
+    aP = np.abs(P)
+    phi = ao[0]
+    y0 = aP @ np.abs(phi)
+    s_rho = np.sum(np.abs(phi) * y0, axis=0)
+    s_grad = np.sqrt(sum((2 * np.sum(np.abs(ao[1 + k]) * y0, axis=0)) ** 2 for k in range(3)))
+    lapphi = ao[4] + ao[7] + ao[9]
+    s_tau = 0.5 * sum(np.sum(np.abs(ao[1 + k]) * (aP @ np.abs(ao[1 + k])), axis=0) for k in range(3))
+    s_lap = 2 * np.sum(np.abs(lapphi) * y0, axis=0) + 4 * s_tau
+    return s_rho, s_grad, s_lap, s_tau
+
+
+def _check(got, ref, scale, tol, mask):
+    a = np.asarray(got).reshape(len(scale), -1)
+    b = np.asarray(ref).reshape(len(scale), -1)
+    err = np.linalg.norm(a - b, axis=1) / scale
+    worst = float(err[mask].max())
+    assert worst <= tol, worst
+    return worst
+
+
+@pytest.mark.parametrize("workload,npts", [("water", 6000), ("organic60", 20000)])
+def test_mixed_matches_fp64_and_oracle(ct, workload, npts):
+    from chemtoolscuda_amd import synthetic
+
+    w = synthetic.WORKLOADS[workload]()
+    rng = np.random.default_rng(21)
+    pts = w.points[rng.choice(w.points.shape[0], size=min(npts, w.points.shape[0]), replace=False)]
+    mol = ct.molecule_from_basis(w.basis, w.P, w.charges)
+    names = ["rho", "grad", "lap", "tau_pd"]
+    mix = mol.compute(pts, names, precision="mixed")
+    rep = mol.last_report()
+    f64 = mol.compute(pts, names, precision="fp64")
+    rho = f64["rho"]
+    mask = rho >= 1e-8
+    rel = np.abs(mix["rho"] - rho) / np.abs(rho)
+    assert rel[mask].max() <= 1e-6, rel[mask].max()
+    # condition-scaled errors on a subsample (the oracle's AO arrays are large)
+    sub = np.arange(0, pts.shape[0], max(1, pts.shape[0] // 1500))
+    s_rho, s_grad, s_lap, s_tau = _condition_scales(w.basis.flat(), w.P, pts[sub])
+    m = mask[sub]
+    _check(mix["grad"][sub], f64["grad"][sub], s_grad, 1e-6, m)
+    _check(mix["lap"][sub], f64["lap"][sub], s_lap, 1e-6, m)
+    _check(mix["tau_pd"][sub], f64["tau_pd"][sub], s_tau, 1e-6, m)
+    # and the FP64 path still agrees with the oracle on the same points
+    ref = oracle.density(w.basis.flat(), w.P, pts[sub], rho=True)
+    assert np.max(np.abs(f64["rho"][sub] - ref["rho"]) / np.abs(ref["rho"])) < 1e-12
+    print(f"{workload}: max rel err rho {rel[mask].max():.2e}, re-evaluated {rep['n_reevaluated']} "
+          f"of {pts.shape[0]}, max estimate {rep['max_error_estimate']:.2e}")
+
+
+def test_mixed_hessian_and_rho_modes(ct):
+    from chemtoolscuda_amd import synthetic
+
+    w = synthetic.water(3000)
+    mol = ct.molecule_from_basis(w.basis, w.P, w.charges)
+    pts = w.points
+    f64 = mol.compute(pts, ["rho", "grad", "hess"], precision="fp64")
+    mix = mol.compute(pts, ["rho", "grad", "hess"], precision="mixed")
+    mask = f64["rho"] >= 1e-8
+    assert (np.abs(mix["rho"] - f64["rho"]) / f64["rho"])[mask].max() <= 1e-6
+    hscale = np.abs(f64["hess"]).reshape(len(pts), -1).max(axis=1) + np.abs(f64["rho"])
+    assert (np.abs(mix["hess"] - f64["hess"]).reshape(len(pts), -1).max(axis=1) / hscale)[mask].max() <= 2e-6
+    for names in (["rho"], ["rho", "grad"]):
+        m2 = mol.compute(pts, names, precision="mixed")
+        assert (np.abs(m2["rho"] - f64["rho"]) / f64["rho"])[mask].max() <= 1e-6
+
+
+def test_forced_reevaluation_reproduces_fp64(ct):
+    """A zero-width tolerance flags every point: the mixed call then returns the FP64 results."""
+    from chemtoolscuda_amd import synthetic
+
+    w = synthetic.water(2500)
+    mol = ct.molecule_from_basis(w.basis, w.P, w.charges)
+    names = ["rho", "grad", "lap", "tau_pd"]
+    f64 = mol.compute(w.points, names, precision="fp64")
+    mix = mol.compute(w.points, names, precision="mixed", mixed_tol=1e-300)
+    rep = mol.last_report()
+    floor_ok = np.abs(f64["rho"]) >= 1e-10
+    assert rep["n_reevaluated"] == int(floor_ok.sum())
+    for k in names:
+        assert np.array_equal(mix[k][floor_ok], f64[k][floor_ok]), k
