@@ -114,8 +114,9 @@ struct FlagArgs {
   double gamma;          // error estimate = gamma * B (B = |phi|^T |P| |phi|)
   double tol;            // relative tolerance on rho
   double rho_floor;      // points with |rho| below this are not re-evaluated
-  int32_t* count;        // number of flagged points
-  int32_t* index;        // flagged point indices (chunk-relative)
+  int32_t* count;        // number of flagged points (appended to, not reset, across chunks)
+  int32_t* index;        // flagged point indices
+  int64_t index_base;    // added to the chunk-relative index of a flagged point
   unsigned long long* max_est;  // max estimate / |rho| over accepted points (bit pattern of a double)
 };
 void launch_finalize_mixed(const FinalizeArgs& a, const FlagArgs& f, hipStream_t s);

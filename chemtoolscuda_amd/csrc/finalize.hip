@@ -101,7 +101,7 @@ __global__ __launch_bounds__(256) void finalize_mixed_kernel(FinalizeArgs a, Fla
   const double est = f.gamma * bound;
   if (!(est <= f.tol * arho)) {  // also catches NaN
     const int k = atomicAdd(f.count, 1);
-    f.index[k] = (int32_t)p;
+    f.index[k] = (int32_t)(f.index_base + p);
   } else {
     const double rel = est / arho;
     atomicMax(f.max_est, (unsigned long long)__double_as_longlong(rel));

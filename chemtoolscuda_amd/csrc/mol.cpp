@@ -402,7 +402,21 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
   const auto* meta = static_cast<const ShellMeta*>(d_meta_.get());
   const auto* geom = static_cast<const ShellGeom*>(d_geom_.get());
   const auto* prim = static_cast<const PrimData*>(d_prim_.get());
-  if (mixed) CT_HIP_CHECK(hipMemsetAsync(d_maxest, 0, 8, stream_));
+  // With device-resident points and outputs, flagged points of all chunks are
+  // collected in one list and re-evaluated in a single FP64 pass at the end
+  // (one large launch instead of one small launch per chunk).
+  bool all_dev = pts_dev && n < (int64_t(1) << 31);
+  for (size_t k = 0; k < NS; ++k) all_dev = all_dev && out_dev[k];
+  const bool batch = mixed && all_dev;
+  int32_t* list = flag_idx;
+  if (batch) {
+    if (!d_flags_.reserve(sizeof(int32_t) * (size_t)n)) throw Error(CT_E_NOMEM, "cannot allocate the flag list");
+    list = static_cast<int32_t*>(d_flags_.get());
+  }
+  if (mixed) {
+    CT_HIP_CHECK(hipMemsetAsync(d_maxest, 0, 8, stream_));
+    CT_HIP_CHECK(hipMemsetAsync(d_count, 0, 4, stream_));
+  }
 
   int64_t chunks = 0, reevaluated = 0;
   for (int64_t start = 0; start < n; start += Nc) {
@@ -448,7 +462,6 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
       launch_finalize(fa, stream_);
       timer_.end(stream_);
     } else {
-      CT_HIP_CHECK(hipMemsetAsync(d_count, 0, 4, stream_));
       timer_.begin(KernelTimer::AO, stream_);
       launch_ao_f32(mode, meta, geom, prim, nshell_, M_, Mp_, pts, nc, ld, static_cast<float*>(ao), 50.0f, stream_);
       timer_.end(stream_);
@@ -461,19 +474,23 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
       fl.tol = par.mixed_tol;
       fl.rho_floor = par.rho_floor;
       fl.count = d_count;
-      fl.index = flag_idx;
+      fl.index = list;
+      fl.index_base = batch ? start : 0;
       fl.max_est = d_maxest;
       timer_.begin(KernelTimer::FINALIZE, stream_);
       launch_finalize_mixed(fa, fl, stream_);
       timer_.end(stream_);
-      CT_HIP_CHECK(hipMemcpyAsync(h_small, d_count, 4, hipMemcpyDeviceToHost, stream_));
-      CT_HIP_CHECK(hipStreamSynchronize(stream_));
-      const int64_t count = (int64_t) * reinterpret_cast<int32_t*>(h_small);
-      if (count > 0) {
-        timer_.begin(KernelTimer::REEVAL, stream_);
-        reevaluate(mode, pts, count, flag_idx, fa);
-        timer_.end(stream_);
-        reevaluated += count;
+      if (!batch) {
+        CT_HIP_CHECK(hipMemcpyAsync(h_small, d_count, 4, hipMemcpyDeviceToHost, stream_));
+        CT_HIP_CHECK(hipStreamSynchronize(stream_));
+        const int64_t count = (int64_t) * reinterpret_cast<int32_t*>(h_small);
+        if (count > 0) {
+          timer_.begin(KernelTimer::REEVAL, stream_);
+          reevaluate(mode, pts, count, flag_idx, fa);
+          timer_.end(stream_);
+          reevaluated += count;
+        }
+        CT_HIP_CHECK(hipMemsetAsync(d_count, 0, 4, stream_));
       }
     }
     for (size_t k = 0; k < NS; ++k) {
@@ -482,6 +499,28 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
                                   sizeof(double) * kSlots[k].width * nc, hipMemcpyDeviceToHost, stream_));
     }
     ++chunks;
+  }
+  if (batch) {
+    CT_HIP_CHECK(hipMemcpyAsync(h_small, d_count, 4, hipMemcpyDeviceToHost, stream_));
+    CT_HIP_CHECK(hipStreamSynchronize(stream_));
+    const int64_t count = (int64_t) * reinterpret_cast<int32_t*>(h_small);
+    if (count > 0) {
+      FinalizeArgs fa;
+      std::memset(&fa, 0, sizeof(fa));
+      fa.mode = mode;
+      fa.q = q;
+      fa.par = par;
+      for (size_t k = 0; k < NS; ++k)
+        if (q & kSlots[k].bit) fa.out.*(kSlots[k].field) = out.*(kSlots[k].field);
+      // the FP64 re-evaluation runs in chunks sized like the FP64 path's
+      const int64_t per64 = (int64_t)nder * Mp_ * 8 + (int64_t)ndot * 8 + 24;
+      const int64_t c64 = chunk_points(per64, count);
+      timer_.begin(KernelTimer::REEVAL, stream_);
+      for (int64_t s0 = 0; s0 < count; s0 += c64)
+        reevaluate(mode, points, std::min(c64, count - s0), list + s0, fa);
+      timer_.end(stream_);
+      reevaluated = count;
+    }
   }
   if (mixed) CT_HIP_CHECK(hipMemcpyAsync(h_small + 1, d_maxest, 8, hipMemcpyDeviceToHost, stream_));
   CT_HIP_CHECK(hipStreamSynchronize(stream_));

@@ -107,3 +107,24 @@ def test_forced_reevaluation_reproduces_fp64(ct):
     assert rep["n_reevaluated"] == int(floor_ok.sum())
     for k in names:
         assert np.array_equal(mix[k][floor_ok], f64[k][floor_ok]), k
+
+
+def test_mixed_error_model_is_conservative(ct):
+    """Without re-evaluation, every point's actual error stays below the estimate
+    gamma * |phi|^T|P||phi| that drives the re-evaluation (checked through the
+    reported maximum estimate/|rho| over accepted points)."""
+    from chemtoolscuda_amd import synthetic
+
+    w = synthetic.organic60()
+    rng = np.random.default_rng(5)
+    pts = w.points[rng.choice(w.points.shape[0], size=30000, replace=False)]
+    mol = ct.molecule_from_basis(w.basis, w.P, w.charges)
+    raw = mol.compute(pts, ["rho"], precision="mixed", mixed_tol=1e30)
+    rep = mol.last_report()
+    f64 = mol.compute(pts, ["rho"], precision="fp64")
+    mask = np.abs(f64["rho"]) >= 1e-10
+    rel = np.abs(raw["rho"] - f64["rho"])[mask] / np.abs(f64["rho"][mask])
+    print(f"no re-evaluation: max rel err {rel.max():.2e}, p99.9 {np.quantile(rel, 0.999):.2e}, "
+          f"max estimate {rep['max_error_estimate']:.2e}, reevaluated {rep['n_reevaluated']}")
+    assert rep["n_reevaluated"] == 0
+    assert rel.max() <= rep["max_error_estimate"]
