@@ -132,11 +132,6 @@ def derivative_exprs(q, L, dset):
     return out
 
 
-def _cxx(e):
-    code = sp.ccode(sp.nsimplify(e) if False else e)
-    return code
-
-
 def _num(expr):
     """Replace exact constants by double literals so the emitted code is plain."""
     return expr.xreplace({n: sp.Float(n, 20) for n in expr.atoms(sp.Number) if not n.is_Integer})
@@ -211,9 +206,40 @@ def main():
         parts.append("")
     with open(os.path.join(OUT_DIR, "ao_poly.h"), "w") as f:
         f.write("\n".join(parts))
+    write_shell_monomials()
     with open(os.path.join(OUT_DIR, "ao_flops.json"), "w") as f:
         json.dump({"shell_types": SHELL_TYPES, "flops": flops}, f, indent=1)
     print("wrote", os.path.join(OUT_DIR, "ao_poly.h"))
+
+
+def write_shell_monomials():
+    """Host table: each shell component as a sum of Cartesian monomials (with
+    the angular normalisation factor), used by the ESP pre-contraction."""
+    rows, offsets = [], [0]
+    for code, l in enumerate(SHELL_TYPES):
+        for ic, q in enumerate(components(l)):
+            poly = sp.Poly(sp.expand(q), x, y, z)
+            for (a, b, c), coef in zip(poly.monoms(), poly.coeffs()):
+                rows.append(f"    {{{float(coef)!r}, {a}, {b}, {c}, {ic}}},")
+        offsets.append(len(rows))
+    text = ["// GENERATED by chemtoolscuda_amd/codegen/gen_ao.py -- do not edit.",
+            "// Shell components as sums of Cartesian monomials x^a y^b z^c, including the",
+            "// angular part of the L2 normalisation (the same polynomials the AO kernels use).",
+            "#pragma once",
+            "#include <cstdint>",
+            "namespace ct {",
+            "struct MonoTerm {",
+            "  double c;",
+            "  int8_t a, b, z, comp;",
+            "};",
+            f"constexpr int kShellMonoOffset[{len(offsets)}] = {{{', '.join(map(str, offsets))}}};",
+            "constexpr MonoTerm kShellMonos[] = {",
+            *rows,
+            "};",
+            "}  // namespace ct",
+            ""]
+    with open(os.path.join(OUT_DIR, "shell_monomials.h"), "w") as f:
+        f.write("\n".join(text))
 
 
 if __name__ == "__main__":

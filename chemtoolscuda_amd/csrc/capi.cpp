@@ -198,14 +198,7 @@ int ct_norm_rows(const double* v, int64_t n, double* out) {
 }
 
 int ct_esp(ct_mol* mol, const double* points, int64_t n, int32_t precision, double* out) {
-  return guard([&] {
-    (void)M(mol);
-    (void)points;
-    (void)n;
-    (void)precision;
-    (void)out;
-    throw ct::Error(CT_E_UNSUPPORTED, "ESP kernel not built yet");
-  });
+  return guard([&] { M(mol).esp(points, n, precision, out); });
 }
 
 int ct_backward(ct_mol* mol, const double* points, int64_t n, const double* dL_drho, int32_t precision,

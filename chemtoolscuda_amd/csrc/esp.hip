@@ -1,0 +1,139 @@
+// ESP kernel: one lane per point, all lanes walk the same list of primitive
+// pairs (wave-uniform record addresses -> scalar loads of the coefficients),
+// grouped by total angular momentum L so each group runs fully unrolled code.
+//
+// Boys function F_0..F_L(T) in FP64, a method of its own: for T < 40 a
+// tenth-order Taylor expansion about the nearest point of a 0.125-spaced table
+// of F_m (Helgaker, Jorgensen & Olsen sec. 9.8.2; table built on the host by
+// the convergent series), followed by the stable downward recursion
+// F_n = (2T F_{n+1} + e^{-T}) / (2n+1); for T >= 40, F_0 = sqrt(pi/T)/2
+// (erfc(sqrt(40)) < 1e-18) and the upward recursion
+// F_{n+1} = ((2n+1) F_n - e^{-T}) / (2T), stable there.  (The reference's
+// erf-based upward recursion, include/boys_functions.cuh:54-72, is NaN at
+// T = 0 and unstable at small T.)  The table lives in LDS.
+//
+// ESP(C) = sum_A Z_A / |C - R_A| - sum_pairs S_pair(C)
+// (reference src/evaluate_electrostatic.cu:10356-10371 for the nuclear term).
+
+#include "esp.h"
+
+namespace ct {
+
+namespace {
+
+template <int L>
+__device__ __forceinline__ void boys(double T, const double* __restrict__ tab, double (&F)[L + 1]) {
+  if (T < kBoysTmax) {
+    const int g = (int)(T * (1.0 / kBoysStep) + 0.5);
+    const double dT = T - g * kBoysStep;
+    const double* row = tab + g * kBoysM + L;
+    double f = 0.0, term = 1.0;
+#pragma unroll
+    for (int k = 0; k <= kBoysK; ++k) {
+      f += row[k] * term;
+      term *= -dT / (k + 1);
+    }
+    F[L] = f;
+    if constexpr (L > 0) {
+      const double e = exp(-T);
+#pragma unroll
+      for (int n = L - 1; n >= 0; --n) F[n] = (2.0 * T * F[n + 1] + e) / (2 * n + 1);
+    }
+  } else {
+    F[0] = 0.5 * sqrt(M_PI / T);
+    if constexpr (L > 0) {
+      const double e = exp(-T);
+      const double inv2T = 0.5 / T;
+#pragma unroll
+      for (int n = 0; n < L; ++n) F[n + 1] = ((2 * n + 1) * F[n] - e) * inv2T;
+    }
+  }
+}
+
+template <int L>
+__device__ __forceinline__ double pair_value(const double* __restrict__ rec, double cx, double cy, double cz,
+                                             const double* __restrict__ tab) {
+  const double X = rec[0] - cx, Y = rec[1] - cy, Z = rec[2] - cz, p = rec[3];
+  const double T = p * (X * X + Y * Y + Z * Z);
+  double F[L + 1];
+  boys<L>(T, tab, F);
+  double xp[L + 1], yp[L + 1], zp[L + 1];
+  xp[0] = yp[0] = zp[0] = 1.0;
+#pragma unroll
+  for (int i = 1; i <= L; ++i) {
+    xp[i] = xp[i - 1] * X;
+    yp[i] = yp[i - 1] * Y;
+    zp[i] = zp[i - 1] * Z;
+  }
+  const double* c = rec + 4;
+  double s = 0.0;
+  int idx = 0;
+#pragma unroll
+  for (int N = 0; N <= L; ++N) {
+    double poly = 0.0;
+#pragma unroll
+    for (int tot = (2 * N - L > 0 ? 2 * N - L : 0); tot <= N; ++tot)
+#pragma unroll
+      for (int i = tot; i >= 0; --i)
+#pragma unroll
+        for (int j = tot - i; j >= 0; --j) poly += c[idx++] * (xp[i] * yp[j] * zp[tot - i - j]);
+    s += F[N] * poly;
+  }
+  return s;
+}
+
+template <int L>
+__device__ __forceinline__ double segment_sum(const double* __restrict__ rec, int count, double cx, double cy,
+                                              double cz, const double* __restrict__ tab) {
+  constexpr int stride = 4 + esp_ncoef(L);
+  double s = 0.0;
+  for (int k = 0; k < count; ++k) s += pair_value<L>(rec + (int64_t)k * stride, cx, cy, cz, tab);
+  return s;
+}
+
+__global__ __launch_bounds__(256) void esp_kernel(const double* __restrict__ pts, int64_t n,
+                                                  const double* __restrict__ records,
+                                                  const EspSegment* __restrict__ segs, int nseg,
+                                                  const double* __restrict__ atoms, int natom,
+                                                  const double* __restrict__ boys_tab, double* __restrict__ out) {
+  __shared__ double tab[kBoysNG * kBoysM];
+  for (int k = threadIdx.x; k < kBoysNG * kBoysM; k += blockDim.x) tab[k] = boys_tab[k];
+  __syncthreads();
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t pc = p < n ? p : n - 1;
+  const double cx = pts[3 * pc], cy = pts[3 * pc + 1], cz = pts[3 * pc + 2];
+  double el = 0.0;
+  for (int sg = 0; sg < nseg; ++sg) {
+    const EspSegment seg = segs[sg];
+    const double* rec = records + seg.off;
+    switch (seg.L) {
+      case 0: el += segment_sum<0>(rec, seg.count, cx, cy, cz, tab); break;
+      case 1: el += segment_sum<1>(rec, seg.count, cx, cy, cz, tab); break;
+      case 2: el += segment_sum<2>(rec, seg.count, cx, cy, cz, tab); break;
+      case 3: el += segment_sum<3>(rec, seg.count, cx, cy, cz, tab); break;
+      case 4: el += segment_sum<4>(rec, seg.count, cx, cy, cz, tab); break;
+      case 5: el += segment_sum<5>(rec, seg.count, cx, cy, cz, tab); break;
+      case 6: el += segment_sum<6>(rec, seg.count, cx, cy, cz, tab); break;
+      case 7: el += segment_sum<7>(rec, seg.count, cx, cy, cz, tab); break;
+      default: el += segment_sum<8>(rec, seg.count, cx, cy, cz, tab); break;
+    }
+  }
+  double nuc = 0.0;
+  for (int a = 0; a < natom; ++a) {
+    const double dx = cx - atoms[4 * a], dy = cy - atoms[4 * a + 1], dz = cz - atoms[4 * a + 2];
+    nuc += atoms[4 * a + 3] / sqrt(dx * dx + dy * dy + dz * dz);
+  }
+  if (p < n) out[p] = nuc - el;
+}
+
+}  // namespace
+
+void launch_esp(const double* pts, int64_t n, const double* records, const EspSegment* segs, int nseg,
+                const double* atoms, int natom, const double* boys_tab, double* out, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(esp_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, pts, n, records, segs, nseg,
+                     atoms, natom, boys_tab, out);
+  CT_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace ct

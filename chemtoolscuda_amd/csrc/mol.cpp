@@ -10,6 +10,8 @@
 
 #include "mol.h"
 
+#include "esp.h"
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -308,6 +310,7 @@ void Mol::set_coordinates(const double* xyz) {
   else
     std::memcpy(xyz_.data(), xyz, sizeof(double) * 3 * natom_);
   upload_geometry();
+  esp_valid_ = false;
 }
 
 void Mol::set_density_matrix(const double* P) {
@@ -318,6 +321,63 @@ void Mol::set_density_matrix(const double* P) {
   else
     std::memcpy(P_.data(), P, sizeof(double) * (size_t)M_ * M_);
   upload_P();
+  esp_valid_ = false;
+}
+
+void Mol::prepare_esp() {
+  if (esp_valid_) return;
+  const EspData d = build_esp_pairs(basis_, xyz_, P_, 1e-18);
+  std::vector<double> atoms(4 * (size_t)natom_);
+  for (int a = 0; a < natom_; ++a) {
+    for (int k = 0; k < 3; ++k) atoms[4 * a + k] = xyz_[3 * a + k];
+    atoms[4 * a + 3] = charges_[a];
+  }
+  if (!d_boys_.get()) {
+    const std::vector<double> tab = build_boys_table();
+    if (!d_boys_.reserve(sizeof(double) * tab.size())) throw Error(CT_E_NOMEM, "cannot allocate the Boys table");
+    CT_HIP_CHECK(hipMemcpy(d_boys_.get(), tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice));
+  }
+  if (!d_esp_rec_.reserve(sizeof(double) * std::max<size_t>(1, d.records.size())) ||
+      !d_esp_seg_.reserve(sizeof(EspSegment) * std::max<size_t>(1, d.segs.size())) ||
+      !d_esp_atoms_.reserve(sizeof(double) * atoms.size()))
+    throw Error(CT_E_NOMEM, "cannot allocate the ESP pair table");
+  if (!d.records.empty())
+    CT_HIP_CHECK(hipMemcpy(d_esp_rec_.get(), d.records.data(), sizeof(double) * d.records.size(), hipMemcpyHostToDevice));
+  if (!d.segs.empty())
+    CT_HIP_CHECK(hipMemcpy(d_esp_seg_.get(), d.segs.data(), sizeof(EspSegment) * d.segs.size(), hipMemcpyHostToDevice));
+  CT_HIP_CHECK(hipMemcpy(d_esp_atoms_.get(), atoms.data(), sizeof(double) * atoms.size(), hipMemcpyHostToDevice));
+  esp_nseg_ = (int)d.segs.size();
+  esp_npairs_ = d.npairs - d.screened;
+  esp_valid_ = true;
+}
+
+void Mol::esp(const double* points, int64_t n, int precision, double* out) {
+  CT_REQUIRE(n >= 0, "negative number of points");
+  CT_REQUIRE(precision == CT_PREC_FP64 || precision == CT_PREC_MIXED, "unknown precision");
+  if (n == 0) return;
+  CT_REQUIRE(points != nullptr && out != nullptr, "null pointer");
+  CT_HIP_CHECK(hipStreamSynchronize(stream_));
+  prepare_esp();
+  const bool pd = is_device_pointer(points), od = is_device_pointer(out);
+  const double* dp = points;
+  double* dout = out;
+  if (!pd || !od) {
+    if (!d_esp_io_.reserve(sizeof(double) * 4 * (size_t)n)) throw Error(CT_E_NOMEM, "cannot allocate ESP staging");
+    double* io = static_cast<double*>(d_esp_io_.get());
+    if (!pd) {
+      CT_HIP_CHECK(hipMemcpyAsync(io, points, sizeof(double) * 3 * n, hipMemcpyHostToDevice, stream_));
+      dp = io;
+    }
+    if (!od) dout = io + 3 * n;
+  }
+  timer_.begin(KernelTimer::ESP, stream_);
+  launch_esp(dp, n, static_cast<const double*>(d_esp_rec_.get()), static_cast<const EspSegment*>(d_esp_seg_.get()),
+             esp_nseg_, static_cast<const double*>(d_esp_atoms_.get()), natom_,
+             static_cast<const double*>(d_boys_.get()), dout, stream_);
+  timer_.end(stream_);
+  if (!od) CT_HIP_CHECK(hipMemcpyAsync(out, dout, sizeof(double) * n, hipMemcpyDeviceToHost, stream_));
+  CT_HIP_CHECK(hipStreamSynchronize(stream_));
+  timer_.collect();
 }
 
 void Mol::synchronize() { CT_HIP_CHECK(hipStreamSynchronize(stream_)); }

@@ -107,6 +107,8 @@ class Mol {
 
   void eval(const double* points, int64_t n, uint32_t q, int precision, const ct_eval_params& par,
             const ct_outputs& out, ct_eval_report* rep);
+  void esp(const double* points, int64_t n, int precision, double* out);
+  int64_t esp_pairs() const { return esp_npairs_; }
 
  private:
   void upload_geometry();
@@ -131,6 +133,12 @@ class Mol {
   PinnedBuffer h_stage_;   // host staging for pageable inputs/outputs
   int64_t ws_limit_ = int64_t(8) << 30;
   KernelTimer timer_;
+  // ESP: Hermite pre-contraction of P, rebuilt when P or the centres change
+  bool esp_valid_ = false;
+  int esp_nseg_ = 0;
+  int64_t esp_npairs_ = 0;
+  DeviceBuffer d_esp_rec_, d_esp_seg_, d_esp_atoms_, d_boys_, d_esp_io_;
+  void prepare_esp();
 };
 
 // true when ptr is device (HBM) memory usable directly by kernels
