@@ -11,6 +11,15 @@ import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 
+# One HIP runtime per process: PyTorch-ROCm ships its own libamdhip64 (same
+# soname, linked by file name).  Loading it first makes the native library
+# bind to that copy; the other order would map two runtimes and PyTorch's
+# would then find no device.
+try:
+    import torch  # noqa: F401
+except ImportError:  # pragma: no cover - the library itself does not need torch
+    pass
+
 try:
     from . import _chemtools_amd as _native  # noqa: F401
 except ImportError as exc:  # pragma: no cover - exercised only on an unbuilt tree

@@ -52,6 +52,10 @@ _lib = None
 def lib():
     global _lib
     if _lib is None:
+        try:  # share PyTorch's HIP runtime when it is installed (see __init__.py)
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} missing: build with `python -m chemtoolscuda_amd.build_ext`")
         L = ctypes.CDLL(LIB_PATH)

@@ -1,0 +1,174 @@
+// Adjoint of rho (no reference counterpart; used by the autograd wrapper):
+//
+//   dL/dP_ij  = sum_g w_g phi_i(g) phi_j(g)                 (w_g = dL/drho_g)
+//   dL/dR_A   = -2 sum_{i on A} sum_g w_g grad phi_i(g) (P phi(g))_i
+//
+// Two FP64 matrix-core kernels per point chunk:
+//   * gram_f64: G = Phi diag(w) Phi^T, split over point slices.  One wave owns
+//     a 64 x 64 tile of G and one slice of points (the GEMM's K); each slice
+//     writes its own partial tile, and a reduction adds the slices in a fixed
+//     order (deterministic).
+//   * contract_adjoint_f64: the forward contraction's GEMM Y = P phi with an
+//     epilogue that sums w_g d_k phi_i(g) Y_i(g) over the wave's points per
+//     row i and adds the row totals into T[i][k] with FP64 atomics (the order
+//     of those few additions per row is not fixed: results agree to the last
+//     bits only).
+// The per-atom sums of -2 T happen on the host.
+
+#include "ct_internal.h"
+
+namespace ct {
+
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+constexpr int kGramSlice = 2048;  // points per partial (K of one wave)
+
+// MFMA 16x16x4 f64 layout (lane l, r = l % 16, q = l / 16): A[r][q], B[q][r],
+// D[q + 4j][r].
+__global__ __launch_bounds__(64) void gram_f64_kernel(const double* __restrict__ ao, int mp, int64_t ld,
+                                                      const double* __restrict__ w, int64_t npts,
+                                                      double* __restrict__ part, int ntile) {
+  const int lane = threadIdx.x;
+  const int r = lane & 15, q = lane >> 4;
+  const int tile = blockIdx.x % (ntile * ntile);
+  const int slice = blockIdx.x / (ntile * ntile);
+  const int I = (tile / ntile) * 64, J = (tile % ntile) * 64;
+  const int64_t pbeg = (int64_t)slice * kGramSlice;
+  const int64_t pend = pbeg + kGramSlice < npts ? pbeg + kGramSlice : npts;
+  d4 acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+  for (int64_t p0 = pbeg; p0 < pend; p0 += 4) {
+    const int64_t p = p0 + q;
+    const bool ok = p < pend;
+    const double wp = ok ? w[p] : 0.0;
+    double a[4], b[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      a[k] = ok ? ao[(int64_t)(I + 16 * k + r) * ld + p] : 0.0;
+      b[k] = ok ? wp * ao[(int64_t)(J + 16 * k + r) * ld + p] : 0.0;
+    }
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+      for (int cb = 0; cb < 4; ++cb) acc[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb], b[cb], acc[rb][cb], 0, 0, 0);
+  }
+  double* out = part + (int64_t)slice * mp * mp;
+#pragma unroll
+  for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) out[(int64_t)(I + 16 * rb + q + 4 * j) * mp + J + 16 * cb + r] = acc[rb][cb][j];
+}
+
+__global__ __launch_bounds__(256) void gram_reduce_kernel(const double* __restrict__ part, int nslice, int64_t count,
+                                                          double* __restrict__ G) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  double s = G[k];
+  for (int sl = 0; sl < nslice; ++sl) s += part[(int64_t)sl * count + k];
+  G[k] = s;
+}
+
+// Y = P phi (64 points per wave, like the forward MODE_GRAD kernel) with the
+// row-wise weighted epilogue.
+__global__ __launch_bounds__(256) void contract_adjoint_f64_kernel(const double* __restrict__ P, int mp,
+                                                                   const double* __restrict__ ao, int64_t ld,
+                                                                   const double* __restrict__ w, int64_t npts,
+                                                                   double* __restrict__ T) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int r = lane & 15, q = lane >> 4;
+  const int64_t p0 = ((int64_t)blockIdx.x * 4 + wave) * 64;
+  const int64_t sd = (int64_t)mp * ld;
+  const double* bptr[4];
+  double wl[4];
+#pragma unroll
+  for (int cb = 0; cb < 4; ++cb) {
+    bptr[cb] = ao + (int64_t)q * ld + p0 + 16 * cb + r;
+    const int64_t p = p0 + 16 * cb + r;
+    wl[cb] = p < npts ? w[p] : 0.0;
+  }
+  for (int i0 = 0; i0 < mp; i0 += 64) {
+    d4 acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+    const double* aptr = P + (int64_t)q * mp + i0 + r;
+    for (int k0 = 0; k0 < mp; k0 += 4) {
+      double a[4], b[4];
+#pragma unroll
+      for (int rb = 0; rb < 4; ++rb) a[rb] = aptr[(int64_t)k0 * mp + 16 * rb];
+#pragma unroll
+      for (int cb = 0; cb < 4; ++cb) b[cb] = bptr[cb][(int64_t)k0 * ld];
+#pragma unroll
+      for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb)
+          acc[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb], b[cb], acc[rb][cb], 0, 0, 0);
+    }
+    // row sums over this wave's 64 points: lane (q, r) holds rows i0+16rb+q+4j
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int64_t i = i0 + 16 * rb + q + 4 * j;
+        double s[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb) {
+          const int64_t p = p0 + 16 * cb + r;
+          const double y = acc[rb][cb][j] * wl[cb];
+#pragma unroll
+          for (int k = 0; k < 3; ++k) s[k] += ao[(1 + k) * sd + i * ld + p] * y;
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          double v = s[k];
+          v += __shfl_xor(v, 1);
+          v += __shfl_xor(v, 2);
+          v += __shfl_xor(v, 4);
+          v += __shfl_xor(v, 8);
+          s[k] = v;
+        }
+        if (r == 0) {
+#pragma unroll
+          for (int k = 0; k < 3; ++k) atomicAdd(&T[i * 3 + k], s[k]);
+        }
+      }
+    }
+  }
+}
+
+}  // namespace
+
+int gram_slices(int64_t npts) { return (int)((npts + kGramSlice - 1) / kGramSlice); }
+
+void launch_gram_f64(const double* ao, int mp, int64_t ld, const double* w, int64_t npts, double* part,
+                     hipStream_t s) {
+  const int ntile = mp / 64;
+  const int nslice = gram_slices(npts);
+  hipLaunchKernelGGL(gram_f64_kernel, dim3((unsigned)(ntile * ntile * nslice)), dim3(64), 0, s, ao, mp, ld, w, npts,
+                     part, ntile);
+  CT_HIP_CHECK(hipGetLastError());
+}
+
+void launch_gram_reduce(const double* part, int nslice, int64_t count, double* G, hipStream_t s) {
+  hipLaunchKernelGGL(gram_reduce_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, part, nslice, count,
+                     G);
+  CT_HIP_CHECK(hipGetLastError());
+}
+
+void launch_contract_adjoint_f64(const double* P, int mp, const double* ao, int64_t ld, const double* w,
+                                 int64_t npts, double* T, hipStream_t s) {
+  hipLaunchKernelGGL(contract_adjoint_f64_kernel, dim3((unsigned)(ld / 256)), dim3(256), 0, s, P, mp, ao, ld, w, npts,
+                     T);
+  CT_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace ct

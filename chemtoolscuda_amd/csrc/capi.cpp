@@ -203,16 +203,7 @@ int ct_esp(ct_mol* mol, const double* points, int64_t n, int32_t precision, doub
 
 int ct_backward(ct_mol* mol, const double* points, int64_t n, const double* dL_drho, int32_t precision,
                 double* dL_dP, double* dL_dR) {
-  return guard([&] {
-    (void)M(mol);
-    (void)points;
-    (void)n;
-    (void)dL_drho;
-    (void)precision;
-    (void)dL_dP;
-    (void)dL_dR;
-    throw ct::Error(CT_E_UNSUPPORTED, "adjoint kernels not built yet");
-  });
+  return guard([&] { M(mol).backward(points, n, dL_drho, precision, dL_dP, dL_dR); });
 }
 
 int ct_mol_synchronize(ct_mol* mol) {

@@ -123,4 +123,12 @@ void launch_finalize_mixed(const FinalizeArgs& a, const FlagArgs& f, hipStream_t
 void launch_gather_points(const double* pts, const int32_t* index, int64_t n, double* out, hipStream_t s);
 void launch_norm_rows(const double* v, int64_t n, double* out, hipStream_t s);
 
+// Adjoint.
+int gram_slices(int64_t npts);
+void launch_gram_f64(const double* ao, int mp, int64_t ld, const double* w, int64_t npts, double* part,
+                     hipStream_t s);
+void launch_gram_reduce(const double* part, int nslice, int64_t count, double* G, hipStream_t s);
+void launch_contract_adjoint_f64(const double* P, int mp, const double* ao, int64_t ld, const double* w,
+                                 int64_t npts, double* T, hipStream_t s);
+
 }  // namespace ct

@@ -351,6 +351,84 @@ void Mol::prepare_esp() {
   esp_valid_ = true;
 }
 
+void Mol::backward(const double* points, int64_t n, const double* w, int precision, double* dP, double* dR) {
+  CT_REQUIRE(n >= 0, "negative number of points");
+  CT_REQUIRE(precision == CT_PREC_FP64 || precision == CT_PREC_MIXED, "unknown precision");
+  CT_REQUIRE(n == 0 || (points != nullptr && w != nullptr), "null points or weights");
+  CT_HIP_CHECK(hipStreamSynchronize(stream_));
+  const int mode = MODE_GRAD;
+  const int nder = mode_nder(mode);
+  const int64_t per_point = (int64_t)nder * Mp_ * 8 + 24 + 8;
+  int64_t Nc = n > 0 ? chunk_points(per_point, n) : kPointAlign;
+  while (!d_ws_.reserve((size_t)Nc * per_point)) {
+    if (Nc <= kPointAlign) throw Error(CT_E_NOMEM, "cannot allocate the adjoint workspace");
+    Nc = std::max<int64_t>(kPointAlign, Nc / 2 / kPointAlign * kPointAlign);
+  }
+  const int nslice_max = gram_slices(Nc);
+  const size_t gsize = (size_t)Mp_ * Mp_;
+  const size_t adj_bytes = sizeof(double) * (gsize + (size_t)Mp_ * 3 + (size_t)nslice_max * gsize);
+  if (!d_adj_.reserve(adj_bytes)) throw Error(CT_E_NOMEM, "cannot allocate the adjoint accumulators");
+  double* G = static_cast<double*>(d_adj_.get());
+  double* T = G + gsize;
+  double* part = T + (size_t)Mp_ * 3;
+  CT_HIP_CHECK(hipMemsetAsync(G, 0, sizeof(double) * (gsize + (size_t)Mp_ * 3), stream_));
+  double* ao = static_cast<double*>(d_ws_.get());
+  double* pts_stage = ao + (size_t)nder * Mp_ * Nc;
+  double* w_stage = pts_stage + 3 * (size_t)Nc;
+  const bool pd = is_device_pointer(points), wd = is_device_pointer(w);
+  const auto* meta = static_cast<const ShellMeta*>(d_meta_.get());
+  const auto* geom = static_cast<const ShellGeom*>(d_geom_.get());
+  const auto* prim = static_cast<const PrimData*>(d_prim_.get());
+  for (int64_t start = 0; start < n; start += Nc) {
+    const int64_t nc = std::min(Nc, n - start);
+    const int64_t ld = round_up(nc, kPointAlign);
+    const double* pts = points + 3 * start;
+    const double* wc = w + start;
+    if (!pd) {
+      CT_HIP_CHECK(hipMemcpyAsync(pts_stage, pts, sizeof(double) * 3 * nc, hipMemcpyHostToDevice, stream_));
+      pts = pts_stage;
+    }
+    if (!wd) {
+      CT_HIP_CHECK(hipMemcpyAsync(w_stage, wc, sizeof(double) * nc, hipMemcpyHostToDevice, stream_));
+      wc = w_stage;
+    }
+    timer_.begin(KernelTimer::BACKWARD, stream_);
+    launch_ao_f64(mode, meta, geom, prim, nshell_, M_, Mp_, pts, nc, ld, ao, 700.0, stream_);
+    launch_gram_f64(ao, Mp_, ld, wc, nc, part, stream_);
+    launch_gram_reduce(part, gram_slices(nc), (int64_t)gsize, G, stream_);
+    launch_contract_adjoint_f64(static_cast<const double*>(d_P_.get()), Mp_, ao, ld, wc, nc, T, stream_);
+    timer_.end(stream_);
+  }
+  std::vector<double> Gh(gsize), Th((size_t)Mp_ * 3);
+  CT_HIP_CHECK(hipMemcpyAsync(Gh.data(), G, sizeof(double) * gsize, hipMemcpyDeviceToHost, stream_));
+  CT_HIP_CHECK(hipMemcpyAsync(Th.data(), T, sizeof(double) * Th.size(), hipMemcpyDeviceToHost, stream_));
+  CT_HIP_CHECK(hipStreamSynchronize(stream_));
+  timer_.collect();
+  if (dP) {
+    std::vector<double> out((size_t)M_ * M_);
+    for (int i = 0; i < M_; ++i)
+      for (int j = 0; j < M_; ++j) out[(size_t)i * M_ + j] = Gh[(size_t)i * Mp_ + j];
+    if (is_device_pointer(dP))
+      CT_HIP_CHECK(hipMemcpy(dP, out.data(), sizeof(double) * out.size(), hipMemcpyHostToDevice));
+    else
+      std::memcpy(dP, out.data(), sizeof(double) * out.size());
+  }
+  if (dR) {
+    std::vector<double> out(3 * (size_t)natom_, 0.0);
+    for (int s = 0; s < nshell_; ++s) {
+      const int a = basis_.shell_atom[s];
+      for (int f = 0; f < shell_nfunc(basis_.shell_l[s]); ++f) {
+        const int i = meta_[s].first + f;
+        for (int k = 0; k < 3; ++k) out[3 * a + k] -= 2.0 * Th[(size_t)i * 3 + k];
+      }
+    }
+    if (is_device_pointer(dR))
+      CT_HIP_CHECK(hipMemcpy(dR, out.data(), sizeof(double) * out.size(), hipMemcpyHostToDevice));
+    else
+      std::memcpy(dR, out.data(), sizeof(double) * out.size());
+  }
+}
+
 void Mol::esp(const double* points, int64_t n, int precision, double* out) {
   CT_REQUIRE(n >= 0, "negative number of points");
   CT_REQUIRE(precision == CT_PREC_FP64 || precision == CT_PREC_MIXED, "unknown precision");

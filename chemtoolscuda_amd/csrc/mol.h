@@ -108,6 +108,7 @@ class Mol {
   void eval(const double* points, int64_t n, uint32_t q, int precision, const ct_eval_params& par,
             const ct_outputs& out, ct_eval_report* rep);
   void esp(const double* points, int64_t n, int precision, double* out);
+  void backward(const double* points, int64_t n, const double* w, int precision, double* dP, double* dR);
   int64_t esp_pairs() const { return esp_npairs_; }
 
  private:
@@ -138,6 +139,7 @@ class Mol {
   int esp_nseg_ = 0;
   int64_t esp_npairs_ = 0;
   DeviceBuffer d_esp_rec_, d_esp_seg_, d_esp_atoms_, d_boys_, d_esp_io_;
+  DeviceBuffer d_adj_;     // adjoint: G, T, slice partials
   void prepare_esp();
 };
 
