@@ -31,6 +31,11 @@ if ROOT not in sys.path:
 
 PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16": 2.5e15}
 
+# HBM bytes per contraction launch (FETCH_SIZE + WRITE_SIZE, calibrated against
+# a 1 GiB device copy), measured with rocprofv3 --pmc on this workload; see
+# profiles/contract_{mixed,f64}_organic60.md for the commands and raw counts.
+MEASURED_TRAFFIC = {("organic60", "mixed"): 15.1e9 + 0.19e9, ("organic60", "fp64"): 77.2e9 + 0.021e9}
+
 
 def parse():
     ap = argparse.ArgumentParser()
@@ -42,6 +47,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU-baseline work (0 disables)")
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64-vs-mixed comparison run")
     ap.add_argument("--workspace-mb", type=int, default=0, help="per-chunk workspace budget (0: library default)")
+    ap.add_argument("--no-extra", action="store_true", help="skip the fwd+bwd and ESP secondary measurements")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
+                    help="only run a device-to-device copy of MB MiB (calibrates FETCH_SIZE/WRITE_SIZE counters)")
     return ap.parse_args()
 
 
@@ -65,6 +73,47 @@ def cpu_baseline(w, target_s):
                       f"({dt:.1f} s, OpenMP, {oracle.max_threads()} threads)"}
 
 
+def fwd_bwd(torch, capi, synthetic, precision, reps=3):
+    """Differentiation workload: 40 atoms, M = 416, 5e5 points; rho forward
+    plus adjoint (dL/dP, dL/dR) per step, inputs resident in HBM."""
+    w = synthetic.diff40()
+    order = synthetic.morton_order(w.points)
+    pts = torch.from_numpy(np.ascontiguousarray(w.points[order])).cuda()
+    wt = torch.from_numpy(np.ascontiguousarray(w.extra["weights"][order])).cuda()
+    n = pts.shape[0]
+    h = capi.Handle(w.basis.flat(), w.P, w.charges)
+    rho = torch.empty(n, dtype=torch.float64, device="cuda")
+    dP = torch.empty((w.basis.nbasis,) * 2, dtype=torch.float64, device="cuda")
+    dR = torch.empty((w.basis.natom, 3), dtype=torch.float64, device="cuda")
+
+    def step():
+        h.eval(pts.data_ptr(), n, {"rho": rho.data_ptr()}, precision)
+        h.backward(pts.data_ptr(), n, wt.data_ptr(), dP.data_ptr(), dR.data_ptr(), precision)
+
+    step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        step()
+    torch.cuda.synchronize()
+    return n * reps / (time.perf_counter() - t0)
+
+
+def esp_rate(torch, capi, synthetic, reps=3):
+    """ESP workload: 30 atoms, M = 312, 2e5 points on the 1.4 x vdW surface."""
+    w = synthetic.esp30()
+    pts = torch.from_numpy(np.ascontiguousarray(w.points)).cuda()
+    out = torch.empty(pts.shape[0], dtype=torch.float64, device="cuda")
+    h = capi.Handle(w.basis.flat(), w.P, w.charges)
+    h.esp(pts.data_ptr(), pts.shape[0], out.data_ptr())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        h.esp(pts.data_ptr(), pts.shape[0], out.data_ptr())
+    torch.cuda.synchronize()
+    return pts.shape[0] * reps / (time.perf_counter() - t0)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -74,6 +123,15 @@ def main():
     import torch
 
     from chemtoolscuda_amd import capi, synthetic
+
+    if args.copy_calibration:
+        a = torch.empty(args.copy_calibration << 20, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": a.numel()}))
+        return
 
     meta = json.load(open(os.path.join(ROOT, "BASELINE.json")))
     w = synthetic.WORKLOADS[args.workload]()
@@ -122,7 +180,7 @@ def main():
     achieved = flops_pt * pts_per_launch / (contract_ms * 1e-3)
     peak_key = "fp64" if precision == "fp64" else "bf16"
     roofline = {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK[peak_key] / 1e12, "unit": "TFLOP/s",
-                "frac": achieved / PEAK[peak_key], "traffic": None,
+                "frac": achieved / PEAK[peak_key], "traffic": MEASURED_TRAFFIC.get((w.name, precision)),
                 "kernel": "contract_" + ("f64" if precision == "fp64" else "mixed"),
                 "avg_launch_ms": contract_ms, "points_per_launch": pts_per_launch,
                 "flops_per_point": flops_pt}
@@ -151,6 +209,12 @@ def main():
             err = torch.linalg.vector_norm(outs[q] - ref[q], dim=1) / torch.clamp(scales[q], min=1e-300)
             extra[f"max_rel_err_{q}"] = float(err[mask].max())
         extra["n_reevaluated"] = rep.get("n_reevaluated") if rep else None
+
+    if not args.no_extra:
+        extra["fwd_bwd_points_per_s"] = {p: fwd_bwd(torch, capi, synthetic, p) for p in ("fp64", "mixed")}
+        extra["fwd_bwd_workload"] = "diff40: 40 atoms, M=416, 500000 points, rho + dL/dP + dL/dR per step"
+        extra["esp_points_per_s"] = esp_rate(torch, capi, synthetic)
+        extra["esp_workload"] = "esp30: 30 atoms, M=312, 200000 points on the 1.4 x vdW surface, FP64"
 
     kernel_ms = {k: stats[f"{k}_ms"] / args.steps for k in ("ao", "contract", "finalize", "reeval")}
     line = {
