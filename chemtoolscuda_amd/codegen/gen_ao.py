@@ -148,9 +148,8 @@ def gen_function(l, dset):
     ops = sum(sp.count_ops(v) for _, v in repl) + sum(sp.count_ops(e) for e in reduced)
     name = f"ao_shell_{'c' if l >= 0 else 'p'}{L}_{dset}"
     lines = [f"// l = {l}: {len(comps)} functions x {nslot} slots, {int(ops)} ops after CSE",
-             "template <typename T>",
-             f"__device__ __forceinline__ void {name}(T x, T y, T z, T s0, T s1, T s2, T* __restrict__ o,",
-             "    int64_t si, int64_t sd) {"]
+             "template <typename T, typename St>",
+             f"__device__ __forceinline__ void {name}(T x, T y, T z, T s0, T s1, T s2, St&& st) {{"]
     used = set()
     for e in list(reduced) + [v for _, v in repl]:
         used |= {str(s) for s in e.free_symbols}
@@ -162,7 +161,7 @@ def gen_function(l, dset):
     for ic in range(len(comps)):
         for d in range(nslot):
             e = reduced[ic * nslot + d]
-            lines.append(f"  o[{d} * sd + {ic} * si] = {sp.ccode(e)};")
+            lines.append(f"  st({d}, {ic}, {sp.ccode(e)});")
     lines.append("}")
     src = "\n".join(lines)
     # literal doubles must stay doubles for T=double and become floats for T=float:
@@ -193,13 +192,14 @@ def main():
             flops[f"{l}:{dset}"] = {"ops": ops, "nfunc": nc}
     # dispatch helper: shell type code (index into SHELL_TYPES) -> generated function
     for dset in DERIV_SETS:
-        parts.append("template <typename T>")
+        parts.append("// st(slot, component, value) stores one AO value of the shell")
+        parts.append("template <typename T, typename St>")
         parts.append(f"__device__ __forceinline__ void ao_shell_dispatch_{dset}(int code, T x, T y, T z, T s0, T s1,")
-        parts.append("    T s2, T* __restrict__ o, int64_t si, int64_t sd) {")
+        parts.append("    T s2, St&& st) {")
         parts.append("  switch (code) {")
         for code, l in enumerate(SHELL_TYPES):
             fn = f"ao_shell_{'c' if l >= 0 else 'p'}{abs(l)}_{dset}"
-            parts.append(f"    case {code}: {fn}<T>(x, y, z, s0, s1, s2, o, si, sd); break;")
+            parts.append(f"    case {code}: {fn}<T>(x, y, z, s0, s1, s2, st); break;")
         parts.append("    default: break;")
         parts.append("  }")
         parts.append("}")

@@ -17,9 +17,13 @@
 //   * a shell is skipped for the whole wave when every lane has
 //     a_min r^2 > screen (exp underflow region in parity mode).
 //
-// Output layout: ao[(d * Mp + i) * ld + p], d = derivative slot, i = basis
-// function, p = point in the chunk (points contiguous -> coalesced stores and
-// directly the B operand layout of the MFMA contraction).
+// Output layouts (d = derivative slot, i = basis function, p = point in the
+// chunk): FP64 ao[(d * Mp + i) * ld + p], points contiguous (coalesced stores,
+// the B operand layout of the FP64 contraction); FP32 in row quads
+// (ao_quad_index): the four stores of a quad fill 16 contiguous bytes per
+// point, which the mixed contraction reads with one 16-byte load.
+
+#include <type_traits>
 
 #include "ct_internal.h"
 #include "generated/ao_poly.h"
@@ -27,6 +31,8 @@
 namespace ct {
 
 namespace {
+
+typedef float f4 __attribute__((ext_vector_type(4)));
 
 // exp(-a r^2) with the argument formed in FP64.  For the FP32 path only the
 // reduced argument goes through the FP32 exponential: with t = a r^2 / ln 2,
@@ -56,7 +62,27 @@ __global__ __launch_bounds__(256) void ao_eval_kernel(const ShellMeta* __restric
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // p < ld by construction
   const int64_t pc = p < npts ? p : (npts - 1);
   const double px = pts[3 * pc + 0], py = pts[3 * pc + 1], pz = pts[3 * pc + 2];
-  const int64_t sd = (int64_t)nbasis_pad * ld;  // stride between derivative slots
+  constexpr int nder = mode_nder(MODE);
+  constexpr bool quads = sizeof(T) == 4;
+  const int64_t mq = nbasis_pad >> 2;
+  // FP32 (row quads): the four rows of a quad are assembled in registers and
+  // stored with one 16-byte store per lane once the quad is complete (shell
+  // boundaries need not align with quads: the partial quad carries over).
+  T qb[quads ? nder : 1][4];
+#pragma unroll
+  for (int d = 0; d < (quads ? nder : 1); ++d)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) qb[d][e] = T(0);
+  auto store_quad = [&](int d, int quad) {
+    if constexpr (quads)
+      *reinterpret_cast<f4*>(ao + ((int64_t)d * mq + quad) * ld * 4 + p * 4) = f4{qb[d][0], qb[d][1], qb[d][2], qb[d][3]};
+  };
+  auto dispatch = [&](int code, T x, T y, T z, T s0, T s1, T s2, auto&& st) {
+    if constexpr (MODE == MODE_RHO) ao_shell_dispatch_val<T>(code, x, y, z, s0, s1, s2, st);
+    if constexpr (MODE == MODE_GRAD) ao_shell_dispatch_grad<T>(code, x, y, z, s0, s1, s2, st);
+    if constexpr (MODE == MODE_LAP) ao_shell_dispatch_lap<T>(code, x, y, z, s0, s1, s2, st);
+    if constexpr (MODE == MODE_HESS) ao_shell_dispatch_hess<T>(code, x, y, z, s0, s1, s2, st);
+  };
 
   for (int s = 0; s < nshell; ++s) {
     const ShellMeta m = meta[s];
@@ -79,16 +105,52 @@ __global__ __launch_bounds__(256) void ao_eval_kernel(const ShellMeta* __restric
         }
       }
     }
-    T* o = ao + (int64_t)m.first * ld + p;
-    if constexpr (MODE == MODE_RHO) ao_shell_dispatch_val<T>(m.code, x, y, z, s0, s1, s2, o, ld, sd);
-    if constexpr (MODE == MODE_GRAD) ao_shell_dispatch_grad<T>(m.code, x, y, z, s0, s1, s2, o, ld, sd);
-    if constexpr (MODE == MODE_LAP) ao_shell_dispatch_lap<T>(m.code, x, y, z, s0, s1, s2, o, ld, sd);
-    if constexpr (MODE == MODE_HESS) ao_shell_dispatch_hess<T>(m.code, x, y, z, s0, s1, s2, o, ld, sd);
+    const int first = m.first;
+    if constexpr (!quads) {
+      auto st = [&](int d, int ic, T v) { ao[((int64_t)d * nbasis_pad + first + ic) * ld + p] = v; };
+      dispatch(m.code, x, y, z, s0, s1, s2, st);
+    } else {
+      // the slot of component ic in its quad is (first + ic) & 3: compile-time
+      // per alignment, so qb stays in registers
+      auto run = [&](auto atag) {
+        constexpr int A = decltype(atag)::value;
+        auto st = [&](int d, int ic, T v) {
+          const int sl = (A + ic) & 3;
+          qb[d][sl] = v;
+          if (sl == 3) store_quad(d, (first + ic) >> 2);
+        };
+        dispatch(m.code, x, y, z, s0, s1, s2, st);
+      };
+      switch (first & 3) {
+        case 0: run(std::integral_constant<int, 0>{}); break;
+        case 1: run(std::integral_constant<int, 1>{}); break;
+        case 2: run(std::integral_constant<int, 2>{}); break;
+        default: run(std::integral_constant<int, 3>{}); break;
+      }
+    }
   }
   // zero rows of the padded basis so the contraction never reads garbage
-  constexpr int nder = mode_nder(MODE);
-  for (int d = 0; d < nder; ++d)
-    for (int i = nbasis; i < nbasis_pad; ++i) ao[((int64_t)d * nbasis_pad + i) * ld + p] = T(0);
+  if constexpr (!quads) {
+    for (int d = 0; d < nder; ++d)
+      for (int i = nbasis; i < nbasis_pad; ++i) ao[((int64_t)d * nbasis_pad + i) * ld + p] = T(0);
+  } else {
+    const int a = nbasis & 3;
+    if (a != 0) {
+#pragma unroll
+      for (int d = 0; d < nder; ++d) {
+        qb[d][3] = T(0);
+        if (a < 3) qb[d][2] = T(0);
+        if (a < 2) qb[d][1] = T(0);
+        store_quad(d, nbasis >> 2);
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < nder; ++d) {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) qb[d][e] = T(0);
+      for (int quad = (nbasis + 3) >> 2; quad < mq; ++quad) store_quad(d, quad);
+    }
+  }
 }
 
 }  // namespace

@@ -81,6 +81,14 @@ constexpr int mode_nder(int m) { return m == MODE_RHO ? 1 : m == MODE_GRAD ? 4 :
 //         h_kl = d_kl phi . P phi,  t_kl = d_k phi . P d_l phi
 constexpr int mode_ndot(int m) { return m == MODE_RHO ? 1 : m == MODE_GRAD ? 4 : m == MODE_LAP ? 8 : 16; }
 
+// AO layouts.  FP64 path: ao[(d * mp + i) * ld + p] (points contiguous).
+// Mixed path (FP32, "row quads"): four consecutive basis functions of one
+// point are 16 contiguous bytes, quads of one array and row block follow each
+// other point by point, so a 16-byte load yields four rows of one point.
+__host__ __device__ inline int64_t ao_quad_index(int d, int i, int64_t p, int mp, int64_t ld) {
+  return (((int64_t)d * (mp >> 2) + (i >> 2)) * ld + p) * 4 + (i & 3);
+}
+
 // ---------------------------------------------------------------------------
 // Kernel launchers (defined in the .hip files).
 struct FinalizeArgs {
@@ -108,6 +116,9 @@ void launch_ao_f32(int mode, const ShellMeta* meta, const ShellGeom* geom, const
                    int nbasis, int nbasis_pad, const double* pts, int64_t npts, int64_t ld, float* ao, float screen,
                    hipStream_t s);
 int mixed_points_per_block(int mode);
+// Reorders three row-major bf16 planes of P ([3][mp][mp]) into the MFMA
+// fragment order the mixed contraction loads (host side, same size).
+void mixed_fragment_order_P(const uint16_t* planes, int mp, uint16_t* out);
 void launch_contract_mixed(int mode, const uint16_t* Pp, int nbasis_pad, const float* ao, int64_t ld, double* part,
                            hipStream_t s);
 struct FlagArgs {

@@ -298,8 +298,10 @@ void Mol::upload_P() {
     Pb[plane + k] = m;
     Pb[2 * plane + k] = l;
   }
-  if (!d_Pb_.reserve(sizeof(uint16_t) * Pb.size())) throw Error(CT_E_NOMEM, "cannot allocate the bf16 density matrix");
-  CT_HIP_CHECK(hipMemcpy(d_Pb_.get(), Pb.data(), sizeof(uint16_t) * Pb.size(), hipMemcpyHostToDevice));
+  std::vector<uint16_t> Pf(Pb.size());
+  mixed_fragment_order_P(Pb.data(), Mp_, Pf.data());
+  if (!d_Pb_.reserve(sizeof(uint16_t) * Pf.size())) throw Error(CT_E_NOMEM, "cannot allocate the bf16 density matrix");
+  CT_HIP_CHECK(hipMemcpy(d_Pb_.get(), Pf.data(), sizeof(uint16_t) * Pf.size(), hipMemcpyHostToDevice));
 }
 
 void Mol::set_coordinates(const double* xyz) {

@@ -126,7 +126,7 @@ class Mol {
   int device_ = 0;
   hipStream_t stream_ = nullptr;
   DeviceBuffer d_meta_, d_geom_, d_prim_, d_P_;
-  DeviceBuffer d_Pb_;      // mixed path: bf16 pieces of P (hi, mid, lo), [3][Mp][Mp]
+  DeviceBuffer d_Pb_;      // mixed path: bf16 pieces of P (hi, mid, lo) in MFMA fragment order
   DeviceBuffer d_ws_;      // AO + dots + staged points / outputs
   DeviceBuffer d_ws64_;    // mixed path: FP64 re-evaluation of flagged points
   DeviceBuffer d_flags_;   // mixed path: flagged point list of a whole call (device outputs)
