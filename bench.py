@@ -100,18 +100,99 @@ def fwd_bwd(torch, capi, synthetic, precision, reps=3):
 
 
 def esp_rate(torch, capi, synthetic, reps=3):
-    """ESP workload: 30 atoms, M = 312, 2e5 points on the 1.4 x vdW surface."""
+    """ESP workload: 30 atoms, M = 312, 2e5 points on the 1.4 x vdW surface;
+    points/s of the FP64 and mixed kernels and the mixed kernel's largest error
+    relative to |V_nuc| + |V_el| (the ESP crosses zero where they cancel)."""
     w = synthetic.esp30()
     pts = torch.from_numpy(np.ascontiguousarray(w.points)).cuda()
-    out = torch.empty(pts.shape[0], dtype=torch.float64, device="cuda")
+    n = pts.shape[0]
     h = capi.Handle(w.basis.flat(), w.P, w.charges)
-    h.esp(pts.data_ptr(), pts.shape[0], out.data_ptr())
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        h.esp(pts.data_ptr(), pts.shape[0], out.data_ptr())
-    torch.cuda.synchronize()
-    return pts.shape[0] * reps / (time.perf_counter() - t0)
+    res, outs = {}, {}
+    for prec in ("fp64", "mixed"):
+        out = torch.empty(n, dtype=torch.float64, device="cuda")
+        h.esp(pts.data_ptr(), n, out.data_ptr(), prec)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            h.esp(pts.data_ptr(), n, out.data_ptr(), prec)
+        torch.cuda.synchronize()
+        res[f"{prec}_points_per_s"] = n * reps / (time.perf_counter() - t0)
+        outs[prec] = out
+    xyz = torch.from_numpy(w.basis.atom_xyz).cuda()
+    Z = torch.from_numpy(np.asarray(w.charges, dtype=np.float64)).cuda()
+    nuc = (Z[None, :] / torch.cdist(pts, xyz)).sum(dim=1)
+    scale = nuc.abs() + (nuc - outs["fp64"]).abs()
+    res["mixed_max_rel_err"] = float(((outs["mixed"] - outs["fp64"]).abs() / scale).max())
+    res["mixed_over_fp64"] = res["mixed_points_per_s"] / res["fp64_points_per_s"]
+    res["workload"] = f"esp30: 30 atoms, M={w.basis.nbasis}, {n} points on the 1.4 x vdW surface"
+    return res
+
+
+def _err_scales(torch, ref):
+    """Per-point error scales: rho, tau pointwise; grad against max(|grad|, sqrt(8 rho tau))
+    (the Cauchy-Schwarz scale of d_k phi.P phi for a PSD P: grad vanishes at critical
+    points); lap against |lap - 4 tau| + 4 tau, the size of its two summands; the Hessian
+    H = A + T (A_kl = 2 d_kl phi.P phi, T_kl = 2 d_k phi.P d_l phi, |T_kl| <= tr T = 4 tau)
+    against max|H_kl| + 8 tau >= max |A_kl| + |T_kl|; RDG against
+    s (grad scale / |grad| + 4/3); Shannon against rho (|ln rho| + 1)."""
+    rho = ref["rho"][:, 0].abs()
+    out = {"rho": rho}
+    if "tau_pd" in ref:
+        tau = ref["tau_pd"][:, 0].abs()
+        out["tau_pd"] = tau
+        gn = torch.linalg.vector_norm(ref["grad"], dim=1)
+        gs = torch.maximum(gn, torch.sqrt(8 * rho * tau))
+        out["grad"] = gs
+        lap_s = (ref["lap"][:, 0] - 4 * tau).abs() + 4 * tau
+        out["lap"] = lap_s
+        if "hess" in ref:
+            out["hess"] = ref["hess"].abs().amax(dim=1) + 8 * tau
+        if "rdg" in ref:
+            out["rdg"] = ref["rdg"][:, 0].abs() * (gs / torch.clamp(gn, min=1e-300) + 4.0 / 3.0)
+    if "shannon" in ref:
+        out["shannon"] = rho * (torch.log(torch.clamp(rho, min=1e-300)).abs() + 1.0)
+    return out
+
+
+def _max_errors(torch, outs, ref, names):
+    scales = _err_scales(torch, ref)
+    mask = ref["rho"][:, 0] >= 1e-8
+    res = {}
+    for q in names:
+        d = (outs[q] - ref[q]).abs().amax(dim=1) if q == "hess" else torch.linalg.vector_norm(outs[q] - ref[q], dim=1)
+        err = d / torch.clamp(scales[q], min=1e-300)
+        res[q] = float(err[mask].max())
+    return res
+
+
+def large_rate(torch, capi, synthetic, n_points=200_000):
+    """300-atom workload (M = 5400 Cartesian, f on H, f+g on heavy atoms):
+    Hessian, RDG and Shannon (with rho, grad, lap, tau_PD) in the mixed and
+    FP64 paths, points/s of each and the mixed path's per-quantity errors."""
+    w = synthetic.large300(n_points)
+    order = synthetic.morton_order(w.points)
+    pts = torch.from_numpy(np.ascontiguousarray(w.points[order])).cuda()
+    n = pts.shape[0]
+    h = capi.Handle(w.basis.flat(), w.P, w.charges)
+    qs = ["rho", "grad", "hess", "lap", "tau_pd", "rdg", "shannon"]
+    res, outs = {}, {}
+    for prec in ("mixed", "fp64"):
+        o = {q: torch.empty((n, capi.WIDTH.get(q, 1)), dtype=torch.float64, device="cuda") for q in qs}
+        ptrs = {q: t.data_ptr() for q, t in o.items()}
+        h.eval(pts.data_ptr(), n, ptrs, prec)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rep = h.eval(pts.data_ptr(), n, ptrs, prec)
+        torch.cuda.synchronize()
+        res[f"{prec}_points_per_s"] = n / (time.perf_counter() - t0)
+        if prec == "mixed":
+            res["n_reevaluated"] = rep["n_reevaluated"]
+        outs[prec] = o
+    res["mixed_over_fp64"] = res["mixed_points_per_s"] / res["fp64_points_per_s"]
+    res["max_rel_err"] = _max_errors(torch, outs["mixed"], outs["fp64"], ["rho", "hess", "rdg", "shannon"])
+    res["workload"] = (f"large300: 300 atoms, M={w.basis.nbasis}, {n} points (per-atom Gamma(3,1) radii), "
+                       "rho+grad+Hessian+lap+tau_PD+RDG+Shannon")
+    return res
 
 
 def main():
@@ -193,28 +274,15 @@ def main():
         torch.cuda.synchronize()
         extra["fp64_points_per_s"] = n / (time.perf_counter() - t1)
         extra["mixed_over_fp64"] = value / extra["fp64_points_per_s"]
-        rho = ref["rho"][:, 0]
-        mask = rho >= 1e-8
-        # rho, tau: pointwise relative.  grad: against max(|grad|, sqrt(8 rho tau)) (the
-        # Cauchy-Schwarz scale of d_k phi.P phi for a PSD P: grad vanishes at critical
-        # points).  lap: against |lap - 4 tau| + 4 tau, the size of its two summands.
-        tau = ref["tau_pd"][:, 0]
-        scales = {
-            "rho": rho.abs(),
-            "grad": torch.maximum(torch.linalg.vector_norm(ref["grad"], dim=1), torch.sqrt(8 * rho.abs() * tau.abs())),
-            "lap": (ref["lap"][:, 0] - 4 * tau).abs() + 4 * tau.abs(),
-            "tau_pd": tau.abs(),
-        }
-        for q in quantities:
-            err = torch.linalg.vector_norm(outs[q] - ref[q], dim=1) / torch.clamp(scales[q], min=1e-300)
-            extra[f"max_rel_err_{q}"] = float(err[mask].max())
+        for q, v in _max_errors(torch, outs, ref, quantities).items():
+            extra[f"max_rel_err_{q}"] = v
         extra["n_reevaluated"] = rep.get("n_reevaluated") if rep else None
 
     if not args.no_extra:
         extra["fwd_bwd_points_per_s"] = {p: fwd_bwd(torch, capi, synthetic, p) for p in ("fp64", "mixed")}
         extra["fwd_bwd_workload"] = "diff40: 40 atoms, M=416, 500000 points, rho + dL/dP + dL/dR per step"
-        extra["esp_points_per_s"] = esp_rate(torch, capi, synthetic)
-        extra["esp_workload"] = "esp30: 30 atoms, M=312, 200000 points on the 1.4 x vdW surface, FP64"
+        extra["esp"] = esp_rate(torch, capi, synthetic)
+        extra["large300"] = large_rate(torch, capi, synthetic)
 
     kernel_ms = {k: stats[f"{k}_ms"] / args.steps for k in ("ao", "contract", "finalize", "reeval")}
     line = {

@@ -14,14 +14,20 @@
 //   * normalisation is folded into the coefficients on the host;
 //   * every output element is written exactly once (no += on uninitialised
 //     memory), rows beyond M are written as zeros for the padded contraction;
-//   * a shell is skipped for the whole wave when every lane has
-//     a_min r^2 > screen (exp underflow region in parity mode).
+//   * a shell, and within a shell a primitive, is skipped for the whole wave
+//     when every lane has a r^2 > screen (exp underflow region in parity
+//     mode; below FP32 resolution of any density above the re-evaluation
+//     floor in the mixed path);
+//   * values are computed in FP64 for both paths; the mixed path rounds them
+//     once to FP32 on the store (0.5 ulp per value: the FP32 AO error that
+//     enters the mixed path's error estimate is only this rounding).
 //
 // Output layouts (d = derivative slot, i = basis function, p = point in the
 // chunk): FP64 ao[(d * Mp + i) * ld + p], points contiguous (coalesced stores,
 // the B operand layout of the FP64 contraction); FP32 in row quads
-// (ao_quad_index): the four stores of a quad fill 16 contiguous bytes per
-// point, which the mixed contraction reads with one 16-byte load.
+// (ao_quad_index): the four values of a quad are assembled in registers and
+// stored as 16 contiguous bytes per point, which the mixed contraction reads
+// with one 16-byte load.
 
 #include <type_traits>
 
@@ -34,45 +40,28 @@ namespace {
 
 typedef float f4 __attribute__((ext_vector_type(4)));
 
-// exp(-a r^2) with the argument formed in FP64.  For the FP32 path only the
-// reduced argument goes through the FP32 exponential: with t = a r^2 / ln 2,
-// n = rint(t), exp(-a r^2) = 2^-(t - n) * 2^-n, so the FP32 rounding of the
-// argument (|t - n| <= 1/2) costs at most ~2^-25 relative instead of
-// |a r^2| * 2^-24 (which reaches 5e-6 near the exp underflow limit).
-template <typename T>
-__device__ __forceinline__ T gauss_exp(double a, double r2d);
-template <>
-__device__ __forceinline__ double gauss_exp<double>(double a, double r2d) {
-  return exp(-a * r2d);
-}
-template <>
-__device__ __forceinline__ float gauss_exp<float>(double a, double r2d) {
-  const double t = a * r2d * 1.4426950408889634074;
-  const double n = rint(t);
-  const float f = (float)(t - n);
-  return ldexpf(exp2f(-f), -(int)n);
-}
-
-template <typename T, int MODE>
+// TS: storage type (double: FP64 path, row-major; float: mixed path, row quads).
+template <typename TS, int MODE>
 __global__ __launch_bounds__(256) void ao_eval_kernel(const ShellMeta* __restrict__ meta,
                                                       const ShellGeom* __restrict__ geom,
                                                       const PrimData* __restrict__ prim, int nshell, int nbasis,
                                                       int nbasis_pad, const double* __restrict__ pts, int64_t npts,
-                                                      int64_t ld, T* __restrict__ ao, T screen) {
+                                                      int64_t ld, TS* __restrict__ ao, double screen) {
+  using T = double;
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // p < ld by construction
   const int64_t pc = p < npts ? p : (npts - 1);
   const double px = pts[3 * pc + 0], py = pts[3 * pc + 1], pz = pts[3 * pc + 2];
   constexpr int nder = mode_nder(MODE);
-  constexpr bool quads = sizeof(T) == 4;
+  constexpr bool quads = sizeof(TS) == 4;
   const int64_t mq = nbasis_pad >> 2;
   // FP32 (row quads): the four rows of a quad are assembled in registers and
   // stored with one 16-byte store per lane once the quad is complete (shell
   // boundaries need not align with quads: the partial quad carries over).
-  T qb[quads ? nder : 1][4];
+  TS qb[quads ? nder : 1][4];
 #pragma unroll
   for (int d = 0; d < (quads ? nder : 1); ++d)
 #pragma unroll
-    for (int e = 0; e < 4; ++e) qb[d][e] = T(0);
+    for (int e = 0; e < 4; ++e) qb[d][e] = TS(0);
   auto store_quad = [&](int d, int quad) {
     if constexpr (quads)
       *reinterpret_cast<f4*>(ao + ((int64_t)d * mq + quad) * ld * 4 + p * 4) = f4{qb[d][0], qb[d][1], qb[d][2], qb[d][3]};
@@ -89,14 +78,16 @@ __global__ __launch_bounds__(256) void ao_eval_kernel(const ShellMeta* __restric
     const ShellGeom g = geom[s];
     const double dx = px - g.cx, dy = py - g.cy, dz = pz - g.cz;
     const double r2d = dx * dx + dy * dy + dz * dz;
-    const T x = (T)dx, y = (T)dy, z = (T)dz;
+    const T x = dx, y = dy, z = dz;
     T s0 = 0, s1 = 0, s2 = 0;
-    const bool far = g.amin * r2d > (double)screen;
+    const bool far = g.amin * r2d > screen;
     if (!__all(far)) {
       for (int k = 0; k < m.nprim; ++k) {
         const PrimData pd = prim[m.prim_off + k];
-        const T a = (T)pd.a;
-        const T e = (T)pd.c * gauss_exp<T>(pd.a, r2d);
+        const T a = pd.a;
+        const T ar2 = a * r2d;
+        if (__all(ar2 > screen)) continue;
+        const T e = pd.c * exp(-ar2);
         s0 += e;
         if constexpr (MODE != MODE_RHO) {
           const T m2a = T(-2) * a;
@@ -107,7 +98,7 @@ __global__ __launch_bounds__(256) void ao_eval_kernel(const ShellMeta* __restric
     }
     const int first = m.first;
     if constexpr (!quads) {
-      auto st = [&](int d, int ic, T v) { ao[((int64_t)d * nbasis_pad + first + ic) * ld + p] = v; };
+      auto st = [&](int d, int ic, T v) { ao[((int64_t)d * nbasis_pad + first + ic) * ld + p] = (TS)v; };
       dispatch(m.code, x, y, z, s0, s1, s2, st);
     } else {
       // the slot of component ic in its quad is (first + ic) & 3: compile-time
@@ -116,7 +107,7 @@ __global__ __launch_bounds__(256) void ao_eval_kernel(const ShellMeta* __restric
         constexpr int A = decltype(atag)::value;
         auto st = [&](int d, int ic, T v) {
           const int sl = (A + ic) & 3;
-          qb[d][sl] = v;
+          qb[d][sl] = (TS)v;
           if (sl == 3) store_quad(d, (first + ic) >> 2);
         };
         dispatch(m.code, x, y, z, s0, s1, s2, st);
@@ -132,22 +123,22 @@ __global__ __launch_bounds__(256) void ao_eval_kernel(const ShellMeta* __restric
   // zero rows of the padded basis so the contraction never reads garbage
   if constexpr (!quads) {
     for (int d = 0; d < nder; ++d)
-      for (int i = nbasis; i < nbasis_pad; ++i) ao[((int64_t)d * nbasis_pad + i) * ld + p] = T(0);
+      for (int i = nbasis; i < nbasis_pad; ++i) ao[((int64_t)d * nbasis_pad + i) * ld + p] = TS(0);
   } else {
     const int a = nbasis & 3;
     if (a != 0) {
 #pragma unroll
       for (int d = 0; d < nder; ++d) {
-        qb[d][3] = T(0);
-        if (a < 3) qb[d][2] = T(0);
-        if (a < 2) qb[d][1] = T(0);
+        qb[d][3] = TS(0);
+        if (a < 3) qb[d][2] = TS(0);
+        if (a < 2) qb[d][1] = TS(0);
         store_quad(d, nbasis >> 2);
       }
     }
 #pragma unroll
     for (int d = 0; d < nder; ++d) {
 #pragma unroll
-      for (int e = 0; e < 4; ++e) qb[d][e] = T(0);
+      for (int e = 0; e < 4; ++e) qb[d][e] = TS(0);
       for (int quad = (nbasis + 3) >> 2; quad < mq; ++quad) store_quad(d, quad);
     }
   }
@@ -181,7 +172,7 @@ void launch_ao_f64(int mode, const ShellMeta* meta, const ShellGeom* geom, const
 }
 
 void launch_ao_f32(int mode, const ShellMeta* meta, const ShellGeom* geom, const PrimData* prim, int nshell,
-                   int nbasis, int nbasis_pad, const double* pts, int64_t npts, int64_t ld, float* ao, float screen,
+                   int nbasis, int nbasis_pad, const double* pts, int64_t npts, int64_t ld, float* ao, double screen,
                    hipStream_t s) {
   const dim3 block(256), grid((unsigned)(ld / 256));
   switch (mode) {

@@ -1,8 +1,8 @@
 // Mixed-precision contraction: FP32 AO values, P and the AOs split into three
 // bf16 pieces each (x = hi + mid + lo exactly for FP32 x), six bf16 MFMA
-// products per tile (lo*hi, mid*mid, hi*lo, mid*hi, hi*mid, hi*hi)
-// accumulated in FP32 -- FP32-quality Y = P [phi | d phi] at bf16 matrix-core
-// rates -- and FP64 per-point dot products in the epilogue.
+// products per tile (lo*hi, mid*mid, hi*lo, mid*hi, hi*mid, hi*hi) summed per
+// k-step and accumulated in FP32 -- FP32-quality Y = P [phi | d phi] at bf16
+// matrix-core rates -- and FP64 per-point dot products in the epilogue.
 //
 // A seventh product |P_hi| * |phi_hi| (bf16, FP32 accumulation; bf16(|x|) =
 // |bf16(x)|, so both are sign-bit masks of the hi pieces) gives the
@@ -47,20 +47,23 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u4 __attribute__((ext_vector_type(4)));
 
 constexpr int TR = 128;       // rows of P per workgroup
-constexpr int TC = 128;       // GEMM columns per workgroup
 constexpr int KB = 32;        // k per stage (= MFMA depth)
 constexpr int LROW = KB + 8;  // bf16 per LDS row (80 bytes: staggers banks)
 constexpr int NPL = 3;        // pieces: hi, mid, lo
 constexpr int NW = 4;         // waves per workgroup
 constexpr int RW = TR / NW;   // rows per wave
 constexpr int NRB = RW / 16;  // 16-row MFMA blocks per wave
-constexpr int NCB = TC / 16;  // 16-column MFMA blocks
 
 __device__ __forceinline__ bf16x8 as_bf(u4 v) { return __builtin_bit_cast(bf16x8, v); }
 
+// GEMM columns per workgroup: 128 = 32 points x 4 column types in the lap /
+// Hessian modes; 64 points of one type in the rho / grad modes, where every
+// column also carries the bound accumulator.
 template <int MODE>
 struct Cols {
   static constexpr int NT = (MODE == MODE_RHO || MODE == MODE_GRAD) ? 1 : 4;
+  static constexpr int TC = NT == 1 ? 64 : 128;
+  static constexpr int NCB = TC / 16;  // 16-column MFMA blocks
   static constexpr int PTS = TC / NT;  // points per workgroup
   __host__ __device__ static constexpr int type(int c) { return NT == 1 ? 0 : c / PTS; }
   __host__ __device__ static constexpr int point(int c) { return NT == 1 ? c : c % PTS; }
@@ -100,6 +103,7 @@ __global__ __launch_bounds__(256, 2) void contract_mixed_kernel(const u4* __rest
                                                                 double* __restrict__ part, int nrt,
                                                                 int64_t nptile) {
   using C = Cols<MODE>;
+  constexpr int TC = C::TC, NCB = C::NCB;
   constexpr int ND = mode_ndot(MODE);
   constexpr int ND1 = ND + 1;
   constexpr int NBC = C::NT == 1 ? NCB : 2;  // column blocks of type phi (they carry the bound)
@@ -126,30 +130,32 @@ __global__ __launch_bounds__(256, 2) void contract_mixed_kernel(const u4* __rest
   // at Pf[((pl * mp/16 + b16) * nk + kb) * 64 + lane].
   const u4* abase = Pf + (int64_t)((i0 + RW * w) / 16) * nk * 64 + lane;
   const int64_t aplane = (int64_t)(mp / 16) * nk * 64;
-  u4 areg[2][NPL][NRB];
-  auto load_a = [&](auto buf, int kb) {
-    constexpr int b = decltype(buf)::value;
+  u4 areg[NPL][NRB];
+  auto load_a = [&](int kb) {
 #pragma unroll
     for (int pl = 0; pl < NPL; ++pl)
 #pragma unroll
-      for (int rb = 0; rb < NRB; ++rb) areg[b][pl][rb] = abase[pl * aplane + ((int64_t)rb * nk + kb) * 64];
+      for (int rb = 0; rb < NRB; ++rb) areg[pl][rb] = abase[pl * aplane + ((int64_t)rb * nk + kb) * 64];
   };
 
-  // B: two (column, 8-row octet) units per thread; an octet is two row quads.
-  const int bc = tid & (TC - 1), bo = tid >> 7;
+  // B: TC/64 (column, 8-row octet) units per thread, unit u = tid + 256 u
+  // -> column u % TC, octet u / TC; an octet is two row quads.
+  constexpr int NU = TC / 64;
+  const int bc = tid & (TC - 1), bo = tid / TC;
   const float* bsrc = ao + ((int64_t)C::type(bc) * mq * ld + pbase + C::point(bc)) * 4;
-  f4 rbuf[2][2];
+  f4 rbuf[NU][2];
   auto load_b = [&](int kb) {
 #pragma unroll
-    for (int u = 0; u < 2; ++u)
+    for (int u = 0; u < NU; ++u)
 #pragma unroll
       for (int h = 0; h < 2; ++h)
-        rbuf[u][h] = *reinterpret_cast<const f4*>(bsrc + (int64_t)(8 * kb + 2 * (bo + 2 * u) + h) * ld * 4);
+        rbuf[u][h] =
+            *reinterpret_cast<const f4*>(bsrc + (int64_t)(8 * kb + 2 * (bo + (256 / TC) * u) + h) * ld * 4);
   };
   // exact three-piece split with the hardware RNE pack (v_cvt_pk_bf16_f32)
   auto store_b = [&](int s) {
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
+    for (int u = 0; u < NU; ++u) {
       u4 vh, vm, vl;
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
@@ -166,28 +172,30 @@ __global__ __launch_bounds__(256, 2) void contract_mixed_kernel(const u4* __rest
         vm[e] = __builtin_bit_cast(uint32_t, mm);
         vl[e] = __builtin_bit_cast(uint32_t, ll);
       }
-      const int k = 8 * (bo + 2 * u);
+      const int k = 8 * (bo + (256 / TC) * u);
       *reinterpret_cast<u4*>(&Bs[s][0][bc][k]) = vh;
       *reinterpret_cast<u4*>(&Bs[s][1][bc][k]) = vm;
       *reinterpret_cast<u4*>(&Bs[s][2][bc][k]) = vl;
     }
   };
 
-  f4 acc[NRB][NCB], accb[NRB][NBC];
+  // acc: hi*hi products; accs: the five smaller products (2^-8 of the size
+  // and less).  Kept apart, the FP32 rounding of the running sums happens once
+  // per k-step at the full magnitude instead of six times (the largest error
+  // source of the path); Y = acc + accs is formed exactly in FP64.
+  f4 acc[NRB][NCB], accs[NRB][NCB], accb[NRB][NBC];
 #pragma unroll
   for (int a = 0; a < NRB; ++a) {
 #pragma unroll
-    for (int b = 0; b < NCB; ++b) acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+    for (int b = 0; b < NCB; ++b) acc[a][b] = accs[a][b] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int b = 0; b < NBC; ++b) accb[a][b] = f4{0.f, 0.f, 0.f, 0.f};
   }
 
   const u4 absmask = u4{0x7FFF7FFFu, 0x7FFF7FFFu, 0x7FFF7FFFu, 0x7FFF7FFFu};
   // Column blocks in pairs: 2 x NRB independent accumulators per product, so
-  // dependent MFMAs are four issues apart.  Products with the smallest pieces
-  // come first so the large hi*hi term is added last.
-  auto compute = [&](int s, auto buf) {
-    constexpr int b = decltype(buf)::value;
+  // dependent MFMAs are four issues apart.
+  auto compute = [&](int s) {
 #pragma unroll
     for (int cp = 0; cp < NCB / 2; ++cp) {
       bf16x8 bf[2][NPL];
@@ -196,49 +204,51 @@ __global__ __launch_bounds__(256, 2) void contract_mixed_kernel(const u4* __rest
 #pragma unroll
         for (int pl = 0; pl < NPL; ++pl)
           bf[c][pl] = as_bf(*reinterpret_cast<const u4*>(&Bs[s][pl][16 * (2 * cp + c) + r][8 * q]));
-      constexpr int pa[6] = {2, 1, 0, 1, 0, 0};
-      constexpr int pb[6] = {0, 1, 2, 0, 1, 0};
+      constexpr int pa[5] = {2, 1, 0, 1, 0};
+      constexpr int pb[5] = {0, 1, 2, 0, 1};
 #pragma unroll
-      for (int pr = 0; pr < 6; ++pr)
+      for (int pr = 0; pr < 5; ++pr)
 #pragma unroll
         for (int rb = 0; rb < NRB; ++rb)
 #pragma unroll
           for (int c = 0; c < 2; ++c)
-            acc[rb][2 * cp + c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(areg[b][pa[pr]][rb]), bf[c][pb[pr]],
-                                                                           acc[rb][2 * cp + c], 0, 0, 0);
+            accs[rb][2 * cp + c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(areg[pa[pr]][rb]), bf[c][pb[pr]],
+                                                                            accs[rb][2 * cp + c], 0, 0, 0);
+#pragma unroll
+      for (int rb = 0; rb < NRB; ++rb)
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+          acc[rb][2 * cp + c] =
+              __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(areg[0][rb]), bf[c][0], acc[rb][2 * cp + c], 0, 0, 0);
       if (2 * cp < NBC) {
 #pragma unroll
         for (int rb = 0; rb < NRB; ++rb)
 #pragma unroll
           for (int c = 0; c < 2; ++c)
             accb[rb][(2 * cp + c) % NBC] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                as_bf(areg[b][0][rb] & absmask), as_bf(__builtin_bit_cast(u4, bf[c][0]) & absmask),
+                as_bf(areg[0][rb] & absmask), as_bf(__builtin_bit_cast(u4, bf[c][0]) & absmask),
                 accb[rb][(2 * cp + c) % NBC], 0, 0, 0);
       }
     }
   };
 
-  // Pipeline: B global loads run one k-step ahead of their LDS store, A
-  // fragments two k-steps ahead of their use.
+  // Pipeline: B global loads run one k-step ahead of their LDS store; the A
+  // fragments of the next k-step are loaded after the last use of the current
+  // ones (the other workgroup on the CU covers the wait).
   load_b(0);
-  load_a(IC<0>{}, 0);
+  load_a(0);
   store_b(0);
   load_b(1);
-  load_a(IC<1>{}, 1);
   __syncthreads();
-  auto step = [&](int kb, auto buf) {
+  for (int kb = 0; kb < nk; ++kb) {
     const int s = kb & 1;
-    compute(s, buf);
-    if (kb + 1 < nk) store_b(s ^ 1);
-    if (kb + 2 < nk) {
-      load_b(kb + 2);
-      load_a(buf, kb + 2);
+    compute(s);
+    if (kb + 1 < nk) {
+      load_a(kb + 1);
+      store_b(s ^ 1);
     }
+    if (kb + 2 < nk) load_b(kb + 2);
     __syncthreads();
-  };
-  for (int kb = 0; kb < nk; kb += 2) {
-    step(kb, IC<0>{});
-    step(kb + 1, IC<1>{});
   }
 
   // ---------------------------------------------------------------- epilogue
@@ -261,7 +271,7 @@ __global__ __launch_bounds__(256, 2) void contract_mixed_kernel(const u4* __rest
         v[k] = *reinterpret_cast<const f4*>(ao + (((int64_t)dot_array<MODE>(t, k) * mq + quad0 + 4 * rb) * ld + p) * 4);
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const double y = (double)acc[rb][CB][j];
+        const double y = (double)acc[rb][CB][j] + (double)accs[rb][CB][j];
 #pragma unroll
         for (int k = 0; k < nd; ++k) d[k] += (double)v[k][j] * y;
         if constexpr (bnd) d[kMaxDots - 1] += fabs((double)v[0][j]) * (double)accb[rb][CB % NBC][j];
@@ -288,10 +298,12 @@ __global__ __launch_bounds__(256, 2) void contract_mixed_kernel(const u4* __rest
   per_cb(IC<1>{});
   per_cb(IC<2>{});
   per_cb(IC<3>{});
-  per_cb(IC<4>{});
-  per_cb(IC<5>{});
-  per_cb(IC<6>{});
-  per_cb(IC<7>{});
+  if constexpr (NCB == 8) {
+    per_cb(IC<4>{});
+    per_cb(IC<5>{});
+    per_cb(IC<6>{});
+    per_cb(IC<7>{});
+  }
   __syncthreads();
   if (tid < TC) {
     const int c = tid;
@@ -314,7 +326,9 @@ __global__ __launch_bounds__(256, 2) void contract_mixed_kernel(const u4* __rest
 
 }  // namespace
 
-int mixed_points_per_block(int mode) { return (mode == MODE_RHO || mode == MODE_GRAD) ? TC : TC / 4; }
+int mixed_points_per_block(int mode) {
+  return mode == MODE_RHO ? Cols<MODE_RHO>::PTS : mode == MODE_GRAD ? Cols<MODE_GRAD>::PTS : Cols<MODE_LAP>::PTS;
+}
 
 void mixed_fragment_order_P(const uint16_t* planes, int mp, uint16_t* out) {
   // out[((pl * mp/16 + b16) * nk + kb) * 64 + lane][e] = plane pl of

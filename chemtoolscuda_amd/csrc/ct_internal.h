@@ -60,9 +60,11 @@ struct PrimData {
 };
 
 // Error model of the mixed path: |rho_mixed - rho| <= gamma * |phi|^T |P| |phi|.
-// FP32 AO rounding contributes ~2u|phi|^T|P||phi| (u = 2^-24) and the
-// three-piece bf16 products with FP32 accumulation a few u more; 2^-21 leaves
-// margin over the largest ratio observed.
+// Rounding the FP64-computed AOs to FP32 contributes <= u|phi|^T|P||phi|
+// (u = 2^-24), the terms the three-piece bf16 split drops <= 2u, FP32
+// accumulation the rest; the largest ratio error / (u |phi|^T|P||phi|)
+// measured on the 60- and 300-atom workloads is below 6, so 2^-21 = 8u
+// leaves a margin of 1.4.
 constexpr double kMixedGamma = 4.76837158203125e-07;  // 2^-21
 
 // Granularities.
@@ -113,7 +115,7 @@ void launch_finalize(const FinalizeArgs& a, hipStream_t s);
 
 // Mixed path.
 void launch_ao_f32(int mode, const ShellMeta* meta, const ShellGeom* geom, const PrimData* prim, int nshell,
-                   int nbasis, int nbasis_pad, const double* pts, int64_t npts, int64_t ld, float* ao, float screen,
+                   int nbasis, int nbasis_pad, const double* pts, int64_t npts, int64_t ld, float* ao, double screen,
                    hipStream_t s);
 int mixed_points_per_block(int mode);
 // Reorders three row-major bf16 planes of P ([3][mp][mp]) into the MFMA

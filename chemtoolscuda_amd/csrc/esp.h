@@ -49,6 +49,7 @@ struct EspSegment {
   int32_t L;
   int32_t count;  // primitive pairs in the segment
   int64_t off;    // offset (doubles) of the first record; a record is [Qx, Qy, Qz, p, g...]
+  int64_t off32;  // mixed precision: offset (floats) of the segment's FP32 g coefficients
 };
 
 struct EspData {
@@ -62,7 +63,14 @@ EspData build_esp_pairs(const HostBasis& basis, const std::vector<double>& xyz, 
                         double screen);
 std::vector<double> build_boys_table();
 
-void launch_esp(const double* pts, int64_t n, const double* records, const EspSegment* segs, int nseg,
-                const double* atoms, int natom, const double* boys_tab, double* out, hipStream_t s);
+// FP32 copy of the g coefficients (esp_ncoef(L) floats per pair, segment by
+// segment) for the mixed-precision kernel; fills EspSegment::off32.
+std::vector<float> esp_coefficients_f32(const std::vector<double>& records, std::vector<EspSegment>& segs);
+
+// records32 == nullptr: all-FP64 kernel.  Otherwise the mixed kernel: the
+// pair geometry (Q - C, T = p |Q - C|^2), the Boys function and the sums over
+// pairs in FP64, the Hermite polynomial sum_ijk g X^i Y^j Z^k in FP32.
+void launch_esp(const double* pts, int64_t n, const double* records, const float* records32, const EspSegment* segs,
+                int nseg, const double* atoms, int natom, const double* boys_tab, double* out, hipStream_t s);
 
 }  // namespace ct

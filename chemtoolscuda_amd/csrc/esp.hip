@@ -14,6 +14,8 @@
 //
 // ESP(C) = sum_A Z_A / |C - R_A| - sum_pairs S_pair(C)
 // (reference src/evaluate_electrostatic.cu:10356-10371 for the nuclear term).
+// Two instantiations: all-FP64, and mixed (Boys function, pair geometry and
+// accumulation in FP64, Hermite polynomial in FP32).
 
 #include "esp.h"
 
@@ -50,49 +52,63 @@ __device__ __forceinline__ void boys(double T, const double* __restrict__ tab, d
   }
 }
 
-template <int L>
-__device__ __forceinline__ double pair_value(const double* __restrict__ rec, double cx, double cy, double cz,
-                                             const double* __restrict__ tab) {
+// One primitive pair: F_0..F_L(T) in FP64, then sum_N F_N poly_N with the
+// polynomial in TP (double, or float for the mixed kernel: X, Y, Z and the
+// coefficients rounded to FP32 once, F_N * poly_N and the sums over pairs in
+// FP64).
+template <int L, typename TP>
+__device__ __forceinline__ double pair_value(const double* __restrict__ rec, const TP* __restrict__ c, double cx,
+                                             double cy, double cz, const double* __restrict__ tab) {
   const double X = rec[0] - cx, Y = rec[1] - cy, Z = rec[2] - cz, p = rec[3];
   const double T = p * (X * X + Y * Y + Z * Z);
   double F[L + 1];
   boys<L>(T, tab, F);
-  double xp[L + 1], yp[L + 1], zp[L + 1];
-  xp[0] = yp[0] = zp[0] = 1.0;
+  TP xp[L + 1], yp[L + 1], zp[L + 1];
+  xp[0] = yp[0] = zp[0] = TP(1);
 #pragma unroll
   for (int i = 1; i <= L; ++i) {
-    xp[i] = xp[i - 1] * X;
-    yp[i] = yp[i - 1] * Y;
-    zp[i] = zp[i - 1] * Z;
+    xp[i] = xp[i - 1] * (TP)X;
+    yp[i] = yp[i - 1] * (TP)Y;
+    zp[i] = zp[i - 1] * (TP)Z;
   }
-  const double* c = rec + 4;
   double s = 0.0;
   int idx = 0;
 #pragma unroll
   for (int N = 0; N <= L; ++N) {
-    double poly = 0.0;
+    TP poly = TP(0);
 #pragma unroll
     for (int tot = (2 * N - L > 0 ? 2 * N - L : 0); tot <= N; ++tot)
 #pragma unroll
       for (int i = tot; i >= 0; --i)
 #pragma unroll
         for (int j = tot - i; j >= 0; --j) poly += c[idx++] * (xp[i] * yp[j] * zp[tot - i - j]);
-    s += F[N] * poly;
+    s += F[N] * (double)poly;
   }
   return s;
 }
 
-template <int L>
-__device__ __forceinline__ double segment_sum(const double* __restrict__ rec, int count, double cx, double cy,
-                                              double cz, const double* __restrict__ tab) {
-  constexpr int stride = 4 + esp_ncoef(L);
+template <int L, typename TP>
+__device__ __forceinline__ double segment_sum(const double* __restrict__ rec, const TP* __restrict__ coef, int count,
+                                              double cx, double cy, double cz, const double* __restrict__ tab) {
+  constexpr int nc = esp_ncoef(L);
+  constexpr int stride = 4 + nc;
   double s = 0.0;
-  for (int k = 0; k < count; ++k) s += pair_value<L>(rec + (int64_t)k * stride, cx, cy, cz, tab);
+  for (int k = 0; k < count; ++k) {
+    const double* r = rec + (int64_t)k * stride;
+    const TP* c;
+    if constexpr (sizeof(TP) == 8)
+      c = reinterpret_cast<const TP*>(r + 4);
+    else
+      c = coef + (int64_t)k * nc;
+    s += pair_value<L, TP>(r, c, cx, cy, cz, tab);
+  }
   return s;
 }
 
+template <typename TP>
 __global__ __launch_bounds__(256) void esp_kernel(const double* __restrict__ pts, int64_t n,
                                                   const double* __restrict__ records,
+                                                  const float* __restrict__ records32,
                                                   const EspSegment* __restrict__ segs, int nseg,
                                                   const double* __restrict__ atoms, int natom,
                                                   const double* __restrict__ boys_tab, double* __restrict__ out) {
@@ -106,16 +122,17 @@ __global__ __launch_bounds__(256) void esp_kernel(const double* __restrict__ pts
   for (int sg = 0; sg < nseg; ++sg) {
     const EspSegment seg = segs[sg];
     const double* rec = records + seg.off;
+    const TP* c32 = reinterpret_cast<const TP*>(records32 + (sizeof(TP) == 4 ? seg.off32 : 0));
     switch (seg.L) {
-      case 0: el += segment_sum<0>(rec, seg.count, cx, cy, cz, tab); break;
-      case 1: el += segment_sum<1>(rec, seg.count, cx, cy, cz, tab); break;
-      case 2: el += segment_sum<2>(rec, seg.count, cx, cy, cz, tab); break;
-      case 3: el += segment_sum<3>(rec, seg.count, cx, cy, cz, tab); break;
-      case 4: el += segment_sum<4>(rec, seg.count, cx, cy, cz, tab); break;
-      case 5: el += segment_sum<5>(rec, seg.count, cx, cy, cz, tab); break;
-      case 6: el += segment_sum<6>(rec, seg.count, cx, cy, cz, tab); break;
-      case 7: el += segment_sum<7>(rec, seg.count, cx, cy, cz, tab); break;
-      default: el += segment_sum<8>(rec, seg.count, cx, cy, cz, tab); break;
+      case 0: el += segment_sum<0, TP>(rec, c32, seg.count, cx, cy, cz, tab); break;
+      case 1: el += segment_sum<1, TP>(rec, c32, seg.count, cx, cy, cz, tab); break;
+      case 2: el += segment_sum<2, TP>(rec, c32, seg.count, cx, cy, cz, tab); break;
+      case 3: el += segment_sum<3, TP>(rec, c32, seg.count, cx, cy, cz, tab); break;
+      case 4: el += segment_sum<4, TP>(rec, c32, seg.count, cx, cy, cz, tab); break;
+      case 5: el += segment_sum<5, TP>(rec, c32, seg.count, cx, cy, cz, tab); break;
+      case 6: el += segment_sum<6, TP>(rec, c32, seg.count, cx, cy, cz, tab); break;
+      case 7: el += segment_sum<7, TP>(rec, c32, seg.count, cx, cy, cz, tab); break;
+      default: el += segment_sum<8, TP>(rec, c32, seg.count, cx, cy, cz, tab); break;
     }
   }
   double nuc = 0.0;
@@ -128,11 +145,16 @@ __global__ __launch_bounds__(256) void esp_kernel(const double* __restrict__ pts
 
 }  // namespace
 
-void launch_esp(const double* pts, int64_t n, const double* records, const EspSegment* segs, int nseg,
-                const double* atoms, int natom, const double* boys_tab, double* out, hipStream_t s) {
+void launch_esp(const double* pts, int64_t n, const double* records, const float* records32, const EspSegment* segs,
+                int nseg, const double* atoms, int natom, const double* boys_tab, double* out, hipStream_t s) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(esp_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, pts, n, records, segs, nseg,
-                     atoms, natom, boys_tab, out);
+  const dim3 grid((unsigned)((n + 255) / 256)), block(256);
+  if (records32 == nullptr)
+    hipLaunchKernelGGL(esp_kernel<double>, grid, block, 0, s, pts, n, records, records32, segs, nseg, atoms, natom,
+                       boys_tab, out);
+  else
+    hipLaunchKernelGGL(esp_kernel<float>, grid, block, 0, s, pts, n, records, records32, segs, nseg, atoms, natom,
+                       boys_tab, out);
   CT_HIP_CHECK(hipGetLastError());
 }
 

@@ -200,8 +200,22 @@ EspData build_esp_pairs(const HostBasis& b, const std::vector<double>& xyz, cons
     sg.L = kv.first;
     sg.count = (int32_t)countL[kv.first];
     sg.off = (int64_t)out.records.size();
+    sg.off32 = 0;
     out.records.insert(out.records.end(), kv.second.begin(), kv.second.end());
     out.segs.push_back(sg);
+  }
+  return out;
+}
+
+std::vector<float> esp_coefficients_f32(const std::vector<double>& records, std::vector<EspSegment>& segs) {
+  std::vector<float> out;
+  for (auto& sg : segs) {
+    const int nc = esp_ncoef(sg.L);
+    sg.off32 = (int64_t)out.size();
+    for (int64_t k = 0; k < sg.count; ++k) {
+      const double* g = records.data() + sg.off + k * (4 + nc) + 4;
+      for (int c = 0; c < nc; ++c) out.push_back((float)g[c]);
+    }
   }
   return out;
 }

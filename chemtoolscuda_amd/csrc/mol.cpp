@@ -328,7 +328,8 @@ void Mol::set_density_matrix(const double* P) {
 
 void Mol::prepare_esp() {
   if (esp_valid_) return;
-  const EspData d = build_esp_pairs(basis_, xyz_, P_, 1e-18);
+  EspData d = build_esp_pairs(basis_, xyz_, P_, 1e-18);
+  const std::vector<float> c32 = esp_coefficients_f32(d.records, d.segs);
   std::vector<double> atoms(4 * (size_t)natom_);
   for (int a = 0; a < natom_; ++a) {
     for (int k = 0; k < 3; ++k) atoms[4 * a + k] = xyz_[3 * a + k];
@@ -341,8 +342,11 @@ void Mol::prepare_esp() {
   }
   if (!d_esp_rec_.reserve(sizeof(double) * std::max<size_t>(1, d.records.size())) ||
       !d_esp_seg_.reserve(sizeof(EspSegment) * std::max<size_t>(1, d.segs.size())) ||
-      !d_esp_atoms_.reserve(sizeof(double) * atoms.size()))
+      !d_esp_atoms_.reserve(sizeof(double) * atoms.size()) ||
+      !d_esp_rec32_.reserve(sizeof(float) * std::max<size_t>(1, c32.size())))
     throw Error(CT_E_NOMEM, "cannot allocate the ESP pair table");
+  if (!c32.empty())
+    CT_HIP_CHECK(hipMemcpy(d_esp_rec32_.get(), c32.data(), sizeof(float) * c32.size(), hipMemcpyHostToDevice));
   if (!d.records.empty())
     CT_HIP_CHECK(hipMemcpy(d_esp_rec_.get(), d.records.data(), sizeof(double) * d.records.size(), hipMemcpyHostToDevice));
   if (!d.segs.empty())
@@ -451,9 +455,10 @@ void Mol::esp(const double* points, int64_t n, int precision, double* out) {
     if (!od) dout = io + 3 * n;
   }
   timer_.begin(KernelTimer::ESP, stream_);
-  launch_esp(dp, n, static_cast<const double*>(d_esp_rec_.get()), static_cast<const EspSegment*>(d_esp_seg_.get()),
-             esp_nseg_, static_cast<const double*>(d_esp_atoms_.get()), natom_,
-             static_cast<const double*>(d_boys_.get()), dout, stream_);
+  const float* rec32 = precision == CT_PREC_MIXED ? static_cast<const float*>(d_esp_rec32_.get()) : nullptr;
+  launch_esp(dp, n, static_cast<const double*>(d_esp_rec_.get()), rec32,
+             static_cast<const EspSegment*>(d_esp_seg_.get()), esp_nseg_, static_cast<const double*>(d_esp_atoms_.get()),
+             natom_, static_cast<const double*>(d_boys_.get()), dout, stream_);
   timer_.end(stream_);
   if (!od) CT_HIP_CHECK(hipMemcpyAsync(out, dout, sizeof(double) * n, hipMemcpyDeviceToHost, stream_));
   CT_HIP_CHECK(hipStreamSynchronize(stream_));
@@ -603,7 +608,7 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
       timer_.end(stream_);
     } else {
       timer_.begin(KernelTimer::AO, stream_);
-      launch_ao_f32(mode, meta, geom, prim, nshell_, M_, Mp_, pts, nc, ld, static_cast<float*>(ao), 50.0f, stream_);
+      launch_ao_f32(mode, meta, geom, prim, nshell_, M_, Mp_, pts, nc, ld, static_cast<float*>(ao), 50.0, stream_);
       timer_.end(stream_);
       timer_.begin(KernelTimer::CONTRACT, stream_);
       launch_contract_mixed(mode, static_cast<const uint16_t*>(d_Pb_.get()), Mp_, static_cast<const float*>(ao), ld,

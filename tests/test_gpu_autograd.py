@@ -13,9 +13,12 @@ Mixed forward pass: for dL/dP the residual is formed as rho(P - P_t) (rho is
 linear in P, so for fixed centres this is the same function of P): the
 FP32-quality contraction is then accurate to ~1e-7 of the residual itself and
 the per-component 1e-6 criterion holds.  For dL/dR_A (rho_t fixed, so no such
-rewrite) the forward rounding (~1e-7 of rho) enters a residual that is ~1e-3
-of rho; the check is then relative to the gradient's scale, max |dL/dR|, at
-5e-6 (measured ~2e-6 on this workload).
+rewrite) the mixed forward's error, |d rho_g| <= 1e-6 rho_g, enters
+dL/drho_g = 2 w_g (rho_g - rho_t,g) directly; the backward kernels are FP64,
+so the gradient can differ from the FP64 finite difference by at most
+sum_g 2 w_g 1e-6 rho_g |d rho_g / d R_A,k|, which the test computes per
+component (d rho_g / d R_A = -2 sum_{i on A} grad phi_i (P phi)_i with the
+oracle's AOs) and adds to the 1e-6 finite-difference tolerance.
 """
 import numpy as np
 import pytest
@@ -119,13 +122,30 @@ def test_autograd_against_finite_differences(ct, precision):
         def central(E, h):
             return (loss(ref_model, P, R + h * E) - loss(ref_model, P, R - h * E)).item() / (2 * h)
 
+        fwd_bound = np.zeros_like(gR)
+        if precision == "mixed":
+            fwd_bound = _mixed_forward_bound(w, ref_model.density(P.detach(), R0, pts).cpu().numpy())
         for A in rng.choice(w.basis.natom, size=4, replace=False):
             for k in range(3):
                 E = torch.zeros_like(R)
                 E[A, k] = 1.0
                 fd = (4 * central(E, 1e-5) - central(E, 2e-5)) / 3
                 an = gR[A, k]
-                if precision == "mixed":
-                    assert abs(fd - an) <= 5e-6 * np.abs(gR).max(), (A, k, fd, an)
-                else:
-                    assert abs(fd - an) <= 1e-6 * max(abs(fd), 1e-3 * np.abs(gR).max()), (A, k, fd, an)
+                tol = 1e-6 * max(abs(fd), 1e-3 * np.abs(gR).max()) + fwd_bound[A, k]
+                assert abs(fd - an) <= tol, (A, k, fd, an, fwd_bound[A, k])
+
+
+def _mixed_forward_bound(w, rho):
+    """sum_g 2 w_g 1e-6 |rho_g| |d rho_g / d R_A,k| for every centre and axis."""
+    flat = w.basis.flat()
+    ao = oracle.ao(flat, w.points, 1)
+    Pphi = w.P @ ao[0]
+    atom_of = np.repeat([s.atom for s in w.basis.shells], [s.nfunc for s in w.basis.shells])
+    wt = 2.0 * w.extra["weights"] * 1e-6 * np.abs(rho)
+    out = np.zeros((w.basis.natom, 3))
+    for k in range(3):
+        per_fn = -2.0 * ao[1 + k] * Pphi  # (M, n): contribution of function i to d rho / d R_atom(i)
+        drho = np.zeros((w.basis.natom, w.points.shape[0]))
+        np.add.at(drho, atom_of, per_fn)
+        out[:, k] = np.abs(drho) @ wt
+    return out

@@ -85,3 +85,26 @@ def test_esp_cache_invalidation_and_edge_cases(ct):
     _check(c, oracle.esp(w.basis.with_coordinates(xyz2).flat(), P2, w.charges, pts), xyz2, w.charges, pts)
     assert not np.allclose(a, b)
     assert mol.compute_electrostatic_potential(np.zeros((0, 3))).shape == (0,)
+
+
+@pytest.mark.parametrize("case", ["water", "esp30", "atom_c_g_pure_orbital.fchk", "ch4.fchk"])
+def test_esp_mixed_against_fp64(ct, case):
+    """Mixed ESP (Boys function, geometry and accumulation in FP64, Hermite
+    polynomial in FP32) against the FP64 kernel, relative to |V_nuc| + |V_el|."""
+    from chemtoolscuda_amd import synthetic
+
+    if case.endswith(".fchk"):
+        path = os.path.join(fchk_py.FCHK_DIR, case)
+        d = fchk_py.load(path)
+        mol = ct.Molecule(path)
+        xyz, Z = d["basis"].atom_xyz, d["charges"]
+        pts = _points(xyz, 400, 6)
+    else:
+        w = synthetic.WORKLOADS[case]()
+        mol = ct.molecule_from_basis(w.basis, w.P, w.charges)
+        xyz, Z = w.basis.atom_xyz, w.charges
+        pts = w.points[:4000] if case == "esp30" else _points(xyz, 2000, 6)
+    ref = mol.compute_electrostatic_potential(pts)
+    got = mol.compute_electrostatic_potential(pts, precision="mixed")
+    err = _check(got, ref, xyz, Z, pts, tol=1e-6)
+    print(f"{case}: mixed ESP max error / (|V_nuc| + |V_el|) = {err:.2e}")

@@ -128,3 +128,32 @@ def test_mixed_error_model_is_conservative(ct):
           f"max estimate {rep['max_error_estimate']:.2e}, reevaluated {rep['n_reevaluated']}")
     assert rep["n_reevaluated"] == 0
     assert rel.max() <= rep["max_error_estimate"]
+
+
+@pytest.mark.parametrize("workload,npts", [("water", 3000), ("organic60", 3000), ("large300", 400)])
+def test_mixed_error_model_pointwise(ct, workload, npts):
+    """Without re-evaluation, the error of every quantity stays below
+    gamma * (its condition scale), gamma = 2^-21 -- the estimate the finaliser
+    applies to rho.  Printed: the largest ratio error / (u * scale), u = 2^-24,
+    per quantity."""
+    from chemtoolscuda_amd import synthetic
+
+    w = synthetic.large300(npts) if workload == "large300" else synthetic.WORKLOADS[workload]()
+    rng = np.random.default_rng(8)
+    pts = w.points[rng.choice(w.points.shape[0], size=min(npts, w.points.shape[0]), replace=False)]
+    mol = ct.molecule_from_basis(w.basis, w.P, w.charges)
+    names = ["rho", "grad", "lap", "tau_pd"]
+    raw = mol.compute(pts, names, precision="mixed", mixed_tol=1e30)
+    assert mol.last_report()["n_reevaluated"] == 0
+    f64 = mol.compute(pts, names, precision="fp64")
+    scales = _condition_scales(w.basis.flat(), w.P, pts)
+    u = 2.0 ** -24
+    ratios = {}
+    for k, sc in zip(names, scales):
+        a = np.asarray(raw[k]).reshape(len(pts), -1)
+        b = np.asarray(f64[k]).reshape(len(pts), -1)
+        ratios[k] = float(np.max(np.linalg.norm(a - b, axis=1) / (u * sc)))
+    print(f"{workload} (M={w.basis.nbasis}): max error / (u * condition scale): "
+          + ", ".join(f"{k} {v:.2f}" for k, v in ratios.items()))
+    for k, v in ratios.items():
+        assert v <= 8.0, (k, v)
