@@ -22,7 +22,8 @@ INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(HERE, "libchemtools_amd.so")
 ARCH = os.environ.get("CT_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["ao_kernels.hip", "contract_f64.hip", "contract_mixed.hip", "finalize.hip", "esp.hip", "adjoint.hip"]
+HIP_SOURCES = ["ao_kernels.hip", "contract_f64.hip", "contract_mixed.hip", "finalize.hip", "esp.hip", "adjoint.hip",
+               "adjoint_mixed.hip"]
 HOST_SOURCES = ["mol.cpp", "capi.cpp", "fchk.cpp", "esp_host.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -143,5 +143,11 @@ void launch_gram_f64(const double* ao, int mp, int64_t ld, const double* w, int6
 void launch_gram_reduce(const double* part, int nslice, int64_t count, double* G, hipStream_t s);
 void launch_contract_adjoint_f64(const double* P, int mp, const double* ao, int64_t ld, const double* w,
                                  int64_t npts, double* T, hipStream_t s);
+// Mixed adjoint (FP32 row-quad AOs, split-bf16 MFMA, FP64 partials).
+void gram_mixed_plan(int64_t npts, int mp, int* nslice, int64_t* slice_len);
+void launch_gram_mixed(const float* ao, int mp, int64_t ld, const double* w, int64_t npts, const int32_t* tiles,
+                       int nslice, int64_t slice_len, double* part, hipStream_t s);
+void launch_contract_adjoint_mixed(const uint16_t* Pf, int mp, const float* ao, int64_t ld, const double* w,
+                                   int64_t npts, double* T, hipStream_t s);
 
 }  // namespace ct

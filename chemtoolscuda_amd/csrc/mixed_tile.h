@@ -1,0 +1,204 @@
+// Split-bf16 tile engine shared by the mixed-precision contraction and the
+// mixed adjoint: Y = P[i0:i0+128, :] * B for TC GEMM columns, FP32-quality,
+// on v_mfma_f32_16x16x32_bf16.
+//
+// P and the FP32 columns are split exactly into three bf16 pieces
+// (x = hi + mid + lo), and Y is the sum of the six products with the
+// largest terms (hi*hi, hi*mid, mid*hi, hi*lo, lo*hi, mid*mid; the dropped
+// ones are <= 2^-23 of |P||B|).  hi*hi goes to one FP32 accumulator, the
+// five smaller products to another: the FP32 rounding of the running sum at
+// full magnitude then happens once per k-step instead of six times, the
+// largest error source of the path.  An optional seventh product
+// |P_hi| * |B_hi| (sign-bit masks of the hi pieces) accumulates the
+// magnitude sum that scales the rounding errors.
+//
+// One workgroup = 4 waves, two workgroups per CU; wave w owns rows
+// 32w..32w+31 of the 128-row block and all TC columns.  K advances 32 per
+// step:
+//   A (P pieces): stored on the host in MFMA fragment order
+//     (mixed_fragment_order_P), so each wave loads its 16 x 32 fragments
+//     straight into registers with one coalesced 1 KiB load per (piece,
+//     16-row block); the next step's fragments are loaded after the last use
+//     of the current ones (the other workgroup on the CU covers the wait);
+//   B: FP32 row quads (ao_quad_index layout) read one k-step ahead, split on
+//     the way into double-buffered LDS, column-major with 80-byte rows
+//     (conflict-free 16-byte fragment reads).
+//
+// MFMA 16x16x32 bf16 lane layout (lane l, r = l % 16, q = l / 16), measured:
+//   A: A[r][8q + e], B: B[8q + e][r] (e = 0..7), D: D[4q + j][r] (j = 0..3).
+#pragma once
+
+#include "ct_internal.h"
+
+namespace ct {
+namespace mixed {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+
+constexpr int TR = 128;       // rows of P per workgroup
+constexpr int KB = 32;        // k per stage (= MFMA depth)
+constexpr int LROW = KB + 8;  // bf16 per LDS row (80 bytes: staggers banks)
+constexpr int NPL = 3;        // pieces: hi, mid, lo
+constexpr int NW = 4;         // waves per workgroup
+constexpr int RW = TR / NW;   // rows per wave
+constexpr int NRB = RW / 16;  // 16-row MFMA blocks per wave
+
+__device__ __forceinline__ bf16x8 as_bf(u4 v) { return __builtin_bit_cast(bf16x8, v); }
+
+// Exact three-piece split of two FP32 values (hardware RNE pack,
+// v_cvt_pk_bf16_f32): the packed hi, mid and lo pairs.
+struct Split2 {
+  uint32_t h, m, l;
+};
+__device__ __forceinline__ Split2 split2(float x0, float x1) {
+  bf16x2 hh, mm, ll;
+  hh[0] = (__bf16)x0;
+  hh[1] = (__bf16)x1;
+  const float r0 = x0 - (float)hh[0], r1 = x1 - (float)hh[1];
+  mm[0] = (__bf16)r0;
+  mm[1] = (__bf16)r1;
+  ll[0] = (__bf16)(r0 - (float)mm[0]);
+  ll[1] = (__bf16)(r1 - (float)mm[1]);
+  return Split2{__builtin_bit_cast(uint32_t, hh), __builtin_bit_cast(uint32_t, mm), __builtin_bit_cast(uint32_t, ll)};
+}
+
+template <int TC, int NBC>
+struct YTile {
+  static constexpr int NCB = TC / 16;
+  f4 hh[NRB][NCB];                     // hi*hi products
+  f4 sm[NRB][NCB];                     // the five smaller products
+  f4 bound[NRB][NBC > 0 ? NBC : 1];    // |P_hi| |B_hi| for column blocks 0..NBC-1
+  // Y in FP64 (exact sum of the two accumulators)
+  __device__ __forceinline__ double y(int rb, int cb, int j) const {
+    return (double)hh[rb][cb][j] + (double)sm[rb][cb][j];
+  }
+};
+
+// Computes the tile.  bsrc: this thread's B column at row quad 0, i.e. the
+// column tid % TC; its row quad k lies at bsrc + k * ld * 4.  Bs: the
+// workgroup's LDS.  Ends with a barrier (Bs free for reuse).
+template <int TC, int NBC>
+__device__ __forceinline__ void y_tile(const u4* __restrict__ Pf, int mp, int64_t ld, int i0, int tid,
+                                       const float* __restrict__ bsrc, uint16_t (&Bs)[2][NPL][TC][LROW],
+                                       YTile<TC, NBC>& t) {
+  constexpr int NCB = TC / 16;
+  const int lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int r = lane & 15, q = lane >> 4;
+  const int nk = mp / KB;
+
+  // A: fragment (piece pl, 16-row block b16, k-step kb) is the 64 x 16 bytes
+  // at Pf[((pl * mp/16 + b16) * nk + kb) * 64 + lane].
+  const u4* abase = Pf + (int64_t)((i0 + RW * w) / 16) * nk * 64 + lane;
+  const int64_t aplane = (int64_t)(mp / 16) * nk * 64;
+  u4 areg[NPL][NRB];
+  auto load_a = [&](int kb) {
+#pragma unroll
+    for (int pl = 0; pl < NPL; ++pl)
+#pragma unroll
+      for (int rb = 0; rb < NRB; ++rb) areg[pl][rb] = abase[pl * aplane + ((int64_t)rb * nk + kb) * 64];
+  };
+
+  // B: TC/64 (column, 8-row octet) units per thread, unit u = tid + 256 u
+  // -> column u % TC, octet u / TC; an octet is two row quads.
+  constexpr int NU = TC / 64;
+  const int bc = tid & (TC - 1), bo = tid / TC;
+  f4 rbuf[NU][2];
+  auto load_b = [&](int kb) {
+#pragma unroll
+    for (int u = 0; u < NU; ++u)
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+        rbuf[u][h] =
+            *reinterpret_cast<const f4*>(bsrc + (int64_t)(8 * kb + 2 * (bo + (256 / TC) * u) + h) * ld * 4);
+  };
+  auto store_b = [&](int s) {
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+      u4 vh, vm, vl;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const Split2 sp = split2(rbuf[u][e >> 1][2 * (e & 1)], rbuf[u][e >> 1][2 * (e & 1) + 1]);
+        vh[e] = sp.h;
+        vm[e] = sp.m;
+        vl[e] = sp.l;
+      }
+      const int k = 8 * (bo + (256 / TC) * u);
+      *reinterpret_cast<u4*>(&Bs[s][0][bc][k]) = vh;
+      *reinterpret_cast<u4*>(&Bs[s][1][bc][k]) = vm;
+      *reinterpret_cast<u4*>(&Bs[s][2][bc][k]) = vl;
+    }
+  };
+
+#pragma unroll
+  for (int a = 0; a < NRB; ++a) {
+#pragma unroll
+    for (int b = 0; b < NCB; ++b) t.hh[a][b] = t.sm[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int b = 0; b < (NBC > 0 ? NBC : 1); ++b) t.bound[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+  }
+
+  const u4 absmask = u4{0x7FFF7FFFu, 0x7FFF7FFFu, 0x7FFF7FFFu, 0x7FFF7FFFu};
+  // Column blocks in pairs: 2 x NRB independent accumulators per product, so
+  // dependent MFMAs are four issues apart.
+  auto compute = [&](int s) {
+#pragma unroll
+    for (int cp = 0; cp < NCB / 2; ++cp) {
+      bf16x8 bf[2][NPL];
+#pragma unroll
+      for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int pl = 0; pl < NPL; ++pl)
+          bf[c][pl] = as_bf(*reinterpret_cast<const u4*>(&Bs[s][pl][16 * (2 * cp + c) + r][8 * q]));
+      constexpr int pa[5] = {2, 1, 0, 1, 0};
+      constexpr int pb[5] = {0, 1, 2, 0, 1};
+#pragma unroll
+      for (int pr = 0; pr < 5; ++pr)
+#pragma unroll
+        for (int rb = 0; rb < NRB; ++rb)
+#pragma unroll
+          for (int c = 0; c < 2; ++c)
+            t.sm[rb][2 * cp + c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(areg[pa[pr]][rb]), bf[c][pb[pr]],
+                                                                            t.sm[rb][2 * cp + c], 0, 0, 0);
+#pragma unroll
+      for (int rb = 0; rb < NRB; ++rb)
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+          t.hh[rb][2 * cp + c] =
+              __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(areg[0][rb]), bf[c][0], t.hh[rb][2 * cp + c], 0, 0, 0);
+      if constexpr (NBC > 0) {
+        if (2 * cp < NBC) {
+#pragma unroll
+          for (int rb = 0; rb < NRB; ++rb)
+#pragma unroll
+            for (int c = 0; c < 2; ++c)
+              t.bound[rb][(2 * cp + c) % NBC] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                  as_bf(areg[0][rb] & absmask), as_bf(__builtin_bit_cast(u4, bf[c][0]) & absmask),
+                  t.bound[rb][(2 * cp + c) % NBC], 0, 0, 0);
+        }
+      }
+    }
+  };
+
+  load_b(0);
+  load_a(0);
+  store_b(0);
+  load_b(1);
+  __syncthreads();
+  for (int kb = 0; kb < nk; ++kb) {
+    const int s = kb & 1;
+    compute(s);
+    if (kb + 1 < nk) {
+      load_a(kb + 1);
+      store_b(s ^ 1);
+    }
+    if (kb + 2 < nk) load_b(kb + 2);
+    __syncthreads();
+  }
+}
+
+}  // namespace mixed
+}  // namespace ct

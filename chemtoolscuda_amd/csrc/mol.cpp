@@ -364,13 +364,16 @@ void Mol::backward(const double* points, int64_t n, const double* w, int precisi
   CT_HIP_CHECK(hipStreamSynchronize(stream_));
   const int mode = MODE_GRAD;
   const int nder = mode_nder(mode);
-  const int64_t per_point = (int64_t)nder * Mp_ * 8 + 24 + 8;
+  const bool mixed = precision == CT_PREC_MIXED;
+  const int64_t per_point = (int64_t)nder * Mp_ * (mixed ? 4 : 8) + 24 + 8;
   int64_t Nc = n > 0 ? chunk_points(per_point, n) : kPointAlign;
   while (!d_ws_.reserve((size_t)Nc * per_point)) {
     if (Nc <= kPointAlign) throw Error(CT_E_NOMEM, "cannot allocate the adjoint workspace");
     Nc = std::max<int64_t>(kPointAlign, Nc / 2 / kPointAlign * kPointAlign);
   }
-  const int nslice_max = gram_slices(Nc);
+  int nslice_max = gram_slices(Nc);
+  int64_t slice_len = 0;
+  if (mixed) gram_mixed_plan(Nc, Mp_, &nslice_max, &slice_len);
   const size_t gsize = (size_t)Mp_ * Mp_;
   const size_t adj_bytes = sizeof(double) * (gsize + (size_t)Mp_ * 3 + (size_t)nslice_max * gsize);
   if (!d_adj_.reserve(adj_bytes)) throw Error(CT_E_NOMEM, "cannot allocate the adjoint accumulators");
@@ -378,8 +381,22 @@ void Mol::backward(const double* points, int64_t n, const double* w, int precisi
   double* T = G + gsize;
   double* part = T + (size_t)Mp_ * 3;
   CT_HIP_CHECK(hipMemsetAsync(G, 0, sizeof(double) * (gsize + (size_t)Mp_ * 3), stream_));
-  double* ao = static_cast<double*>(d_ws_.get());
-  double* pts_stage = ao + (size_t)nder * Mp_ * Nc;
+  const int32_t* tiles = nullptr;
+  if (mixed) {
+    // upper-triangle 128 x 128 tiles of the Gram matrix, (I, J) with I <= J
+    const int nb = Mp_ / kRowBlock;
+    std::vector<int32_t> tl;
+    for (int I = 0; I < nb; ++I)
+      for (int J = I; J < nb; ++J) {
+        tl.push_back(I);
+        tl.push_back(J);
+      }
+    if (!d_tiles_.reserve(sizeof(int32_t) * tl.size())) throw Error(CT_E_NOMEM, "cannot allocate the tile list");
+    CT_HIP_CHECK(hipMemcpy(d_tiles_.get(), tl.data(), sizeof(int32_t) * tl.size(), hipMemcpyHostToDevice));
+    tiles = static_cast<const int32_t*>(d_tiles_.get());
+  }
+  char* ws = static_cast<char*>(d_ws_.get());
+  double* pts_stage = reinterpret_cast<double*>(ws + (size_t)nder * Mp_ * Nc * (mixed ? 4 : 8));
   double* w_stage = pts_stage + 3 * (size_t)Nc;
   const bool pd = is_device_pointer(points), wd = is_device_pointer(w);
   const auto* meta = static_cast<const ShellMeta*>(d_meta_.get());
@@ -399,10 +416,22 @@ void Mol::backward(const double* points, int64_t n, const double* w, int precisi
       wc = w_stage;
     }
     timer_.begin(KernelTimer::BACKWARD, stream_);
-    launch_ao_f64(mode, meta, geom, prim, nshell_, M_, Mp_, pts, nc, ld, ao, 700.0, stream_);
-    launch_gram_f64(ao, Mp_, ld, wc, nc, part, stream_);
-    launch_gram_reduce(part, gram_slices(nc), (int64_t)gsize, G, stream_);
-    launch_contract_adjoint_f64(static_cast<const double*>(d_P_.get()), Mp_, ao, ld, wc, nc, T, stream_);
+    if (!mixed) {
+      double* ao = reinterpret_cast<double*>(ws);
+      launch_ao_f64(mode, meta, geom, prim, nshell_, M_, Mp_, pts, nc, ld, ao, 700.0, stream_);
+      launch_gram_f64(ao, Mp_, ld, wc, nc, part, stream_);
+      launch_gram_reduce(part, gram_slices(nc), (int64_t)gsize, G, stream_);
+      launch_contract_adjoint_f64(static_cast<const double*>(d_P_.get()), Mp_, ao, ld, wc, nc, T, stream_);
+    } else {
+      float* ao = reinterpret_cast<float*>(ws);
+      launch_ao_f32(mode, meta, geom, prim, nshell_, M_, Mp_, pts, nc, ld, ao, 50.0, stream_);
+      int nslice = 0;
+      int64_t len = 0;
+      gram_mixed_plan(nc, Mp_, &nslice, &len);
+      launch_gram_mixed(ao, Mp_, ld, wc, nc, tiles, nslice, len, part, stream_);
+      launch_gram_reduce(part, nslice, (int64_t)gsize, G, stream_);
+      launch_contract_adjoint_mixed(static_cast<const uint16_t*>(d_Pb_.get()), Mp_, ao, ld, wc, nc, T, stream_);
+    }
     timer_.end(stream_);
   }
   std::vector<double> Gh(gsize), Th((size_t)Mp_ * 3);

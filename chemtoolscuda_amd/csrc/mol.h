@@ -138,6 +138,7 @@ class Mol {
   bool esp_valid_ = false;
   int esp_nseg_ = 0;
   int64_t esp_npairs_ = 0;
+  DeviceBuffer d_tiles_;    // mixed adjoint: upper-triangle Gram tile list
   DeviceBuffer d_esp_rec_, d_esp_rec32_, d_esp_seg_, d_esp_atoms_, d_boys_, d_esp_io_;
   DeviceBuffer d_adj_;     // adjoint: G, T, slice partials
   void prepare_esp();

@@ -75,6 +75,37 @@ def test_backward_chunked_and_host_device_inputs(ct):
     assert _rel(dP.cpu().numpy(), a[0]) < 1e-13 and _rel(dR.cpu().numpy(), a[1]) < 1e-12
 
 
+@pytest.mark.parametrize("workload,npts", [("water", 3000), ("diff40", 3000)])
+def test_mixed_backward_against_fp64(ct, workload, npts):
+    """Mixed adjoint (FP32 AOs, split-bf16 Gram matrix and adjoint contraction,
+    FP64 partial sums) against the FP64 adjoint, per element relative to the
+    condition scale of each sum: sum_g |w_g| |phi_i| |phi_j| for dL/dP and
+    sum_g |w_g| 2 sum_{i on A} |d_k phi_i| (|P| |phi|)_i for dL/dR (floored at
+    1e-14 of the largest), at 1e-6."""
+    from chemtoolscuda_amd import synthetic
+
+    w = synthetic.WORKLOADS[workload](npts)
+    mol = ct.molecule_from_basis(w.basis, w.P, w.charges)
+    wt = np.random.default_rng(3).uniform(-1.0, 1.0, size=w.points.shape[0])
+    dP64, dR64 = mol.backward(w.points, wt)
+    dPm, dRm = mol.backward(w.points, wt, precision="mixed")
+    ao = oracle.ao(w.basis.flat(), w.points, 1)
+    a0 = np.abs(ao[0])
+    sP = (a0 * np.abs(wt)) @ a0.T
+    # pairs of far-apart functions have a zero or negligible scale (their
+    # products underflow): floor the scale at 1e-14 of the largest
+    eP = np.max(np.abs(dPm - dP64) / np.maximum(sP, 1e-14 * sP.max()))
+    y = np.abs(w.P) @ a0
+    atom_of = np.repeat([s.atom for s in w.basis.shells], [s.nfunc for s in w.basis.shells])
+    sR = np.zeros_like(dR64)
+    for k in range(3):
+        per_fn = 2.0 * np.abs(ao[1 + k]) * y @ np.abs(wt)
+        np.add.at(sR[:, k], atom_of, per_fn)
+    eR = np.max(np.abs(dRm - dR64) / np.maximum(sR, 1e-14 * sR.max()))
+    print(f"{workload}: mixed adjoint max error / condition scale: dP {eP:.2e}, dR {eR:.2e}")
+    assert eP <= 1e-6 and eR <= 1e-6, (eP, eR)
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_autograd_against_finite_differences(ct, precision):
     import torch
