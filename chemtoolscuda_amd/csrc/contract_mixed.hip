@@ -94,9 +94,14 @@ __global__ __launch_bounds__(256, 2) void contract_mixed_kernel(const u4* __rest
   const int i0 = rt * TR;
   const int64_t mq = mp / 4;  // row quads per AO array
 
+  // Lap mode: the derivative columns enter only tau_kk = d_k phi^T P d_k phi,
+  // a quadratic form, so they use the symmetric-P shortcut of y_tile (0.6 of
+  // their K range for five row blocks).  The Hessian mode's t_kl (k != l) are
+  // not quadratic forms and use the full K.
+  constexpr bool SYM = MODE == MODE_LAP;
   YTile<TC, NBC> yt;
   const float* bsrc = ao + ((int64_t)C::type(tid & (TC - 1)) * mq * ld + pbase + C::point(tid & (TC - 1))) * 4;
-  y_tile<TC, NBC>(Pf, mp, ld, i0, tid, bsrc, Bs, yt);
+  y_tile<TC, NBC, SYM>(Pf, mp, ld, i0, tid, bsrc, Bs, yt, C::type(tid & (TC - 1)) != 0, i0 / KB, (i0 + TR) / KB);
 
   // ---------------------------------------------------------------- epilogue
   double* red = reinterpret_cast<double*>(&Bs[0][0][0][0]);  // [NW][TC][kMaxDots]

@@ -80,10 +80,16 @@ struct YTile {
 // Computes the tile.  bsrc: this thread's B column at row quad 0, i.e. the
 // column tid % TC; its row quad k lies at bsrc + k * ld * 4.  Bs: the
 // workgroup's LDS.  Ends with a barrier (Bs free for reuse).
-template <int TC, int NBC>
+//
+// SYM (symmetric P, columns 32..TC-1 used only in quadratic forms
+// x^T P x): by sum_IJ x_I^T P_IJ x_J = sum_I x_I^T (P_II x_I + 2 sum_{J>I}
+// P_IJ x_J), those columns need only the k-steps from k_lo (the row block's
+// diagonal block) on, with their B values doubled from k_dbl (right of the
+// diagonal block; exact in bf16).  bsym: this thread's column is one of them.
+template <int TC, int NBC, bool SYM = false>
 __device__ __forceinline__ void y_tile(const u4* __restrict__ Pf, int mp, int64_t ld, int i0, int tid,
                                        const float* __restrict__ bsrc, uint16_t (&Bs)[2][NPL][TC][LROW],
-                                       YTile<TC, NBC>& t) {
+                                       YTile<TC, NBC>& t, bool bsym = false, int k_lo = 0, int k_dbl = 0) {
   constexpr int NCB = TC / 16;
   const int lane = tid & 63;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -108,6 +114,7 @@ __device__ __forceinline__ void y_tile(const u4* __restrict__ Pf, int mp, int64_
   const int bc = tid & (TC - 1), bo = tid / TC;
   f4 rbuf[NU][2];
   auto load_b = [&](int kb) {
+    if (SYM && bsym && kb < k_lo) return;
 #pragma unroll
     for (int u = 0; u < NU; ++u)
 #pragma unroll
@@ -115,13 +122,15 @@ __device__ __forceinline__ void y_tile(const u4* __restrict__ Pf, int mp, int64_
         rbuf[u][h] =
             *reinterpret_cast<const f4*>(bsrc + (int64_t)(8 * kb + 2 * (bo + (256 / TC) * u) + h) * ld * 4);
   };
-  auto store_b = [&](int s) {
+  auto store_b = [&](int s, int kb) {
+    if (SYM && bsym && kb < k_lo) return;
+    const float sc = (SYM && bsym && kb >= k_dbl) ? 2.0f : 1.0f;
 #pragma unroll
     for (int u = 0; u < NU; ++u) {
       u4 vh, vm, vl;
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const Split2 sp = split2(rbuf[u][e >> 1][2 * (e & 1)], rbuf[u][e >> 1][2 * (e & 1) + 1]);
+        const Split2 sp = split2(sc * rbuf[u][e >> 1][2 * (e & 1)], sc * rbuf[u][e >> 1][2 * (e & 1) + 1]);
         vh[e] = sp.h;
         vm[e] = sp.m;
         vl[e] = sp.l;
@@ -144,9 +153,11 @@ __device__ __forceinline__ void y_tile(const u4* __restrict__ Pf, int mp, int64_
   const u4 absmask = u4{0x7FFF7FFFu, 0x7FFF7FFFu, 0x7FFF7FFFu, 0x7FFF7FFFu};
   // Column blocks in pairs: 2 x NRB independent accumulators per product, so
   // dependent MFMAs are four issues apart.
-  auto compute = [&](int s) {
+  auto compute = [&](int s, int kb) {
+    const bool phi_only = SYM && kb < k_lo;  // wave-uniform
 #pragma unroll
     for (int cp = 0; cp < NCB / 2; ++cp) {
+      if (cp > 0 && phi_only) break;
       bf16x8 bf[2][NPL];
 #pragma unroll
       for (int c = 0; c < 2; ++c)
@@ -185,15 +196,15 @@ __device__ __forceinline__ void y_tile(const u4* __restrict__ Pf, int mp, int64_
 
   load_b(0);
   load_a(0);
-  store_b(0);
+  store_b(0, 0);
   load_b(1);
   __syncthreads();
   for (int kb = 0; kb < nk; ++kb) {
     const int s = kb & 1;
-    compute(s);
+    compute(s, kb);
     if (kb + 1 < nk) {
       load_a(kb + 1);
-      store_b(s ^ 1);
+      store_b(s ^ 1, kb + 1);
     }
     if (kb + 2 < nk) load_b(kb + 2);
     __syncthreads();
