@@ -192,14 +192,16 @@ def main():
             flops[f"{l}:{dset}"] = {"ops": ops, "nfunc": nc}
     # dispatch helper: shell type code (index into SHELL_TYPES) -> generated function
     for dset in DERIV_SETS:
-        parts.append("// st(slot, component, value) stores one AO value of the shell")
-        parts.append("template <typename T, typename St>")
+        parts.append("// st(slot, component, value) stores one AO value of the shell; shell types")
+        parts.append("// with |l| > LMAX are compiled out (the kernel is instantiated per basis max L,")
+        parts.append("// so a basis without f or g shells does not pay their register pressure)")
+        parts.append("template <typename T, int LMAX, typename St>")
         parts.append(f"__device__ __forceinline__ void ao_shell_dispatch_{dset}(int code, T x, T y, T z, T s0, T s1,")
         parts.append("    T s2, St&& st) {")
         parts.append("  switch (code) {")
         for code, l in enumerate(SHELL_TYPES):
             fn = f"ao_shell_{'c' if l >= 0 else 'p'}{abs(l)}_{dset}"
-            parts.append(f"    case {code}: {fn}<T>(x, y, z, s0, s1, s2, st); break;")
+            parts.append(f"    case {code}: if constexpr ({abs(l)} <= LMAX) {fn}<T>(x, y, z, s0, s1, s2, st); break;")
         parts.append("    default: break;")
         parts.append("  }")
         parts.append("}")

@@ -41,7 +41,8 @@ namespace {
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 // TS: storage type (double: FP64 path, row-major; float: mixed path, row quads).
-template <typename TS, int MODE>
+// LMAX: largest |l| of the basis (the code of larger shells is compiled out).
+template <typename TS, int MODE, int LMAX>
 __global__ __launch_bounds__(256) void ao_eval_kernel(const ShellMeta* __restrict__ meta,
                                                       const ShellGeom* __restrict__ geom,
                                                       const PrimData* __restrict__ prim, int nshell, int nbasis,
@@ -67,10 +68,10 @@ __global__ __launch_bounds__(256) void ao_eval_kernel(const ShellMeta* __restric
       *reinterpret_cast<f4*>(ao + ((int64_t)d * mq + quad) * ld * 4 + p * 4) = f4{qb[d][0], qb[d][1], qb[d][2], qb[d][3]};
   };
   auto dispatch = [&](int code, T x, T y, T z, T s0, T s1, T s2, auto&& st) {
-    if constexpr (MODE == MODE_RHO) ao_shell_dispatch_val<T>(code, x, y, z, s0, s1, s2, st);
-    if constexpr (MODE == MODE_GRAD) ao_shell_dispatch_grad<T>(code, x, y, z, s0, s1, s2, st);
-    if constexpr (MODE == MODE_LAP) ao_shell_dispatch_lap<T>(code, x, y, z, s0, s1, s2, st);
-    if constexpr (MODE == MODE_HESS) ao_shell_dispatch_hess<T>(code, x, y, z, s0, s1, s2, st);
+    if constexpr (MODE == MODE_RHO) ao_shell_dispatch_val<T, LMAX>(code, x, y, z, s0, s1, s2, st);
+    if constexpr (MODE == MODE_GRAD) ao_shell_dispatch_grad<T, LMAX>(code, x, y, z, s0, s1, s2, st);
+    if constexpr (MODE == MODE_LAP) ao_shell_dispatch_lap<T, LMAX>(code, x, y, z, s0, s1, s2, st);
+    if constexpr (MODE == MODE_HESS) ao_shell_dispatch_hess<T, LMAX>(code, x, y, z, s0, s1, s2, st);
   };
 
   for (int s = 0; s < nshell; ++s) {
@@ -144,56 +145,54 @@ __global__ __launch_bounds__(256) void ao_eval_kernel(const ShellMeta* __restric
   }
 }
 
-}  // namespace
-
-void launch_ao_f64(int mode, const ShellMeta* meta, const ShellGeom* geom, const PrimData* prim, int nshell,
-                   int nbasis, int nbasis_pad, const double* pts, int64_t npts, int64_t ld, double* ao,
-                   double screen, hipStream_t s) {
+template <typename TS, int MODE>
+void launch_lmax(int lmax, const ShellMeta* meta, const ShellGeom* geom, const PrimData* prim, int nshell, int nbasis,
+                 int nbasis_pad, const double* pts, int64_t npts, int64_t ld, TS* ao, double screen, hipStream_t s) {
   const dim3 block(256), grid((unsigned)(ld / 256));
-  switch (mode) {
-    case MODE_RHO:
-      hipLaunchKernelGGL((ao_eval_kernel<double, MODE_RHO>), grid, block, 0, s, meta, geom, prim, nshell, nbasis,
-                         nbasis_pad, pts, npts, ld, ao, screen);
-      break;
-    case MODE_GRAD:
-      hipLaunchKernelGGL((ao_eval_kernel<double, MODE_GRAD>), grid, block, 0, s, meta, geom, prim, nshell, nbasis,
-                         nbasis_pad, pts, npts, ld, ao, screen);
-      break;
-    case MODE_LAP:
-      hipLaunchKernelGGL((ao_eval_kernel<double, MODE_LAP>), grid, block, 0, s, meta, geom, prim, nshell, nbasis,
-                         nbasis_pad, pts, npts, ld, ao, screen);
-      break;
-    default:
-      hipLaunchKernelGGL((ao_eval_kernel<double, MODE_HESS>), grid, block, 0, s, meta, geom, prim, nshell, nbasis,
-                         nbasis_pad, pts, npts, ld, ao, screen);
-      break;
-  }
+  if (lmax <= 2)
+    hipLaunchKernelGGL((ao_eval_kernel<TS, MODE, 2>), grid, block, 0, s, meta, geom, prim, nshell, nbasis, nbasis_pad,
+                       pts, npts, ld, ao, screen);
+  else if (lmax == 3)
+    hipLaunchKernelGGL((ao_eval_kernel<TS, MODE, 3>), grid, block, 0, s, meta, geom, prim, nshell, nbasis, nbasis_pad,
+                       pts, npts, ld, ao, screen);
+  else
+    hipLaunchKernelGGL((ao_eval_kernel<TS, MODE, 4>), grid, block, 0, s, meta, geom, prim, nshell, nbasis, nbasis_pad,
+                       pts, npts, ld, ao, screen);
   CT_HIP_CHECK(hipGetLastError());
 }
 
-void launch_ao_f32(int mode, const ShellMeta* meta, const ShellGeom* geom, const PrimData* prim, int nshell,
-                   int nbasis, int nbasis_pad, const double* pts, int64_t npts, int64_t ld, float* ao, double screen,
-                   hipStream_t s) {
-  const dim3 block(256), grid((unsigned)(ld / 256));
+template <typename TS>
+void launch_mode(int mode, int lmax, const ShellMeta* meta, const ShellGeom* geom, const PrimData* prim, int nshell,
+                 int nbasis, int nbasis_pad, const double* pts, int64_t npts, int64_t ld, TS* ao, double screen,
+                 hipStream_t s) {
   switch (mode) {
     case MODE_RHO:
-      hipLaunchKernelGGL((ao_eval_kernel<float, MODE_RHO>), grid, block, 0, s, meta, geom, prim, nshell, nbasis,
-                         nbasis_pad, pts, npts, ld, ao, screen);
+      launch_lmax<TS, MODE_RHO>(lmax, meta, geom, prim, nshell, nbasis, nbasis_pad, pts, npts, ld, ao, screen, s);
       break;
     case MODE_GRAD:
-      hipLaunchKernelGGL((ao_eval_kernel<float, MODE_GRAD>), grid, block, 0, s, meta, geom, prim, nshell, nbasis,
-                         nbasis_pad, pts, npts, ld, ao, screen);
+      launch_lmax<TS, MODE_GRAD>(lmax, meta, geom, prim, nshell, nbasis, nbasis_pad, pts, npts, ld, ao, screen, s);
       break;
     case MODE_LAP:
-      hipLaunchKernelGGL((ao_eval_kernel<float, MODE_LAP>), grid, block, 0, s, meta, geom, prim, nshell, nbasis,
-                         nbasis_pad, pts, npts, ld, ao, screen);
+      launch_lmax<TS, MODE_LAP>(lmax, meta, geom, prim, nshell, nbasis, nbasis_pad, pts, npts, ld, ao, screen, s);
       break;
     default:
-      hipLaunchKernelGGL((ao_eval_kernel<float, MODE_HESS>), grid, block, 0, s, meta, geom, prim, nshell, nbasis,
-                         nbasis_pad, pts, npts, ld, ao, screen);
+      launch_lmax<TS, MODE_HESS>(lmax, meta, geom, prim, nshell, nbasis, nbasis_pad, pts, npts, ld, ao, screen, s);
       break;
   }
-  CT_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace
+
+void launch_ao_f64(int mode, int lmax, const ShellMeta* meta, const ShellGeom* geom, const PrimData* prim, int nshell,
+                   int nbasis, int nbasis_pad, const double* pts, int64_t npts, int64_t ld, double* ao,
+                   double screen, hipStream_t s) {
+  launch_mode<double>(mode, lmax, meta, geom, prim, nshell, nbasis, nbasis_pad, pts, npts, ld, ao, screen, s);
+}
+
+void launch_ao_f32(int mode, int lmax, const ShellMeta* meta, const ShellGeom* geom, const PrimData* prim, int nshell,
+                   int nbasis, int nbasis_pad, const double* pts, int64_t npts, int64_t ld, float* ao, double screen,
+                   hipStream_t s) {
+  launch_mode<float>(mode, lmax, meta, geom, prim, nshell, nbasis, nbasis_pad, pts, npts, ld, ao, screen, s);
 }
 
 }  // namespace ct

@@ -106,7 +106,7 @@ struct FinalizeArgs {
   ct_outputs out;   // device pointers (full-length arrays, indexed with out_off)
 };
 
-void launch_ao_f64(int mode, const ShellMeta* meta, const ShellGeom* geom, const PrimData* prim, int nshell,
+void launch_ao_f64(int mode, int lmax, const ShellMeta* meta, const ShellGeom* geom, const PrimData* prim, int nshell,
                    int nbasis, int nbasis_pad, const double* pts, int64_t npts, int64_t ld, double* ao,
                    double screen, hipStream_t s);
 void launch_contract_f64(int mode, const double* P, int nbasis_pad, const double* ao, int64_t ld, double* dots,
@@ -114,7 +114,7 @@ void launch_contract_f64(int mode, const double* P, int nbasis_pad, const double
 void launch_finalize(const FinalizeArgs& a, hipStream_t s);
 
 // Mixed path.
-void launch_ao_f32(int mode, const ShellMeta* meta, const ShellGeom* geom, const PrimData* prim, int nshell,
+void launch_ao_f32(int mode, int lmax, const ShellMeta* meta, const ShellGeom* geom, const PrimData* prim, int nshell,
                    int nbasis, int nbasis_pad, const double* pts, int64_t npts, int64_t ld, float* ao, double screen,
                    hipStream_t s);
 int mixed_points_per_block(int mode);

@@ -213,6 +213,7 @@ Mol::Mol(const HostBasis& basis, const double* atom_xyz, const double* charges, 
                    basis.exps.size() == basis.coefs.size(),
                "primitive offsets out of range");
     meta_[s] = ShellMeta{shell_code(l), basis.shell_nprim[s], basis.shell_prim_off[s], M_};
+    lmax_ = std::max(lmax_, std::abs(l));
     M_ += shell_nfunc(l);
   }
   basis_.nbasis = M_;
@@ -418,13 +419,13 @@ void Mol::backward(const double* points, int64_t n, const double* w, int precisi
     timer_.begin(KernelTimer::BACKWARD, stream_);
     if (!mixed) {
       double* ao = reinterpret_cast<double*>(ws);
-      launch_ao_f64(mode, meta, geom, prim, nshell_, M_, Mp_, pts, nc, ld, ao, 700.0, stream_);
+      launch_ao_f64(mode, lmax_, meta, geom, prim, nshell_, M_, Mp_, pts, nc, ld, ao, 700.0, stream_);
       launch_gram_f64(ao, Mp_, ld, wc, nc, part, stream_);
       launch_gram_reduce(part, gram_slices(nc), (int64_t)gsize, G, stream_);
       launch_contract_adjoint_f64(static_cast<const double*>(d_P_.get()), Mp_, ao, ld, wc, nc, T, stream_);
     } else {
       float* ao = reinterpret_cast<float*>(ws);
-      launch_ao_f32(mode, meta, geom, prim, nshell_, M_, Mp_, pts, nc, ld, ao, 50.0, stream_);
+      launch_ao_f32(mode, lmax_, meta, geom, prim, nshell_, M_, Mp_, pts, nc, ld, ao, 50.0, stream_);
       int nslice = 0;
       int64_t len = 0;
       gram_mixed_plan(nc, Mp_, &nslice, &len);
@@ -513,7 +514,7 @@ void Mol::reevaluate(int mode, const double* pts, int64_t count, const int32_t* 
   double* dots = ao + (size_t)nder * Mp_ * ld;
   double* gp = dots + (size_t)ndot * ld;
   launch_gather_points(pts, index, count, gp, stream_);
-  launch_ao_f64(mode, static_cast<const ShellMeta*>(d_meta_.get()), static_cast<const ShellGeom*>(d_geom_.get()),
+  launch_ao_f64(mode, lmax_, static_cast<const ShellMeta*>(d_meta_.get()), static_cast<const ShellGeom*>(d_geom_.get()),
                 static_cast<const PrimData*>(d_prim_.get()), nshell_, M_, Mp_, gp, count, ld, ao, 700.0, stream_);
   launch_contract_f64(mode, static_cast<const double*>(d_P_.get()), Mp_, ao, ld, dots, stream_);
   FinalizeArgs f = fin;
@@ -626,7 +627,7 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
 
     if (!mixed) {
       timer_.begin(KernelTimer::AO, stream_);
-      launch_ao_f64(mode, meta, geom, prim, nshell_, M_, Mp_, pts, nc, ld, static_cast<double*>(ao), 700.0, stream_);
+      launch_ao_f64(mode, lmax_, meta, geom, prim, nshell_, M_, Mp_, pts, nc, ld, static_cast<double*>(ao), 700.0, stream_);
       timer_.end(stream_);
       timer_.begin(KernelTimer::CONTRACT, stream_);
       launch_contract_f64(mode, static_cast<const double*>(d_P_.get()), Mp_, static_cast<const double*>(ao), ld,
@@ -637,7 +638,7 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
       timer_.end(stream_);
     } else {
       timer_.begin(KernelTimer::AO, stream_);
-      launch_ao_f32(mode, meta, geom, prim, nshell_, M_, Mp_, pts, nc, ld, static_cast<float*>(ao), 50.0, stream_);
+      launch_ao_f32(mode, lmax_, meta, geom, prim, nshell_, M_, Mp_, pts, nc, ld, static_cast<float*>(ao), 50.0, stream_);
       timer_.end(stream_);
       timer_.begin(KernelTimer::CONTRACT, stream_);
       launch_contract_mixed(mode, static_cast<const uint16_t*>(d_Pb_.get()), Mp_, static_cast<const float*>(ao), ld,

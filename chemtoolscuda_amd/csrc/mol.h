@@ -119,6 +119,7 @@ class Mol {
 
   HostBasis basis_;
   int M_ = 0, Mp_ = 0, natom_ = 0, nshell_ = 0;
+  int lmax_ = 0;  // largest |l| of the basis: selects the AO kernel instantiation
   std::vector<double> xyz_, charges_, P_;
   std::vector<int32_t> numbers_;
   std::vector<ShellMeta> meta_;
