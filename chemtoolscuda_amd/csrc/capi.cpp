@@ -175,6 +175,26 @@ int ct_eval(ct_mol* mol, const double* points, int64_t n, uint32_t quantities, i
   });
 }
 
+int ct_eval_cubic(ct_mol* mol, const double* origin, const double* axes, const int64_t* shape, uint32_t quantities,
+                  int32_t precision, const ct_eval_params* params, const ct_outputs* out, ct_eval_report* report) {
+  return guard([&] {
+    CT_REQUIRE(out != nullptr && origin != nullptr && axes != nullptr && shape != nullptr, "null argument");
+    ct::CubicGrid g;
+    for (int k = 0; k < 3; ++k) {
+      CT_REQUIRE(shape[k] > 0, "grid shape must be positive");
+      g.origin[k] = origin[k];
+      g.shape[k] = shape[k];
+    }
+    for (int k = 0; k < 9; ++k) g.axes[k] = axes[k];
+    CT_REQUIRE(shape[0] <= (int64_t(1) << 62) / shape[1] / shape[2], "grid too large");
+    ct_eval_params p{};
+    if (params) p = *params;
+    if (p.mixed_tol <= 0) p.mixed_tol = 1e-6;
+    if (p.rho_floor <= 0) p.rho_floor = 1e-10;
+    M(mol).eval(nullptr, shape[0] * shape[1] * shape[2], quantities, precision, p, *out, report, &g);
+  });
+}
+
 int ct_norm_rows(const double* v, int64_t n, double* out) {
   return guard([&] {
     CT_REQUIRE(n >= 0, "negative size");

@@ -134,6 +134,16 @@ struct FlagArgs {
 };
 void launch_finalize_mixed(const FinalizeArgs& a, const FlagArgs& f, hipStream_t s);
 void launch_gather_points(const double* pts, const int32_t* index, int64_t n, double* out, hipStream_t s);
+
+// Uniform (possibly skewed) grid: point (i, j, k) = origin + i a0 + j a1 + k a2,
+// a_r = axes[3r .. 3r+2]; linear index (i * n1 + j) * n2 + k (k fastest).
+struct CubicGrid {
+  double origin[3];
+  double axes[9];
+  int64_t shape[3];
+};
+// Writes the points of linear indices start .. start+n-1 as (n, 3) row-major.
+void launch_cubic_points(const CubicGrid& g, int64_t start, int64_t n, double* out, hipStream_t s);
 void launch_norm_rows(const double* v, int64_t n, double* out, hipStream_t s);
 
 // Adjoint.

@@ -119,6 +119,18 @@ __global__ __launch_bounds__(256) void gather_points_kernel(const double* __rest
   out[3 * k + 2] = pts[3 * p + 2];
 }
 
+__global__ __launch_bounds__(256) void cubic_points_kernel(CubicGrid g, int64_t start, int64_t n,
+                                                           double* __restrict__ out) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int64_t lin = start + k;
+  const double iz = (double)(lin % g.shape[2]);
+  const double iy = (double)((lin / g.shape[2]) % g.shape[1]);
+  const double ix = (double)(lin / (g.shape[2] * g.shape[1]));
+#pragma unroll
+  for (int c = 0; c < 3; ++c) out[3 * k + c] = g.origin[c] + ix * g.axes[c] + iy * g.axes[3 + c] + iz * g.axes[6 + c];
+}
+
 __global__ __launch_bounds__(256) void norm_rows_kernel(const double* __restrict__ v, int64_t n,
                                                         double* __restrict__ out) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -144,6 +156,12 @@ void launch_finalize_mixed(const FinalizeArgs& a, const FlagArgs& f, hipStream_t
 void launch_gather_points(const double* pts, const int32_t* index, int64_t n, double* out, hipStream_t s) {
   if (n <= 0) return;
   hipLaunchKernelGGL(gather_points_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, pts, index, n, out);
+  CT_HIP_CHECK(hipGetLastError());
+}
+
+void launch_cubic_points(const CubicGrid& g, int64_t start, int64_t n, double* out, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(cubic_points_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g, start, n, out);
   CT_HIP_CHECK(hipGetLastError());
 }
 

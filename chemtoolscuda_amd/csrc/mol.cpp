@@ -527,7 +527,7 @@ void Mol::reevaluate(int mode, const double* pts, int64_t count, const int32_t* 
 }
 
 void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const ct_eval_params& par,
-               const ct_outputs& out, ct_eval_report* rep) {
+               const ct_outputs& out, ct_eval_report* rep, const CubicGrid* cube) {
   CT_REQUIRE(n >= 0, "negative number of points");
   CT_REQUIRE(q != 0, "no quantity requested");
   CT_REQUIRE(precision == CT_PREC_FP64 || precision == CT_PREC_MIXED, "unknown precision");
@@ -535,13 +535,13 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
     if ((q & s.bit) && out.*(s.field) == nullptr) throw Error(CT_E_INVALID, "output array missing for a requested quantity");
   if (rep) *rep = ct_eval_report{n, 0, 0, 0.0};
   if (n == 0) return;
-  CT_REQUIRE(points != nullptr, "null points");
+  CT_REQUIRE(points != nullptr || cube != nullptr, "null points");
 
   const bool mixed = precision == CT_PREC_MIXED;
   const int mode = select_mode(q);
   const int nder = mode_nder(mode), ndot = mode_ndot(mode);
   const int nrt = Mp_ / kRowBlock;
-  const bool pts_dev = is_device_pointer(points);
+  const bool pts_dev = cube == nullptr && is_device_pointer(points);
   constexpr size_t NS = sizeof(kSlots) / sizeof(kSlots[0]);
   bool out_dev[NS];
   int stage_width = 0;
@@ -580,7 +580,7 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
   // With device-resident points and outputs, flagged points of all chunks are
   // collected in one list and re-evaluated in a single FP64 pass at the end
   // (one large launch instead of one small launch per chunk).
-  bool all_dev = pts_dev && n < (int64_t(1) << 31);
+  bool all_dev = pts_dev && n < (int64_t(1) << 31);  // (grid points are not materialised: per-chunk re-evaluation)
   for (size_t k = 0; k < NS; ++k) all_dev = all_dev && out_dev[k];
   const bool batch = mixed && all_dev;
   int32_t* list = flag_idx;
@@ -597,8 +597,11 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
   for (int64_t start = 0; start < n; start += Nc) {
     const int64_t nc = std::min(Nc, n - start);
     const int64_t ld = round_up(nc, kPointAlign);
-    const double* pts = points + 3 * start;
-    if (!pts_dev) {
+    const double* pts = cube ? nullptr : points + 3 * start;
+    if (cube) {
+      launch_cubic_points(*cube, start, nc, pts_stage, stream_);
+      pts = pts_stage;
+    } else if (!pts_dev) {
       CT_HIP_CHECK(hipMemcpyAsync(pts_stage, pts, sizeof(double) * 3 * nc, hipMemcpyHostToDevice, stream_));
       pts = pts_stage;
     }

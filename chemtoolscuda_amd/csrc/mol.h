@@ -105,8 +105,10 @@ class Mol {
   void synchronize();
   KernelTimer& timer() { return timer_; }
 
+  // points: (n, 3) host or device; or points == nullptr and cube != nullptr:
+  // the points of a uniform grid, generated on the device chunk by chunk.
   void eval(const double* points, int64_t n, uint32_t q, int precision, const ct_eval_params& par,
-            const ct_outputs& out, ct_eval_report* rep);
+            const ct_outputs& out, ct_eval_report* rep, const CubicGrid* cube = nullptr);
   void esp(const double* points, int64_t n, int precision, double* out);
   void backward(const double* points, int64_t n, const double* w, int precision, double* dP, double* dR);
   int64_t esp_pairs() const { return esp_npairs_; }

@@ -118,61 +118,38 @@ class Molecule {
   // Generic multi-quantity evaluation: returns {name: array}.
   py::dict compute(const Points& pts, const std::vector<std::string>& names, const std::string& precision,
                    double a, double b, double alpha, double mixed_tol) {
-    static const std::map<std::string, uint32_t> bits = {
-        {"rho", CT_Q_RHO},         {"grad", CT_Q_GRAD},
-        {"hess", CT_Q_HESS},       {"lap", CT_Q_LAP},
-        {"tau_pd", CT_Q_TAU_PD},   {"rdg", CT_Q_RDG},
-        {"shannon", CT_Q_SHANNON}, {"weizsacker", CT_Q_WEIZSACKER},
-        {"thomas_fermi", CT_Q_THOMAS_FERMI}, {"gge", CT_Q_GGE},
-        {"general_ked", CT_Q_GENERAL_KED},   {"ked_alpha", CT_Q_KED_ALPHA}};
     const int64_t n = n_points(pts);
-    uint32_t q = 0;
-    for (const auto& nm : names) {
-      auto it = bits.find(nm);
-      if (it == bits.end()) throw std::invalid_argument("unknown quantity '" + nm + "'");
-      q |= it->second;
-    }
-    ct_outputs out;
-    std::memset(&out, 0, sizeof(out));
-    py::dict res;
-    std::vector<py::array_t<double>> keep;
-    auto make = [&](const char* nm, uint32_t bit, double*& slot, std::vector<py::ssize_t> shape) {
-      if (!(q & bit)) return;
-      py::array_t<double> arr(shape);
-      slot = arr.mutable_data();
-      res[nm] = arr;
-      keep.push_back(arr);
-    };
-    make("rho", CT_Q_RHO, out.rho, {n});
-    make("grad", CT_Q_GRAD, out.grad, {n, 3});
-    make("hess", CT_Q_HESS, out.hess, {n, 3, 3});
-    make("lap", CT_Q_LAP, out.lap, {n});
-    make("tau_pd", CT_Q_TAU_PD, out.tau_pd, {n});
-    make("rdg", CT_Q_RDG, out.rdg, {n});
-    make("shannon", CT_Q_SHANNON, out.shannon, {n});
-    make("weizsacker", CT_Q_WEIZSACKER, out.weizsacker, {n});
-    make("thomas_fermi", CT_Q_THOMAS_FERMI, out.thomas_fermi, {n});
-    make("gge", CT_Q_GGE, out.gge, {n});
-    make("general_ked", CT_Q_GENERAL_KED, out.general_ked, {n});
-    make("ked_alpha", CT_Q_KED_ALPHA, out.ked_alpha, {n});
-    ct_eval_params par{};
-    par.gge_a = a;
-    par.gge_b = b;
-    par.ked_a = alpha;
-    par.ked_alpha = alpha;
-    par.mixed_tol = mixed_tol;
-    ct_eval_report rep{};
-    const int prec = parse_precision(precision);
-    int st;
-    {
-      py::gil_scoped_release nogil;
-      st = ct_eval(h_, pts.data(), n, q, prec, &par, &out, &rep);
-    }
-    check(st);
-    last_report_ = rep;
-    return res;
+    return compute_impl(pts.data(), nullptr, {n}, names, precision, a, b, alpha, mixed_tol);
   }
 
+  // The same on a uniform grid generated on the device: point (i, j, k) =
+  // origin + i axes[0] + j axes[1] + k axes[2]; arrays of shape (n0, n1, n2, ...).
+  py::dict compute_on_cubic_grid(py::array_t<double, py::array::c_style | py::array::forcecast> origin,
+                                 py::array_t<double, py::array::c_style | py::array::forcecast> axes,
+                                 py::array_t<int64_t, py::array::c_style | py::array::forcecast> shape,
+                                 const std::vector<std::string>& names, const std::string& precision, double a,
+                                 double b, double alpha, double mixed_tol) {
+    if (origin.size() != 3 || axes.size() != 9 || shape.size() != 3)
+      throw std::invalid_argument("origin must have 3 entries, axes 3 x 3, shape 3");
+    Cube c;
+    std::memcpy(c.origin, origin.data(), sizeof(c.origin));
+    std::memcpy(c.axes, axes.data(), sizeof(c.axes));
+    std::memcpy(c.shape, shape.data(), sizeof(c.shape));
+    for (int k = 0; k < 3; ++k)
+      if (c.shape[k] <= 0) throw std::invalid_argument("grid shape must be positive");
+    return compute_impl(nullptr, &c, {c.shape[0], c.shape[1], c.shape[2]}, names, precision, a, b, alpha,
+                        mixed_tol);
+  }
+
+  // compute_electron_density_cubic (src/pymolecule.cu:59-71); `disp` (progress
+  // printing in the reference) is accepted and ignored.
+  py::array_t<double> density_cubic(py::array_t<double, py::array::c_style | py::array::forcecast> origin,
+                                    py::array_t<double, py::array::c_style | py::array::forcecast> axes,
+                                    py::array_t<int64_t, py::array::c_style | py::array::forcecast> shape, bool,
+                                    const std::string& precision) {
+    py::dict d = compute_on_cubic_grid(origin, axes, shape, {"rho"}, precision, 0, 0, 0, 0);
+    return d["rho"].cast<py::array_t<double>>();
+  }
   py::array_t<double> one(const Points& pts, uint32_t bit, std::vector<py::ssize_t> extra,
                           const std::string& precision, double a = 0, double b = 0, double alpha = 0) {
     const int64_t n = n_points(pts);
@@ -254,6 +231,74 @@ class Molecule {
   uintptr_t handle() const { return reinterpret_cast<uintptr_t>(h_); }
 
  private:
+  struct Cube {
+    double origin[3];
+    double axes[9];
+    int64_t shape[3];
+  };
+
+  py::dict compute_impl(const double* pts, const Cube* cube, std::vector<py::ssize_t> lead,
+                        const std::vector<std::string>& names, const std::string& precision, double a, double b,
+                        double alpha, double mixed_tol) {
+    static const std::map<std::string, uint32_t> bits = {
+        {"rho", CT_Q_RHO},         {"grad", CT_Q_GRAD},
+        {"hess", CT_Q_HESS},       {"lap", CT_Q_LAP},
+        {"tau_pd", CT_Q_TAU_PD},   {"rdg", CT_Q_RDG},
+        {"shannon", CT_Q_SHANNON}, {"weizsacker", CT_Q_WEIZSACKER},
+        {"thomas_fermi", CT_Q_THOMAS_FERMI}, {"gge", CT_Q_GGE},
+        {"general_ked", CT_Q_GENERAL_KED},   {"ked_alpha", CT_Q_KED_ALPHA}};
+    int64_t n = 1;
+    for (auto e : lead) n *= e;
+    uint32_t q = 0;
+    for (const auto& nm : names) {
+      auto it = bits.find(nm);
+      if (it == bits.end()) throw std::invalid_argument("unknown quantity '" + nm + "'");
+      q |= it->second;
+    }
+    ct_outputs out;
+    std::memset(&out, 0, sizeof(out));
+    py::dict res;
+    auto make = [&](const char* nm, uint32_t bit, double*& slot, std::vector<py::ssize_t> tail) {
+      if (!(q & bit)) return;
+      std::vector<py::ssize_t> shape = lead;
+      shape.insert(shape.end(), tail.begin(), tail.end());
+      py::array_t<double> arr(shape);
+      slot = arr.mutable_data();
+      res[nm] = arr;
+    };
+    make("rho", CT_Q_RHO, out.rho, {});
+    make("grad", CT_Q_GRAD, out.grad, {3});
+    make("hess", CT_Q_HESS, out.hess, {3, 3});
+    make("lap", CT_Q_LAP, out.lap, {});
+    make("tau_pd", CT_Q_TAU_PD, out.tau_pd, {});
+    make("rdg", CT_Q_RDG, out.rdg, {});
+    make("shannon", CT_Q_SHANNON, out.shannon, {});
+    make("weizsacker", CT_Q_WEIZSACKER, out.weizsacker, {});
+    make("thomas_fermi", CT_Q_THOMAS_FERMI, out.thomas_fermi, {});
+    make("gge", CT_Q_GGE, out.gge, {});
+    make("general_ked", CT_Q_GENERAL_KED, out.general_ked, {});
+    make("ked_alpha", CT_Q_KED_ALPHA, out.ked_alpha, {});
+    ct_eval_params par{};
+    par.gge_a = a;
+    par.gge_b = b;
+    par.ked_a = alpha;
+    par.ked_alpha = alpha;
+    par.mixed_tol = mixed_tol;
+    ct_eval_report rep{};
+    const int prec = parse_precision(precision);
+    int st;
+    {
+      py::gil_scoped_release nogil;
+      if (cube)
+        st = ct_eval_cubic(h_, cube->origin, cube->axes, cube->shape, q, prec, &par, &out, &rep);
+      else
+        st = ct_eval(h_, pts, n, q, prec, &par, &out, &rep);
+    }
+    check(st);
+    last_report_ = rep;
+    return res;
+  }
+
   ct_mol* h_ = nullptr;
   std::string path_;
   std::vector<int32_t> numbers_override_;
@@ -349,6 +394,17 @@ PYBIND11_MODULE(_chemtools_amd, m) {
       .def("compute", &Molecule::compute, py::arg("points"), py::arg("quantities"), py::arg("precision") = P64,
            py::arg("a") = 0.0, py::arg("b") = 0.0, py::arg("alpha") = 0.0, py::arg("mixed_tol") = 0.0,
            "Evaluate several quantities in one fused pass; returns a dict.")
+      .def("compute_on_cubic_grid", &Molecule::compute_on_cubic_grid, py::arg("origin"), py::arg("axes"),
+           py::arg("shape"), py::arg("quantities"), py::arg("precision") = P64, py::arg("a") = 0.0,
+           py::arg("b") = 0.0, py::arg("alpha") = 0.0, py::arg("mixed_tol") = 0.0,
+           "Several quantities on the uniform grid origin + i axes[0] + j axes[1] + k axes[2], generated on the "
+           "GPU; arrays of shape (n0, n1, n2, ...).")
+      .def("compute_electron_density_on_cubic_grid", &Molecule::density_cubic, py::arg("origin"), py::arg("axes"),
+           py::arg("shape"), py::arg("disp") = false, py::arg("precision") = P64,
+           "Electron density on a uniform grid, shape (n0, n1, n2) (the reference's test spelling).")
+      .def("compute_electron_density_cubic", &Molecule::density_cubic, py::arg("origin"), py::arg("axes"),
+           py::arg("shape"), py::arg("disp") = false, py::arg("precision") = P64,
+           "Electron density on a uniform grid, shape (n0, n1, n2) (Molecule::compute_electron_density_cubic).")
       .def("backward", &Molecule::backward, py::arg("points"), py::arg("dL_drho"), py::arg("precision") = P64,
            "Adjoint of rho: returns (dL/dP, dL/dR).")
       .def("set_coordinates", &Molecule::set_coordinates)

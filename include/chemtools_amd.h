@@ -27,6 +27,8 @@
  *                            compute_thomas_fermi_energy_density, compute_ked_gradient_expansion_general,
  *                            compute_general_ked, compute_shannon_information_density
  *                            (src/evaluate_densbased.cu:44-223)
+ *   ct_eval_cubic            evaluate_electron_density_on_cubic (src/evaluate_density.cu:425-660),
+ *                            Molecule::compute_electron_density_cubic (src/pymolecule.cu:59-71)
  *   ct_norm_rows             compute_norm_of_3d_vector (src/evaluate_densbased.cu:10-41)
  *   ct_esp                   compute_electrostatic_potential_over_points (src/evaluate_electrostatic.cu:10161-10372)
  *   ct_backward              (no reference counterpart: adjoint of rho w.r.t. P and the centres)
@@ -138,6 +140,15 @@ int ct_mol_set_workspace_limit(ct_mol* mol, int64_t bytes);
  * non-null; others are ignored.  report may be NULL. */
 int ct_eval(ct_mol* mol, const double* points, int64_t n, uint32_t quantities, int32_t precision,
             const ct_eval_params* params, const ct_outputs* out, ct_eval_report* report);
+
+/* The same quantities on a uniform grid generated on the device (no point
+ * upload): point (i, j, k) = origin + i axes[0..2] + j axes[3..5] + k axes[6..8]
+ * for 0 <= i < shape[0], 0 <= j < shape[1], 0 <= k < shape[2]; outputs are
+ * laid out with k fastest, (shape[0], shape[1], shape[2], width).
+ * Replaces evaluate_electron_density_on_cubic (src/evaluate_density.cu:425-660)
+ * and Molecule::compute_electron_density_cubic (src/pymolecule.cu:59-71). */
+int ct_eval_cubic(ct_mol* mol, const double* origin, const double* axes, const int64_t* shape, uint32_t quantities,
+                  int32_t precision, const ct_eval_params* params, const ct_outputs* out, ct_eval_report* report);
 
 /* |v| of each row of an (n,3) array. */
 int ct_norm_rows(const double* v, int64_t n, double* out);
