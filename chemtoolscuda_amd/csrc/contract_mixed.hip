@@ -99,9 +99,11 @@ __global__ __launch_bounds__(256, 2) void contract_mixed_kernel(const u4* __rest
   // their K range for five row blocks).  The Hessian mode's t_kl (k != l) are
   // not quadratic forms and use the full K.
   constexpr bool SYM = MODE == MODE_LAP;
-  YTile<TC, NBC> yt;
+  constexpr bool F64 = MODE != MODE_HESS;  // the Hessian mode has no registers for FP64 sums
+  YTile<TC, NBC, F64> yt;
   const float* bsrc = ao + ((int64_t)C::type(tid & (TC - 1)) * mq * ld + pbase + C::point(tid & (TC - 1))) * 4;
-  y_tile<TC, NBC, SYM>(Pf, mp, ld, i0, tid, bsrc, Bs, yt, C::type(tid & (TC - 1)) != 0, i0 / KB, (i0 + TR) / KB);
+  y_tile<TC, NBC, SYM, F64>(Pf, mp, ld, i0, tid, bsrc, Bs, yt, C::type(tid & (TC - 1)) != 0, i0 / KB,
+                            (i0 + TR) / KB);
 
   // ---------------------------------------------------------------- epilogue
   double* red = reinterpret_cast<double*>(&Bs[0][0][0][0]);  // [NW][TC][kMaxDots]

@@ -59,13 +59,17 @@ struct PrimData {
   double c;
 };
 
-// Error model of the mixed path: |rho_mixed - rho| <= gamma * |phi|^T |P| |phi|.
-// Rounding the FP64-computed AOs to FP32 contributes <= u|phi|^T|P||phi|
-// (u = 2^-24), the terms the three-piece bf16 split drops <= 2u, FP32
-// accumulation the rest; the largest ratio error / (u |phi|^T|P||phi|)
-// measured on the 60- and 300-atom workloads is below 6, so 2^-21 = 8u
-// leaves a margin of 1.4.
-constexpr double kMixedGamma = 4.76837158203125e-07;  // 2^-21
+// Error model of the mixed path: |rho_mixed - rho| <= gamma * |phi|^T |P| |phi|
+// (u = 2^-24).  Rounding the FP64-computed AOs and P to FP32 contributes
+// ~2.1u |phi|^T|P||phi|, the terms the three-piece bf16 split drops <= 2u in
+// the worst case (~0 in practice), FP32 accumulation the rest.  Largest ratio
+// error / (u |phi|^T|P||phi|) measured on the 25-, 624- and 5400-function
+// workloads: 3.3 with FP64 running sums (rho / grad / lap modes), 6.8 with the
+// two FP32 accumulators of the Hessian mode; gamma leaves a margin of 1.8 and
+// 1.2 over them.
+constexpr double kMixedGammaF64Sums = 3.5762786865234375e-07;  // 6u
+constexpr double kMixedGammaF32Sums = 4.76837158203125e-07;    // 8u
+
 
 // Granularities.
 constexpr int kRowBlock = 128;     // basis padding: rows of P per contraction block
@@ -73,6 +77,7 @@ constexpr int kPointAlign = 256;   // chunk stride alignment (points)
 
 // Evaluation modes of the fused contraction (which GEMM columns and dots).
 enum Mode : int { MODE_RHO = 0, MODE_GRAD = 1, MODE_LAP = 2, MODE_HESS = 3 };
+inline double mixed_gamma(int mode) { return mode == MODE_HESS ? kMixedGammaF32Sums : kMixedGammaF64Sums; }
 // AO arrays evaluated per mode: val; val+grad; val+grad+lap; val+grad+hess.
 constexpr int mode_nder(int m) { return m == MODE_RHO ? 1 : m == MODE_GRAD ? 4 : m == MODE_LAP ? 5 : 10; }
 // Per-point dot products produced by the contraction epilogue.

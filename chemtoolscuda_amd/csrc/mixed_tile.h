@@ -5,10 +5,8 @@
 // P and the FP32 columns are split exactly into three bf16 pieces
 // (x = hi + mid + lo), and Y is the sum of the six products with the
 // largest terms (hi*hi, hi*mid, mid*hi, hi*lo, lo*hi, mid*mid; the dropped
-// ones are <= 2^-23 of |P||B|).  hi*hi goes to one FP32 accumulator, the
-// five smaller products to another: the FP32 rounding of the running sum at
-// full magnitude then happens once per k-step instead of six times, the
-// largest error source of the path.  An optional seventh product
+// ones are <= 2^-23 of |P||B|), accumulated as described at YTile.  An
+// optional seventh product
 // |P_hi| * |B_hi| (sign-bit masks of the hi pieces) accumulates the
 // magnitude sum that scales the rounding errors.
 //
@@ -65,15 +63,28 @@ __device__ __forceinline__ Split2 split2(float x0, float x1) {
   return Split2{__builtin_bit_cast(uint32_t, hh), __builtin_bit_cast(uint32_t, mm), __builtin_bit_cast(uint32_t, ll)};
 }
 
-template <int TC, int NBC>
+// Accumulation of the six products, two schemes:
+//   F64 = true:  per k-step and column-block pair the six products are summed
+//     from zero in FP32 (their 32-term MFMA sums and five adds at the size of
+//     one k-step's contribution) and the chunk is added to FP64 running sums:
+//     the only FP32 rounding at full magnitude is gone (error ~2.4u |P||B|,
+//     close to the 2.1u floor of rounding the inputs to FP32); costs 2x the
+//     accumulator registers.
+//   F64 = false: hi*hi in one FP32 accumulator, the five smaller products in
+//     another (~3.7u |P||B|), for the Hessian mode's larger epilogue.
+template <int TC, int NBC, bool F64 = true>
 struct YTile {
   static constexpr int NCB = TC / 16;
-  f4 hh[NRB][NCB];                     // hi*hi products
-  f4 sm[NRB][NCB];                     // the five smaller products
-  f4 bound[NRB][NBC > 0 ? NBC : 1];    // |P_hi| |B_hi| for column blocks 0..NBC-1
-  // Y in FP64 (exact sum of the two accumulators)
+  static constexpr bool kF64 = F64;
+  double yd[F64 ? NRB : 1][F64 ? NCB : 1][4];  // F64: running sums
+  f4 hh[F64 ? 1 : NRB][F64 ? 1 : NCB];         // !F64: hi*hi products
+  f4 sm[F64 ? 1 : NRB][F64 ? 1 : NCB];         // !F64: the five smaller products
+  f4 bound[NRB][NBC > 0 ? NBC : 1];            // |P_hi| |B_hi| for column blocks 0..NBC-1
   __device__ __forceinline__ double y(int rb, int cb, int j) const {
-    return (double)hh[rb][cb][j] + (double)sm[rb][cb][j];
+    if constexpr (F64)
+      return yd[rb][cb][j];
+    else
+      return (double)hh[rb][cb][j] + (double)sm[rb][cb][j];
   }
 };
 
@@ -86,10 +97,10 @@ struct YTile {
 // P_IJ x_J), those columns need only the k-steps from k_lo (the row block's
 // diagonal block) on, with their B values doubled from k_dbl (right of the
 // diagonal block; exact in bf16).  bsym: this thread's column is one of them.
-template <int TC, int NBC, bool SYM = false>
+template <int TC, int NBC, bool SYM, bool F64>
 __device__ __forceinline__ void y_tile(const u4* __restrict__ Pf, int mp, int64_t ld, int i0, int tid,
                                        const float* __restrict__ bsrc, uint16_t (&Bs)[2][NPL][TC][LROW],
-                                       YTile<TC, NBC>& t, bool bsym = false, int k_lo = 0, int k_dbl = 0) {
+                                       YTile<TC, NBC, F64>& t, bool bsym = false, int k_lo = 0, int k_dbl = 0) {
   constexpr int NCB = TC / 16;
   const int lane = tid & 63;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -145,7 +156,14 @@ __device__ __forceinline__ void y_tile(const u4* __restrict__ Pf, int mp, int64_
 #pragma unroll
   for (int a = 0; a < NRB; ++a) {
 #pragma unroll
-    for (int b = 0; b < NCB; ++b) t.hh[a][b] = t.sm[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+    for (int b = 0; b < NCB; ++b) {
+      if constexpr (F64) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) t.yd[a][b][j] = 0.0;
+      } else {
+        t.hh[a][b] = t.sm[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+      }
+    }
 #pragma unroll
     for (int b = 0; b < (NBC > 0 ? NBC : 1); ++b) t.bound[a][b] = f4{0.f, 0.f, 0.f, 0.f};
   }
@@ -164,22 +182,46 @@ __device__ __forceinline__ void y_tile(const u4* __restrict__ Pf, int mp, int64_
 #pragma unroll
         for (int pl = 0; pl < NPL; ++pl)
           bf[c][pl] = as_bf(*reinterpret_cast<const u4*>(&Bs[s][pl][16 * (2 * cp + c) + r][8 * q]));
-      constexpr int pa[5] = {2, 1, 0, 1, 0};
-      constexpr int pb[5] = {0, 1, 2, 0, 1};
+      if constexpr (F64) {
+        constexpr int pa[6] = {2, 1, 0, 1, 0, 0};
+        constexpr int pb[6] = {0, 1, 2, 0, 1, 0};
+        f4 tc[NRB][2];
 #pragma unroll
-      for (int pr = 0; pr < 5; ++pr)
+        for (int rb = 0; rb < NRB; ++rb)
+#pragma unroll
+          for (int c = 0; c < 2; ++c) tc[rb][c] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int pr = 0; pr < 6; ++pr)
+#pragma unroll
+          for (int rb = 0; rb < NRB; ++rb)
+#pragma unroll
+            for (int c = 0; c < 2; ++c)
+              tc[rb][c] =
+                  __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(areg[pa[pr]][rb]), bf[c][pb[pr]], tc[rb][c], 0, 0, 0);
 #pragma unroll
         for (int rb = 0; rb < NRB; ++rb)
 #pragma unroll
           for (int c = 0; c < 2; ++c)
-            t.sm[rb][2 * cp + c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(areg[pa[pr]][rb]), bf[c][pb[pr]],
-                                                                            t.sm[rb][2 * cp + c], 0, 0, 0);
 #pragma unroll
-      for (int rb = 0; rb < NRB; ++rb)
+            for (int j = 0; j < 4; ++j) t.yd[rb][2 * cp + c][j] += (double)tc[rb][c][j];
+      } else {
+        constexpr int pa[5] = {2, 1, 0, 1, 0};
+        constexpr int pb[5] = {0, 1, 2, 0, 1};
 #pragma unroll
-        for (int c = 0; c < 2; ++c)
-          t.hh[rb][2 * cp + c] =
-              __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(areg[0][rb]), bf[c][0], t.hh[rb][2 * cp + c], 0, 0, 0);
+        for (int pr = 0; pr < 5; ++pr)
+#pragma unroll
+          for (int rb = 0; rb < NRB; ++rb)
+#pragma unroll
+            for (int c = 0; c < 2; ++c)
+              t.sm[rb][2 * cp + c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                  as_bf(areg[pa[pr]][rb]), bf[c][pb[pr]], t.sm[rb][2 * cp + c], 0, 0, 0);
+#pragma unroll
+        for (int rb = 0; rb < NRB; ++rb)
+#pragma unroll
+          for (int c = 0; c < 2; ++c)
+            t.hh[rb][2 * cp + c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(areg[0][rb]), bf[c][0],
+                                                                            t.hh[rb][2 * cp + c], 0, 0, 0);
+      }
       if constexpr (NBC > 0) {
         if (2 * cp < NBC) {
 #pragma unroll

@@ -648,7 +648,7 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
                             dots, stream_);
       timer_.end(stream_);
       FlagArgs fl;
-      fl.gamma = kMixedGamma;
+      fl.gamma = mixed_gamma(mode);
       fl.tol = par.mixed_tol;
       fl.rho_floor = par.rho_floor;
       fl.count = d_count;

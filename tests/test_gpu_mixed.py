@@ -133,9 +133,9 @@ def test_mixed_error_model_is_conservative(ct):
 @pytest.mark.parametrize("workload,npts", [("water", 3000), ("organic60", 3000), ("large300", 400)])
 def test_mixed_error_model_pointwise(ct, workload, npts):
     """Without re-evaluation, the error of every quantity stays below
-    gamma * (its condition scale), gamma = 2^-21 -- the estimate the finaliser
-    applies to rho.  Printed: the largest ratio error / (u * scale), u = 2^-24,
-    per quantity."""
+    gamma * (its condition scale), gamma = 6u for these (FP64 running-sum)
+    modes -- the estimate the finaliser applies to rho.  Printed: the largest
+    ratio error / (u * scale), u = 2^-24, per quantity."""
     from chemtoolscuda_amd import synthetic
 
     w = synthetic.large300(npts) if workload == "large300" else synthetic.WORKLOADS[workload]()
@@ -156,4 +156,4 @@ def test_mixed_error_model_pointwise(ct, workload, npts):
     print(f"{workload} (M={w.basis.nbasis}): max error / (u * condition scale): "
           + ", ".join(f"{k} {v:.2f}" for k, v in ratios.items()))
     for k, v in ratios.items():
-        assert v <= 8.0, (k, v)
+        assert v <= 6.0, (k, v)
