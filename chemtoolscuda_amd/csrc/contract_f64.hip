@@ -39,7 +39,10 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 
 constexpr int kIBlock = 64;  // rows of P per wave pass (4 MFMA row blocks); the basis padding is a multiple
 
-template <int MODE>
+// ISPLIT: one 64-row block of P per wave (blockIdx.y), each writing its own
+// partial dots [block][dot][point] that reduce_row_blocks adds in order --
+// enough waves for small point counts (the mixed path's re-evaluation).
+template <int MODE, bool ISPLIT>
 __global__ __launch_bounds__(256) void contract_f64_kernel(const double* __restrict__ P, int mp,
                                                            const double* __restrict__ ao, int64_t ld,
                                                            double* __restrict__ dots) {
@@ -64,7 +67,10 @@ __global__ __launch_bounds__(256) void contract_f64_kernel(const double* __restr
 #pragma unroll
   for (int k = 0; k < NACC; ++k) dacc[k] = 0.0;
 
-  for (int i0 = 0; i0 < mp; i0 += kIBlock) {
+  const int i_begin = ISPLIT ? (int)blockIdx.y * kIBlock : 0;
+  const int i_end = ISPLIT ? i_begin + kIBlock : mp;
+  if (ISPLIT) dots += (int64_t)blockIdx.y * ND * ld;
+  for (int i0 = i_begin; i0 < i_end; i0 += kIBlock) {
     d4 acc[4][4];
 #pragma unroll
     for (int rb = 0; rb < 4; ++rb)
@@ -161,26 +167,54 @@ __global__ __launch_bounds__(256) void contract_f64_kernel(const double* __restr
   }
 }
 
+__global__ __launch_bounds__(256) void reduce_row_blocks_kernel(const double* __restrict__ part, int nblk,
+                                                                int64_t count, double* __restrict__ out) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  double s = 0.0;
+  for (int b = 0; b < nblk; ++b) s += part[(int64_t)b * count + k];
+  out[k] = s;
+}
+
+template <bool ISPLIT>
+void launch_modes(int mode, const double* P, int nbasis_pad, const double* ao, int64_t ld, double* dots,
+                  hipStream_t s) {
+  const int ppw = (mode == MODE_RHO || mode == MODE_GRAD) ? 64 : 16;
+  const dim3 block(256), grid((unsigned)(ld / (4 * ppw)), ISPLIT ? (unsigned)(nbasis_pad / kIBlock) : 1u);
+  switch (mode) {
+    case MODE_RHO:
+      hipLaunchKernelGGL((contract_f64_kernel<MODE_RHO, ISPLIT>), grid, block, 0, s, P, nbasis_pad, ao, ld, dots);
+      break;
+    case MODE_GRAD:
+      hipLaunchKernelGGL((contract_f64_kernel<MODE_GRAD, ISPLIT>), grid, block, 0, s, P, nbasis_pad, ao, ld, dots);
+      break;
+    case MODE_LAP:
+      hipLaunchKernelGGL((contract_f64_kernel<MODE_LAP, ISPLIT>), grid, block, 0, s, P, nbasis_pad, ao, ld, dots);
+      break;
+    default:
+      hipLaunchKernelGGL((contract_f64_kernel<MODE_HESS, ISPLIT>), grid, block, 0, s, P, nbasis_pad, ao, ld, dots);
+      break;
+  }
+  CT_HIP_CHECK(hipGetLastError());
+}
+
 }  // namespace
 
 void launch_contract_f64(int mode, const double* P, int nbasis_pad, const double* ao, int64_t ld, double* dots,
                          hipStream_t s) {
-  const int ppw = (mode == MODE_RHO || mode == MODE_GRAD) ? 64 : 16;
-  const dim3 block(256), grid((unsigned)(ld / (4 * ppw)));
-  switch (mode) {
-    case MODE_RHO:
-      hipLaunchKernelGGL(contract_f64_kernel<MODE_RHO>, grid, block, 0, s, P, nbasis_pad, ao, ld, dots);
-      break;
-    case MODE_GRAD:
-      hipLaunchKernelGGL(contract_f64_kernel<MODE_GRAD>, grid, block, 0, s, P, nbasis_pad, ao, ld, dots);
-      break;
-    case MODE_LAP:
-      hipLaunchKernelGGL(contract_f64_kernel<MODE_LAP>, grid, block, 0, s, P, nbasis_pad, ao, ld, dots);
-      break;
-    default:
-      hipLaunchKernelGGL(contract_f64_kernel<MODE_HESS>, grid, block, 0, s, P, nbasis_pad, ao, ld, dots);
-      break;
-  }
+  launch_modes<false>(mode, P, nbasis_pad, ao, ld, dots, s);
+}
+
+int64_t contract_f64_split_workspace(int mode, int nbasis_pad, int64_t ld) {
+  return (int64_t)(nbasis_pad / kIBlock) * mode_ndot(mode) * ld;
+}
+
+void launch_contract_f64_split(int mode, const double* P, int nbasis_pad, const double* ao, int64_t ld,
+                               double* part, double* dots, hipStream_t s) {
+  launch_modes<true>(mode, P, nbasis_pad, ao, ld, part, s);
+  const int64_t count = (int64_t)mode_ndot(mode) * ld;
+  hipLaunchKernelGGL(reduce_row_blocks_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, part,
+                     nbasis_pad / kIBlock, count, dots);
   CT_HIP_CHECK(hipGetLastError());
 }
 

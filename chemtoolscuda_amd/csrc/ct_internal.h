@@ -117,6 +117,12 @@ void launch_ao_f64(int mode, int lmax, const ShellMeta* meta, const ShellGeom* g
 void launch_contract_f64(int mode, const double* P, int nbasis_pad, const double* ao, int64_t ld, double* dots,
                          hipStream_t s);
 void launch_finalize(const FinalizeArgs& a, hipStream_t s);
+// The same contraction with one 64-row block of P per wave and an ordered
+// sum of the blocks' partial dots (for small point counts); part holds
+// contract_f64_split_workspace() doubles.
+int64_t contract_f64_split_workspace(int mode, int nbasis_pad, int64_t ld);
+void launch_contract_f64_split(int mode, const double* P, int nbasis_pad, const double* ao, int64_t ld,
+                               double* part, double* dots, hipStream_t s);
 
 // Mixed path.
 void launch_ao_f32(int mode, int lmax, const ShellMeta* meta, const ShellGeom* geom, const PrimData* prim, int nshell,

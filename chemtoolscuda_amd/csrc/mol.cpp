@@ -508,15 +508,22 @@ int64_t Mol::chunk_points(int64_t per_point, int64_t n) const {
 void Mol::reevaluate(int mode, const double* pts, int64_t count, const int32_t* index, const FinalizeArgs& fin) {
   const int nder = mode_nder(mode), ndot = mode_ndot(mode);
   const int64_t ld = round_up(count, kPointAlign);
-  const size_t bytes = (size_t)ld * ((size_t)nder * Mp_ * 8 + (size_t)ndot * 8 + 24);
+  // few points: one wave per (16 or 64 points, 64-row block of P) so the GPU fills
+  const bool split = ld < 65536;
+  const size_t split_doubles = split ? (size_t)contract_f64_split_workspace(mode, Mp_, ld) : 0;
+  const size_t bytes = (size_t)ld * ((size_t)nder * Mp_ * 8 + (size_t)ndot * 8 + 24) + split_doubles * 8;
   if (!d_ws64_.reserve(bytes)) throw Error(CT_E_NOMEM, "cannot allocate the FP64 re-evaluation workspace");
   double* ao = static_cast<double*>(d_ws64_.get());
   double* dots = ao + (size_t)nder * Mp_ * ld;
   double* gp = dots + (size_t)ndot * ld;
+  double* part = gp + 3 * (size_t)ld;
   launch_gather_points(pts, index, count, gp, stream_);
   launch_ao_f64(mode, lmax_, static_cast<const ShellMeta*>(d_meta_.get()), static_cast<const ShellGeom*>(d_geom_.get()),
                 static_cast<const PrimData*>(d_prim_.get()), nshell_, M_, Mp_, gp, count, ld, ao, 700.0, stream_);
-  launch_contract_f64(mode, static_cast<const double*>(d_P_.get()), Mp_, ao, ld, dots, stream_);
+  if (split)
+    launch_contract_f64_split(mode, static_cast<const double*>(d_P_.get()), Mp_, ao, ld, part, dots, stream_);
+  else
+    launch_contract_f64(mode, static_cast<const double*>(d_P_.get()), Mp_, ao, ld, dots, stream_);
   FinalizeArgs f = fin;
   f.dots = dots;
   f.index = index;

@@ -94,7 +94,9 @@ def test_mixed_hessian_and_rho_modes(ct):
 
 
 def test_forced_reevaluation_reproduces_fp64(ct):
-    """A zero-width tolerance flags every point: the mixed call then returns the FP64 results."""
+    """A zero-width tolerance flags every point: the mixed call then returns the
+    FP64 results (to rounding: few re-evaluated points use the row-block-split
+    FP64 contraction, whose partial sums are added in another order)."""
     from chemtoolscuda_amd import synthetic
 
     w = synthetic.water(2500)
@@ -106,7 +108,10 @@ def test_forced_reevaluation_reproduces_fp64(ct):
     floor_ok = np.abs(f64["rho"]) >= 1e-10
     assert rep["n_reevaluated"] == int(floor_ok.sum())
     for k in names:
-        assert np.array_equal(mix[k][floor_ok], f64[k][floor_ok]), k
+        a = np.asarray(mix[k][floor_ok]).reshape(int(floor_ok.sum()), -1)
+        b = np.asarray(f64[k][floor_ok]).reshape(int(floor_ok.sum()), -1)
+        den = np.maximum(np.linalg.norm(b, axis=1), 1e-10 * np.abs(b).max())
+        assert np.max(np.linalg.norm(a - b, axis=1) / den) < 1e-13, k
 
 
 def test_mixed_error_model_is_conservative(ct):
