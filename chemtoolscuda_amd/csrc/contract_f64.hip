@@ -84,23 +84,38 @@ __global__ __launch_bounds__(256) void contract_f64_kernel(const double* __restr
 #pragma unroll
     for (int cb = 0; cb < 4; ++cb) b[cb] = bptr[cb][0];
 
-    for (int k0 = 0; k0 < mp; k0 += 4) {
+    // Lap mode: the derivative columns (cb 1..3) enter only tau_kk =
+    // d_k phi^T P d_k phi, a quadratic form; by the symmetry of P,
+    // sum_IJ x_I^T P_IJ x_J = sum_I x_I^T (P_II x_I + 2 sum_{J>I} P_IJ x_J),
+    // so they need only k >= i0, with the values doubled (exactly) right of
+    // the diagonal block.
+    auto kstep = [&](int k0, bool all_cols, bool dbl) {
       // prefetch the next k-step (clamped on the last step; the value is unused)
       const int kn = (k0 + 4 < mp) ? k0 + 4 : k0;
       double an[4], bn[4];
 #pragma unroll
       for (int rb = 0; rb < 4; ++rb) an[rb] = aptr[(int64_t)kn * mp + 16 * rb];
 #pragma unroll
-      for (int cb = 0; cb < 4; ++cb) bn[cb] = bptr[cb][(int64_t)kn * ld];
+      for (int cb = 0; cb < 4; ++cb)
+        if (cb == 0 || all_cols || kn >= i0) bn[cb] = bptr[cb][(int64_t)kn * ld];
 #pragma unroll
       for (int rb = 0; rb < 4; ++rb)
 #pragma unroll
         for (int cb = 0; cb < 4; ++cb)
-          acc[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb], b[cb], acc[rb][cb], 0, 0, 0);
+          if (cb == 0 || all_cols)
+            acc[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb], (cb > 0 && dbl) ? 2.0 * b[cb] : b[cb],
+                                                               acc[rb][cb], 0, 0, 0);
 #pragma unroll
       for (int rb = 0; rb < 4; ++rb) a[rb] = an[rb];
 #pragma unroll
       for (int cb = 0; cb < 4; ++cb) b[cb] = bn[cb];
+    };
+    if constexpr (MODE == MODE_LAP) {
+      for (int k0 = 0; k0 < i0; k0 += 4) kstep(k0, false, false);
+      for (int k0 = i0; k0 < i0 + kIBlock; k0 += 4) kstep(k0, true, false);
+      for (int k0 = i0 + kIBlock; k0 < mp; k0 += 4) kstep(k0, true, true);
+    } else {
+      for (int k0 = 0; k0 < mp; k0 += 4) kstep(k0, true, false);
     }
 
     // epilogue: dot the Y tile with the AO rows i0..i0+63 of this wave's points
