@@ -34,7 +34,7 @@ PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16": 2.5e15}
 # HBM bytes per contraction launch (FETCH_SIZE + WRITE_SIZE, calibrated against
 # a 1 GiB device copy), measured with rocprofv3 --pmc on this workload; see
 # profiles/contract_{mixed,f64}_organic60.md for the commands and raw counts.
-MEASURED_TRAFFIC = {("organic60", "mixed"): 18.69e9 + 0.19e9, ("organic60", "fp64"): 77.1e9 + 0.021e9}
+MEASURED_TRAFFIC = {("organic60", "mixed"): 18.08e9 + 0.19e9, ("organic60", "fp64"): 53.7e9 + 0.021e9}
 
 
 def parse():
