@@ -25,39 +25,54 @@ namespace {
 
 using namespace mixed;
 
-// GEMM columns per workgroup: 128 = 32 points x 4 column types in the lap /
-// Hessian modes; 64 points of one type in the rho / grad modes, where every
-// column also carries the bound accumulator.
-template <int MODE>
+// Tile variants.  A workgroup owns a 128-row block of P and TC columns:
+//   V_RHO, V_GRAD: 64 points of phi;
+//   V_LAPPHI:      64 points of phi, full K -- the lap mode's rho, grad and
+//                  lap phi . P phi dots (and the bound);
+//   V_LAPDER<d>:   128 points of one derivative column type d (1..3), only
+//                  the diagonal-and-right blocks of P (the symmetric shortcut
+//                  of y_tile) -- the lap mode's tau_dd = d_d phi^T P d_d phi;
+//   V_HESS:        32 points x {phi, d_x phi, d_y phi, d_z phi}, full K (the
+//                  t_kl with k != l are not quadratic forms).
+// The two lap variants are two launches writing disjoint dot slots of the
+// same partials array.
+enum Var : int { V_RHO, V_GRAD, V_LAPPHI, V_LAPDER, V_HESS };
+
+template <int V>
 struct Cols {
-  static constexpr int NT = (MODE == MODE_RHO || MODE == MODE_GRAD) ? 1 : 4;
-  static constexpr int TC = NT == 1 ? 64 : 128;
-  static constexpr int NCB = TC / 16;  // 16-column MFMA blocks
-  static constexpr int PTS = TC / NT;  // points per workgroup
-  __host__ __device__ static constexpr int type(int c) { return NT == 1 ? 0 : c / PTS; }
-  __host__ __device__ static constexpr int point(int c) { return NT == 1 ? c : c % PTS; }
+  static constexpr int MODE = V == V_RHO ? MODE_RHO : V == V_GRAD ? MODE_GRAD : V == V_HESS ? MODE_HESS : MODE_LAP;
+  static constexpr int NT = V == V_HESS ? 4 : 1;          // column types per tile
+  static constexpr int TC = V == V_HESS || V == V_LAPDER ? 128 : 64;
+  static constexpr int NCB = TC / 16;                     // 16-column MFMA blocks
+  static constexpr int PTS = TC / NT;                     // points per workgroup
+  static constexpr int NBC = V == V_LAPDER ? 0 : V == V_HESS ? 2 : NCB;  // column blocks with the bound
+  static constexpr bool F64 = V != V_HESS;  // FP64 running sums (the Hessian tile lacks the registers)
+  // column type (0 = phi, 1..3 = d_x, d_y, d_z phi) of column c; dtype: the
+  // derivative type of a V_LAPDER workgroup
+  __host__ __device__ static constexpr int type(int c, int dtype) { return NT == 4 ? c / PTS : V == V_LAPDER ? dtype : 0; }
+  __host__ __device__ static constexpr int point(int c) { return NT == 4 ? c % PTS : c; }
 };
 
-// Dots produced from a GEMM column of type t (phi, d_x phi, d_y phi, d_z phi):
-// how many, and for the k-th one its slot in the dot layout and the AO array
-// it pairs the column with.
-template <int MODE>
+// Dots produced from a GEMM column of type t: how many, and for the k-th one
+// its slot in the mode's dot layout and the AO array it pairs the column with.
+template <int V>
 __host__ __device__ constexpr int ndots_of(int t) {
-  if (MODE == MODE_RHO) return 1;
-  if (MODE == MODE_GRAD) return 4;
-  if (MODE == MODE_LAP) return t == 0 ? 5 : 1;
+  if (V == V_RHO) return 1;
+  if (V == V_GRAD) return 4;
+  if (V == V_LAPPHI) return 5;
+  if (V == V_LAPDER) return 1;
   return t == 0 ? 10 : t;  // HESS: t_xy.. lists grow with t
 }
-template <int MODE>
+template <int V>
 __host__ __device__ constexpr int dot_slot(int t, int k) {
-  if (MODE == MODE_RHO || MODE == MODE_GRAD || t == 0) return k;
-  if (MODE == MODE_LAP) return 4 + t;
+  if (V == V_RHO || V == V_GRAD || V == V_LAPPHI || t == 0) return k;
+  if (V == V_LAPDER) return 4 + t;
   return (t == 1 ? 10 : t == 2 ? 11 : 13) + k;  // t_xx | t_xy t_yy | t_xz t_yz t_zz
 }
-template <int MODE>
+template <int V>
 __host__ __device__ constexpr int dot_array(int t, int k) {
-  if (MODE == MODE_RHO || MODE == MODE_GRAD || t == 0) return k;
-  if (MODE == MODE_LAP) return t;
+  if (V == V_RHO || V == V_GRAD || V == V_LAPPHI || t == 0) return k;
+  if (V == V_LAPDER) return t;
   return 1 + k;
 }
 
@@ -66,16 +81,15 @@ using IC = std::integral_constant<int, N>;
 
 constexpr int kMaxDots = 11;  // 10 Hessian-mode dots of the phi column + the bound
 
-template <int MODE>
+template <int V, int DT>
 __global__ __launch_bounds__(256, 2) void contract_mixed_kernel(const u4* __restrict__ Pf, int mp,
                                                                 const float* __restrict__ ao, int64_t ld,
                                                                 double* __restrict__ part, int nrt,
                                                                 int64_t nptile) {
-  using C = Cols<MODE>;
-  constexpr int TC = C::TC, NCB = C::NCB;
-  constexpr int ND = mode_ndot(MODE);
+  using C = Cols<V>;
+  constexpr int TC = C::TC, NCB = C::NCB, NBC = C::NBC;
+  constexpr int ND = mode_ndot(C::MODE);
   constexpr int ND1 = ND + 1;
-  constexpr int NBC = C::NT == 1 ? NCB : 2;  // column blocks of type phi (they carry the bound)
   __shared__ __attribute__((aligned(16))) uint16_t Bs[2][NPL][TC][LROW];
 
   const int tid = threadIdx.x;
@@ -85,34 +99,35 @@ __global__ __launch_bounds__(256, 2) void contract_mixed_kernel(const u4* __rest
   // XCD-aware tile order: workgroup g runs on XCD g % 8 (round-robin
   // dispatch); the nrt row blocks of one point tile get consecutive slots on
   // the same XCD so its AO tile is fetched from HBM once and re-read from L2.
+  // V_LAPDER: DT is the derivative type (1..3).
   const int g = blockIdx.x;
+  constexpr int dtype = DT;
   const int xcd = g & 7, slot = g >> 3;
   const int rt = slot % nrt;
   const int64_t ptile = (int64_t)(slot / nrt) * 8 + xcd;
   if (ptile >= nptile) return;
   const int64_t pbase = ptile * C::PTS;
   const int i0 = rt * TR;
+  const int nk = mp / KB;
   const int64_t mq = mp / 4;  // row quads per AO array
 
-  // Lap mode: the derivative columns enter only tau_kk = d_k phi^T P d_k phi,
-  // a quadratic form, so they use the symmetric-P shortcut of y_tile (0.6 of
-  // their K range for five row blocks).  The Hessian mode's t_kl (k != l) are
-  // not quadratic forms and use the full K.
-  constexpr bool SYM = MODE == MODE_LAP;
-  constexpr bool F64 = MODE != MODE_HESS;  // the Hessian mode has no registers for FP64 sums
-  YTile<TC, NBC, F64> yt;
-  const float* bsrc = ao + ((int64_t)C::type(tid & (TC - 1)) * mq * ld + pbase + C::point(tid & (TC - 1))) * 4;
-  y_tile<TC, NBC, SYM, F64>(Pf, mp, ld, i0, tid, bsrc, Bs, yt, C::type(tid & (TC - 1)) != 0, i0 / KB,
-                            (i0 + TR) / KB);
+  YTile<TC, NBC, C::F64> yt;
+  const int bc = tid & (TC - 1);
+  const float* bsrc = ao + ((int64_t)C::type(bc, dtype) * mq * ld + pbase + C::point(bc)) * 4;
+  if constexpr (V == V_LAPDER)
+    y_tile<TC, NBC, C::F64>(Pf, mp, ld, i0, tid, bsrc, Bs, yt, i0 / KB, (i0 + TR) / KB);
+  else
+    y_tile<TC, NBC, C::F64>(Pf, mp, ld, i0, tid, bsrc, Bs, yt, 0, nk);
 
   // ---------------------------------------------------------------- epilogue
   double* red = reinterpret_cast<double*>(&Bs[0][0][0][0]);  // [NW][TC][kMaxDots]
   const int64_t quad0 = (i0 + RW * w) / 4 + q;
   auto per_cb = [&](auto cbtag) {
     constexpr int CB = decltype(cbtag)::value;
-    constexpr int t = C::type(16 * CB);
-    constexpr int nd = ndots_of<MODE>(t);
+    constexpr int tc = C::type(16 * CB, 0);  // compile-time type (0 for V_LAPDER)
+    constexpr int nd = ndots_of<V>(tc);
     constexpr bool bnd = CB < NBC;
+    constexpr int t = V == V_LAPDER ? dtype : tc;
     const int64_t p = pbase + C::point(16 * CB) + r;
     double d[kMaxDots];
 #pragma unroll
@@ -122,7 +137,7 @@ __global__ __launch_bounds__(256, 2) void contract_mixed_kernel(const u4* __rest
       f4 v[nd];
 #pragma unroll
       for (int k = 0; k < nd; ++k)
-        v[k] = *reinterpret_cast<const f4*>(ao + (((int64_t)dot_array<MODE>(t, k) * mq + quad0 + 4 * rb) * ld + p) * 4);
+        v[k] = *reinterpret_cast<const f4*>(ao + (((int64_t)dot_array<V>(t, k) * mq + quad0 + 4 * rb) * ld + p) * 4);
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const double y = yt.y(rb, CB, j);
@@ -160,28 +175,36 @@ __global__ __launch_bounds__(256, 2) void contract_mixed_kernel(const u4* __rest
   __syncthreads();
   if (tid < TC) {
     const int c = tid;
-    const int t = C::type(c);
-    const int nd = ndots_of<MODE>(t);
+    const int tc = C::type(c, 0);
+    const int t = V == V_LAPDER ? dtype : tc;
+    const int nd = ndots_of<V>(tc);
+    const bool has_bound = NBC > 0 && tc == 0;
     const int64_t p = pbase + C::point(c);
 #pragma unroll
     for (int k = 0; k < kMaxDots; ++k) {
-      const bool bound = (k == kMaxDots - 1) && t == 0;
+      const bool bound = (k == kMaxDots - 1) && has_bound;
       if (k < nd || bound) {
         double sum = 0.0;
 #pragma unroll
         for (int ww = 0; ww < NW; ++ww) sum += red[((int64_t)ww * TC + c) * kMaxDots + k];
-        const int slot_k = bound ? ND : dot_slot<MODE>(t, k);
+        const int slot_k = bound ? ND : dot_slot<V>(t, k);
         part[((int64_t)rt * ND1 + slot_k) * ld + p] = sum;
       }
     }
   }
 }
 
-}  // namespace
-
-int mixed_points_per_block(int mode) {
-  return mode == MODE_RHO ? Cols<MODE_RHO>::PTS : mode == MODE_GRAD ? Cols<MODE_GRAD>::PTS : Cols<MODE_LAP>::PTS;
+template <int V, int DT = 0>
+void launch_var(const u4* P, int mp, const float* ao, int64_t ld, double* part, hipStream_t s) {
+  using C = Cols<V>;
+  const int nrt = mp / TR;
+  const int64_t nptile = ld / C::PTS;  // ld is a multiple of 256
+  const dim3 block(256), grid((unsigned)(((nptile + 7) / 8) * 8 * nrt));
+  hipLaunchKernelGGL((contract_mixed_kernel<V, DT>), grid, block, 0, s, P, mp, ao, ld, part, nrt, nptile);
+  CT_HIP_CHECK(hipGetLastError());
 }
+
+}  // namespace
 
 void mixed_fragment_order_P(const uint16_t* planes, int mp, uint16_t* out) {
   // out[((pl * mp/16 + b16) * nk + kb) * 64 + lane][e] = plane pl of
@@ -200,26 +223,25 @@ void mixed_fragment_order_P(const uint16_t* planes, int mp, uint16_t* out) {
 void launch_contract_mixed(int mode, const uint16_t* Pf, int nbasis_pad, const float* ao, int64_t ld, double* part,
                            hipStream_t s) {
   CT_REQUIRE(nbasis_pad % TR == 0, "padded basis size must be a multiple of 128");
-  const int ppb = mixed_points_per_block(mode);
-  const int nrt = nbasis_pad / TR;
-  const int64_t nptile = ld / ppb;
-  const dim3 block(256), grid((unsigned)(((nptile + 7) / 8) * 8 * nrt));
+  CT_REQUIRE(ld % 256 == 0, "point stride must be a multiple of 256");
   const u4* P = reinterpret_cast<const u4*>(Pf);
   switch (mode) {
     case MODE_RHO:
-      hipLaunchKernelGGL(contract_mixed_kernel<MODE_RHO>, grid, block, 0, s, P, nbasis_pad, ao, ld, part, nrt, nptile);
+      launch_var<V_RHO>(P, nbasis_pad, ao, ld, part, s);
       break;
     case MODE_GRAD:
-      hipLaunchKernelGGL(contract_mixed_kernel<MODE_GRAD>, grid, block, 0, s, P, nbasis_pad, ao, ld, part, nrt, nptile);
+      launch_var<V_GRAD>(P, nbasis_pad, ao, ld, part, s);
       break;
     case MODE_LAP:
-      hipLaunchKernelGGL(contract_mixed_kernel<MODE_LAP>, grid, block, 0, s, P, nbasis_pad, ao, ld, part, nrt, nptile);
+      launch_var<V_LAPPHI>(P, nbasis_pad, ao, ld, part, s);
+      launch_var<V_LAPDER, 1>(P, nbasis_pad, ao, ld, part, s);
+      launch_var<V_LAPDER, 2>(P, nbasis_pad, ao, ld, part, s);
+      launch_var<V_LAPDER, 3>(P, nbasis_pad, ao, ld, part, s);
       break;
     default:
-      hipLaunchKernelGGL(contract_mixed_kernel<MODE_HESS>, grid, block, 0, s, P, nbasis_pad, ao, ld, part, nrt, nptile);
+      launch_var<V_HESS>(P, nbasis_pad, ao, ld, part, s);
       break;
   }
-  CT_HIP_CHECK(hipGetLastError());
 }
 
 }  // namespace ct

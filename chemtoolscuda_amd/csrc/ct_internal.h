@@ -128,7 +128,6 @@ void launch_contract_f64_split(int mode, const double* P, int nbasis_pad, const 
 void launch_ao_f32(int mode, int lmax, const ShellMeta* meta, const ShellGeom* geom, const PrimData* prim, int nshell,
                    int nbasis, int nbasis_pad, const double* pts, int64_t npts, int64_t ld, float* ao, double screen,
                    hipStream_t s);
-int mixed_points_per_block(int mode);
 // Reorders three row-major bf16 planes of P ([3][mp][mp]) into the MFMA
 // fragment order the mixed contraction loads (host side, same size).
 void mixed_fragment_order_P(const uint16_t* planes, int mp, uint16_t* out);

@@ -92,15 +92,15 @@ struct YTile {
 // column tid % TC; its row quad k lies at bsrc + k * ld * 4.  Bs: the
 // workgroup's LDS.  Ends with a barrier (Bs free for reuse).
 //
-// SYM (symmetric P, columns 32..TC-1 used only in quadratic forms
-// x^T P x): by sum_IJ x_I^T P_IJ x_J = sum_I x_I^T (P_II x_I + 2 sum_{J>I}
-// P_IJ x_J), those columns need only the k-steps from k_lo (the row block's
-// diagonal block) on, with their B values doubled from k_dbl (right of the
-// diagonal block; exact in bf16).  bsym: this thread's column is one of them.
-template <int TC, int NBC, bool SYM, bool F64>
+// K runs over the k-steps k_begin .. mp/32 - 1, the B values doubled (exactly)
+// from k-step k_dbl on.  With k_begin = the row block's own diagonal block and
+// k_dbl = the block after it this gives x_I^T (P_II x_I + 2 sum_{J>I} P_IJ x_J),
+// whose sum over the row blocks I is the quadratic form x^T P x of a
+// symmetric P with 0.6 of the work (five row blocks).
+template <int TC, int NBC, bool F64>
 __device__ __forceinline__ void y_tile(const u4* __restrict__ Pf, int mp, int64_t ld, int i0, int tid,
                                        const float* __restrict__ bsrc, uint16_t (&Bs)[2][NPL][TC][LROW],
-                                       YTile<TC, NBC, F64>& t, bool bsym = false, int k_lo = 0, int k_dbl = 0) {
+                                       YTile<TC, NBC, F64>& t, int k_begin, int k_dbl) {
   constexpr int NCB = TC / 16;
   const int lane = tid & 63;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -125,7 +125,6 @@ __device__ __forceinline__ void y_tile(const u4* __restrict__ Pf, int mp, int64_
   const int bc = tid & (TC - 1), bo = tid / TC;
   f4 rbuf[NU][2];
   auto load_b = [&](int kb) {
-    if (SYM && bsym && kb < k_lo) return;
 #pragma unroll
     for (int u = 0; u < NU; ++u)
 #pragma unroll
@@ -134,8 +133,7 @@ __device__ __forceinline__ void y_tile(const u4* __restrict__ Pf, int mp, int64_
             *reinterpret_cast<const f4*>(bsrc + (int64_t)(8 * kb + 2 * (bo + (256 / TC) * u) + h) * ld * 4);
   };
   auto store_b = [&](int s, int kb) {
-    if (SYM && bsym && kb < k_lo) return;
-    const float sc = (SYM && bsym && kb >= k_dbl) ? 2.0f : 1.0f;
+    const float sc = kb >= k_dbl ? 2.0f : 1.0f;
 #pragma unroll
     for (int u = 0; u < NU; ++u) {
       u4 vh, vm, vl;
@@ -171,11 +169,9 @@ __device__ __forceinline__ void y_tile(const u4* __restrict__ Pf, int mp, int64_
   const u4 absmask = u4{0x7FFF7FFFu, 0x7FFF7FFFu, 0x7FFF7FFFu, 0x7FFF7FFFu};
   // Column blocks in pairs: 2 x NRB independent accumulators per product, so
   // dependent MFMAs are four issues apart.
-  auto compute = [&](int s, int kb) {
-    const bool phi_only = SYM && kb < k_lo;  // wave-uniform
+  auto compute = [&](int s) {
 #pragma unroll
     for (int cp = 0; cp < NCB / 2; ++cp) {
-      if (cp > 0 && phi_only) break;
       bf16x8 bf[2][NPL];
 #pragma unroll
       for (int c = 0; c < 2; ++c)
@@ -236,19 +232,22 @@ __device__ __forceinline__ void y_tile(const u4* __restrict__ Pf, int mp, int64_
     }
   };
 
-  load_b(0);
-  load_a(0);
-  store_b(0, 0);
-  load_b(1);
+  // The loads run unconditionally, clamped to the last k-step on the final
+  // iterations (those values are unused; the LDS buffer written then is the
+  // idle one): conditional loads would keep two copies of the B registers
+  // alive across the k-step.
+  load_b(k_begin);
+  load_a(k_begin);
+  store_b(0, k_begin);
+  load_b(k_begin + 1 < nk ? k_begin + 1 : k_begin);
   __syncthreads();
-  for (int kb = 0; kb < nk; ++kb) {
-    const int s = kb & 1;
-    compute(s, kb);
-    if (kb + 1 < nk) {
-      load_a(kb + 1);
-      store_b(s ^ 1, kb + 1);
-    }
-    if (kb + 2 < nk) load_b(kb + 2);
+  for (int kb = k_begin; kb < nk; ++kb) {
+    const int s = (kb - k_begin) & 1;
+    compute(s);
+    const int k1 = kb + 1 < nk ? kb + 1 : kb;
+    load_a(k1);
+    store_b(s ^ 1, k1);
+    load_b(kb + 2 < nk ? kb + 2 : nk - 1);
     __syncthreads();
   }
 }
