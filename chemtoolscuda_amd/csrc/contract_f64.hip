@@ -77,18 +77,21 @@ __global__ __launch_bounds__(256) void contract_f64_kernel(const double* __restr
 #pragma unroll
       for (int cb = 0; cb < 4; ++cb) acc[rb][cb] = d4{0.0, 0.0, 0.0, 0.0};
 
+    // Quadratic forms x^T P x need only the diagonal-and-right blocks: by the
+    // symmetry of P, sum_IJ x_I^T P_IJ x_J = sum_I x_I^T (P_II x_I +
+    // 2 sum_{J>I} P_IJ x_J), so those columns use k >= i0 only, with their
+    // values doubled (exactly) right of the diagonal block.  Rho mode: every
+    // column (rho = phi^T P phi).  Lap mode: the derivative columns (cb 1..3),
+    // which enter only tau_kk = d_k phi^T P d_k phi.
+    constexpr bool TRI_ALL = MODE == MODE_RHO;
+    const int kstart = TRI_ALL ? i0 : 0;
     const double* aptr = P + (int64_t)q * mp + i0 + r;
     double a[4], b[4];
 #pragma unroll
-    for (int rb = 0; rb < 4; ++rb) a[rb] = aptr[16 * rb];
+    for (int rb = 0; rb < 4; ++rb) a[rb] = aptr[(int64_t)kstart * mp + 16 * rb];
 #pragma unroll
-    for (int cb = 0; cb < 4; ++cb) b[cb] = bptr[cb][0];
+    for (int cb = 0; cb < 4; ++cb) b[cb] = bptr[cb][(int64_t)kstart * ld];
 
-    // Lap mode: the derivative columns (cb 1..3) enter only tau_kk =
-    // d_k phi^T P d_k phi, a quadratic form; by the symmetry of P,
-    // sum_IJ x_I^T P_IJ x_J = sum_I x_I^T (P_II x_I + 2 sum_{J>I} P_IJ x_J),
-    // so they need only k >= i0, with the values doubled (exactly) right of
-    // the diagonal block.
     auto kstep = [&](int k0, bool all_cols, bool dbl) {
       // prefetch the next k-step (clamped on the last step; the value is unused)
       const int kn = (k0 + 4 < mp) ? k0 + 4 : k0;
@@ -103,15 +106,15 @@ __global__ __launch_bounds__(256) void contract_f64_kernel(const double* __restr
 #pragma unroll
         for (int cb = 0; cb < 4; ++cb)
           if (cb == 0 || all_cols)
-            acc[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb], (cb > 0 && dbl) ? 2.0 * b[cb] : b[cb],
-                                                               acc[rb][cb], 0, 0, 0);
+            acc[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                a[rb], ((TRI_ALL || cb > 0) && dbl) ? 2.0 * b[cb] : b[cb], acc[rb][cb], 0, 0, 0);
 #pragma unroll
       for (int rb = 0; rb < 4; ++rb) a[rb] = an[rb];
 #pragma unroll
       for (int cb = 0; cb < 4; ++cb) b[cb] = bn[cb];
     };
-    if constexpr (MODE == MODE_LAP) {
-      for (int k0 = 0; k0 < i0; k0 += 4) kstep(k0, false, false);
+    if constexpr (MODE == MODE_LAP || TRI_ALL) {
+      for (int k0 = kstart; k0 < i0; k0 += 4) kstep(k0, false, false);  // lap: the phi column left of i0
       for (int k0 = i0; k0 < i0 + kIBlock; k0 += 4) kstep(k0, true, false);
       for (int k0 = i0 + kIBlock; k0 < mp; k0 += 4) kstep(k0, true, true);
     } else {

@@ -8,12 +8,10 @@
 // for FP64 re-evaluation.
 //
 // Decomposition: one workgroup owns a 128-row block of P and TC GEMM
-// columns: 32 points x {phi, d_x phi, d_y phi, d_z phi} (lap / Hessian
-// modes, TC = 128) or 64 points of phi (rho / grad modes).  The epilogue dots
-// each wave's rows with the matching AO row quads (one 16-byte load per
-// quad) in FP64, reduces over lane groups and the four waves in a fixed
-// order, and writes one partial per (row block, dot, point); the finaliser
-// sums the row blocks.
+// columns (the variants below).  The epilogue dots each wave's rows with the
+// matching AO row quads (one 16-byte load per quad) in FP64, reduces over
+// lane groups and the four waves in a fixed order, and writes one partial per
+// (row block, dot, point); the finaliser sums the row blocks.
 
 #include <type_traits>
 
@@ -26,12 +24,13 @@ namespace {
 using namespace mixed;
 
 // Tile variants.  A workgroup owns a 128-row block of P and TC columns:
-//   V_RHO, V_GRAD: 64 points of phi;
+//   V_RHO:         64 points of phi, triangle of P only (y_tile's TRI: rho and
+//                  the bound are quadratic forms);
+//   V_GRAD:        64 points of phi, full K;
 //   V_LAPPHI:      64 points of phi, full K -- the lap mode's rho, grad and
 //                  lap phi . P phi dots (and the bound);
-//   V_LAPDER<d>:   128 points of one derivative column type d (1..3), only
-//                  the diagonal-and-right blocks of P (the symmetric shortcut
-//                  of y_tile) -- the lap mode's tau_dd = d_d phi^T P d_d phi;
+//   V_LAPDER<d>:   128 points of one derivative column type d (1..3), triangle
+//                  of P only -- the lap mode's tau_dd = d_d phi^T P d_d phi;
 //   V_HESS:        32 points x {phi, d_x phi, d_y phi, d_z phi}, full K (the
 //                  t_kl with k != l are not quadratic forms).
 // The two lap variants are two launches writing disjoint dot slots of the
@@ -47,6 +46,7 @@ struct Cols {
   static constexpr int PTS = TC / NT;                     // points per workgroup
   static constexpr int NBC = V == V_LAPDER ? 0 : V == V_HESS ? 2 : NCB;  // column blocks with the bound
   static constexpr bool F64 = V != V_HESS;  // FP64 running sums (the Hessian tile lacks the registers)
+  static constexpr bool TRI = V == V_RHO || V == V_LAPDER;  // quadratic forms only: triangle of P
   // column type (0 = phi, 1..3 = d_x, d_y, d_z phi) of column c; dtype: the
   // derivative type of a V_LAPDER workgroup
   __host__ __device__ static constexpr int type(int c, int dtype) { return NT == 4 ? c / PTS : V == V_LAPDER ? dtype : 0; }
@@ -108,16 +108,12 @@ __global__ __launch_bounds__(256, 2) void contract_mixed_kernel(const u4* __rest
   if (ptile >= nptile) return;
   const int64_t pbase = ptile * C::PTS;
   const int i0 = rt * TR;
-  const int nk = mp / KB;
   const int64_t mq = mp / 4;  // row quads per AO array
 
   YTile<TC, NBC, C::F64> yt;
   const int bc = tid & (TC - 1);
   const float* bsrc = ao + ((int64_t)C::type(bc, dtype) * mq * ld + pbase + C::point(bc)) * 4;
-  if constexpr (V == V_LAPDER)
-    y_tile<TC, NBC, C::F64>(Pf, mp, ld, i0, tid, bsrc, Bs, yt, i0 / KB, (i0 + TR) / KB);
-  else
-    y_tile<TC, NBC, C::F64>(Pf, mp, ld, i0, tid, bsrc, Bs, yt, 0, nk);
+  y_tile<TC, NBC, C::F64, C::TRI>(Pf, mp, ld, i0, tid, bsrc, Bs, yt, C::TRI ? i0 / KB : 0);
 
   // ---------------------------------------------------------------- epilogue
   double* red = reinterpret_cast<double*>(&Bs[0][0][0][0]);  // [NW][TC][kMaxDots]

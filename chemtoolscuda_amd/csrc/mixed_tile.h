@@ -92,15 +92,20 @@ struct YTile {
 // column tid % TC; its row quad k lies at bsrc + k * ld * 4.  Bs: the
 // workgroup's LDS.  Ends with a barrier (Bs free for reuse).
 //
-// K runs over the k-steps k_begin .. mp/32 - 1, the B values doubled (exactly)
-// from k-step k_dbl on.  With k_begin = the row block's own diagonal block and
-// k_dbl = the block after it this gives x_I^T (P_II x_I + 2 sum_{J>I} P_IJ x_J),
-// whose sum over the row blocks I is the quadratic form x^T P x of a
-// symmetric P with 0.6 of the work (five row blocks).
-template <int TC, int NBC, bool F64>
+// K runs over the k-steps k_begin .. mp/32 - 1.  TRI (symmetric P, columns
+// used only in quadratic forms x^T P x): k_begin is the row block's diagonal
+// k-step and wave w, whose 32 rows are k-step kd = k_begin + w, uses the
+// k-steps from kd on, adding the products of k-steps right of kd twice.  By
+// x^T P x = sum_I x_I^T (P_II x_I + 2 sum_{J>I} P_IJ x_J) over 32-row blocks I,
+// summing the tile's dots over all row blocks gives the full quadratic form
+// for about half the work.  The bound product is treated the same way
+// (|x|^T |P| |x| is a quadratic form too), its doubling applied exactly to
+// the |P_hi| operand (bf16 exponent + 1).
+template <int TC, int NBC, bool F64, bool TRI>
 __device__ __forceinline__ void y_tile(const u4* __restrict__ Pf, int mp, int64_t ld, int i0, int tid,
                                        const float* __restrict__ bsrc, uint16_t (&Bs)[2][NPL][TC][LROW],
-                                       YTile<TC, NBC, F64>& t, int k_begin, int k_dbl) {
+                                       YTile<TC, NBC, F64>& t, int k_begin) {
+  static_assert(!TRI || F64, "the triangle needs the FP64 running sums");
   constexpr int NCB = TC / 16;
   const int lane = tid & 63;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -132,14 +137,13 @@ __device__ __forceinline__ void y_tile(const u4* __restrict__ Pf, int mp, int64_
         rbuf[u][h] =
             *reinterpret_cast<const f4*>(bsrc + (int64_t)(8 * kb + 2 * (bo + (256 / TC) * u) + h) * ld * 4);
   };
-  auto store_b = [&](int s, int kb) {
-    const float sc = kb >= k_dbl ? 2.0f : 1.0f;
+  auto store_b = [&](int s) {
 #pragma unroll
     for (int u = 0; u < NU; ++u) {
       u4 vh, vm, vl;
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const Split2 sp = split2(sc * rbuf[u][e >> 1][2 * (e & 1)], sc * rbuf[u][e >> 1][2 * (e & 1) + 1]);
+        const Split2 sp = split2(rbuf[u][e >> 1][2 * (e & 1)], rbuf[u][e >> 1][2 * (e & 1) + 1]);
         vh[e] = sp.h;
         vm[e] = sp.m;
         vl[e] = sp.l;
@@ -167,9 +171,11 @@ __device__ __forceinline__ void y_tile(const u4* __restrict__ Pf, int mp, int64_
   }
 
   const u4 absmask = u4{0x7FFF7FFFu, 0x7FFF7FFFu, 0x7FFF7FFFu, 0x7FFF7FFFu};
+  const u4 exp1 = u4{0x00800080u, 0x00800080u, 0x00800080u, 0x00800080u};  // +1 in both bf16 exponents
   // Column blocks in pairs: 2 x NRB independent accumulators per product, so
-  // dependent MFMAs are four issues apart.
-  auto compute = [&](int s) {
+  // dependent MFMAs are four issues apart.  dbl (TRI only): add the products
+  // twice.
+  auto compute = [&](int s, bool dbl) {
 #pragma unroll
     for (int cp = 0; cp < NCB / 2; ++cp) {
       bf16x8 bf[2][NPL];
@@ -194,12 +200,13 @@ __device__ __forceinline__ void y_tile(const u4* __restrict__ Pf, int mp, int64_
             for (int c = 0; c < 2; ++c)
               tc[rb][c] =
                   __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(areg[pa[pr]][rb]), bf[c][pb[pr]], tc[rb][c], 0, 0, 0);
+        const double sc = (TRI && dbl) ? 2.0 : 1.0;
 #pragma unroll
         for (int rb = 0; rb < NRB; ++rb)
 #pragma unroll
           for (int c = 0; c < 2; ++c)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) t.yd[rb][2 * cp + c][j] += (double)tc[rb][c][j];
+            for (int j = 0; j < 4; ++j) t.yd[rb][2 * cp + c][j] = fma((double)tc[rb][c][j], sc, t.yd[rb][2 * cp + c][j]);
       } else {
         constexpr int pa[5] = {2, 1, 0, 1, 0};
         constexpr int pb[5] = {0, 1, 2, 0, 1};
@@ -221,12 +228,14 @@ __device__ __forceinline__ void y_tile(const u4* __restrict__ Pf, int mp, int64_
       if constexpr (NBC > 0) {
         if (2 * cp < NBC) {
 #pragma unroll
-          for (int rb = 0; rb < NRB; ++rb)
+          for (int rb = 0; rb < NRB; ++rb) {
+            const u4 pabs = (TRI && dbl) ? (areg[0][rb] & absmask) + exp1 : areg[0][rb] & absmask;
 #pragma unroll
             for (int c = 0; c < 2; ++c)
               t.bound[rb][(2 * cp + c) % NBC] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                  as_bf(areg[0][rb] & absmask), as_bf(__builtin_bit_cast(u4, bf[c][0]) & absmask),
-                  t.bound[rb][(2 * cp + c) % NBC], 0, 0, 0);
+                  as_bf(pabs), as_bf(__builtin_bit_cast(u4, bf[c][0]) & absmask), t.bound[rb][(2 * cp + c) % NBC], 0,
+                  0, 0);
+          }
         }
       }
     }
@@ -236,17 +245,18 @@ __device__ __forceinline__ void y_tile(const u4* __restrict__ Pf, int mp, int64_
   // iterations (those values are unused; the LDS buffer written then is the
   // idle one): conditional loads would keep two copies of the B registers
   // alive across the k-step.
+  const int kd = k_begin + w;  // TRI: this wave's diagonal k-step (wave-uniform)
   load_b(k_begin);
-  load_a(k_begin);
-  store_b(0, k_begin);
+  load_a(TRI ? kd : k_begin);
+  store_b(0);
   load_b(k_begin + 1 < nk ? k_begin + 1 : k_begin);
   __syncthreads();
   for (int kb = k_begin; kb < nk; ++kb) {
     const int s = (kb - k_begin) & 1;
-    compute(s);
+    if (!TRI || kb >= kd) compute(s, kb > kd);
     const int k1 = kb + 1 < nk ? kb + 1 : kb;
-    load_a(k1);
-    store_b(s ^ 1, k1);
+    if (!TRI || k1 > kd) load_a(k1);  // A(kd) is loaded before the loop
+    store_b(s ^ 1);
     load_b(kb + 2 < nk ? kb + 2 : nk - 1);
     __syncthreads();
   }

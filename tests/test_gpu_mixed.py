@@ -158,6 +158,12 @@ def test_mixed_error_model_pointwise(ct, workload, npts):
         a = np.asarray(raw[k]).reshape(len(pts), -1)
         b = np.asarray(f64[k]).reshape(len(pts), -1)
         ratios[k] = float(np.max(np.linalg.norm(a - b, axis=1) / (u * sc)))
+    # density-only mode: its own tile (triangle of P, doubled right of each
+    # wave's diagonal block); the FP64 density-only path uses the same shortcut
+    raw_r = mol.compute(pts, ["rho"], precision="mixed", mixed_tol=1e30)["rho"]
+    f64_r = mol.compute(pts, ["rho"], precision="fp64")["rho"]
+    ratios["rho (rho-only mode)"] = float(np.max(np.abs(raw_r - f64_r) / (u * scales[0])))
+    assert np.max(np.abs(f64_r - f64["rho"]) / scales[0]) < 1e-14
     print(f"{workload} (M={w.basis.nbasis}): max error / (u * condition scale): "
           + ", ".join(f"{k} {v:.2f}" for k, v in ratios.items()))
     for k, v in ratios.items():
