@@ -48,4 +48,20 @@ def molecule_from_basis(basis: BasisSet, P, charges=None, numbers=None) -> "Mole
                     np.asarray(numbers, dtype=np.int32))
 
 
-__all__ = ["Molecule", "BasisSet", "Shell", "n_functions", "molecule_from_basis", "LIBRARY_PATH"]
+def read_wavefunction(path: str) -> dict:
+    """Host-side contents of a .fchk / .wfn / .wfx / Molden file as the native
+    readers produce them (shell table, P, coordinates, charges, numbers)."""
+    from .capi import read_wavefunction as _read
+
+    return _read(path)
+
+
+def basis_from_wavefunction(w: dict) -> BasisSet:
+    """``BasisSet`` of a ``read_wavefunction`` result."""
+    shells = [Shell(int(a), int(l), w["exps"][o:o + n], w["coefs"][o:o + n])
+              for a, l, n, o in zip(w["shell_atom"], w["shell_l"], w["shell_nprim"], w["shell_prim_off"])]
+    return BasisSet(shells, w["atom_xyz"], w["numbers"])
+
+
+__all__ = ["Molecule", "BasisSet", "Shell", "n_functions", "molecule_from_basis", "read_wavefunction",
+           "basis_from_wavefunction", "LIBRARY_PATH"]

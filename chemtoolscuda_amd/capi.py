@@ -62,6 +62,11 @@ def lib():
         vp, i32, i64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64
         L.ct_mol_create.argtypes = [ctypes.POINTER(BasisDesc), vp, vp, i32, vp, i32, ctypes.POINTER(vp)]
         L.ct_mol_create_from_fchk.argtypes = [ctypes.c_char_p, ctypes.POINTER(vp)]
+        L.ct_mol_create_from_file.argtypes = [ctypes.c_char_p, ctypes.POINTER(vp)]
+        L.ct_wavefunction_read.argtypes = [ctypes.c_char_p, ctypes.POINTER(vp)]
+        L.ct_wavefunction_sizes.argtypes = [vp, vp, vp, vp, vp]
+        L.ct_wavefunction_copy.argtypes = [vp] * 11
+        L.ct_wavefunction_destroy.argtypes = [vp]
         L.ct_mol_destroy.argtypes = [vp]
         L.ct_eval.argtypes = [vp, vp, i64, ctypes.c_uint32, i32, ctypes.POINTER(EvalParams), ctypes.POINTER(Outputs),
                               ctypes.POINTER(EvalReport)]
@@ -153,3 +158,31 @@ class Handle:
         st = KernelStats()
         check(lib().ct_mol_kernel_stats(self.ptr, ctypes.byref(st)))
         return {k: getattr(st, k) for k, _ in KernelStats._fields_}
+
+
+def read_wavefunction(path: str) -> dict:
+    """Reads a .fchk / .wfn / .wfx / Molden file with the library's native
+    readers, on the host only (no device work): the shell table (the
+    ``BasisSet.flat()`` layout), P, coordinates, nuclear charges and atomic
+    numbers -- exactly what ``Molecule(path)`` uploads."""
+    import numpy as np
+
+    L = lib()
+    h = ctypes.c_void_p()
+    check(L.ct_wavefunction_read(os.fsencode(path), ctypes.byref(h)))
+    try:
+        n = [ctypes.c_int32() for _ in range(4)]
+        check(L.ct_wavefunction_sizes(h, *[ctypes.byref(v) for v in n]))
+        nshell, nprim, nbasis, natom = (v.value for v in n)
+        out = {
+            "shell_atom": np.zeros(nshell, np.int32), "shell_l": np.zeros(nshell, np.int32),
+            "shell_nprim": np.zeros(nshell, np.int32), "shell_prim_off": np.zeros(nshell, np.int32),
+            "exps": np.zeros(nprim), "coefs": np.zeros(nprim), "P": np.zeros((nbasis, nbasis)),
+            "atom_xyz": np.zeros((natom, 3)), "charges": np.zeros(natom), "numbers": np.zeros(natom, np.int32),
+        }
+        keys = ["shell_atom", "shell_l", "shell_nprim", "shell_prim_off", "exps", "coefs", "P", "atom_xyz", "charges",
+                "numbers"]
+        check(L.ct_wavefunction_copy(h, *[out[k].ctypes.data for k in keys]))
+    finally:
+        L.ct_wavefunction_destroy(h)
+    return out

@@ -4,6 +4,7 @@
 // cudaError_t throw (include/cuda_utils.cuh:55-60, seen in Python as
 // "Caught an unknown exception!").
 
+#include <algorithm>
 #include <cstring>
 #include <string>
 
@@ -88,11 +89,13 @@ int ct_mol_create(const ct_basis_desc* basis, const double* atom_xyz, const doub
   });
 }
 
-int ct_mol_create_from_fchk(const char* path, ct_mol** out) {
+namespace {
+
+int create_from(ct::FchkData (*load)(const std::string&), const char* path, ct_mol** out) {
   return guard([&] {
     CT_REQUIRE(out != nullptr && path != nullptr, "null argument");
     *out = nullptr;
-    ct::FchkData d = ct::load_fchk(path);
+    ct::FchkData d = load(path);
     auto* h = new ct_mol{nullptr};
     try {
       h->impl = new ct::Mol(d.basis, d.atom_xyz.data(), d.charges.data(), d.numbers.data(),
@@ -102,6 +105,64 @@ int ct_mol_create_from_fchk(const char* path, ct_mol** out) {
       throw;
     }
     *out = h;
+  });
+}
+
+}  // namespace
+
+int ct_mol_create_from_fchk(const char* path, ct_mol** out) { return create_from(ct::load_fchk, path, out); }
+
+int ct_mol_create_from_file(const char* path, ct_mol** out) { return create_from(ct::load_wavefunction, path, out); }
+
+struct ct_wavefunction {
+  ct::FchkData d;
+};
+
+int ct_wavefunction_read(const char* path, ct_wavefunction** out) {
+  return guard([&] {
+    CT_REQUIRE(out != nullptr && path != nullptr, "null argument");
+    *out = nullptr;
+    auto* w = new ct_wavefunction{ct::load_wavefunction(path)};
+    *out = w;
+  });
+}
+
+int ct_wavefunction_destroy(ct_wavefunction* wfn) {
+  return guard([&] { delete wfn; });
+}
+
+int ct_wavefunction_sizes(const ct_wavefunction* wfn, int32_t* nshell, int32_t* nprim, int32_t* nbasis,
+                          int32_t* natom) {
+  return guard([&] {
+    CT_REQUIRE(wfn != nullptr, "null wavefunction");
+    const auto& d = wfn->d;
+    if (nshell) *nshell = (int32_t)d.basis.shell_l.size();
+    if (nprim) *nprim = (int32_t)d.basis.exps.size();
+    if (nbasis) *nbasis = (int32_t)d.basis.nbasis;
+    if (natom) *natom = (int32_t)d.numbers.size();
+  });
+}
+
+int ct_wavefunction_copy(const ct_wavefunction* wfn, int32_t* shell_atom, int32_t* shell_l, int32_t* shell_nprim,
+                         int32_t* shell_prim_off, double* exps, double* coefs, double* P, double* atom_xyz,
+                         double* charges, int32_t* numbers) {
+  return guard([&] {
+    CT_REQUIRE(wfn != nullptr, "null wavefunction");
+    const auto& d = wfn->d;
+    const auto& b = d.basis;
+    auto put = [](auto* dst, const auto& src) {
+      if (dst) std::copy(src.begin(), src.end(), dst);
+    };
+    put(shell_atom, b.shell_atom);
+    put(shell_l, b.shell_l);
+    put(shell_nprim, b.shell_nprim);
+    put(shell_prim_off, b.shell_prim_off);
+    put(exps, b.exps);
+    put(coefs, b.coefs);
+    put(P, d.P);
+    put(atom_xyz, d.atom_xyz);
+    put(charges, d.charges);
+    put(numbers, d.numbers);
   });
 }
 

@@ -24,6 +24,12 @@ struct FchkData {
 
 // Native reader for Gaussian formatted checkpoint files (fchk.cpp).
 FchkData load_fchk(const std::string& path);
+// AIM .wfn / .wfx and Molden readers (wavefunction.cpp), and the dispatch on
+// the file extension that also covers .fchk.
+FchkData load_wfn(const std::string& path);
+FchkData load_wfx(const std::string& path);
+FchkData load_molden(const std::string& path);
+FchkData load_wavefunction(const std::string& path);
 
 class DeviceBuffer {
  public:

@@ -45,7 +45,7 @@ int64_t n_points(const Points& pts) {
 
 class Molecule {
  public:
-  explicit Molecule(const std::string& path) : path_(path) { check(ct_mol_create_from_fchk(path.c_str(), &h_)); }
+  explicit Molecule(const std::string& path) : path_(path) { check(ct_mol_create_from_file(path.c_str(), &h_)); }
 
   Molecule(py::array_t<int32_t, py::array::c_style | py::array::forcecast> shell_atom,
            py::array_t<int32_t, py::array::c_style | py::array::forcecast> shell_l,

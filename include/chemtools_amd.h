@@ -17,6 +17,9 @@
  *   ct_mol_create            chemtools::Molecule::Molecule + IOData
  *                            (src/pymolecule.cu:18-23, include/iodata.h:18-101)
  *   ct_mol_create_from_fchk  get_molecular_basis_from_fchk (src/iodata.cpp:15-211)
+ *   ct_mol_create_from_file  the same, for every format its iodata.load_one call
+ *                            (src/iodata.cpp:17-20) reads that this library supports:
+ *                            .fchk, AIM .wfn / .wfx, Molden
  *   ct_eval                  evaluate_electron_density_on_any_grid (src/evaluate_density.cu:663-840),
  *                            evaluate_electron_density_gradient (src/evaluate_gradient.cu:1184-1371),
  *                            evaluate_electron_density_hessian (src/evaluate_hessian.cu:1893-2307),
@@ -120,6 +123,23 @@ typedef struct {
 int ct_mol_create(const ct_basis_desc* basis, const double* atom_xyz, const double* charges, int32_t natom,
                   const double* P, int32_t nbasis, ct_mol** out);
 int ct_mol_create_from_fchk(const char* path, ct_mol** out);
+/* Format from the extension: .fchk/.fch, .wfn, .wfx, .molden/.molden.input;
+ * CT_E_UNSUPPORTED for other files. */
+int ct_mol_create_from_file(const char* path, ct_mol** out);
+
+/* Host-only access to what a file holds (no device work): the shell table in
+ * ct_basis_desc conventions, P (nbasis x nbasis, row-major), coordinates
+ * (bohr), nuclear charges and atomic numbers.  Read, query the sizes, copy
+ * into caller arrays (any pointer may be NULL), destroy.  Same formats and
+ * reference interface as ct_mol_create_from_file. */
+typedef struct ct_wavefunction ct_wavefunction;
+int ct_wavefunction_read(const char* path, ct_wavefunction** out);
+int ct_wavefunction_sizes(const ct_wavefunction* wfn, int32_t* nshell, int32_t* nprim, int32_t* nbasis,
+                          int32_t* natom);
+int ct_wavefunction_copy(const ct_wavefunction* wfn, int32_t* shell_atom, int32_t* shell_l, int32_t* shell_nprim,
+                         int32_t* shell_prim_off, double* exps, double* coefs, double* P, double* atom_xyz,
+                         double* charges, int32_t* numbers);
+int ct_wavefunction_destroy(ct_wavefunction* wfn);
 int ct_mol_destroy(ct_mol* mol);
 
 int ct_mol_nbasis(const ct_mol* mol);
