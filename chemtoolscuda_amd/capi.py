@@ -29,7 +29,8 @@ class BasisDesc(ctypes.Structure):
 
 class EvalParams(ctypes.Structure):
     _fields_ = [("gge_a", ctypes.c_double), ("gge_b", ctypes.c_double), ("ked_a", ctypes.c_double),
-                ("ked_alpha", ctypes.c_double), ("mixed_tol", ctypes.c_double), ("rho_floor", ctypes.c_double)]
+                ("ked_alpha", ctypes.c_double), ("mixed_tol", ctypes.c_double), ("rho_floor", ctypes.c_double),
+                ("route", ctypes.c_int32)]
 
 
 class Outputs(ctypes.Structure):
@@ -38,7 +39,7 @@ class Outputs(ctypes.Structure):
 
 class EvalReport(ctypes.Structure):
     _fields_ = [("n_points", ctypes.c_int64), ("n_reevaluated", ctypes.c_int64), ("n_chunks", ctypes.c_int64),
-                ("max_error_estimate", ctypes.c_double)]
+                ("max_error_estimate", ctypes.c_double), ("factor_rank", ctypes.c_int64)]
 
 
 class KernelStats(ctypes.Structure):

@@ -23,7 +23,8 @@ namespace {
 
 using namespace mixed;
 
-// Tile variants.  A workgroup owns a 128-row block of P and TC columns:
+// Tile variants.  A workgroup owns a 128-row block of A and TC columns.
+// Density route, A = P:
 //   V_RHO:         64 points of phi, triangle of P only (y_tile's TRI: rho and
 //                  the bound are quadratic forms);
 //   V_GRAD:        64 points of phi, full K;
@@ -33,24 +34,42 @@ using namespace mixed;
 //                  of P only -- the lap mode's tau_dd = d_d phi^T P d_d phi;
 //   V_HESS:        32 points x {phi, d_x phi, d_y phi, d_z phi}, full K (the
 //                  t_kl with k != l are not quadratic forms).
-// The two lap variants are two launches writing disjoint dot slots of the
-// same partials array.
-enum Var : int { V_RHO, V_GRAD, V_LAPPHI, V_LAPDER, V_HESS };
+// Factor route, A = L^T with P = L L^T, so that the tile holds psi = L^T phi
+// and its derivatives for 128 columns of L, and every dot is a lane-local
+// sum over those columns (phi^T P x = psi^T (L^T x)):
+//   V_F1:          64 points of phi: rho = psi . psi and the bound;
+//   V_F4<mode>:    32 points x {phi, d_x phi, d_y phi, d_z phi}: rho,
+//                  g_k = psi . d_k psi, the bound, and in the lap mode
+//                  t_kk = d_k psi . d_k psi;
+//   V_F2:          32 points x {phi, lap phi}: the lap mode's psi . lap psi.
+// The error bound of the factor route is 2 sum_j |psi_j| (|L|^T |phi|)_j,
+// the first-order propagation of the per-column bound of psi into
+// rho = sum_j psi_j^2.
+// Launches of one mode write disjoint dot slots of the same partials array.
+enum Var : int { V_RHO, V_GRAD, V_LAPPHI, V_LAPDER, V_HESS, V_F1, V_F4, V_F2 };
 
-template <int V>
+// DT: the derivative type of a V_LAPDER launch, the mode of a V_F4 launch.
+template <int V, int DT>
 struct Cols {
-  static constexpr int MODE = V == V_RHO ? MODE_RHO : V == V_GRAD ? MODE_GRAD : V == V_HESS ? MODE_HESS : MODE_LAP;
-  static constexpr int NT = V == V_HESS ? 4 : 1;          // column types per tile
-  static constexpr int TC = V == V_HESS || V == V_LAPDER ? 128 : 64;
+  static constexpr bool FAC = V >= V_F1;
+  static constexpr int MODE = V == V_RHO || V == V_F1 ? MODE_RHO
+                              : V == V_GRAD           ? MODE_GRAD
+                              : V == V_HESS           ? MODE_HESS
+                              : V == V_F4             ? DT
+                                                      : MODE_LAP;
+  static constexpr int NT = V == V_HESS || V == V_F4 ? 4 : V == V_F2 ? 2 : 1;  // column types per tile
+  static constexpr int TC = V == V_HESS || V == V_LAPDER || V == V_F4 ? 128 : 64;
   static constexpr int NCB = TC / 16;                     // 16-column MFMA blocks
   static constexpr int PTS = TC / NT;                     // points per workgroup
-  static constexpr int NBC = V == V_LAPDER ? 0 : V == V_HESS ? 2 : NCB;  // column blocks with the bound
+  static constexpr int NBC = V == V_LAPDER || V == V_F2 ? 0 : V == V_HESS || V == V_F4 ? 2 : NCB;  // blocks with the bound
   static constexpr bool F64 = V != V_HESS;  // FP64 running sums (the Hessian tile lacks the registers)
   static constexpr bool TRI = V == V_RHO || V == V_LAPDER;  // quadratic forms only: triangle of P
-  // column type (0 = phi, 1..3 = d_x, d_y, d_z phi) of column c; dtype: the
-  // derivative type of a V_LAPDER workgroup
-  __host__ __device__ static constexpr int type(int c, int dtype) { return NT == 4 ? c / PTS : V == V_LAPDER ? dtype : 0; }
-  __host__ __device__ static constexpr int point(int c) { return NT == 4 ? c % PTS : c; }
+  // AO array (0 = phi, 1..3 = d_x, d_y, d_z phi, 4 = lap phi in the lap mode)
+  // of column c
+  __host__ __device__ static constexpr int type(int c) {
+    return V == V_F2 ? (c < PTS ? 0 : 4) : NT == 4 ? c / PTS : V == V_LAPDER ? DT : 0;
+  }
+  __host__ __device__ static constexpr int point(int c) { return NT > 1 ? c % PTS : c; }
 };
 
 // Dots produced from a GEMM column of type t: how many, and for the k-th one
@@ -81,12 +100,78 @@ using IC = std::integral_constant<int, N>;
 
 constexpr int kMaxDots = 11;  // 10 Hessian-mode dots of the phi column + the bound
 
+// Epilogue of the factor route: per point, lane-local FP64 sums over the
+// tile's rows (columns of L) of products of the point's psi columns, then over
+// the four lane groups and the four waves in a fixed order; one partial per
+// (row block, dot, point).  Dots: V_F1 [rho, bound]; V_F4 [rho, gx, gy, gz,
+// (txx, tyy, tzz in the lap mode,) bound]; V_F2 [lap phi . P phi].
+template <int V, int DT, class Tile>
+__device__ __forceinline__ void factor_epilogue(const Tile& yt, uint16_t* lds, double* __restrict__ part,
+                                                int64_t ld, int rt, int64_t pbase, int tid) {
+  using C = Cols<V, DT>;
+  constexpr int ND = mode_ndot(C::MODE), ND1 = ND + 1;
+  constexpr int NPB = C::PTS / 16;  // 16-point column blocks per column type
+  constexpr bool LAP = C::MODE == MODE_LAP;
+  constexpr int NDF = V == V_F1 ? 2 : V == V_F2 ? 1 : (LAP ? 8 : 5);
+  const int lane = tid & 63, w = tid >> 6, r = lane & 15, q = lane >> 4;
+  double* red = reinterpret_cast<double*>(lds);  // [NW][PTS][NDF]
+#pragma unroll
+  for (int pb = 0; pb < NPB; ++pb) {
+    double d[NDF];
+#pragma unroll
+    for (int k = 0; k < NDF; ++k) d[k] = 0.0;
+#pragma unroll
+    for (int rb = 0; rb < NRB; ++rb) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const double y0 = yt.y(rb, pb, j);
+        if constexpr (V == V_F2) {
+          d[0] += y0 * yt.y(rb, NPB + pb, j);
+        } else {
+          d[0] += y0 * y0;
+          d[NDF - 1] += 2.0 * fabs(y0) * (double)yt.bound[rb][pb][j];
+          if constexpr (V == V_F4) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+              const double yk = yt.y(rb, (k + 1) * NPB + pb, j);
+              d[1 + k] += y0 * yk;
+              if constexpr (LAP) d[4 + k] += yk * yk;
+            }
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < NDF; ++k) {
+      double x = d[k];
+      x += __shfl_xor(x, 16);
+      x += __shfl_xor(x, 32);
+      d[k] = x;
+    }
+    if (q == 0) {
+#pragma unroll
+      for (int k = 0; k < NDF; ++k) red[((int64_t)w * C::PTS + 16 * pb + r) * NDF + k] = d[k];
+    }
+  }
+  __syncthreads();
+  if (tid < C::PTS) {
+#pragma unroll
+    for (int k = 0; k < NDF; ++k) {
+      double sum = 0.0;
+#pragma unroll
+      for (int ww = 0; ww < NW; ++ww) sum += red[((int64_t)ww * C::PTS + tid) * NDF + k];
+      const int slot_k = V == V_F2 ? 4 : k == NDF - 1 ? ND : (k < 4 ? k : k + 1);
+      part[((int64_t)rt * ND1 + slot_k) * ld + pbase + tid] = sum;
+    }
+  }
+}
+
 template <int V, int DT>
-__global__ __launch_bounds__(256, 2) void contract_mixed_kernel(const u4* __restrict__ Pf, int mp,
+__global__ __launch_bounds__(256, 2) void contract_mixed_kernel(const u4* __restrict__ Pf, int rows, int mp,
                                                                 const float* __restrict__ ao, int64_t ld,
                                                                 double* __restrict__ part, int nrt,
                                                                 int64_t nptile) {
-  using C = Cols<V>;
+  using C = Cols<V, DT>;
   constexpr int TC = C::TC, NCB = C::NCB, NBC = C::NBC;
   constexpr int ND = mode_ndot(C::MODE);
   constexpr int ND1 = ND + 1;
@@ -112,15 +197,19 @@ __global__ __launch_bounds__(256, 2) void contract_mixed_kernel(const u4* __rest
 
   YTile<TC, NBC, C::F64> yt;
   const int bc = tid & (TC - 1);
-  const float* bsrc = ao + ((int64_t)C::type(bc, dtype) * mq * ld + pbase + C::point(bc)) * 4;
-  y_tile<TC, NBC, C::F64, C::TRI>(Pf, mp, ld, i0, tid, bsrc, Bs, yt, C::TRI ? i0 / KB : 0);
+  const float* bsrc = ao + ((int64_t)C::type(bc) * mq * ld + pbase + C::point(bc)) * 4;
+  y_tile<TC, NBC, C::F64, C::TRI>(Pf, rows, mp, ld, i0, tid, bsrc, Bs, yt, C::TRI ? i0 / KB : 0);
+  if constexpr (C::FAC) {
+    factor_epilogue<V, DT>(yt, &Bs[0][0][0][0], part, ld, rt, pbase, tid);
+    return;
+  } else {
 
   // ---------------------------------------------------------------- epilogue
   double* red = reinterpret_cast<double*>(&Bs[0][0][0][0]);  // [NW][TC][kMaxDots]
   const int64_t quad0 = (i0 + RW * w) / 4 + q;
   auto per_cb = [&](auto cbtag) {
     constexpr int CB = decltype(cbtag)::value;
-    constexpr int tc = C::type(16 * CB, 0);  // compile-time type (0 for V_LAPDER)
+    constexpr int tc = V == V_LAPDER ? 0 : C::type(16 * CB);  // compile-time column type (0 for V_LAPDER)
     constexpr int nd = ndots_of<V>(tc);
     constexpr bool bnd = CB < NBC;
     constexpr int t = V == V_LAPDER ? dtype : tc;
@@ -171,7 +260,7 @@ __global__ __launch_bounds__(256, 2) void contract_mixed_kernel(const u4* __rest
   __syncthreads();
   if (tid < TC) {
     const int c = tid;
-    const int tc = C::type(c, 0);
+    const int tc = V == V_LAPDER ? 0 : C::type(c);
     const int t = V == V_LAPDER ? dtype : tc;
     const int nd = ndots_of<V>(tc);
     const bool has_bound = NBC > 0 && tc == 0;
@@ -188,32 +277,35 @@ __global__ __launch_bounds__(256, 2) void contract_mixed_kernel(const u4* __rest
       }
     }
   }
+  }  // density route
 }
 
+// A: rows x mp (P: rows = mp); the workgroups cover rows / 128 row blocks
+// times ld / PTS point tiles.
 template <int V, int DT = 0>
-void launch_var(const u4* P, int mp, const float* ao, int64_t ld, double* part, hipStream_t s) {
-  using C = Cols<V>;
-  const int nrt = mp / TR;
+void launch_var(const u4* A, int rows, int mp, const float* ao, int64_t ld, double* part, hipStream_t s) {
+  using C = Cols<V, DT>;
+  const int nrt = rows / TR;
   const int64_t nptile = ld / C::PTS;  // ld is a multiple of 256
   const dim3 block(256), grid((unsigned)(((nptile + 7) / 8) * 8 * nrt));
-  hipLaunchKernelGGL((contract_mixed_kernel<V, DT>), grid, block, 0, s, P, mp, ao, ld, part, nrt, nptile);
+  hipLaunchKernelGGL((contract_mixed_kernel<V, DT>), grid, block, 0, s, A, rows, mp, ao, ld, part, nrt, nptile);
   CT_HIP_CHECK(hipGetLastError());
 }
 
 }  // namespace
 
-void mixed_fragment_order_P(const uint16_t* planes, int mp, uint16_t* out) {
-  // out[((pl * mp/16 + b16) * nk + kb) * 64 + lane][e] = plane pl of
-  // P[16 b16 + lane % 16][32 kb + 8 (lane / 16) + e]
-  const int nk = mp / KB, nb = mp / 16;
-  const size_t plane = (size_t)mp * mp;
+void mixed_fragment_order(const uint16_t* planes, int rows, int cols, uint16_t* out) {
+  // out[((pl * rows/16 + b16) * nk + kb) * 64 + lane][e] = plane pl of
+  // A[16 b16 + lane % 16][32 kb + 8 (lane / 16) + e]
+  const int nk = cols / KB, nb = rows / 16;
+  const size_t plane = (size_t)rows * cols;
   size_t o = 0;
   for (int pl = 0; pl < NPL; ++pl)
     for (int b16 = 0; b16 < nb; ++b16)
       for (int kb = 0; kb < nk; ++kb)
         for (int l = 0; l < 64; ++l)
           for (int e = 0; e < 8; ++e)
-            out[o++] = planes[pl * plane + (size_t)(16 * b16 + (l & 15)) * mp + 32 * kb + 8 * (l >> 4) + e];
+            out[o++] = planes[pl * plane + (size_t)(16 * b16 + (l & 15)) * cols + 32 * kb + 8 * (l >> 4) + e];
 }
 
 void launch_contract_mixed(int mode, const uint16_t* Pf, int nbasis_pad, const float* ao, int64_t ld, double* part,
@@ -221,22 +313,44 @@ void launch_contract_mixed(int mode, const uint16_t* Pf, int nbasis_pad, const f
   CT_REQUIRE(nbasis_pad % TR == 0, "padded basis size must be a multiple of 128");
   CT_REQUIRE(ld % 256 == 0, "point stride must be a multiple of 256");
   const u4* P = reinterpret_cast<const u4*>(Pf);
+  const int mp = nbasis_pad;
   switch (mode) {
     case MODE_RHO:
-      launch_var<V_RHO>(P, nbasis_pad, ao, ld, part, s);
+      launch_var<V_RHO>(P, mp, mp, ao, ld, part, s);
       break;
     case MODE_GRAD:
-      launch_var<V_GRAD>(P, nbasis_pad, ao, ld, part, s);
+      launch_var<V_GRAD>(P, mp, mp, ao, ld, part, s);
       break;
     case MODE_LAP:
-      launch_var<V_LAPPHI>(P, nbasis_pad, ao, ld, part, s);
-      launch_var<V_LAPDER, 1>(P, nbasis_pad, ao, ld, part, s);
-      launch_var<V_LAPDER, 2>(P, nbasis_pad, ao, ld, part, s);
-      launch_var<V_LAPDER, 3>(P, nbasis_pad, ao, ld, part, s);
+      launch_var<V_LAPPHI>(P, mp, mp, ao, ld, part, s);
+      launch_var<V_LAPDER, 1>(P, mp, mp, ao, ld, part, s);
+      launch_var<V_LAPDER, 2>(P, mp, mp, ao, ld, part, s);
+      launch_var<V_LAPDER, 3>(P, mp, mp, ao, ld, part, s);
       break;
     default:
-      launch_var<V_HESS>(P, nbasis_pad, ao, ld, part, s);
+      launch_var<V_HESS>(P, mp, mp, ao, ld, part, s);
       break;
+  }
+}
+
+void launch_contract_mixed_factor(int mode, const uint16_t* Lf, int rows_pad, int nbasis_pad, const float* ao,
+                                  int64_t ld, double* part, hipStream_t s) {
+  CT_REQUIRE(nbasis_pad % TR == 0 && rows_pad % TR == 0 && rows_pad > 0, "factor / basis padding must be multiples of 128");
+  CT_REQUIRE(ld % 256 == 0, "point stride must be a multiple of 256");
+  const u4* L = reinterpret_cast<const u4*>(Lf);
+  switch (mode) {
+    case MODE_RHO:
+      launch_var<V_F1>(L, rows_pad, nbasis_pad, ao, ld, part, s);
+      break;
+    case MODE_GRAD:
+      launch_var<V_F4, MODE_GRAD>(L, rows_pad, nbasis_pad, ao, ld, part, s);
+      break;
+    case MODE_LAP:
+      launch_var<V_F4, MODE_LAP>(L, rows_pad, nbasis_pad, ao, ld, part, s);
+      launch_var<V_F2>(L, rows_pad, nbasis_pad, ao, ld, part, s);
+      break;
+    default:
+      throw Error(CT_E_INVALID, "the factor route covers the rho, grad and lap modes");
   }
 }
 

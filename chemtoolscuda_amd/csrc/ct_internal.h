@@ -69,6 +69,11 @@ struct PrimData {
 // 1.2 over them.
 constexpr double kMixedGammaF64Sums = 3.5762786865234375e-07;  // 6u
 constexpr double kMixedGammaF32Sums = 4.76837158203125e-07;    // 8u
+// Factor route (P = L L^T, psi = L^T phi, rho = psi . psi): the bound is
+// B' = 2 sum_j |psi_j| (|L|^T |phi|)_j >= 2 rho, the first-order propagation
+// of the per-column error of psi; largest ratio error / (u B') measured on the
+// same workloads: 2.0, so gamma = 3u leaves a margin of 1.5.
+constexpr double kMixedGammaFactor = 1.78813934326171875e-07;  // 3u
 
 
 // Granularities.
@@ -128,11 +133,25 @@ void launch_contract_f64_split(int mode, const double* P, int nbasis_pad, const 
 void launch_ao_f32(int mode, int lmax, const ShellMeta* meta, const ShellGeom* geom, const PrimData* prim, int nshell,
                    int nbasis, int nbasis_pad, const double* pts, int64_t npts, int64_t ld, float* ao, double screen,
                    hipStream_t s);
-// Reorders three row-major bf16 planes of P ([3][mp][mp]) into the MFMA
-// fragment order the mixed contraction loads (host side, same size).
-void mixed_fragment_order_P(const uint16_t* planes, int mp, uint16_t* out);
+// Reorders three row-major bf16 planes of a rows x cols matrix
+// ([3][rows][cols]; rows a multiple of 16, cols of 32) into the MFMA fragment
+// order of the mixed tile engine's A operand (host side, same size).
+void mixed_fragment_order(const uint16_t* planes, int rows, int cols, uint16_t* out);
 void launch_contract_mixed(int mode, const uint16_t* Pp, int nbasis_pad, const float* ao, int64_t ld, double* part,
                            hipStream_t s);
+// The same partial dots from a factor P = L L^T (ct_eval_params.route): Lf
+// holds the bf16 pieces of L^T (rows_pad x nbasis_pad, fragment order), and
+// the partials run over rows_pad / 128 blocks of the factor's columns.
+// Modes rho, grad and lap.
+void launch_contract_mixed_factor(int mode, const uint16_t* Lf, int rows_pad, int nbasis_pad, const float* ao,
+                                  int64_t ld, double* part, hipStream_t s);
+// Pivoted Cholesky factorisation of the symmetric n x n matrix A (row stride
+// lda) on the host: A = L L^T with L (n x rank, column k at L[k * n ...]),
+// columns in pivot order, stopping when the largest remaining diagonal
+// element is <= rel_tol times the largest diagonal element of A.  Returns
+// false when A is not positive semidefinite to that tolerance or its rank
+// exceeds rmax.
+bool pivoted_cholesky(const double* A, int n, int lda, int rmax, double rel_tol, std::vector<double>& L, int& rank);
 struct FlagArgs {
   double gamma;          // error estimate = gamma * B (B = |phi|^T |P| |phi|)
   double tol;            // relative tolerance on rho

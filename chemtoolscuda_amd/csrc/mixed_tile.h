@@ -1,6 +1,6 @@
 // Split-bf16 tile engine shared by the mixed-precision contraction and the
-// mixed adjoint: Y = P[i0:i0+128, :] * B for TC GEMM columns, FP32-quality,
-// on v_mfma_f32_16x16x32_bf16.
+// mixed adjoint: Y = A[i0:i0+128, :] * B for TC GEMM columns, FP32-quality,
+// on v_mfma_f32_16x16x32_bf16 (A = P, or the transposed factor of P).
 //
 // P and the FP32 columns are split exactly into three bf16 pieces
 // (x = hi + mid + lo), and Y is the sum of the six products with the
@@ -14,7 +14,7 @@
 // 32w..32w+31 of the 128-row block and all TC columns.  K advances 32 per
 // step:
 //   A (P pieces): stored on the host in MFMA fragment order
-//     (mixed_fragment_order_P), so each wave loads its 16 x 32 fragments
+//     (mixed_fragment_order), so each wave loads its 16 x 32 fragments
 //     straight into registers with one coalesced 1 KiB load per (piece,
 //     16-row block); the next step's fragments are loaded after the last use
 //     of the current ones (the other workgroup on the CU covers the wait);
@@ -102,7 +102,7 @@ struct YTile {
 // (|x|^T |P| |x| is a quadratic form too), its doubling applied exactly to
 // the |P_hi| operand (bf16 exponent + 1).
 template <int TC, int NBC, bool F64, bool TRI>
-__device__ __forceinline__ void y_tile(const u4* __restrict__ Pf, int mp, int64_t ld, int i0, int tid,
+__device__ __forceinline__ void y_tile(const u4* __restrict__ Pf, int rows, int mp, int64_t ld, int i0, int tid,
                                        const float* __restrict__ bsrc, uint16_t (&Bs)[2][NPL][TC][LROW],
                                        YTile<TC, NBC, F64>& t, int k_begin) {
   static_assert(!TRI || F64, "the triangle needs the FP64 running sums");
@@ -112,10 +112,11 @@ __device__ __forceinline__ void y_tile(const u4* __restrict__ Pf, int mp, int64_
   const int r = lane & 15, q = lane >> 4;
   const int nk = mp / KB;
 
-  // A: fragment (piece pl, 16-row block b16, k-step kb) is the 64 x 16 bytes
-  // at Pf[((pl * mp/16 + b16) * nk + kb) * 64 + lane].
+  // A (rows x mp: P, or the transposed factor of P): fragment (piece pl,
+  // 16-row block b16, k-step kb) is the 64 x 16 bytes at
+  // Pf[((pl * rows/16 + b16) * nk + kb) * 64 + lane].
   const u4* abase = Pf + (int64_t)((i0 + RW * w) / 16) * nk * 64 + lane;
-  const int64_t aplane = (int64_t)(mp / 16) * nk * 64;
+  const int64_t aplane = (int64_t)(rows / 16) * nk * 64;
   u4 areg[NPL][NRB];
   auto load_a = [&](int kb) {
 #pragma unroll

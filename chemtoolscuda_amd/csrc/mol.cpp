@@ -279,6 +279,46 @@ float bf16_float(uint16_t h) {
 }
 }  // namespace
 
+bool pivoted_cholesky(const double* A, int n, int lda, int rmax, double rel_tol, std::vector<double>& L, int& rank) {
+  L.clear();
+  rank = 0;
+  std::vector<double> d(n);
+  double dmax = 0.0;
+  for (int i = 0; i < n; ++i) {
+    d[i] = A[(size_t)i * lda + i];
+    dmax = std::max(dmax, std::fabs(d[i]));
+  }
+  if (dmax == 0.0) return true;
+  const double tol = rel_tol * dmax;
+  std::vector<char> used(n, 0);
+  for (;;) {
+    int p = -1;
+    for (int i = 0; i < n; ++i)
+      if (!used[i] && (p < 0 || d[i] > d[p])) p = i;
+    if (p < 0 || d[p] <= tol) break;
+    if (rank >= rmax) return false;
+    // column rank: (A[:, p] - L L[p, :]^T) / sqrt(d[p]); rows pivoted earlier are 0
+    const double piv = std::sqrt(d[p]);
+    std::vector<double> col(n, 0.0);
+    for (int i = 0; i < n; ++i) {
+      if (used[i]) continue;
+      double v = A[(size_t)i * lda + p];
+      for (int k = 0; k < rank; ++k) v -= L[(size_t)k * n + i] * L[(size_t)k * n + p];
+      col[i] = v / piv;
+    }
+    col[p] = piv;
+    for (int i = 0; i < n; ++i)
+      if (!used[i]) d[i] -= col[i] * col[i];
+    used[p] = 1;
+    L.insert(L.end(), col.begin(), col.end());
+    ++rank;
+  }
+  // positive semidefinite: what is left of the diagonal must not be negative
+  for (int i = 0; i < n; ++i)
+    if (!used[i] && d[i] < -tol) return false;
+  return true;
+}
+
 void Mol::upload_P() {
   // symmetrise: the descriptors depend on the symmetric part of P only
   std::vector<double> Pp((size_t)Mp_ * Mp_, 0.0);
@@ -300,9 +340,62 @@ void Mol::upload_P() {
     Pb[2 * plane + k] = l;
   }
   std::vector<uint16_t> Pf(Pb.size());
-  mixed_fragment_order_P(Pb.data(), Mp_, Pf.data());
+  mixed_fragment_order(Pb.data(), Mp_, Mp_, Pf.data());
   if (!d_Pb_.reserve(sizeof(uint16_t) * Pf.size())) throw Error(CT_E_NOMEM, "cannot allocate the bf16 density matrix");
   CT_HIP_CHECK(hipMemcpy(d_Pb_.get(), Pf.data(), sizeof(uint16_t) * Pf.size(), hipMemcpyHostToDevice));
+
+  // mixed path, factor route: P = L L^T (pivoted Cholesky, to 1e-12 of the
+  // largest diagonal element), kept when P is positive semidefinite and its
+  // rank is at most about half the basis (beyond that the density route needs
+  // fewer products in every mode); L^T padded to a multiple of 128 rows, in
+  // FP32 split into three bf16 pieces like P.
+  factor_rank_ = -1;
+  std::vector<double> L;
+  int rank = 0;
+  if (pivoted_cholesky(Pp.data(), M_, Mp_, Mp_ / 2 + 1, 1e-12, L, rank)) {
+    factor_rank_ = rank;
+    factor_rows_ = std::max(kRowBlock, (rank + kRowBlock - 1) / kRowBlock * kRowBlock);
+    const size_t fplane = (size_t)factor_rows_ * Mp_;
+    std::vector<uint16_t> Lb(3 * fplane, 0);
+    for (int j = 0; j < rank; ++j)
+      for (int i = 0; i < M_; ++i) {
+        const float x = (float)L[(size_t)j * M_ + i];  // row j of L^T
+        const uint16_t h = bf16_bits(x);
+        const float r1 = x - bf16_float(h);
+        const uint16_t m = bf16_bits(r1);
+        const uint16_t l = bf16_bits(r1 - bf16_float(m));
+        const size_t k = (size_t)j * Mp_ + i;
+        Lb[k] = h;
+        Lb[fplane + k] = m;
+        Lb[2 * fplane + k] = l;
+      }
+    std::vector<uint16_t> Lf(Lb.size());
+    mixed_fragment_order(Lb.data(), factor_rows_, Mp_, Lf.data());
+    if (!d_Lb_.reserve(sizeof(uint16_t) * Lf.size())) throw Error(CT_E_NOMEM, "cannot allocate the bf16 factor of P");
+    CT_HIP_CHECK(hipMemcpy(d_Lb_.get(), Lf.data(), sizeof(uint16_t) * Lf.size(), hipMemcpyHostToDevice));
+  }
+}
+
+bool Mol::factor_route(int mode, int route) const {
+  if (route == CT_ROUTE_DENSITY) return false;
+  const bool avail = factor_rank_ >= 0 && mode != MODE_HESS;
+  if (route == CT_ROUTE_FACTOR) {
+    if (!avail)
+      throw Error(CT_E_INVALID, mode == MODE_HESS ? "the factor route does not cover the Hessian"
+                                                  : "factor route unavailable: P is not positive semidefinite "
+                                                    "with rank at most half the basis");
+    return true;
+  }
+  if (!avail) return false;
+  // matrix-core products per point, in units of Mp^2 x (the six split
+  // products): density route rho 0.525 (triangle), grad 1, lap 1 + 3 x 0.525;
+  // factor route 1, 4, 4 + 2 columns of rows / Mp
+  const double f = (double)factor_rows_ / Mp_;
+  switch (mode) {
+    case MODE_RHO: return f < 0.525;
+    case MODE_GRAD: return 4 * f < 1.0;
+    default: return 6 * f < 2.575;
+  }
 }
 
 void Mol::set_coordinates(const double* xyz) {
@@ -540,14 +633,15 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
   CT_REQUIRE(precision == CT_PREC_FP64 || precision == CT_PREC_MIXED, "unknown precision");
   for (const OutSlot& s : kSlots)
     if ((q & s.bit) && out.*(s.field) == nullptr) throw Error(CT_E_INVALID, "output array missing for a requested quantity");
-  if (rep) *rep = ct_eval_report{n, 0, 0, 0.0};
+  if (rep) *rep = ct_eval_report{n, 0, 0, 0.0, -1};
   if (n == 0) return;
   CT_REQUIRE(points != nullptr || cube != nullptr, "null points");
 
   const bool mixed = precision == CT_PREC_MIXED;
   const int mode = select_mode(q);
   const int nder = mode_nder(mode), ndot = mode_ndot(mode);
-  const int nrt = Mp_ / kRowBlock;
+  const bool use_factor = mixed && factor_route(mode, par.route);
+  const int nrt = (use_factor ? factor_rows_ : Mp_) / kRowBlock;  // row blocks of the mixed partials
   const bool pts_dev = cube == nullptr && is_device_pointer(points);
   constexpr size_t NS = sizeof(kSlots) / sizeof(kSlots[0]);
   bool out_dev[NS];
@@ -651,11 +745,15 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
       launch_ao_f32(mode, lmax_, meta, geom, prim, nshell_, M_, Mp_, pts, nc, ld, static_cast<float*>(ao), 50.0, stream_);
       timer_.end(stream_);
       timer_.begin(KernelTimer::CONTRACT, stream_);
-      launch_contract_mixed(mode, static_cast<const uint16_t*>(d_Pb_.get()), Mp_, static_cast<const float*>(ao), ld,
-                            dots, stream_);
+      if (use_factor)
+        launch_contract_mixed_factor(mode, static_cast<const uint16_t*>(d_Lb_.get()), factor_rows_, Mp_,
+                                     static_cast<const float*>(ao), ld, dots, stream_);
+      else
+        launch_contract_mixed(mode, static_cast<const uint16_t*>(d_Pb_.get()), Mp_, static_cast<const float*>(ao),
+                              ld, dots, stream_);
       timer_.end(stream_);
       FlagArgs fl;
-      fl.gamma = mixed_gamma(mode);
+      fl.gamma = use_factor ? kMixedGammaFactor : mixed_gamma(mode);
       fl.tol = par.mixed_tol;
       fl.rho_floor = par.rho_floor;
       fl.count = d_count;
@@ -714,6 +812,7 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
     rep->n_chunks = chunks;
     rep->n_reevaluated = reevaluated;
     if (mixed) std::memcpy(&rep->max_error_estimate, h_small + 1, 8);
+    rep->factor_rank = use_factor ? factor_rank_ : -1;
   }
 }
 

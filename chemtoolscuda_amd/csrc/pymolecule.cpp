@@ -117,9 +117,9 @@ class Molecule {
 
   // Generic multi-quantity evaluation: returns {name: array}.
   py::dict compute(const Points& pts, const std::vector<std::string>& names, const std::string& precision,
-                   double a, double b, double alpha, double mixed_tol) {
+                   double a, double b, double alpha, double mixed_tol, const std::string& route) {
     const int64_t n = n_points(pts);
-    return compute_impl(pts.data(), nullptr, {n}, names, precision, a, b, alpha, mixed_tol);
+    return compute_impl(pts.data(), nullptr, {n}, names, precision, a, b, alpha, mixed_tol, route);
   }
 
   // The same on a uniform grid generated on the device: point (i, j, k) =
@@ -128,7 +128,7 @@ class Molecule {
                                  py::array_t<double, py::array::c_style | py::array::forcecast> axes,
                                  py::array_t<int64_t, py::array::c_style | py::array::forcecast> shape,
                                  const std::vector<std::string>& names, const std::string& precision, double a,
-                                 double b, double alpha, double mixed_tol) {
+                                 double b, double alpha, double mixed_tol, const std::string& route) {
     if (origin.size() != 3 || axes.size() != 9 || shape.size() != 3)
       throw std::invalid_argument("origin must have 3 entries, axes 3 x 3, shape 3");
     Cube c;
@@ -138,7 +138,7 @@ class Molecule {
     for (int k = 0; k < 3; ++k)
       if (c.shape[k] <= 0) throw std::invalid_argument("grid shape must be positive");
     return compute_impl(nullptr, &c, {c.shape[0], c.shape[1], c.shape[2]}, names, precision, a, b, alpha,
-                        mixed_tol);
+                        mixed_tol, route);
   }
 
   // compute_electron_density_cubic (src/pymolecule.cu:59-71); `disp` (progress
@@ -147,7 +147,7 @@ class Molecule {
                                     py::array_t<double, py::array::c_style | py::array::forcecast> axes,
                                     py::array_t<int64_t, py::array::c_style | py::array::forcecast> shape, bool,
                                     const std::string& precision) {
-    py::dict d = compute_on_cubic_grid(origin, axes, shape, {"rho"}, precision, 0, 0, 0, 0);
+    py::dict d = compute_on_cubic_grid(origin, axes, shape, {"rho"}, precision, 0, 0, 0, 0, "auto");
     return d["rho"].cast<py::array_t<double>>();
   }
   py::array_t<double> one(const Points& pts, uint32_t bit, std::vector<py::ssize_t> extra,
@@ -225,6 +225,7 @@ class Molecule {
     d["n_reevaluated"] = last_report_.n_reevaluated;
     d["n_chunks"] = last_report_.n_chunks;
     d["max_error_estimate"] = last_report_.max_error_estimate;
+    d["factor_rank"] = last_report_.factor_rank;
     return d;
   }
 
@@ -239,7 +240,7 @@ class Molecule {
 
   py::dict compute_impl(const double* pts, const Cube* cube, std::vector<py::ssize_t> lead,
                         const std::vector<std::string>& names, const std::string& precision, double a, double b,
-                        double alpha, double mixed_tol) {
+                        double alpha, double mixed_tol, const std::string& route) {
     static const std::map<std::string, uint32_t> bits = {
         {"rho", CT_Q_RHO},         {"grad", CT_Q_GRAD},
         {"hess", CT_Q_HESS},       {"lap", CT_Q_LAP},
@@ -284,6 +285,14 @@ class Molecule {
     par.ked_a = alpha;
     par.ked_alpha = alpha;
     par.mixed_tol = mixed_tol;
+    if (route == "auto")
+      par.route = CT_ROUTE_AUTO;
+    else if (route == "density")
+      par.route = CT_ROUTE_DENSITY;
+    else if (route == "factor")
+      par.route = CT_ROUTE_FACTOR;
+    else
+      throw std::invalid_argument("route must be 'auto', 'density' or 'factor'");
     ct_eval_report rep{};
     const int prec = parse_precision(precision);
     int st;
@@ -393,10 +402,12 @@ PYBIND11_MODULE(_chemtools_amd, m) {
       .def("compute_electrostatic_potential", &Molecule::esp, py::arg("points"), py::arg("precision") = P64)
       .def("compute", &Molecule::compute, py::arg("points"), py::arg("quantities"), py::arg("precision") = P64,
            py::arg("a") = 0.0, py::arg("b") = 0.0, py::arg("alpha") = 0.0, py::arg("mixed_tol") = 0.0,
-           "Evaluate several quantities in one fused pass; returns a dict.")
+           py::arg("route") = "auto",
+           "Evaluate several quantities in one fused pass; returns a dict.  route (mixed precision): 'auto', "
+           "'density' (contract with P) or 'factor' (contract with the pivoted Cholesky factor of P).")
       .def("compute_on_cubic_grid", &Molecule::compute_on_cubic_grid, py::arg("origin"), py::arg("axes"),
            py::arg("shape"), py::arg("quantities"), py::arg("precision") = P64, py::arg("a") = 0.0,
-           py::arg("b") = 0.0, py::arg("alpha") = 0.0, py::arg("mixed_tol") = 0.0,
+           py::arg("b") = 0.0, py::arg("alpha") = 0.0, py::arg("mixed_tol") = 0.0, py::arg("route") = "auto",
            "Several quantities on the uniform grid origin + i axes[0] + j axes[1] + k axes[2], generated on the "
            "GPU; arrays of shape (n0, n1, n2, ...).")
       .def("compute_electron_density_on_cubic_grid", &Molecule::density_cubic, py::arg("origin"), py::arg("axes"),

@@ -95,7 +95,21 @@ typedef struct {
   double ked_alpha;        /* CT_Q_KED_ALPHA */
   double mixed_tol;        /* relative error target of the mixed path (0 => 1e-6) */
   double rho_floor;        /* points with rho below this are not re-evaluated (0 => 1e-10) */
+  int32_t route;           /* mixed path: CT_ROUTE_AUTO (0), CT_ROUTE_DENSITY, CT_ROUTE_FACTOR */
 } ct_eval_params;
+
+/* Routes of the mixed contraction (rho, grad and lap quantities).  DENSITY
+ * contracts the AOs with P itself, as the reference does (rho = phi^T P phi).
+ * FACTOR contracts them with a pivoted Cholesky factor P = L L^T computed on
+ * the host when P is set (psi = L^T phi, rho = psi^T psi, grad rho =
+ * 2 psi^T grad psi, ...): rank(P) columns instead of nbasis, available when P
+ * is positive semidefinite of rank at most ~half the basis (SCF densities).
+ * AUTO takes FACTOR when it is available and needs fewer matrix-core
+ * products.  Both routes meet the same mixed_tol through the same FP64
+ * re-evaluation of flagged points; the FP64 path always contracts with P. */
+#define CT_ROUTE_AUTO 0
+#define CT_ROUTE_DENSITY 1
+#define CT_ROUTE_FACTOR 2
 
 typedef struct {
   double* rho;
@@ -117,6 +131,7 @@ typedef struct {
   int64_t n_reevaluated;   /* mixed path: points recomputed in FP64 */
   int64_t n_chunks;
   double max_error_estimate;
+  int64_t factor_rank;     /* mixed path: rank of the factor if the FACTOR route ran, else -1 */
 } ct_eval_report;
 
 /* Lifetime ---------------------------------------------------------------- */

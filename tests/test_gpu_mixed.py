@@ -148,7 +148,7 @@ def test_mixed_error_model_pointwise(ct, workload, npts):
     pts = w.points[rng.choice(w.points.shape[0], size=min(npts, w.points.shape[0]), replace=False)]
     mol = ct.molecule_from_basis(w.basis, w.P, w.charges)
     names = ["rho", "grad", "lap", "tau_pd"]
-    raw = mol.compute(pts, names, precision="mixed", mixed_tol=1e30)
+    raw = mol.compute(pts, names, precision="mixed", mixed_tol=1e30, route="density")
     assert mol.last_report()["n_reevaluated"] == 0
     f64 = mol.compute(pts, names, precision="fp64")
     scales = _condition_scales(w.basis.flat(), w.P, pts)
@@ -160,7 +160,7 @@ def test_mixed_error_model_pointwise(ct, workload, npts):
         ratios[k] = float(np.max(np.linalg.norm(a - b, axis=1) / (u * sc)))
     # density-only mode: its own tile (triangle of P, doubled right of each
     # wave's diagonal block); the FP64 density-only path uses the same shortcut
-    raw_r = mol.compute(pts, ["rho"], precision="mixed", mixed_tol=1e30)["rho"]
+    raw_r = mol.compute(pts, ["rho"], precision="mixed", mixed_tol=1e30, route="density")["rho"]
     f64_r = mol.compute(pts, ["rho"], precision="fp64")["rho"]
     ratios["rho (rho-only mode)"] = float(np.max(np.abs(raw_r - f64_r) / (u * scales[0])))
     assert np.max(np.abs(f64_r - f64["rho"]) / scales[0]) < 1e-14
@@ -168,3 +168,115 @@ def test_mixed_error_model_pointwise(ct, workload, npts):
           + ", ".join(f"{k} {v:.2f}" for k, v in ratios.items()))
     for k, v in ratios.items():
         assert v <= 6.0, (k, v)
+
+
+def _pivoted_cholesky(P, rel_tol=1e-12):
+    """numpy restatement of the library's factorisation of P (host code,
+    chemtoolscuda_amd/csrc/mol.cpp): P = L L^T, columns in pivot order (largest
+    remaining diagonal first), stop at rel_tol of the largest diagonal."""
+    n = P.shape[0]
+    d = np.diag(P).astype(np.float64).copy()
+    tol = rel_tol * np.abs(d).max()
+    used = np.zeros(n, dtype=bool)
+    cols = []
+    while not used.all():
+        p = int(np.argmax(np.where(used, -np.inf, d)))
+        if d[p] <= tol:
+            break
+        L = np.array(cols).T if cols else np.zeros((n, 0))
+        c = (P[:, p] - L @ L[p]) / np.sqrt(d[p])
+        c[used] = 0.0
+        c[p] = np.sqrt(d[p])
+        d[~used] -= c[~used] ** 2
+        used[p] = True
+        cols.append(c)
+    return np.array(cols).T
+
+
+def _factor_scales(flat, L, pts):
+    """Condition scales of the factor route (psi = L^T phi): the first-order
+    propagation of per-column bounds b = |L|^T |x| of every psi column into
+    rho = psi.psi, grad = 2 psi.d psi, tau = 1/2 sum_k d_k psi.d_k psi and
+    lap = 2 (psi.lap psi + sum_k d_k psi.d_k psi)."""
+    ao = oracle.ao(flat, pts, 2)
+    aL = np.abs(L)
+    lapphi = ao[4] + ao[7] + ao[9]
+    psi = [L.T @ ao[k] for k in range(4)]
+    psil = L.T @ lapphi
+    b = [aL.T @ np.abs(ao[k]) for k in range(4)]
+    bl = aL.T @ np.abs(lapphi)
+    s_rho = 2 * np.sum(np.abs(psi[0]) * b[0], axis=0)
+    s_grad = np.sqrt(sum((2 * np.sum(np.abs(psi[k + 1]) * b[0] + np.abs(psi[0]) * b[k + 1], axis=0)) ** 2
+                         for k in range(3)))
+    s_tau = sum(np.sum(np.abs(psi[k + 1]) * b[k + 1], axis=0) for k in range(3))
+    s_lap = 2 * np.sum(np.abs(psil) * b[0] + np.abs(psi[0]) * bl, axis=0) + 4 * s_tau
+    return s_rho, s_grad, s_lap, s_tau
+
+
+@pytest.mark.parametrize("workload,npts", [("water", 3000), ("organic60", 12000), ("large300", 400)])
+def test_factor_route_error_model_pointwise(ct, workload, npts):
+    """Factor route (P = L L^T, psi = L^T phi on the split-bf16 engine):
+    without re-evaluation the error of rho stays below gamma = 3u times the
+    bound the kernel computes and the finaliser applies, 2 sum_j |psi_j|
+    (|L|^T |phi|)_j, and the other quantities below 6u times their
+    factor-route condition scales.  Also the rho-only and grad modes, and the
+    reported rank."""
+    from chemtoolscuda_amd import synthetic
+
+    w = synthetic.large300(npts) if workload == "large300" else synthetic.WORKLOADS[workload]()
+    rng = np.random.default_rng(9)
+    pts = w.points[rng.choice(w.points.shape[0], size=min(npts, w.points.shape[0]), replace=False)]
+    mol = ct.molecule_from_basis(w.basis, w.P, w.charges)
+    P = 0.5 * (w.P + w.P.T)
+    L = _pivoted_cholesky(P)
+    names = ["rho", "grad", "lap", "tau_pd"]
+    raw = mol.compute(pts, names, precision="mixed", mixed_tol=1e30, route="factor")
+    rep = mol.last_report()
+    assert rep["n_reevaluated"] == 0 and rep["factor_rank"] == L.shape[1]
+    f64 = mol.compute(pts, names, precision="fp64")
+    scales = _factor_scales(w.basis.flat(), L, pts)
+    u = 2.0 ** -24
+    ratios = {}
+    for k, sc in zip(names, scales):
+        a = np.asarray(raw[k]).reshape(len(pts), -1)
+        b = np.asarray(f64[k]).reshape(len(pts), -1)
+        ratios[k] = float(np.max(np.linalg.norm(a - b, axis=1) / (u * sc)))
+    for names2 in (["rho"], ["rho", "grad"]):
+        r2 = mol.compute(pts, names2, precision="mixed", mixed_tol=1e30, route="factor")
+        assert mol.last_report()["factor_rank"] == L.shape[1]
+        for k in names2:
+            i = names.index(k)
+            a = np.asarray(r2[k]).reshape(len(pts), -1)
+            b = np.asarray(f64[k]).reshape(len(pts), -1)
+            ratios[f"{k} ({'+'.join(names2)} mode)"] = float(np.max(np.linalg.norm(a - b, axis=1) / (u * scales[i])))
+    print(f"{workload} (M={w.basis.nbasis}, rank {L.shape[1]}): factor route, max error / (u * condition scale): "
+          + ", ".join(f"{k} {v:.2f}" for k, v in ratios.items()))
+    for k, v in ratios.items():
+        assert v <= (3.0 if k.startswith("rho") else 6.0), (k, v)
+    # with re-evaluation: the 1e-6 target on rho
+    mix = mol.compute(pts, ["rho"], precision="mixed", route="factor")["rho"]
+    m = f64["rho"] >= 1e-8
+    assert np.max(np.abs(mix - f64["rho"])[m] / f64["rho"][m]) <= 1e-6
+
+
+def test_route_selection_and_errors(ct):
+    from chemtoolscuda_amd import synthetic
+
+    w = synthetic.water(500)
+    mol = ct.molecule_from_basis(w.basis, w.P, w.charges)
+    mol.compute(w.points, ["rho"], precision="mixed")
+    assert mol.last_report()["factor_rank"] == -1  # rank 5 in a 128-row padding: the density route is cheaper
+    mol.compute(w.points, ["rho"], precision="fp64", route="factor")
+    assert mol.last_report()["factor_rank"] == -1  # the FP64 path always contracts with P
+    with pytest.raises(RuntimeError, match="Hessian"):
+        mol.compute(w.points, ["hess"], precision="mixed", route="factor")
+    # an indefinite P has no factor: the factor route is refused, auto falls back
+    P2 = w.P.copy()
+    P2[0, 0] -= 10.0
+    mol.set_density_matrix(P2)
+    with pytest.raises(RuntimeError, match="not positive semidefinite"):
+        mol.compute(w.points, ["rho"], precision="mixed", route="factor")
+    a = mol.compute(w.points, ["rho"], precision="mixed")["rho"]
+    b = mol.compute(w.points, ["rho"], precision="fp64")["rho"]
+    m = np.abs(b) >= 1e-8
+    assert np.max(np.abs(a - b)[m] / np.abs(b)[m]) <= 1e-6
