@@ -33,10 +33,14 @@ PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16": 2.5e15}
 
 # HBM bytes per contraction call (FETCH_SIZE + WRITE_SIZE, calibrated against
 # a 1 GiB device copy), measured with rocprofv3 --pmc on this workload; see
-# profiles/contract_{mixed,f64}_organic60.md for the commands and raw counts.
-# The mixed call is four kernel dispatches (phi columns, then one per
-# derivative type); its figure is their sum, averaged over the chunks.
-MEASURED_TRAFFIC = {("organic60", "mixed"): 24.1e9 + 0.19e9, ("organic60", "fp64"): 53.7e9 + 0.021e9}
+# profiles/contract_mixed_{factor,density}_organic60.md and
+# profiles/contract_f64_organic60.md for the commands and raw counts.
+# A mixed call is several kernel dispatches (density route: phi columns, then
+# one per derivative type; factor route: two); its figure is their sum,
+# averaged over the chunks.
+MEASURED_TRAFFIC = {("organic60", "mixed", "factor"): 8.44e9 + 0.038e9,
+                    ("organic60", "mixed", "density"): 24.1e9 + 0.19e9,
+                    ("organic60", "fp64", "density"): 53.7e9 + 0.021e9}
 
 
 def parse():
@@ -46,6 +50,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="organic60")
     ap.add_argument("--precision", default="auto", choices=["auto", "fp64", "mixed"])
+    ap.add_argument("--route", default="auto", choices=["auto", "density", "factor"],
+                    help="mixed contraction route (contract with P, or with its pivoted Cholesky factor)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU-baseline work (0 disables)")
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64-vs-mixed comparison run")
     ap.add_argument("--workspace-mb", type=int, default=0, help="per-chunk workspace budget (0: library default)")
@@ -237,8 +243,10 @@ def main():
     if precision == "auto":
         precision = "mixed"
 
+    route = {"auto": 0, "density": 1, "factor": 2}[args.route]
+
     def run(outs, prec):
-        return h.eval(pts.data_ptr(), n, {q: t.data_ptr() for q, t in outs.items()}, prec)
+        return h.eval(pts.data_ptr(), n, {q: t.data_ptr() for q, t in outs.items()}, prec, {"route": route})
 
     outs = alloc()
     for _ in range(args.warmup):
@@ -262,11 +270,20 @@ def main():
     pts_per_launch = n * args.steps / max(stats["contract_calls"], 1)
     achieved = flops_pt * pts_per_launch / (contract_ms * 1e-3)
     peak_key = "fp64" if precision == "fp64" else "bf16"
+    rank = rep.get("factor_rank", -1) if rep else -1
+    used_route = "factor" if rank >= 0 else "density"
     roofline = {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK[peak_key] / 1e12, "unit": "TFLOP/s",
-                "frac": achieved / PEAK[peak_key], "traffic": MEASURED_TRAFFIC.get((w.name, precision)),
+                "frac": achieved / PEAK[peak_key], "traffic": MEASURED_TRAFFIC.get((w.name, precision, used_route)),
                 "kernel": "contract_" + ("f64" if precision == "fp64" else "mixed"),
                 "avg_launch_ms": contract_ms, "points_per_launch": pts_per_launch,
-                "flops_per_point": flops_pt}
+                "flops_per_point": flops_pt, "route": used_route}
+    if rank >= 0:
+        # the factor route's own algorithmic count: psi = L^T [phi, d phi, lap phi]
+        # (5 column types of rank r) and 8 dots of length r per point
+        f_exec = 2.0 * M * rank * 5 + 2.0 * rank * 8
+        a_exec = f_exec * pts_per_launch / (contract_ms * 1e-3)
+        roofline["factor_rank"] = rank
+        roofline["executed"] = {"flops_per_point": f_exec, "achieved": a_exec / 1e12, "frac": a_exec / PEAK[peak_key]}
 
     extra = {}
     if precision == "mixed" and not args.no_compare:
