@@ -41,33 +41,42 @@ using namespace mixed;
 //   V_F4<mode>:    32 points x {phi, d_x phi, d_y phi, d_z phi}: rho,
 //                  g_k = psi . d_k psi, the bound, and in the lap mode
 //                  t_kk = d_k psi . d_k psi;
-//   V_F2:          32 points x {phi, lap phi}: the lap mode's psi . lap psi.
+//   V_F2:          32 points x {phi, lap phi}: the lap mode's psi . lap psi;
+//   V_FH<part>:    32 points x {phi, d_kl phi for three kl}: the Hessian
+//                  mode's h_kl = psi . d_kl psi (part 0: xx, xy, xz; part 1:
+//                  yy, yz, zz); with V_F4<hess>, whose t_kl = d_k psi . d_l psi
+//                  cover all six kl.
 // The error bound of the factor route is 2 sum_j |psi_j| (|L|^T |phi|)_j,
 // the first-order propagation of the per-column bound of psi into
 // rho = sum_j psi_j^2.
 // Launches of one mode write disjoint dot slots of the same partials array.
-enum Var : int { V_RHO, V_GRAD, V_LAPPHI, V_LAPDER, V_HESS, V_F1, V_F4, V_F2 };
+enum Var : int { V_RHO, V_GRAD, V_LAPPHI, V_LAPDER, V_HESS, V_F1, V_F4, V_F2, V_FH };
 
-// DT: the derivative type of a V_LAPDER launch, the mode of a V_F4 launch.
+// DT: the derivative type of a V_LAPDER launch, the mode of a V_F4 launch,
+// the part of a V_FH launch.
 template <int V, int DT>
 struct Cols {
   static constexpr bool FAC = V >= V_F1;
   static constexpr int MODE = V == V_RHO || V == V_F1 ? MODE_RHO
                               : V == V_GRAD           ? MODE_GRAD
-                              : V == V_HESS           ? MODE_HESS
+                              : V == V_HESS || V == V_FH ? MODE_HESS
                               : V == V_F4             ? DT
                                                       : MODE_LAP;
-  static constexpr int NT = V == V_HESS || V == V_F4 ? 4 : V == V_F2 ? 2 : 1;  // column types per tile
-  static constexpr int TC = V == V_HESS || V == V_LAPDER || V == V_F4 ? 128 : 64;
+  static constexpr int NT = V == V_HESS || V == V_F4 || V == V_FH ? 4 : V == V_F2 ? 2 : 1;  // column types per tile
+  static constexpr int TC = V == V_HESS || V == V_LAPDER || V == V_F4 || V == V_FH ? 128 : 64;
   static constexpr int NCB = TC / 16;                     // 16-column MFMA blocks
   static constexpr int PTS = TC / NT;                     // points per workgroup
-  static constexpr int NBC = V == V_LAPDER || V == V_F2 ? 0 : V == V_HESS || V == V_F4 ? 2 : NCB;  // blocks with the bound
+  static constexpr int NBC = V == V_LAPDER || V == V_F2 || V == V_FH ? 0 : V == V_HESS || V == V_F4 ? 2 : NCB;  // bound blocks
   static constexpr bool F64 = V != V_HESS;  // FP64 running sums (the Hessian tile lacks the registers)
   static constexpr bool TRI = V == V_RHO || V == V_LAPDER;  // quadratic forms only: triangle of P
-  // AO array (0 = phi, 1..3 = d_x, d_y, d_z phi, 4 = lap phi in the lap mode)
-  // of column c
+  // AO array of column c: 0 = phi, 1..3 = d_x, d_y, d_z phi, then 4 = lap phi
+  // (lap mode) or 4..9 = d_xx, d_xy, d_xz, d_yy, d_yz, d_zz phi (Hessian mode)
   __host__ __device__ static constexpr int type(int c) {
-    return V == V_F2 ? (c < PTS ? 0 : 4) : NT == 4 ? c / PTS : V == V_LAPDER ? DT : 0;
+    return V == V_F2   ? (c < PTS ? 0 : 4)
+           : V == V_FH ? (c < PTS ? 0 : 4 + 3 * DT + c / PTS - 1)
+           : NT == 4   ? c / PTS
+           : V == V_LAPDER ? DT
+                           : 0;
   }
   __host__ __device__ static constexpr int point(int c) { return NT > 1 ? c % PTS : c; }
 };
@@ -111,8 +120,11 @@ __device__ __forceinline__ void factor_epilogue(const Tile& yt, uint16_t* lds, d
   using C = Cols<V, DT>;
   constexpr int ND = mode_ndot(C::MODE), ND1 = ND + 1;
   constexpr int NPB = C::PTS / 16;  // 16-point column blocks per column type
-  constexpr bool LAP = C::MODE == MODE_LAP;
-  constexpr int NDF = V == V_F1 ? 2 : V == V_F2 ? 1 : (LAP ? 8 : 5);
+  constexpr bool LAP = C::MODE == MODE_LAP, HES = C::MODE == MODE_HESS;
+  // dots per point: V_F1 [rho, bound]; V_F2 [psi.lap psi]; V_FH [psi.d_kl psi
+  // x 3]; V_F4 [rho, g_x, g_y, g_z, (lap: t_xx, t_yy, t_zz | Hessian: t_xx,
+  // t_xy, t_yy, t_xz, t_yz, t_zz,) bound]
+  constexpr int NDF = V == V_F1 ? 2 : V == V_F2 ? 1 : V == V_FH ? 3 : (LAP ? 8 : HES ? 11 : 5);
   const int lane = tid & 63, w = tid >> 6, r = lane & 15, q = lane >> 4;
   double* red = reinterpret_cast<double*>(lds);  // [NW][PTS][NDF]
 #pragma unroll
@@ -127,15 +139,27 @@ __device__ __forceinline__ void factor_epilogue(const Tile& yt, uint16_t* lds, d
         const double y0 = yt.y(rb, pb, j);
         if constexpr (V == V_F2) {
           d[0] += y0 * yt.y(rb, NPB + pb, j);
+        } else if constexpr (V == V_FH) {
+#pragma unroll
+          for (int k = 0; k < 3; ++k) d[k] += y0 * yt.y(rb, (k + 1) * NPB + pb, j);
         } else {
           d[0] += y0 * y0;
           d[NDF - 1] += 2.0 * fabs(y0) * (double)yt.bound[rb][pb][j];
           if constexpr (V == V_F4) {
+            double yk[3];
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-              const double yk = yt.y(rb, (k + 1) * NPB + pb, j);
-              d[1 + k] += y0 * yk;
-              if constexpr (LAP) d[4 + k] += yk * yk;
+              yk[k] = yt.y(rb, (k + 1) * NPB + pb, j);
+              d[1 + k] += y0 * yk[k];
+              if constexpr (LAP) d[4 + k] += yk[k] * yk[k];
+            }
+            if constexpr (HES) {  // t_xx, t_xy, t_yy, t_xz, t_yz, t_zz
+              d[4] += yk[0] * yk[0];
+              d[5] += yk[0] * yk[1];
+              d[6] += yk[1] * yk[1];
+              d[7] += yk[0] * yk[2];
+              d[8] += yk[1] * yk[2];
+              d[9] += yk[2] * yk[2];
             }
           }
         }
@@ -160,7 +184,15 @@ __device__ __forceinline__ void factor_epilogue(const Tile& yt, uint16_t* lds, d
       double sum = 0.0;
 #pragma unroll
       for (int ww = 0; ww < NW; ++ww) sum += red[((int64_t)ww * C::PTS + tid) * NDF + k];
-      const int slot_k = V == V_F2 ? 4 : k == NDF - 1 ? ND : (k < 4 ? k : k + 1);
+      // dot slots of the mode (mode_ndot): lap [rho, g, L0, t_xx, t_yy, t_zz],
+      // Hessian [rho, g, h_xx..h_zz, t_xx, t_xy, t_yy, t_xz, t_yz, t_zz]; the
+      // bound after them
+      const int slot_k = V == V_F2   ? 4
+                         : V == V_FH ? 4 + 3 * DT + k
+                         : k == NDF - 1 ? ND
+                         : k < 4        ? k
+                         : HES          ? k + 6
+                                        : k + 1;
       part[((int64_t)rt * ND1 + slot_k) * ld + pbase + tid] = sum;
     }
   }
@@ -350,7 +382,10 @@ void launch_contract_mixed_factor(int mode, const uint16_t* Lf, int rows_pad, in
       launch_var<V_F2>(L, rows_pad, nbasis_pad, ao, ld, part, s);
       break;
     default:
-      throw Error(CT_E_INVALID, "the factor route covers the rho, grad and lap modes");
+      launch_var<V_F4, MODE_HESS>(L, rows_pad, nbasis_pad, ao, ld, part, s);
+      launch_var<V_FH, 0>(L, rows_pad, nbasis_pad, ao, ld, part, s);
+      launch_var<V_FH, 1>(L, rows_pad, nbasis_pad, ao, ld, part, s);
+      break;
   }
 }
 

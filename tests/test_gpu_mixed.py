@@ -268,8 +268,6 @@ def test_route_selection_and_errors(ct):
     assert mol.last_report()["factor_rank"] == -1  # rank 5 in a 128-row padding: the density route is cheaper
     mol.compute(w.points, ["rho"], precision="fp64", route="factor")
     assert mol.last_report()["factor_rank"] == -1  # the FP64 path always contracts with P
-    with pytest.raises(RuntimeError, match="Hessian"):
-        mol.compute(w.points, ["hess"], precision="mixed", route="factor")
     # an indefinite P has no factor: the factor route is refused, auto falls back
     P2 = w.P.copy()
     P2[0, 0] -= 10.0
@@ -310,3 +308,55 @@ def test_device_resident_chunks_match_host_path(ct):
             assert rep["n_chunks"] >= 6
         for q in names:
             assert np.array_equal(dev[q].cpu().numpy(), host[q]), (limit_mb, q)
+
+
+def _factor_hess_scale(flat, L, pts):
+    """Condition scale of the factor route's Hessian, H_kl = 2 sum_j (psi_j
+    d_kl psi_j + d_k psi_j d_l psi_j): 2 sum_j (|d_kl psi_j| b_j + |psi_j|
+    b_kl,j + |d_k psi_j| b_l,j + |d_l psi_j| b_k,j) with b_x = |L|^T |x|,
+    largest over kl."""
+    ao = oracle.ao(flat, pts, 2)
+    aL = np.abs(L)
+    psi = [L.T @ a for a in ao]
+    b = [aL.T @ np.abs(a) for a in ao]
+    idx = {(0, 0): 4, (0, 1): 5, (0, 2): 6, (1, 1): 7, (1, 2): 8, (2, 2): 9}
+    out = np.zeros(pts.shape[0])
+    for (k, l), i in idx.items():
+        s = 2 * np.sum(np.abs(psi[i]) * b[0] + np.abs(psi[0]) * b[i] + np.abs(psi[1 + k]) * b[1 + l]
+                       + np.abs(psi[1 + l]) * b[1 + k], axis=0)
+        out = np.maximum(out, s)
+    return out
+
+
+@pytest.mark.parametrize("workload,npts", [("water", 2000), ("organic60", 2000), ("large300", 300)])
+def test_factor_route_hessian_pointwise(ct, workload, npts):
+    """Hessian mode on the factor route (three launches: {phi, d phi} for rho,
+    grad and t_kl; {phi, d_kl phi} twice for h_kl): without re-evaluation rho
+    within 3u of its bound, the Hessian within 6u of its factor-route scale;
+    with re-evaluation rho to 1e-6 and rho ln rho to 1e-6 of |rho ln rho| + rho."""
+    from chemtoolscuda_amd import synthetic
+
+    w = synthetic.large300(npts) if workload == "large300" else synthetic.WORKLOADS[workload]()
+    rng = np.random.default_rng(13)
+    pts = w.points[rng.choice(w.points.shape[0], size=min(npts, w.points.shape[0]), replace=False)]
+    mol = ct.molecule_from_basis(w.basis, w.P, w.charges)
+    L = _pivoted_cholesky(0.5 * (w.P + w.P.T))
+    names = ["rho", "grad", "hess"]
+    raw = mol.compute(pts, names, precision="mixed", mixed_tol=1e30, route="factor")
+    assert mol.last_report()["factor_rank"] == L.shape[1]
+    f64 = mol.compute(pts, names + ["rdg", "shannon"], precision="fp64")
+    s_rho, s_grad, _, _ = _factor_scales(w.basis.flat(), L, pts)
+    s_hess = _factor_hess_scale(w.basis.flat(), L, pts)
+    u = 2.0 ** -24
+    r_rho = float(np.max(np.abs(raw["rho"] - f64["rho"]) / (u * s_rho)))
+    r_grad = float(np.max(np.linalg.norm(raw["grad"] - f64["grad"], axis=1) / (u * s_grad)))
+    r_hess = float(np.max(np.abs(raw["hess"] - f64["hess"]).reshape(len(pts), 9).max(axis=1) / (u * s_hess)))
+    print(f"{workload} (M={w.basis.nbasis}, rank {L.shape[1]}): factor route Hessian mode, max error / (u * scale): "
+          f"rho {r_rho:.2f}, grad {r_grad:.2f}, hess {r_hess:.2f}")
+    assert r_rho <= 3.0 and r_grad <= 6.0 and r_hess <= 6.0, (r_rho, r_grad, r_hess)
+    mix = mol.compute(pts, ["rho", "hess", "rdg", "shannon"], precision="mixed", route="factor")
+    m = f64["rho"] >= 1e-8
+    rho = f64["rho"][m]
+    assert np.max(np.abs(mix["rho"][m] - rho) / rho) <= 1e-6
+    sh = f64["shannon"][m]
+    assert np.max(np.abs(mix["shannon"][m] - sh) / (np.abs(sh) + rho)) <= 1e-6
