@@ -216,7 +216,175 @@ void launch_modes(int mode, const double* P, int nbasis_pad, const double* ao, i
   CT_HIP_CHECK(hipGetLastError());
 }
 
+// Factor route in FP64 (P = L L^T, L stored k-major: L[k * rows + j]): one
+// wave owns 64 GEMM columns -- NT column types x 64/NT points -- and all
+// `rows` factor columns, 64 at a time; the tile psi = L^T [columns] stays in
+// the MFMA accumulators and every dot is a lane-local sum over the factor
+// columns (phi^T P x = psi^T (L^T x)), reduced over the four lane groups.
+//   F1:  phi x 64 points (rho mode: rho);
+//   F4:  {phi, d_x phi, d_y phi, d_z phi} x 16 points: rho, g_k and t_kk (lap
+//        mode) or t_kl (Hessian mode);
+//   F2:  {phi, lap phi} x 32 points: psi . lap psi;
+//   FH0, FH1: {phi, d_kl phi for kl = xx, xy, xz | yy, yz, zz} x 16 points: h_kl.
+enum VF64 : int { F64_F1, F64_F4, F64_F2, F64_FH0, F64_FH1 };
+
+template <int VF, int MODE>
+__global__ __launch_bounds__(256) void contract_f64_factor_kernel(const double* __restrict__ L, int rows, int mp,
+                                                                  const double* __restrict__ ao, int64_t ld,
+                                                                  double* __restrict__ dots) {
+  constexpr int NT = VF == F64_F1 ? 1 : VF == F64_F2 ? 2 : 4;
+  constexpr int PPW = 64 / NT;   // points per wave
+  constexpr int NPB = PPW / 16;  // 16-point column blocks per type
+  constexpr bool HES = MODE == MODE_HESS, LAP = MODE == MODE_LAP;
+  // dots per point: F1 [rho]; F2 [L0]; FH [h x 3]; F4 [rho, g_x, g_y, g_z,
+  // (lap: t_xx, t_yy, t_zz | Hessian: t_xx, t_xy, t_yy, t_xz, t_yz, t_zz)]
+  constexpr int NDV = VF == F64_F1 || VF == F64_F2 ? 1 : VF == F64_F4 ? (LAP ? 7 : HES ? 10 : 4) : 3;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int r = lane & 15, q = lane >> 4;
+  const int64_t p0 = ((int64_t)blockIdx.x * 4 + wave) * PPW;
+  const int64_t sd = (int64_t)mp * ld;
+
+  // AO array of type t (0 = phi, 1..3 = d phi, 4 = lap phi | 4..9 = d_kl phi)
+  auto arr = [](int t) {
+    return VF == F64_F2 ? (t == 0 ? 0 : 4) : VF == F64_FH0 ? (t == 0 ? 0 : 3 + t) : VF == F64_FH1 ? (t == 0 ? 0 : 6 + t) : t;
+  };
+  const double* bptr[4];
+#pragma unroll
+  for (int cb = 0; cb < 4; ++cb) bptr[cb] = ao + arr(cb / NPB) * sd + (int64_t)q * ld + p0 + 16 * (cb % NPB) + r;
+
+  double dacc[NPB][NDV];
+#pragma unroll
+  for (int pb = 0; pb < NPB; ++pb)
+#pragma unroll
+    for (int k = 0; k < NDV; ++k) dacc[pb][k] = 0.0;
+
+  for (int i0 = 0; i0 < rows; i0 += 64) {
+    d4 acc[4][4];
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+      for (int cb = 0; cb < 4; ++cb) acc[rb][cb] = d4{0.0, 0.0, 0.0, 0.0};
+    // A[r][q] = L^T[i0 + 16 rb + r][k0 + q] = L[k0 + q][i0 + 16 rb + r]
+    const double* aptr = L + (int64_t)q * rows + i0 + r;
+    double a[4], b[4];
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb) a[rb] = aptr[16 * rb];
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) b[cb] = bptr[cb][0];
+    for (int k0 = 0; k0 < mp; k0 += 4) {
+      const int kn = (k0 + 4 < mp) ? k0 + 4 : k0;  // prefetch (clamped on the last step; unused)
+      double an[4], bn[4];
+#pragma unroll
+      for (int rb = 0; rb < 4; ++rb) an[rb] = aptr[(int64_t)kn * rows + 16 * rb];
+#pragma unroll
+      for (int cb = 0; cb < 4; ++cb) bn[cb] = bptr[cb][(int64_t)kn * ld];
+#pragma unroll
+      for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb) acc[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb], b[cb], acc[rb][cb], 0, 0, 0);
+#pragma unroll
+      for (int rb = 0; rb < 4; ++rb) a[rb] = an[rb];
+#pragma unroll
+      for (int cb = 0; cb < 4; ++cb) b[cb] = bn[cb];
+    }
+    // lane (r, q) holds rows q + 4 j of every 16-row block rb, column r of
+    // every column block: all column types of its point(s)
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+#pragma unroll
+        for (int pb = 0; pb < NPB; ++pb) {
+          const double y0 = acc[rb][pb][j];
+          if constexpr (VF == F64_F1) {
+            dacc[pb][0] += y0 * y0;
+          } else if constexpr (VF == F64_F2) {
+            dacc[pb][0] += y0 * acc[rb][NPB + pb][j];
+          } else if constexpr (VF == F64_FH0 || VF == F64_FH1) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) dacc[pb][k] += y0 * acc[rb][1 + k][j];
+          } else {
+            const double yx = acc[rb][1][j], yy = acc[rb][2][j], yz = acc[rb][3][j];
+            dacc[pb][0] += y0 * y0;
+            dacc[pb][1] += y0 * yx;
+            dacc[pb][2] += y0 * yy;
+            dacc[pb][3] += y0 * yz;
+            if constexpr (LAP) {
+              dacc[pb][4] += yx * yx;
+              dacc[pb][5] += yy * yy;
+              dacc[pb][6] += yz * yz;
+            }
+            if constexpr (HES) {
+              dacc[pb][4] += yx * yx;
+              dacc[pb][5] += yx * yy;
+              dacc[pb][6] += yy * yy;
+              dacc[pb][7] += yx * yz;
+              dacc[pb][8] += yy * yz;
+              dacc[pb][9] += yz * yz;
+            }
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int pb = 0; pb < NPB; ++pb)
+#pragma unroll
+    for (int k = 0; k < NDV; ++k) {
+      double v = dacc[pb][k];
+      v += __shfl_xor(v, 16);
+      v += __shfl_xor(v, 32);
+      dacc[pb][k] = v;
+    }
+  if (q == 0) {
+#pragma unroll
+    for (int pb = 0; pb < NPB; ++pb)
+#pragma unroll
+      for (int k = 0; k < NDV; ++k) {
+        // slots of the mode's dot layout (mode_ndot)
+        const int slot = VF == F64_F2    ? 4
+                         : VF == F64_FH0 ? 4 + k
+                         : VF == F64_FH1 ? 7 + k
+                         : k < 4         ? k
+                         : HES           ? k + 6
+                                         : k + 1;
+        dots[(int64_t)slot * ld + p0 + 16 * pb + r] = dacc[pb][k];
+      }
+  }
+}
+
+template <int VF, int MODE>
+void launch_factor_var(const double* L, int rows, int mp, const double* ao, int64_t ld, double* dots, hipStream_t s) {
+  constexpr int PPW = VF == F64_F1 ? 64 : VF == F64_F2 ? 32 : 16;
+  const dim3 block(256), grid((unsigned)(ld / (4 * PPW)));  // ld is a multiple of 256
+  hipLaunchKernelGGL((contract_f64_factor_kernel<VF, MODE>), grid, block, 0, s, L, rows, mp, ao, ld, dots);
+  CT_HIP_CHECK(hipGetLastError());
+}
+
 }  // namespace
+
+void launch_contract_f64_factor(int mode, const double* L, int rows_pad, int nbasis_pad, const double* ao,
+                                int64_t ld, double* dots, hipStream_t s) {
+  CT_REQUIRE(rows_pad % 64 == 0 && rows_pad > 0 && ld % 256 == 0, "factor padding / point stride");
+  switch (mode) {
+    case MODE_RHO:
+      launch_factor_var<F64_F1, MODE_RHO>(L, rows_pad, nbasis_pad, ao, ld, dots, s);
+      break;
+    case MODE_GRAD:
+      launch_factor_var<F64_F4, MODE_GRAD>(L, rows_pad, nbasis_pad, ao, ld, dots, s);
+      break;
+    case MODE_LAP:
+      launch_factor_var<F64_F4, MODE_LAP>(L, rows_pad, nbasis_pad, ao, ld, dots, s);
+      launch_factor_var<F64_F2, MODE_LAP>(L, rows_pad, nbasis_pad, ao, ld, dots, s);
+      break;
+    default:
+      launch_factor_var<F64_F4, MODE_HESS>(L, rows_pad, nbasis_pad, ao, ld, dots, s);
+      launch_factor_var<F64_FH0, MODE_HESS>(L, rows_pad, nbasis_pad, ao, ld, dots, s);
+      launch_factor_var<F64_FH1, MODE_HESS>(L, rows_pad, nbasis_pad, ao, ld, dots, s);
+      break;
+  }
+}
 
 void launch_contract_f64(int mode, const double* P, int nbasis_pad, const double* ao, int64_t ld, double* dots,
                          hipStream_t s) {

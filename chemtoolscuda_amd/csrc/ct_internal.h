@@ -122,6 +122,10 @@ void launch_ao_f64(int mode, int lmax, const ShellMeta* meta, const ShellGeom* g
 void launch_contract_f64(int mode, const double* P, int nbasis_pad, const double* ao, int64_t ld, double* dots,
                          hipStream_t s);
 void launch_finalize(const FinalizeArgs& a, hipStream_t s);
+// The same FP64 dots from a factor P = L L^T (L: nbasis_pad x rows_pad,
+// k-major, rows_pad a multiple of 64): rank(P) columns instead of nbasis.
+void launch_contract_f64_factor(int mode, const double* L, int rows_pad, int nbasis_pad, const double* ao,
+                                int64_t ld, double* dots, hipStream_t s);
 // The same contraction with one 64-row block of P per wave and an ordered
 // sum of the blocks' partial dots (for small point counts); part holds
 // contract_f64_split_workspace() doubles.

@@ -386,11 +386,19 @@ void Mol::upload_P() {
     mixed_fragment_order(Lb.data(), factor_rows_, Mp_, Lf.data());
     if (!d_Lb_.reserve(sizeof(uint16_t) * Lf.size())) throw Error(CT_E_NOMEM, "cannot allocate the bf16 factor of P");
     CT_HIP_CHECK(hipMemcpy(d_Lb_.get(), Lf.data(), sizeof(uint16_t) * Lf.size(), hipMemcpyHostToDevice));
+    // FP64 copy for the FP64 factor route (re-evaluation of the mixed factor
+    // route's flagged points; FP64 evaluation on request): L k-major,
+    // L64[k * rows + j], padded with zeros
+    std::vector<double> L64((size_t)Mp_ * factor_rows_, 0.0);
+    for (int j = 0; j < rank; ++j)
+      for (int i = 0; i < M_; ++i) L64[(size_t)i * factor_rows_ + j] = L[(size_t)j * M_ + i];
+    if (!d_L64_.reserve(sizeof(double) * L64.size())) throw Error(CT_E_NOMEM, "cannot allocate the factor of P");
+    CT_HIP_CHECK(hipMemcpy(d_L64_.get(), L64.data(), sizeof(double) * L64.size(), hipMemcpyHostToDevice));
   }
 }
 
-bool Mol::factor_route(int mode, int route) const {
-  if (route == CT_ROUTE_DENSITY) return false;
+bool Mol::factor_route(int mode, int route, bool mixed) const {
+  if (route == CT_ROUTE_DENSITY || (!mixed && route == CT_ROUTE_AUTO)) return false;  // FP64: P unless asked
   const bool avail = factor_rank_ >= 0;
   if (route == CT_ROUTE_FACTOR) {
     if (!avail)
@@ -615,11 +623,13 @@ int64_t Mol::chunk_points(int64_t per_point, int64_t n) const {
 
 // FP64 evaluation of `count` points of the current chunk (index = chunk-
 // relative positions), written into the chunk's outputs through the index map.
-void Mol::reevaluate(int mode, const double* pts, int64_t count, const int32_t* index, const FinalizeArgs& fin) {
+void Mol::reevaluate(int mode, const double* pts, int64_t count, const int32_t* index, const FinalizeArgs& fin,
+                     bool factor) {
   const int nder = mode_nder(mode), ndot = mode_ndot(mode);
   const int64_t ld = round_up(count, kPointAlign);
-  // few points: one wave per (16 or 64 points, 64-row block of P) so the GPU fills
-  const bool split = ld < 65536;
+  // few points: one wave per (16 or 64 points, 64-row block of P) so the GPU
+  // fills; the factor route re-evaluates with the FP64 factor kernel
+  const bool split = !factor && ld < 65536;
   const size_t split_doubles = split ? (size_t)contract_f64_split_workspace(mode, Mp_, ld) : 0;
   const size_t bytes = (size_t)ld * ((size_t)nder * Mp_ * 8 + (size_t)ndot * 8 + 24) + split_doubles * 8;
   if (!d_ws64_.reserve(bytes)) throw Error(CT_E_NOMEM, "cannot allocate the FP64 re-evaluation workspace");
@@ -630,7 +640,10 @@ void Mol::reevaluate(int mode, const double* pts, int64_t count, const int32_t* 
   launch_gather_points(pts, index, count, gp, stream_);
   launch_ao_f64(mode, lmax_, static_cast<const ShellMeta*>(d_meta_.get()), static_cast<const ShellGeom*>(d_geom_.get()),
                 static_cast<const PrimData*>(d_prim_.get()), nshell_, M_, Mp_, gp, count, ld, ao, 700.0, stream_);
-  if (split)
+  if (factor)
+    launch_contract_f64_factor(mode, static_cast<const double*>(d_L64_.get()), factor_rows_, Mp_, ao, ld, dots,
+                               stream_);
+  else if (split)
     launch_contract_f64_split(mode, static_cast<const double*>(d_P_.get()), Mp_, ao, ld, part, dots, stream_);
   else
     launch_contract_f64(mode, static_cast<const double*>(d_P_.get()), Mp_, ao, ld, dots, stream_);
@@ -657,7 +670,7 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
   const bool mixed = precision == CT_PREC_MIXED;
   const int mode = select_mode(q);
   const int nder = mode_nder(mode), ndot = mode_ndot(mode);
-  const bool use_factor = mixed && factor_route(mode, par.route);
+  const bool use_factor = factor_route(mode, par.route, mixed);
   const int nrt = (use_factor ? factor_rows_ : Mp_) / kRowBlock;  // row blocks of the mixed partials
   const bool pts_dev = cube == nullptr && is_device_pointer(points);
   constexpr size_t NS = sizeof(kSlots) / sizeof(kSlots[0]);
@@ -770,8 +783,12 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
       launch_ao_f64(mode, lmax_, meta, geom, prim, nshell_, M_, Mp_, pts, nc, ld, static_cast<double*>(ao), 700.0, stream_);
       timer_.end(stream_);
       timer_.begin(KernelTimer::CONTRACT, stream_);
-      launch_contract_f64(mode, static_cast<const double*>(d_P_.get()), Mp_, static_cast<const double*>(ao), ld,
-                          dots, stream_);
+      if (use_factor)
+        launch_contract_f64_factor(mode, static_cast<const double*>(d_L64_.get()), factor_rows_, Mp_,
+                                   static_cast<const double*>(ao), ld, dots, stream_);
+      else
+        launch_contract_f64(mode, static_cast<const double*>(d_P_.get()), Mp_, static_cast<const double*>(ao), ld,
+                            dots, stream_);
       timer_.end(stream_);
       timer_.begin(KernelTimer::FINALIZE, stream_);
       launch_finalize(fa, stream_);
@@ -813,7 +830,7 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
         const int64_t count = (int64_t) * reinterpret_cast<int32_t*>(h_small);
         if (count > 0) {
           timer_.begin(KernelTimer::REEVAL, stream_);
-          reevaluate(mode, pts, count, flag_idx, fa);
+          reevaluate(mode, pts, count, flag_idx, fa, use_factor);
           timer_.end(stream_);
           reevaluated += count;
         }
@@ -844,7 +861,7 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
       const int64_t c64 = chunk_points(per64, count);
       timer_.begin(KernelTimer::REEVAL, stream_);
       for (int64_t s0 = 0; s0 < count; s0 += c64)
-        reevaluate(mode, points, std::min(c64, count - s0), list + s0, fa);
+        reevaluate(mode, points, std::min(c64, count - s0), list + s0, fa, use_factor);
       timer_.end(stream_);
       reevaluated = count;
     }

@@ -123,9 +123,11 @@ class Mol {
   void upload_geometry();
   void upload_P();
   int64_t chunk_points(int64_t per_point, int64_t n) const;
-  void reevaluate(int mode, const double* pts, int64_t count, const int32_t* index, const FinalizeArgs& fin);
-  // mixed path: whether this mode runs on the factor of P (ct_eval_params.route)
-  bool factor_route(int mode, int route) const;
+  void reevaluate(int mode, const double* pts, int64_t count, const int32_t* index, const FinalizeArgs& fin,
+                  bool factor);
+  // whether this mode runs on the factor of P (ct_eval_params.route; the FP64
+  // path only when asked)
+  bool factor_route(int mode, int route, bool mixed) const;
 
   HostBasis basis_;
   int M_ = 0, Mp_ = 0, natom_ = 0, nshell_ = 0;
@@ -145,6 +147,7 @@ class Mol {
   DeviceBuffer d_meta_, d_geom_, d_prim_, d_P_;
   DeviceBuffer d_Pb_;      // mixed path: bf16 pieces of P (hi, mid, lo) in MFMA fragment order
   DeviceBuffer d_Lb_;      // mixed path: bf16 pieces of L^T (P = L L^T) in MFMA fragment order
+  DeviceBuffer d_L64_;     // L in FP64, k-major (nbasis_pad x factor rows)
   int factor_rank_ = -1;   // rank of the factor, -1 when P has none (not PSD or rank too high)
   int factor_rows_ = 0;    // rows of L^T padded to a multiple of 128
   DeviceBuffer d_ws_;      // AO + dots + staged points / outputs

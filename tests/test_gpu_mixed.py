@@ -266,8 +266,10 @@ def test_route_selection_and_errors(ct):
     mol = ct.molecule_from_basis(w.basis, w.P, w.charges)
     mol.compute(w.points, ["rho"], precision="mixed")
     assert mol.last_report()["factor_rank"] == -1  # rank 5 in a 128-row padding: the density route is cheaper
+    mol.compute(w.points, ["rho"], precision="fp64")
+    assert mol.last_report()["factor_rank"] == -1  # the FP64 path contracts with P unless asked
     mol.compute(w.points, ["rho"], precision="fp64", route="factor")
-    assert mol.last_report()["factor_rank"] == -1  # the FP64 path always contracts with P
+    assert mol.last_report()["factor_rank"] == 5
     # an indefinite P has no factor: the factor route is refused, auto falls back
     P2 = w.P.copy()
     P2[0, 0] -= 10.0
@@ -360,3 +362,32 @@ def test_factor_route_hessian_pointwise(ct, workload, npts):
     assert np.max(np.abs(mix["rho"][m] - rho) / rho) <= 1e-6
     sh = f64["shannon"][m]
     assert np.max(np.abs(mix["shannon"][m] - sh) / (np.abs(sh) + rho)) <= 1e-6
+
+
+@pytest.mark.parametrize("workload,npts", [("water", 3000), ("organic60", 4000)])
+def test_fp64_factor_route_matches_density_route(ct, workload, npts):
+    """FP64 on the factor route (route="factor"; also what re-evaluates the
+    mixed factor route's flagged points) against FP64 on P, every mode: the
+    same quantities summed in another order -- rho to 1e-12 relative, the
+    others to 1e-11 of the magnitude of their terms (the Laplacian against
+    |lap - 4 tau| + 4 tau, the rest against their norm floored at 1e-10 of the
+    largest)."""
+    from chemtoolscuda_amd import synthetic
+
+    w = synthetic.WORKLOADS[workload]()
+    rng = np.random.default_rng(14)
+    pts = w.points[rng.choice(w.points.shape[0], size=min(npts, w.points.shape[0]), replace=False)]
+    mol = ct.molecule_from_basis(w.basis, w.P, w.charges)
+    for names in (["rho"], ["rho", "grad"], ["rho", "grad", "lap", "tau_pd"], ["rho", "grad", "hess"]):
+        a = mol.compute(pts, names, precision="fp64", route="factor")
+        assert mol.last_report()["factor_rank"] > 0
+        b = mol.compute(pts, names, precision="fp64")
+        for k in names:
+            x = np.asarray(a[k]).reshape(len(pts), -1)
+            y = np.asarray(b[k]).reshape(len(pts), -1)
+            den = np.maximum(np.linalg.norm(y, axis=1), 1e-10 * np.abs(y).max())
+            if k == "lap":
+                tau = b["tau_pd"]
+                den = np.abs(b["lap"] - 4 * tau) + 4 * tau
+            tol = 1e-12 if k == "rho" else 1e-11
+            assert np.max(np.linalg.norm(x - y, axis=1) / den) < tol, (names, k)
