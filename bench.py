@@ -288,11 +288,22 @@ def main():
     extra = {}
     if precision == "mixed" and not args.no_compare:
         ref = alloc()
+        run(ref, "fp64")  # warm-up (workspace growth) and the reference values
+        torch.cuda.synchronize()
         t1 = time.perf_counter()
         run(ref, "fp64")
         torch.cuda.synchronize()
         extra["fp64_points_per_s"] = n / (time.perf_counter() - t1)
         extra["mixed_over_fp64"] = value / extra["fp64_points_per_s"]
+        if rank >= 0:  # FP64 on the factor route (on request only: the FP64 default contracts with P)
+            tmp = alloc()
+            h.eval(pts.data_ptr(), n, {q: t.data_ptr() for q, t in tmp.items()}, "fp64", {"route": 2})
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            h.eval(pts.data_ptr(), n, {q: t.data_ptr() for q, t in tmp.items()}, "fp64", {"route": 2})
+            torch.cuda.synchronize()
+            extra["fp64_factor_route_points_per_s"] = n / (time.perf_counter() - t2)
+            del tmp
         for q, v in _max_errors(torch, outs, ref, quantities).items():
             extra[f"max_rel_err_{q}"] = v
         extra["n_reevaluated"] = rep.get("n_reevaluated") if rep else None

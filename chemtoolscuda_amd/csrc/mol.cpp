@@ -237,11 +237,6 @@ Mol::Mol(const HostBasis& basis, const double* atom_xyz, const double* charges, 
 
   CT_HIP_CHECK(hipGetDevice(&device_));
   CT_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
-  CT_HIP_CHECK(hipStreamCreateWithFlags(&ao_stream_, hipStreamNonBlocking));
-  for (int b = 0; b < 2; ++b) {
-    CT_HIP_CHECK(hipEventCreateWithFlags(&ev_ao_[b], hipEventDisableTiming));
-    CT_HIP_CHECK(hipEventCreateWithFlags(&ev_free_[b], hipEventDisableTiming));
-  }
   if (!d_meta_.reserve(sizeof(ShellMeta) * nshell_) || !d_prim_.reserve(sizeof(PrimData) * prim_.size()) ||
       !d_geom_.reserve(sizeof(ShellGeom) * nshell_) || !d_P_.reserve(sizeof(double) * (size_t)Mp_ * Mp_))
     throw Error(CT_E_NOMEM, "cannot allocate the device shell table / density matrix");
@@ -252,17 +247,9 @@ Mol::Mol(const HostBasis& basis, const double* atom_xyz, const double* charges, 
 }
 
 Mol::~Mol() {
-  if (ao_stream_) {
-    (void)hipStreamSynchronize(ao_stream_);
-    (void)hipStreamDestroy(ao_stream_);
-  }
   if (stream_) {
     (void)hipStreamSynchronize(stream_);
     (void)hipStreamDestroy(stream_);
-  }
-  for (int b = 0; b < 2; ++b) {
-    if (ev_ao_[b]) (void)hipEventDestroy(ev_ao_[b]);
-    if (ev_free_[b]) (void)hipEventDestroy(ev_free_[b]);
   }
 }
 
@@ -683,18 +670,12 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
 
   // With device-resident points and outputs, flagged points of all chunks are
   // collected in one list and re-evaluated in a single FP64 pass at the end
-  // (one large launch instead of one small launch per chunk), and on the
-  // factor route the AO evaluation of chunk c + 1 overlaps the contraction of
-  // chunk c (two AO buffers, the AO kernels on their own stream; measured
-  // +2 % there, -2 % on the density route's larger contractions, whose chunks
-  // the second buffer halves).
+  // (one large launch instead of one small launch per chunk).
   bool all_dev = pts_dev && n < (int64_t(1) << 31);  // (grid points are not materialised: per-chunk re-evaluation)
   for (size_t k = 0; k < NS; ++k) all_dev = all_dev && out_dev[k];
   const bool batch = mixed && all_dev;
-  const bool overlap = batch && use_factor;
   const int64_t ao_bytes_pt = (int64_t)nder * Mp_ * (mixed ? 4 : 8);
-  const int64_t per_point = mixed ? ao_bytes_pt * (overlap ? 2 : 1) + (int64_t)nrt * (ndot + 1) * 8 + 4 + 24 +
-                                        stage_width * 8
+  const int64_t per_point = mixed ? ao_bytes_pt + (int64_t)nrt * (ndot + 1) * 8 + 4 + 24 + stage_width * 8
                                   : ao_bytes_pt + (int64_t)ndot * 8 + 24 + stage_width * 8;
   int64_t Nc = chunk_points(per_point, n);
   const size_t head = 256;  // flag counter + max estimate
@@ -708,11 +689,6 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
   char* cur = ws + head;
   void* ao = cur;
   cur += (size_t)Nc * ao_bytes_pt;
-  void* ao_buf[2] = {ao, ao};
-  if (overlap) {
-    ao_buf[1] = cur;
-    cur += (size_t)Nc * ao_bytes_pt;
-  }
   double* dots = reinterpret_cast<double*>(cur);
   cur += (size_t)Nc * (mixed ? nrt * (ndot + 1) : ndot) * 8;
   double* pts_stage = reinterpret_cast<double*>(cur);
@@ -734,13 +710,6 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
   if (mixed) {
     CT_HIP_CHECK(hipMemsetAsync(d_maxest, 0, 8, stream_));
     CT_HIP_CHECK(hipMemsetAsync(d_count, 0, 4, stream_));
-  }
-  if (overlap) {
-    // the AO stream starts after everything already queued on the main stream
-    // (earlier calls' use of the workspace); ev_free_[b] then marks buffer b
-    // free for the AO stream after each contraction that reads it
-    CT_HIP_CHECK(hipEventRecord(ev_free_[0], stream_));
-    CT_HIP_CHECK(hipEventRecord(ev_free_[1], stream_));
   }
 
   int64_t chunks = 0, reevaluated = 0;
@@ -794,17 +763,10 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
       launch_finalize(fa, stream_);
       timer_.end(stream_);
     } else {
-      const int b = overlap ? (int)(chunks & 1) : 0;
-      float* aoc = static_cast<float*>(ao_buf[b]);
-      hipStream_t as = overlap ? ao_stream_ : stream_;
-      if (overlap) CT_HIP_CHECK(hipStreamWaitEvent(ao_stream_, ev_free_[b], 0));
-      timer_.begin(KernelTimer::AO, as);
-      launch_ao_f32(mode, lmax_, meta, geom, prim, nshell_, M_, Mp_, pts, nc, ld, aoc, 50.0, as);
-      timer_.end(as);
-      if (overlap) {
-        CT_HIP_CHECK(hipEventRecord(ev_ao_[b], ao_stream_));
-        CT_HIP_CHECK(hipStreamWaitEvent(stream_, ev_ao_[b], 0));
-      }
+      float* aoc = static_cast<float*>(ao);
+      timer_.begin(KernelTimer::AO, stream_);
+      launch_ao_f32(mode, lmax_, meta, geom, prim, nshell_, M_, Mp_, pts, nc, ld, aoc, 50.0, stream_);
+      timer_.end(stream_);
       timer_.begin(KernelTimer::CONTRACT, stream_);
       if (use_factor)
         launch_contract_mixed_factor(mode, static_cast<const uint16_t*>(d_Lb_.get()), factor_rows_, Mp_, aoc, ld,
@@ -812,7 +774,6 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
       else
         launch_contract_mixed(mode, static_cast<const uint16_t*>(d_Pb_.get()), Mp_, aoc, ld, dots, stream_);
       timer_.end(stream_);
-      if (overlap) CT_HIP_CHECK(hipEventRecord(ev_free_[b], stream_));
       FlagArgs fl;
       fl.gamma = use_factor ? kMixedGammaFactor : mixed_gamma(mode);
       fl.tol = par.mixed_tol;

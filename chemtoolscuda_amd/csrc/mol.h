@@ -138,12 +138,6 @@ class Mol {
   std::vector<PrimData> prim_;
   int device_ = 0;
   hipStream_t stream_ = nullptr;
-  // mixed path with device-resident points and outputs: the AO evaluation of
-  // the next chunk runs on its own stream (HBM-bound) while the current chunk
-  // is contracted (matrix-core-bound), into the other of two AO buffers
-  hipStream_t ao_stream_ = nullptr;
-  hipEvent_t ev_ao_[2] = {nullptr, nullptr};    // AO buffer b written
-  hipEvent_t ev_free_[2] = {nullptr, nullptr};  // AO buffer b read by its contraction
   DeviceBuffer d_meta_, d_geom_, d_prim_, d_P_;
   DeviceBuffer d_Pb_;      // mixed path: bf16 pieces of P (hi, mid, lo) in MFMA fragment order
   DeviceBuffer d_Lb_;      // mixed path: bf16 pieces of L^T (P = L L^T) in MFMA fragment order

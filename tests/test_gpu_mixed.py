@@ -283,11 +283,11 @@ def test_route_selection_and_errors(ct):
 
 
 def test_device_resident_chunks_match_host_path(ct):
-    """Device-resident points and outputs take the pipelined path (the AO
-    evaluation of chunk c + 1 on its own stream, overlapping the contraction of
-    chunk c, two AO buffers, flagged points batched over the call); host arrays
-    take the one-stream path.  Per point both run the same kernels, so the
-    results agree bit for bit, also with many small chunks."""
+    """Device-resident points and outputs take the batched path (flagged
+    points of all chunks re-evaluated in one FP64 pass at the end); host arrays
+    re-evaluate chunk by chunk through pinned staging.  Per point both run the
+    same kernels, so the results agree bit for bit, also with many small
+    chunks."""
     import torch
     from chemtoolscuda_amd import capi, synthetic
 
@@ -300,7 +300,7 @@ def test_device_resident_chunks_match_host_path(ct):
     host = {q: np.zeros((n, capi.WIDTH.get(q, 1))) for q in names}
     h.eval(pts.ctypes.data, n, {q: a.ctypes.data for q, a in host.items()}, "mixed")
     dev_pts = torch.from_numpy(pts).cuda()
-    for limit_mb in (0, 64):  # library default (one chunk), then ~2 600 points per chunk
+    for limit_mb in (0, 24):  # library default (one chunk), then ~1 800 points per chunk
         if limit_mb:
             h.set_workspace_limit(limit_mb << 20)
         dev = {q: torch.empty((n, capi.WIDTH.get(q, 1)), dtype=torch.float64, device="cuda") for q in names}
