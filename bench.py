@@ -284,6 +284,14 @@ def main():
         a_exec = f_exec * pts_per_launch / (contract_ms * 1e-3)
         roofline["factor_rank"] = rank
         roofline["executed"] = {"flops_per_point": f_exec, "achieved": a_exec / 1e12, "frac": a_exec / PEAK[peak_key]}
+        # spatial screening: fraction of (point tile, k-step) blocks contracted
+        act = rep.get("active_fraction", 1.0) if rep else 1.0
+        roofline["active_fraction"] = act
+        if act < 1.0:
+            f_scr = 2.0 * M * rank * 5 * act + 2.0 * rank * 8
+            a_scr = f_scr * pts_per_launch / (contract_ms * 1e-3)
+            roofline["executed_screened"] = {"flops_per_point": f_scr, "achieved": a_scr / 1e12,
+                                             "frac": a_scr / PEAK[peak_key]}
 
     extra = {}
     if precision == "mixed" and not args.no_compare:
@@ -314,7 +322,7 @@ def main():
         extra["esp"] = esp_rate(torch, capi, synthetic)
         extra["large300"] = large_rate(torch, capi, synthetic)
 
-    kernel_ms = {k: stats[f"{k}_ms"] / args.steps for k in ("ao", "contract", "finalize", "reeval")}
+    kernel_ms = {k: stats[f"{k}_ms"] / args.steps for k in ("screen", "ao", "contract", "finalize", "reeval")}
     line = {
         "metric": meta["metric"],
         "value": value,

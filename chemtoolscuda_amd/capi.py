@@ -39,12 +39,14 @@ class Outputs(ctypes.Structure):
 
 class EvalReport(ctypes.Structure):
     _fields_ = [("n_points", ctypes.c_int64), ("n_reevaluated", ctypes.c_int64), ("n_chunks", ctypes.c_int64),
-                ("max_error_estimate", ctypes.c_double), ("factor_rank", ctypes.c_int64)]
+                ("max_error_estimate", ctypes.c_double), ("factor_rank", ctypes.c_int64),
+                ("active_fraction", ctypes.c_double)]
 
 
 class KernelStats(ctypes.Structure):
     _fields_ = [(f"{k}_ms", ctypes.c_double) for k in ("ao", "contract", "finalize", "esp", "backward", "reeval")] + \
-               [(f"{k}_calls", ctypes.c_int64) for k in ("ao", "contract", "finalize", "esp", "backward", "reeval")]
+               [(f"{k}_calls", ctypes.c_int64) for k in ("ao", "contract", "finalize", "esp", "backward", "reeval")] + \
+               [("screen_ms", ctypes.c_double), ("screen_calls", ctypes.c_int64)]
 
 
 _lib = None
@@ -76,6 +78,8 @@ def lib():
         L.ct_mol_set_coordinates.argtypes = [vp, vp]
         L.ct_mol_set_density_matrix.argtypes = [vp, vp]
         L.ct_mol_set_workspace_limit.argtypes = [vp, i64]
+        L.ct_mol_set_screening.argtypes = [vp, ctypes.c_double]
+        L.ct_shell_screening_radius.argtypes = [i32, i32, vp, vp, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
         L.ct_mol_synchronize.argtypes = [vp]
         L.ct_mol_enable_profiling.argtypes = [vp, ctypes.c_int]
         L.ct_mol_kernel_stats.argtypes = [vp, ctypes.POINTER(KernelStats)]
@@ -149,6 +153,9 @@ class Handle:
     def set_workspace_limit(self, nbytes: int):
         check(lib().ct_mol_set_workspace_limit(self.ptr, int(nbytes)))
 
+    def set_screening(self, eps: float):
+        check(lib().ct_mol_set_screening(self.ptr, float(eps)))
+
     def synchronize(self):
         check(lib().ct_mol_synchronize(self.ptr))
 
@@ -187,3 +194,15 @@ def read_wavefunction(path: str) -> dict:
     finally:
         L.ct_wavefunction_destroy(h)
     return out
+
+
+def shell_screening_radius(l: int, exps, coefs, eps: float) -> float:
+    """Cutoff radius of one shell (host only): beyond it every function of the
+    shell and its first and second derivatives are at most eps in magnitude."""
+    import numpy as np
+
+    a = np.ascontiguousarray(exps, dtype=np.float64)
+    c = np.ascontiguousarray(coefs, dtype=np.float64)
+    r = ctypes.c_double()
+    check(lib().ct_shell_screening_radius(int(l), a.size, a.ctypes.data, c.ctypes.data, float(eps), ctypes.byref(r)))
+    return r.value

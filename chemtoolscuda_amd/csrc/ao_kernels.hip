@@ -18,6 +18,9 @@
 //     when every lane has a r^2 > screen (exp underflow region in parity
 //     mode; below FP32 resolution of any density above the re-evaluation
 //     floor in the mixed path);
+//   * in the screened layout of the mixed factor route (points in Morton
+//     order, per-tile activity masks) a wave skips, stores included, every
+//     shell that lies only in k-steps the contraction skips for its points;
 //   * values are computed in FP64 for both paths; the mixed path rounds them
 //     once to FP32 on the store (0.5 ulp per value: the FP32 AO error that
 //     enters the mixed path's error estimate is only this rounding).
@@ -47,7 +50,9 @@ __global__ __launch_bounds__(256) void ao_eval_kernel(const ShellMeta* __restric
                                                       const ShellGeom* __restrict__ geom,
                                                       const PrimData* __restrict__ prim, int nshell, int nbasis,
                                                       int nbasis_pad, const double* __restrict__ pts, int64_t npts,
-                                                      int64_t ld, TS* __restrict__ ao, double screen) {
+                                                      int64_t ld, TS* __restrict__ ao, double screen,
+                                                      const uint32_t* __restrict__ kmask, int nwords,
+                                                      int tile_points) {
   using T = double;
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // p < ld by construction
   const int64_t pc = p < npts ? p : (npts - 1);
@@ -74,8 +79,28 @@ __global__ __launch_bounds__(256) void ao_eval_kernel(const ShellMeta* __restric
     if constexpr (MODE == MODE_HESS) ao_shell_dispatch_hess<T, LMAX>(code, x, y, z, s0, s1, s2, st);
   };
 
+  // screened layout: the activity masks of the (one or two) point tiles of
+  // this wave's 64 points
+  const uint32_t* mask0 = nullptr;
+  const uint32_t* mask1 = nullptr;
+  if (kmask) {
+    const int64_t p0 = (int64_t)blockIdx.x * blockDim.x + 64 * __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    mask0 = kmask + (p0 / tile_points) * nwords;
+    mask1 = kmask + ((p0 + 63) / tile_points) * nwords;
+  }
+  constexpr int kNfunc[8] = {1, 3, 6, 10, 15, 5, 7, 9};  // per shell code
+
   for (int s = 0; s < nshell; ++s) {
     const ShellMeta m = meta[s];
+    if (kmask) {
+      // skipped: no k-step that the shell touches is read by the contraction
+      // for these points (a quad it shares with a neighbouring shell lies in
+      // one of those k-steps, so storing it or not does not matter either)
+      const int k0 = m.first >> 5, k1 = (m.first + kNfunc[m.code] - 1) >> 5;
+      const uint32_t b0 = 1u << (k0 & 31), b1 = 1u << (k1 & 31);
+      const bool need = ((mask0[k0 >> 5] | mask1[k0 >> 5]) & b0) || ((mask0[k1 >> 5] | mask1[k1 >> 5]) & b1);
+      if (!need) continue;
+    }
     const ShellGeom g = geom[s];
     const double dx = px - g.cx, dy = py - g.cy, dz = pz - g.cz;
     const double r2d = dx * dx + dy * dy + dz * dz;
@@ -147,36 +172,41 @@ __global__ __launch_bounds__(256) void ao_eval_kernel(const ShellMeta* __restric
 
 template <typename TS, int MODE>
 void launch_lmax(int lmax, const ShellMeta* meta, const ShellGeom* geom, const PrimData* prim, int nshell, int nbasis,
-                 int nbasis_pad, const double* pts, int64_t npts, int64_t ld, TS* ao, double screen, hipStream_t s) {
+                 int nbasis_pad, const double* pts, int64_t npts, int64_t ld, TS* ao, double screen,
+                 const uint32_t* kmask, int nwords, int tile_points, hipStream_t s) {
   const dim3 block(256), grid((unsigned)(ld / 256));
   if (lmax <= 2)
     hipLaunchKernelGGL((ao_eval_kernel<TS, MODE, 2>), grid, block, 0, s, meta, geom, prim, nshell, nbasis, nbasis_pad,
-                       pts, npts, ld, ao, screen);
+                       pts, npts, ld, ao, screen, kmask, nwords, tile_points);
   else if (lmax == 3)
     hipLaunchKernelGGL((ao_eval_kernel<TS, MODE, 3>), grid, block, 0, s, meta, geom, prim, nshell, nbasis, nbasis_pad,
-                       pts, npts, ld, ao, screen);
+                       pts, npts, ld, ao, screen, kmask, nwords, tile_points);
   else
     hipLaunchKernelGGL((ao_eval_kernel<TS, MODE, 4>), grid, block, 0, s, meta, geom, prim, nshell, nbasis, nbasis_pad,
-                       pts, npts, ld, ao, screen);
+                       pts, npts, ld, ao, screen, kmask, nwords, tile_points);
   CT_HIP_CHECK(hipGetLastError());
 }
 
 template <typename TS>
 void launch_mode(int mode, int lmax, const ShellMeta* meta, const ShellGeom* geom, const PrimData* prim, int nshell,
                  int nbasis, int nbasis_pad, const double* pts, int64_t npts, int64_t ld, TS* ao, double screen,
-                 hipStream_t s) {
+                 const uint32_t* kmask, int nwords, int tile_points, hipStream_t s) {
   switch (mode) {
     case MODE_RHO:
-      launch_lmax<TS, MODE_RHO>(lmax, meta, geom, prim, nshell, nbasis, nbasis_pad, pts, npts, ld, ao, screen, s);
+      launch_lmax<TS, MODE_RHO>(lmax, meta, geom, prim, nshell, nbasis, nbasis_pad, pts, npts, ld, ao, screen, kmask,
+                          nwords, tile_points, s);
       break;
     case MODE_GRAD:
-      launch_lmax<TS, MODE_GRAD>(lmax, meta, geom, prim, nshell, nbasis, nbasis_pad, pts, npts, ld, ao, screen, s);
+      launch_lmax<TS, MODE_GRAD>(lmax, meta, geom, prim, nshell, nbasis, nbasis_pad, pts, npts, ld, ao, screen, kmask,
+                          nwords, tile_points, s);
       break;
     case MODE_LAP:
-      launch_lmax<TS, MODE_LAP>(lmax, meta, geom, prim, nshell, nbasis, nbasis_pad, pts, npts, ld, ao, screen, s);
+      launch_lmax<TS, MODE_LAP>(lmax, meta, geom, prim, nshell, nbasis, nbasis_pad, pts, npts, ld, ao, screen, kmask,
+                          nwords, tile_points, s);
       break;
     default:
-      launch_lmax<TS, MODE_HESS>(lmax, meta, geom, prim, nshell, nbasis, nbasis_pad, pts, npts, ld, ao, screen, s);
+      launch_lmax<TS, MODE_HESS>(lmax, meta, geom, prim, nshell, nbasis, nbasis_pad, pts, npts, ld, ao, screen, kmask,
+                          nwords, tile_points, s);
       break;
   }
 }
@@ -186,13 +216,17 @@ void launch_mode(int mode, int lmax, const ShellMeta* meta, const ShellGeom* geo
 void launch_ao_f64(int mode, int lmax, const ShellMeta* meta, const ShellGeom* geom, const PrimData* prim, int nshell,
                    int nbasis, int nbasis_pad, const double* pts, int64_t npts, int64_t ld, double* ao,
                    double screen, hipStream_t s) {
-  launch_mode<double>(mode, lmax, meta, geom, prim, nshell, nbasis, nbasis_pad, pts, npts, ld, ao, screen, s);
+  launch_mode<double>(mode, lmax, meta, geom, prim, nshell, nbasis, nbasis_pad, pts, npts, ld, ao, screen, nullptr, 0,
+                      64, s);
 }
 
 void launch_ao_f32(int mode, int lmax, const ShellMeta* meta, const ShellGeom* geom, const PrimData* prim, int nshell,
                    int nbasis, int nbasis_pad, const double* pts, int64_t npts, int64_t ld, float* ao, double screen,
-                   hipStream_t s) {
-  launch_mode<float>(mode, lmax, meta, geom, prim, nshell, nbasis, nbasis_pad, pts, npts, ld, ao, screen, s);
+                   const uint32_t* kmask, int nwords, int tile_points, hipStream_t s) {
+  CT_REQUIRE(!kmask || (tile_points > 0 && 64 % tile_points == 0 && nwords == (nbasis_pad / 32 + 31) / 32),
+             "activity masks do not match the AO kernel's waves");
+  launch_mode<float>(mode, lmax, meta, geom, prim, nshell, nbasis, nbasis_pad, pts, npts, ld, ao, screen, kmask,
+                     nwords, tile_points, s);
 }
 
 }  // namespace ct

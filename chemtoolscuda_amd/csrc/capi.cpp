@@ -5,6 +5,7 @@
 // "Caught an unknown exception!").
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <string>
 
@@ -221,6 +222,28 @@ int ct_mol_set_workspace_limit(ct_mol* mol, int64_t bytes) {
   return guard([&] {
     CT_REQUIRE(bytes > 0, "workspace limit must be positive");
     M(mol).set_workspace_limit(bytes);
+  });
+}
+
+int ct_mol_set_screening(ct_mol* mol, double eps) {
+  return guard([&] {
+    CT_REQUIRE(eps >= 0.0 && std::isfinite(eps), "screening threshold must be finite and non-negative");
+    M(mol).set_screening(eps);
+  });
+}
+
+int ct_shell_screening_radius(int32_t l, int32_t nprim, const double* exps, const double* coefs, double eps,
+                              double* radius) {
+  return guard([&] {
+    CT_REQUIRE(exps && coefs && radius, "null pointer");
+    CT_REQUIRE(nprim > 0, "shell without primitives");
+    (void)ct::shell_nfunc(l);  // rejects unsupported l
+    std::vector<double> cn(nprim);
+    for (int p = 0; p < nprim; ++p) {
+      CT_REQUIRE(exps[p] > 0.0 && std::isfinite(exps[p]), "exponents must be positive and finite");
+      cn[p] = coefs[p] * ct::prim_radial_norm(exps[p], std::abs(l));
+    }
+    *radius = ct::shell_screen_radius(l, nprim, exps, cn.data(), eps);
   });
 }
 

@@ -202,7 +202,9 @@ template <int V, int DT>
 __global__ __launch_bounds__(256, 2) void contract_mixed_kernel(const u4* __restrict__ Pf, int rows, int mp,
                                                                 const float* __restrict__ ao, int64_t ld,
                                                                 double* __restrict__ part, int nrt,
-                                                                int64_t nptile) {
+                                                                int64_t nptile, const int32_t* __restrict__ klist,
+                                                                int64_t kstride, const int32_t* __restrict__ kcount,
+                                                                int kcount_all) {
   using C = Cols<V, DT>;
   constexpr int TC = C::TC, NCB = C::NCB, NBC = C::NBC;
   constexpr int ND = mode_ndot(C::MODE);
@@ -230,7 +232,11 @@ __global__ __launch_bounds__(256, 2) void contract_mixed_kernel(const u4* __rest
   YTile<TC, NBC, C::F64> yt;
   const int bc = tid & (TC - 1);
   const float* bsrc = ao + ((int64_t)C::type(bc) * mq * ld + pbase + C::point(bc)) * 4;
-  y_tile<TC, NBC, C::F64, C::TRI>(Pf, rows, mp, ld, i0, tid, bsrc, Bs, yt, C::TRI ? i0 / KB : 0);
+  // factor route: the point tile's list of k-steps (screened: its own list
+  // and length; else one shared list of all k-steps, kstride 0)
+  const int32_t* kl = C::FAC ? klist + ptile * kstride : nullptr;
+  const int kc = C::FAC ? (kcount ? kcount[ptile] : kcount_all) : 0;
+  y_tile<TC, NBC, C::F64, C::TRI, C::FAC>(Pf, rows, mp, ld, i0, tid, bsrc, Bs, yt, C::TRI ? i0 / KB : 0, kl, kc);
   if constexpr (C::FAC) {
     factor_epilogue<V, DT>(yt, &Bs[0][0][0][0], part, ld, rt, pbase, tid);
     return;
@@ -315,12 +321,15 @@ __global__ __launch_bounds__(256, 2) void contract_mixed_kernel(const u4* __rest
 // A: rows x mp (P: rows = mp); the workgroups cover rows / 128 row blocks
 // times ld / PTS point tiles.
 template <int V, int DT = 0>
-void launch_var(const u4* A, int rows, int mp, const float* ao, int64_t ld, double* part, hipStream_t s) {
+void launch_var(const u4* A, int rows, int mp, const float* ao, int64_t ld, double* part, hipStream_t s,
+                const int32_t* klist = nullptr, int64_t kstride = 0, const int32_t* kcount = nullptr,
+                int kcount_all = 0) {
   using C = Cols<V, DT>;
   const int nrt = rows / TR;
   const int64_t nptile = ld / C::PTS;  // ld is a multiple of 256
   const dim3 block(256), grid((unsigned)(((nptile + 7) / 8) * 8 * nrt));
-  hipLaunchKernelGGL((contract_mixed_kernel<V, DT>), grid, block, 0, s, A, rows, mp, ao, ld, part, nrt, nptile);
+  hipLaunchKernelGGL((contract_mixed_kernel<V, DT>), grid, block, 0, s, A, rows, mp, ao, ld, part, nrt, nptile, klist,
+                     kstride, kcount, kcount_all);
   CT_HIP_CHECK(hipGetLastError());
 }
 
@@ -365,26 +374,45 @@ void launch_contract_mixed(int mode, const uint16_t* Pf, int nbasis_pad, const f
   }
 }
 
+int factor_tile_points(int mode) {
+  switch (mode) {
+    case MODE_RHO: return Cols<V_F1, 0>::PTS;
+    case MODE_GRAD: return Cols<V_F4, MODE_GRAD>::PTS;
+    case MODE_LAP:
+      static_assert(Cols<V_F4, MODE_LAP>::PTS == Cols<V_F2, 0>::PTS, "one tile size per mode");
+      return Cols<V_F4, MODE_LAP>::PTS;
+    default:
+      static_assert(Cols<V_F4, MODE_HESS>::PTS == Cols<V_FH, 0>::PTS, "one tile size per mode");
+      return Cols<V_F4, MODE_HESS>::PTS;
+  }
+}
+
 void launch_contract_mixed_factor(int mode, const uint16_t* Lf, int rows_pad, int nbasis_pad, const float* ao,
-                                  int64_t ld, double* part, hipStream_t s) {
+                                  int64_t ld, double* part, const int32_t* klist, const int32_t* kcount,
+                                  hipStream_t s) {
+  CT_REQUIRE(klist != nullptr, "the factor route needs a k-step list");
+  // screened: per-tile lists of stride nk and their lengths; else one list
+  // of all nk k-steps shared by every tile
+  const int nk = nbasis_pad / KB;
+  const int64_t kstride = kcount ? nk : 0;
   CT_REQUIRE(nbasis_pad % TR == 0 && rows_pad % TR == 0 && rows_pad > 0, "factor / basis padding must be multiples of 128");
   CT_REQUIRE(ld % 256 == 0, "point stride must be a multiple of 256");
   const u4* L = reinterpret_cast<const u4*>(Lf);
   switch (mode) {
     case MODE_RHO:
-      launch_var<V_F1>(L, rows_pad, nbasis_pad, ao, ld, part, s);
+      launch_var<V_F1>(L, rows_pad, nbasis_pad, ao, ld, part, s, klist, kstride, kcount, nk);
       break;
     case MODE_GRAD:
-      launch_var<V_F4, MODE_GRAD>(L, rows_pad, nbasis_pad, ao, ld, part, s);
+      launch_var<V_F4, MODE_GRAD>(L, rows_pad, nbasis_pad, ao, ld, part, s, klist, kstride, kcount, nk);
       break;
     case MODE_LAP:
-      launch_var<V_F4, MODE_LAP>(L, rows_pad, nbasis_pad, ao, ld, part, s);
-      launch_var<V_F2>(L, rows_pad, nbasis_pad, ao, ld, part, s);
+      launch_var<V_F4, MODE_LAP>(L, rows_pad, nbasis_pad, ao, ld, part, s, klist, kstride, kcount, nk);
+      launch_var<V_F2>(L, rows_pad, nbasis_pad, ao, ld, part, s, klist, kstride, kcount, nk);
       break;
     default:
-      launch_var<V_F4, MODE_HESS>(L, rows_pad, nbasis_pad, ao, ld, part, s);
-      launch_var<V_FH, 0>(L, rows_pad, nbasis_pad, ao, ld, part, s);
-      launch_var<V_FH, 1>(L, rows_pad, nbasis_pad, ao, ld, part, s);
+      launch_var<V_F4, MODE_HESS>(L, rows_pad, nbasis_pad, ao, ld, part, s, klist, kstride, kcount, nk);
+      launch_var<V_FH, 0>(L, rows_pad, nbasis_pad, ao, ld, part, s, klist, kstride, kcount, nk);
+      launch_var<V_FH, 1>(L, rows_pad, nbasis_pad, ao, ld, part, s, klist, kstride, kcount, nk);
       break;
   }
 }

@@ -105,7 +105,8 @@ __host__ __device__ inline int64_t ao_quad_index(int d, int i, int64_t p, int mp
 // Kernel launchers (defined in the .hip files).
 struct FinalizeArgs {
   const double* dots;     // FP64 path: [ndot][ld];  mixed path: partials [nrt][ndot+1][ld]
-  const int32_t* index;   // optional: output index of point p is out_off + index[p]
+  const int32_t* index;   // optional: output index of point p is out_off + index[p] (FP64: the
+                          // re-evaluated points; mixed: the points in Morton order)
   int nrt;                // mixed path: number of 128-row blocks to sum
   int64_t ld;
   int64_t n;        // points in this chunk
@@ -134,9 +135,12 @@ void launch_contract_f64_split(int mode, const double* P, int nbasis_pad, const 
                                double* part, double* dots, hipStream_t s);
 
 // Mixed path.
+// kmask (optional, screened layout): the activity masks of the point tiles of
+// tile_points points (launch_tile_activity); a wave skips a shell, stores
+// included, when no k-step the shell touches is active for its points.
 void launch_ao_f32(int mode, int lmax, const ShellMeta* meta, const ShellGeom* geom, const PrimData* prim, int nshell,
                    int nbasis, int nbasis_pad, const double* pts, int64_t npts, int64_t ld, float* ao, double screen,
-                   hipStream_t s);
+                   const uint32_t* kmask, int nwords, int tile_points, hipStream_t s);
 // Reorders three row-major bf16 planes of a rows x cols matrix
 // ([3][rows][cols]; rows a multiple of 16, cols of 32) into the MFMA fragment
 // order of the mixed tile engine's A operand (host side, same size).
@@ -147,8 +151,12 @@ void launch_contract_mixed(int mode, const uint16_t* Pp, int nbasis_pad, const f
 // holds the bf16 pieces of L^T (rows_pad x nbasis_pad, fragment order), and
 // the partials run over rows_pad / 128 blocks of the factor's columns.
 // Modes rho, grad and lap.
+// k-steps: screened, klist / kcount from launch_tile_activity at
+// factor_tile_points(mode), only the listed k-steps are contracted; else
+// kcount = nullptr and klist = 0 .. nbasis_pad / 32 - 1 for every tile.
 void launch_contract_mixed_factor(int mode, const uint16_t* Lf, int rows_pad, int nbasis_pad, const float* ao,
-                                  int64_t ld, double* part, hipStream_t s);
+                                  int64_t ld, double* part, const int32_t* klist, const int32_t* kcount,
+                                  hipStream_t s);
 // Pivoted Cholesky factorisation of the symmetric n x n matrix A (row stride
 // lda) on the host: A = L L^T with L (n x rank, column k at L[k * n ...]),
 // columns in pivot order, stopping when the largest remaining diagonal
@@ -156,6 +164,47 @@ void launch_contract_mixed_factor(int mode, const uint16_t* Lf, int rows_pad, in
 // false when A is not positive semidefinite to that tolerance or its rank
 // exceeds rmax.
 bool pivoted_cholesky(const double* A, int n, int lda, int rmax, double rel_tol, std::vector<double>& L, int& rank);
+
+// ---------------------------------------------------------------------------
+// Spatial screening of the mixed factor route (screen_host.cpp, screen.hip).
+// sqrt((2a/pi)^{3/2} (4a)^L): radial part of a primitive's L2 normalisation.
+double prim_radial_norm(double a, int L);
+// Distance from the centre beyond which the value, every first and second
+// derivative and the Laplacian of every function of the shell are at most
+// eps in magnitude (cn[p] = c_p N_p; rigorous bound, see screen_host.cpp).
+double shell_screen_radius(int l, int nprim, const double* a, const double* cn, double eps);
+struct ScreenSpan {  // one shell of a k-step: centre and squared cutoff radius
+  double cx, cy, cz, r2;
+};
+// Shell order of the screened AO layout and what the activity kernel reads.
+struct ScreenLayout {
+  std::vector<int> order;     // screened position -> shell
+  std::vector<int> first;     // shell -> first function in the screened layout
+  std::vector<int> func_src;  // screened function -> function of the natural layout
+  std::vector<int> span_off;  // k-step kb -> spans[span_off[kb] .. span_off[kb + 1])
+  std::vector<ScreenSpan> spans;
+};
+ScreenLayout build_screen_layout(const std::vector<int>& shell_l, const std::vector<int>& shell_atom,
+                                 const std::vector<double>& radius, const double* xyz, int nbasis_pad);
+// Lattice of the Morton keys: cell (floor((x - origin) * inv_cell)), 10 bits
+// per axis, clamped.
+struct MortonFrame {
+  double origin[3];
+  double inv_cell;
+};
+size_t screen_sort_workspace(int64_t n);
+// perm[i] = index of the i-th point in Morton order (stable)
+void launch_screen_sort(const double* pts, int64_t n, const MortonFrame& f, uint32_t* keys_in, uint32_t* keys_out,
+                        int32_t* idx_in, int32_t* perm, void* temp, size_t temp_bytes, hipStream_t s);
+// Per tile of tile_points consecutive (sorted) points: active k-steps
+// klist[tile * nk ..] (kcount[tile] of them, increasing), mask
+// kmask[tile * nwords ..], nwords = ceil(nk / 32); active_total += the counts.
+void launch_tile_activity(const double* sorted_pts, int64_t n, int tile_points, int64_t ntile, const int32_t* span_off,
+                          const ScreenSpan* spans, int nk, int32_t* klist, int32_t* kcount, uint32_t* kmask,
+                          int nwords, unsigned long long* active_total, hipStream_t s);
+// Points per tile of the factor route's contraction in this mode (the
+// granularity of the activity lists).
+int factor_tile_points(int mode);
 struct FlagArgs {
   double gamma;          // error estimate = gamma * B (B = |phi|^T |P| |phi|)
   double tol;            // relative tolerance on rho
@@ -164,6 +213,7 @@ struct FlagArgs {
   int32_t* index;        // flagged point indices
   int64_t index_base;    // added to the chunk-relative index of a flagged point
   unsigned long long* max_est;  // max estimate / |rho| over accepted points (bit pattern of a double)
+  double trunc;          // screened factor route: + trunc * sqrt|rho| (bound of the skipped AO blocks)
 };
 void launch_finalize_mixed(const FinalizeArgs& a, const FlagArgs& f, hipStream_t s);
 void launch_gather_points(const double* pts, const int32_t* index, int64_t n, double* out, hipStream_t s);

@@ -95,13 +95,15 @@ __global__ __launch_bounds__(256) void finalize_mixed_kernel(FinalizeArgs a, Fla
       if (d < nd) dot[d] += base[(int64_t)d * a.ld];
     bound += base[(int64_t)nd * a.ld];
   }
-  emit_outputs(a, a.out_off + p, dot);
+  // index (screened layout): the chunk position of the p-th sorted point
+  const int64_t pp = a.index ? (int64_t)a.index[p] : p;
+  emit_outputs(a, a.out_off + pp, dot);
   const double arho = fabs(dot[0]);
   if (arho < f.rho_floor) return;
-  const double est = f.gamma * bound;
+  const double est = f.gamma * bound + f.trunc * sqrt(arho);
   if (!(est <= f.tol * arho)) {  // also catches NaN
     const int k = atomicAdd(f.count, 1);
-    f.index[k] = (int32_t)(f.index_base + p);
+    f.index[k] = (int32_t)(f.index_base + pp);
   } else {
     const double rel = est / arho;
     atomicMax(f.max_est, (unsigned long long)__double_as_longlong(rel));

@@ -101,11 +101,13 @@ struct YTile {
 // for about half the work.  The bound product is treated the same way
 // (|x|^T |P| |x| is a quadratic form too), its doubling applied exactly to
 // the |P_hi| operand (bf16 exponent + 1).
-template <int TC, int NBC, bool F64, bool TRI>
+template <int TC, int NBC, bool F64, bool TRI, bool LIST = false>
 __device__ __forceinline__ void y_tile(const u4* __restrict__ Pf, int rows, int mp, int64_t ld, int i0, int tid,
                                        const float* __restrict__ bsrc, uint16_t (&Bs)[2][NPL][TC][LROW],
-                                       YTile<TC, NBC, F64>& t, int k_begin) {
+                                       YTile<TC, NBC, F64>& t, int k_begin, const int32_t* klist = nullptr,
+                                       int kcount = 0) {
   static_assert(!TRI || F64, "the triangle needs the FP64 running sums");
+  static_assert(!(TRI && LIST), "the triangle walks all k-steps from the diagonal");
   constexpr int NCB = TC / 16;
   const int lane = tid & 63;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -246,20 +248,37 @@ __device__ __forceinline__ void y_tile(const u4* __restrict__ Pf, int rows, int 
   // iterations (those values are unused; the LDS buffer written then is the
   // idle one): conditional loads would keep two copies of the B registers
   // alive across the k-step.
+  // The k-steps: LIST, klist[0 .. kcount) (the k-steps a point tile needs,
+  // increasing; list and length are the same for the whole workgroup, so the
+  // entries are scalar loads, fetched three steps ahead); else
+  // k_begin .. nk - 1.
+  const int kn = LIST ? kcount : nk - k_begin;
+  auto kat = [&](int i) {
+    if constexpr (LIST)
+      return klist[i];
+    else
+      return k_begin + i;
+  };
+  if (kn <= 0) return;
   const int kd = k_begin + w;  // TRI: this wave's diagonal k-step (wave-uniform)
-  load_b(k_begin);
-  load_a(TRI ? kd : k_begin);
+  int kcur = kat(0), knext = kat(kn > 1 ? 1 : 0), knext2 = kat(kn > 2 ? 2 : kn - 1);
+  load_b(kcur);
+  load_a(TRI ? kd : kcur);
   store_b(0);
-  load_b(k_begin + 1 < nk ? k_begin + 1 : k_begin);
+  load_b(knext);
   __syncthreads();
-  for (int kb = k_begin; kb < nk; ++kb) {
-    const int s = (kb - k_begin) & 1;
+  for (int i = 0; i < kn; ++i) {
+    const int kb = kcur;
+    const int k3 = kat(i + 3 < kn ? i + 3 : kn - 1);
+    const int s = i & 1;
     if (!TRI || kb >= kd) compute(s, kb > kd);
-    const int k1 = kb + 1 < nk ? kb + 1 : kb;
-    if (!TRI || k1 > kd) load_a(k1);  // A(kd) is loaded before the loop
+    if (!TRI || knext > kd) load_a(knext);  // A(kd) is loaded before the loop
     store_b(s ^ 1);
-    load_b(kb + 2 < nk ? kb + 2 : nk - 1);
+    load_b(knext2);
     __syncthreads();
+    kcur = knext;
+    knext = knext2;
+    knext2 = k3;
   }
 }
 

@@ -20,8 +20,6 @@ namespace ct {
 
 namespace {
 
-const double kPi = 3.14159265358979323846264338327950288;
-
 int64_t round_up(int64_t v, int64_t a) { return (v + a - 1) / a * a; }
 
 int select_mode(uint32_t q) {
@@ -190,6 +188,8 @@ ct_kernel_stats KernelTimer::stats() const {
   st.esp_calls = calls_[ESP];
   st.backward_calls = calls_[BACKWARD];
   st.reeval_calls = calls_[REEVAL];
+  st.screen_ms = ms_[SCREEN];
+  st.screen_calls = calls_[SCREEN];
   return st;
 }
 
@@ -225,7 +225,7 @@ Mol::Mol(const HostBasis& basis, const double* atom_xyz, const double* charges, 
       const int j = basis.shell_prim_off[s] + k;
       const double a = basis.exps[j];
       CT_REQUIRE(a > 0.0 && std::isfinite(a), "exponents must be positive and finite");
-      prim_[j] = PrimData{a, basis.coefs[j] * std::sqrt(std::pow(2.0 * a / kPi, 1.5) * std::pow(4.0 * a, L))};
+      prim_[j] = PrimData{a, basis.coefs[j] * prim_radial_norm(a, L)};
     }
   }
   xyz_.assign(atom_xyz, atom_xyz + 3 * (size_t)natom);
@@ -262,6 +262,7 @@ void Mol::upload_geometry() {
     geom[s] = ShellGeom{xyz_[3 * a], xyz_[3 * a + 1], xyz_[3 * a + 2], amin};
   }
   CT_HIP_CHECK(hipMemcpy(d_geom_.get(), geom.data(), sizeof(ShellGeom) * nshell_, hipMemcpyHostToDevice));
+  if (screen_built_) build_screen();  // the shell order follows the centres
 }
 
 namespace {
@@ -355,24 +356,6 @@ void Mol::upload_P() {
   if (pivoted_cholesky(Pp.data(), M_, Mp_, Mp_ / 2 + 1, 1e-12, L, rank)) {
     factor_rank_ = rank;
     factor_rows_ = std::max(kRowBlock, (rank + kRowBlock - 1) / kRowBlock * kRowBlock);
-    const size_t fplane = (size_t)factor_rows_ * Mp_;
-    std::vector<uint16_t> Lb(3 * fplane, 0);
-    for (int j = 0; j < rank; ++j)
-      for (int i = 0; i < M_; ++i) {
-        const float x = (float)L[(size_t)j * M_ + i];  // row j of L^T
-        const uint16_t h = bf16_bits(x);
-        const float r1 = x - bf16_float(h);
-        const uint16_t m = bf16_bits(r1);
-        const uint16_t l = bf16_bits(r1 - bf16_float(m));
-        const size_t k = (size_t)j * Mp_ + i;
-        Lb[k] = h;
-        Lb[fplane + k] = m;
-        Lb[2 * fplane + k] = l;
-      }
-    std::vector<uint16_t> Lf(Lb.size());
-    mixed_fragment_order(Lb.data(), factor_rows_, Mp_, Lf.data());
-    if (!d_Lb_.reserve(sizeof(uint16_t) * Lf.size())) throw Error(CT_E_NOMEM, "cannot allocate the bf16 factor of P");
-    CT_HIP_CHECK(hipMemcpy(d_Lb_.get(), Lf.data(), sizeof(uint16_t) * Lf.size(), hipMemcpyHostToDevice));
     // FP64 copy for the FP64 factor route (re-evaluation of the mixed factor
     // route's flagged points; FP64 evaluation on request): L k-major,
     // L64[k * rows + j], padded with zeros
@@ -382,6 +365,122 @@ void Mol::upload_P() {
     if (!d_L64_.reserve(sizeof(double) * L64.size())) throw Error(CT_E_NOMEM, "cannot allocate the factor of P");
     CT_HIP_CHECK(hipMemcpy(d_L64_.get(), L64.data(), sizeof(double) * L64.size(), hipMemcpyHostToDevice));
   }
+  L_.swap(L);
+  if (factor_rank_ < 0) L_.clear();
+  build_screen();
+}
+
+void Mol::set_screening(double eps) {
+  CT_REQUIRE(eps >= 0.0 && std::isfinite(eps), "screening threshold must be finite and non-negative");
+  CT_HIP_CHECK(hipStreamSynchronize(stream_));
+  screen_eps_ = eps;
+  build_screen();
+}
+
+void Mol::build_screen() {
+  screen_built_ = true;
+  // shell order: the screened one, or the natural one when screening is off
+  std::vector<int> shell_l(basis_.shell_l.begin(), basis_.shell_l.end());
+  std::vector<int> shell_atom(basis_.shell_atom.begin(), basis_.shell_atom.end());
+  if (screen_eps_ > 0.0) {
+    std::vector<double> radius(nshell_);
+    for (int s = 0; s < nshell_; ++s) {
+      const ShellMeta& m = meta_[s];
+      std::vector<double> a(m.nprim), cn(m.nprim);
+      for (int k = 0; k < m.nprim; ++k) {
+        a[k] = prim_[m.prim_off + k].a;
+        cn[k] = prim_[m.prim_off + k].c;
+      }
+      radius[s] = shell_screen_radius(shell_l[s], m.nprim, a.data(), cn.data(), screen_eps_);
+    }
+    layout_ = build_screen_layout(shell_l, shell_atom, radius, xyz_.data(), Mp_);
+    // Morton lattice: the atoms' box with a margin of 12 bohr, 1024 cells per
+    // axis, cells of at least 0.25 bohr (points outside clamp to the border)
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int a = 0; a < natom_; ++a)
+      for (int k = 0; k < 3; ++k) {
+        lo[k] = std::min(lo[k], xyz_[3 * a + k]);
+        hi[k] = std::max(hi[k], xyz_[3 * a + k]);
+      }
+    double span = 0.0;
+    for (int k = 0; k < 3; ++k) {
+      frame_.origin[k] = lo[k] - 12.0;
+      span = std::max(span, hi[k] - lo[k] + 24.0);
+    }
+    frame_.inv_cell = 1.0 / std::max(0.25, span / 1024.0);
+  } else {
+    layout_ = ScreenLayout{};
+    layout_.order.resize(nshell_);
+    layout_.first.resize(nshell_);
+    for (int s = 0; s < nshell_; ++s) {
+      layout_.order[s] = s;
+      layout_.first[s] = meta_[s].first;
+    }
+    layout_.func_src.resize(M_);
+    for (int i = 0; i < M_; ++i) layout_.func_src[i] = i;
+  }
+  // shell table in that order
+  std::vector<ShellMeta> sm(nshell_);
+  std::vector<ShellGeom> sg(nshell_);
+  for (int i = 0; i < nshell_; ++i) {
+    const int s = layout_.order[i];
+    sm[i] = ShellMeta{meta_[s].code, meta_[s].nprim, meta_[s].prim_off, layout_.first[s]};
+    const int a = basis_.shell_atom[s];
+    double amin = 1e300;
+    for (int k = 0; k < meta_[s].nprim; ++k) amin = std::min(amin, prim_[meta_[s].prim_off + k].a);
+    sg[i] = ShellGeom{xyz_[3 * a], xyz_[3 * a + 1], xyz_[3 * a + 2], amin};
+  }
+  if (!d_smeta_.reserve(sizeof(ShellMeta) * nshell_) || !d_sgeom_.reserve(sizeof(ShellGeom) * nshell_))
+    throw Error(CT_E_NOMEM, "cannot allocate the screened shell table");
+  CT_HIP_CHECK(hipMemcpy(d_smeta_.get(), sm.data(), sizeof(ShellMeta) * nshell_, hipMemcpyHostToDevice));
+  CT_HIP_CHECK(hipMemcpy(d_sgeom_.get(), sg.data(), sizeof(ShellGeom) * nshell_, hipMemcpyHostToDevice));
+  if (screen_eps_ > 0.0) {
+    if (!d_span_off_.reserve(sizeof(int32_t) * layout_.span_off.size()) ||
+        !d_spans_.reserve(sizeof(ScreenSpan) * std::max<size_t>(1, layout_.spans.size())))
+      throw Error(CT_E_NOMEM, "cannot allocate the screening tables");
+    CT_HIP_CHECK(hipMemcpy(d_span_off_.get(), layout_.span_off.data(), sizeof(int32_t) * layout_.span_off.size(),
+                           hipMemcpyHostToDevice));
+    CT_HIP_CHECK(hipMemcpy(d_spans_.get(), layout_.spans.data(), sizeof(ScreenSpan) * layout_.spans.size(),
+                           hipMemcpyHostToDevice));
+  }
+  if (factor_rank_ < 0) return;
+  // L^T with its columns (basis functions) in that order, FP32 split into
+  // three bf16 pieces, in MFMA fragment order
+  const int rank = factor_rank_;
+  const size_t fplane = (size_t)factor_rows_ * Mp_;
+  std::vector<uint16_t> Lb(3 * fplane, 0);
+  for (int j = 0; j < rank; ++j)
+    for (int i = 0; i < M_; ++i) {
+      const float x = (float)L_[(size_t)j * M_ + layout_.func_src[i]];  // row j of L^T
+      const uint16_t h = bf16_bits(x);
+      const float r1 = x - bf16_float(h);
+      const uint16_t m = bf16_bits(r1);
+      const uint16_t l = bf16_bits(r1 - bf16_float(m));
+      const size_t k = (size_t)j * Mp_ + i;
+      Lb[k] = h;
+      Lb[fplane + k] = m;
+      Lb[2 * fplane + k] = l;
+    }
+  std::vector<uint16_t> Lf(Lb.size());
+  mixed_fragment_order(Lb.data(), factor_rows_, Mp_, Lf.data());
+  if (!d_Lb_.reserve(sizeof(uint16_t) * Lf.size())) throw Error(CT_E_NOMEM, "cannot allocate the bf16 factor of P");
+  CT_HIP_CHECK(hipMemcpy(d_Lb_.get(), Lf.data(), sizeof(uint16_t) * Lf.size(), hipMemcpyHostToDevice));
+  // Truncation bound: a skipped AO value is at most eps, so the skipped part
+  // of psi_j is at most t_j = eps ||L_j||_1 and rho moves by at most
+  // sum_j t_j (2 |psi_j| + t_j) <= 2 eps ||c||_2 sqrt(rho) + O(eps^2)
+  // (Cauchy-Schwarz, c_j = ||L_j||_1); the finaliser adds twice that.
+  double c2 = 0.0;
+  for (int j = 0; j < rank; ++j) {
+    double c = 0.0;
+    for (int i = 0; i < M_; ++i) c += std::fabs(L_[(size_t)j * M_ + i]);
+    c2 += c * c;
+  }
+  screen_trunc_ = 4.0 * screen_eps_ * std::sqrt(c2);
+  // the k-step list of unscreened tiles: every k-step
+  std::vector<int32_t> all(Mp_ / 32);
+  for (size_t k = 0; k < all.size(); ++k) all[k] = (int32_t)k;
+  if (!d_klist_all_.reserve(sizeof(int32_t) * all.size())) throw Error(CT_E_NOMEM, "cannot allocate the k-step list");
+  CT_HIP_CHECK(hipMemcpy(d_klist_all_.get(), all.data(), sizeof(int32_t) * all.size(), hipMemcpyHostToDevice));
 }
 
 bool Mol::factor_route(int mode, int route, bool mixed) const {
@@ -530,7 +629,7 @@ void Mol::backward(const double* points, int64_t n, const double* w, int precisi
       launch_contract_adjoint_f64(static_cast<const double*>(d_P_.get()), Mp_, ao, ld, wc, nc, T, stream_);
     } else {
       float* ao = reinterpret_cast<float*>(ws);
-      launch_ao_f32(mode, lmax_, meta, geom, prim, nshell_, M_, Mp_, pts, nc, ld, ao, 50.0, stream_);
+      launch_ao_f32(mode, lmax_, meta, geom, prim, nshell_, M_, Mp_, pts, nc, ld, ao, 50.0, nullptr, 0, 64, stream_);
       int nslice = 0;
       int64_t len = 0;
       gram_mixed_plan(nc, Mp_, &nslice, &len);
@@ -650,7 +749,7 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
   CT_REQUIRE(precision == CT_PREC_FP64 || precision == CT_PREC_MIXED, "unknown precision");
   for (const OutSlot& s : kSlots)
     if ((q & s.bit) && out.*(s.field) == nullptr) throw Error(CT_E_INVALID, "output array missing for a requested quantity");
-  if (rep) *rep = ct_eval_report{n, 0, 0, 0.0, -1};
+  if (rep) *rep = ct_eval_report{n, 0, 0, 0.0, -1, 1.0};
   if (n == 0) return;
   CT_REQUIRE(points != nullptr || cube != nullptr, "null points");
 
@@ -659,6 +758,11 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
   const int nder = mode_nder(mode), ndot = mode_ndot(mode);
   const bool use_factor = factor_route(mode, par.route, mixed);
   const int nrt = (use_factor ? factor_rows_ : Mp_) / kRowBlock;  // row blocks of the mixed partials
+  // screened layout (mixed factor route): points in Morton order, per-tile
+  // lists of the active k-steps
+  const bool screened = mixed && use_factor && screen_eps_ > 0.0;
+  const int nk = Mp_ / 32, nwords = (nk + 31) / 32;
+  const int tp = screened ? factor_tile_points(mode) : 64;
   const bool pts_dev = cube == nullptr && is_device_pointer(points);
   constexpr size_t NS = sizeof(kSlots) / sizeof(kSlots[0]);
   bool out_dev[NS];
@@ -677,15 +781,22 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
   const int64_t ao_bytes_pt = (int64_t)nder * Mp_ * (mixed ? 4 : 8);
   const int64_t per_point = mixed ? ao_bytes_pt + (int64_t)nrt * (ndot + 1) * 8 + 4 + 24 + stage_width * 8
                                   : ao_bytes_pt + (int64_t)ndot * 8 + 24 + stage_width * 8;
-  int64_t Nc = chunk_points(per_point, n);
-  const size_t head = 256;  // flag counter + max estimate
-  while (!d_ws_.reserve(head + (size_t)Nc * per_point)) {
+  // screened: keys (2 x 4 B), indices (2 x 4 B), sorted points (24 B) per
+  // point; k-step list (2 nk B), count and mask (4 + 4 nwords B) per tile
+  const int64_t per_point_s = screened ? 40 + (4 * nk + 4 + 4 * nwords + tp - 1) / tp : 0;
+  int64_t Nc = chunk_points(per_point + per_point_s, n);
+  const size_t head = 256;  // flag counter + max estimate + active k-step count
+  size_t sort_bytes = 0;
+  for (;;) {
+    sort_bytes = screened ? (screen_sort_workspace(Nc) + 255) / 256 * 256 : 0;
+    if (d_ws_.reserve(head + (size_t)Nc * (per_point + per_point_s) + sort_bytes + 4096)) break;
     if (Nc <= kPointAlign) throw Error(CT_E_NOMEM, "cannot allocate the evaluation workspace");
     Nc = std::max<int64_t>(kPointAlign, Nc / 2 / kPointAlign * kPointAlign);
   }
   char* ws = static_cast<char*>(d_ws_.get());
   int32_t* d_count = reinterpret_cast<int32_t*>(ws);
   unsigned long long* d_maxest = reinterpret_cast<unsigned long long*>(ws + 8);
+  unsigned long long* d_active = reinterpret_cast<unsigned long long*>(ws + 16);
   char* cur = ws + head;
   void* ao = cur;
   cur += (size_t)Nc * ao_bytes_pt;
@@ -696,6 +807,32 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
   double* out_stage = reinterpret_cast<double*>(cur);
   cur += (size_t)Nc * stage_width * 8;
   int32_t* flag_idx = reinterpret_cast<int32_t*>(cur);
+  cur += (size_t)Nc * 4;
+  // screened layout buffers (256-byte aligned pieces)
+  auto take = [&](size_t bytes) {
+    char* p = cur;
+    cur += (bytes + 255) / 256 * 256;
+    return p;
+  };
+  uint32_t *keys_in = nullptr, *keys_out = nullptr, *kmask = nullptr;
+  int32_t *idx_in = nullptr, *perm = nullptr, *kcount = nullptr;
+  int32_t* klist = nullptr;
+  double* spts = nullptr;
+  void* sort_tmp = nullptr;
+  if (screened) {
+    const int64_t ntile_max = Nc / tp;
+    spts = reinterpret_cast<double*>(take((size_t)Nc * 24));
+    keys_in = reinterpret_cast<uint32_t*>(take((size_t)Nc * 4));
+    keys_out = reinterpret_cast<uint32_t*>(take((size_t)Nc * 4));
+    idx_in = reinterpret_cast<int32_t*>(take((size_t)Nc * 4));
+    perm = reinterpret_cast<int32_t*>(take((size_t)Nc * 4));
+    klist = reinterpret_cast<int32_t*>(take((size_t)ntile_max * nk * 4));
+    kcount = reinterpret_cast<int32_t*>(take((size_t)ntile_max * 4));
+    kmask = reinterpret_cast<uint32_t*>(take((size_t)ntile_max * nwords * 4));
+    sort_tmp = take(sort_bytes);
+    CT_REQUIRE(cur <= ws + d_ws_.size(), "screening workspace overflow");
+  }
+  int64_t valid_tiles = 0;
   h_small_.reserve(64);
   auto* h_small = static_cast<int64_t*>(h_small_.get());
 
@@ -710,6 +847,7 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
   if (mixed) {
     CT_HIP_CHECK(hipMemsetAsync(d_maxest, 0, 8, stream_));
     CT_HIP_CHECK(hipMemsetAsync(d_count, 0, 4, stream_));
+    CT_HIP_CHECK(hipMemsetAsync(d_active, 0, 8, stream_));
   }
 
   int64_t chunks = 0, reevaluated = 0;
@@ -764,13 +902,32 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
       timer_.end(stream_);
     } else {
       float* aoc = static_cast<float*>(ao);
+      const double* apts = pts;  // the points the AO kernel and the contraction see
+      if (screened) {
+        timer_.begin(KernelTimer::SCREEN, stream_);
+        launch_screen_sort(pts, nc, frame_, keys_in, keys_out, idx_in, perm, sort_tmp, sort_bytes, stream_);
+        launch_gather_points(pts, perm, nc, spts, stream_);
+        launch_tile_activity(spts, nc, tp, ld / tp, static_cast<const int32_t*>(d_span_off_.get()),
+                             static_cast<const ScreenSpan*>(d_spans_.get()), nk, klist, kcount, kmask, nwords,
+                             d_active, stream_);
+        timer_.end(stream_);
+        apts = spts;
+        fa.index = perm;  // results of sorted point p belong to chunk point perm[p]
+        valid_tiles += (nc + tp - 1) / tp;
+      }
+      // the factor route reads the AOs in the order of its factor (screened
+      // shell order, or the natural one when screening is off)
+      const auto* ameta = use_factor ? static_cast<const ShellMeta*>(d_smeta_.get()) : meta;
+      const auto* ageom = use_factor ? static_cast<const ShellGeom*>(d_sgeom_.get()) : geom;
       timer_.begin(KernelTimer::AO, stream_);
-      launch_ao_f32(mode, lmax_, meta, geom, prim, nshell_, M_, Mp_, pts, nc, ld, aoc, 50.0, stream_);
+      launch_ao_f32(mode, lmax_, ameta, ageom, prim, nshell_, M_, Mp_, apts, nc, ld, aoc, 50.0,
+                    screened ? kmask : nullptr, nwords, tp, stream_);
       timer_.end(stream_);
       timer_.begin(KernelTimer::CONTRACT, stream_);
       if (use_factor)
         launch_contract_mixed_factor(mode, static_cast<const uint16_t*>(d_Lb_.get()), factor_rows_, Mp_, aoc, ld,
-                                     dots, stream_);
+                                     dots, screened ? klist : static_cast<const int32_t*>(d_klist_all_.get()),
+                                     screened ? kcount : nullptr, stream_);
       else
         launch_contract_mixed(mode, static_cast<const uint16_t*>(d_Pb_.get()), Mp_, aoc, ld, dots, stream_);
       timer_.end(stream_);
@@ -782,6 +939,7 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
       fl.index = list;
       fl.index_base = batch ? start : 0;
       fl.max_est = d_maxest;
+      fl.trunc = screened ? screen_trunc_ : 0.0;
       timer_.begin(KernelTimer::FINALIZE, stream_);
       launch_finalize_mixed(fa, fl, stream_);
       timer_.end(stream_);
@@ -827,7 +985,7 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
       reevaluated = count;
     }
   }
-  if (mixed) CT_HIP_CHECK(hipMemcpyAsync(h_small + 1, d_maxest, 8, hipMemcpyDeviceToHost, stream_));
+  if (mixed) CT_HIP_CHECK(hipMemcpyAsync(h_small + 1, d_maxest, 16, hipMemcpyDeviceToHost, stream_));
   CT_HIP_CHECK(hipStreamSynchronize(stream_));
   timer_.collect();
   if (rep) {
@@ -835,6 +993,10 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
     rep->n_reevaluated = reevaluated;
     if (mixed) std::memcpy(&rep->max_error_estimate, h_small + 1, 8);
     rep->factor_rank = use_factor ? factor_rank_ : -1;
+    // k-steps holding basis functions (the padding beyond them is never active)
+    const int64_t nk_basis = (M_ + 31) / 32;
+    if (screened && valid_tiles > 0)
+      rep->active_fraction = (double)(uint64_t)h_small[2] / ((double)valid_tiles * (double)nk_basis);
   }
 }
 

@@ -67,7 +67,7 @@ class PinnedBuffer {
 // their elapsed times when collected.
 class KernelTimer {
  public:
-  enum Kind { AO = 0, CONTRACT, FINALIZE, ESP, BACKWARD, REEVAL, NKIND };
+  enum Kind { AO = 0, CONTRACT, FINALIZE, ESP, BACKWARD, REEVAL, SCREEN, NKIND };
   ~KernelTimer();
   void enable(bool on);
   bool enabled() const { return on_; }
@@ -107,6 +107,11 @@ class Mol {
   void set_coordinates(const double* xyz);  // host or device
   void set_density_matrix(const double* P);  // host or device
   void set_workspace_limit(int64_t bytes) { ws_limit_ = bytes; }
+  // Spatial screening of the mixed factor route: k-steps of basis functions
+  // whose values and derivatives are at most eps at every point of a tile
+  // are skipped (their bound enters the error estimate); 0 turns it off.
+  void set_screening(double eps);
+  double screening() const { return screen_eps_; }
   hipStream_t stream() const { return stream_; }
   void synchronize();
   KernelTimer& timer() { return timer_; }
@@ -122,6 +127,10 @@ class Mol {
  private:
   void upload_geometry();
   void upload_P();
+  // screened layout of the mixed factor route (shell order, bf16 factor in
+  // that order, k-step spans, truncation bounds); after P, the geometry or
+  // the threshold change
+  void build_screen();
   int64_t chunk_points(int64_t per_point, int64_t n) const;
   void reevaluate(int mode, const double* pts, int64_t count, const int32_t* index, const FinalizeArgs& fin,
                   bool factor);
@@ -144,6 +153,16 @@ class Mol {
   DeviceBuffer d_L64_;     // L in FP64, k-major (nbasis_pad x factor rows)
   int factor_rank_ = -1;   // rank of the factor, -1 when P has none (not PSD or rank too high)
   int factor_rows_ = 0;    // rows of L^T padded to a multiple of 128
+  std::vector<double> L_;  // the factor, rank x M (row j = column j of L), natural order
+  // spatial screening of the mixed factor route
+  double screen_eps_ = 1e-15;
+  bool screen_built_ = false;
+  ScreenLayout layout_;    // identity order (and no spans) when screening is off
+  MortonFrame frame_{};
+  DeviceBuffer d_smeta_, d_sgeom_;     // shell table in the screened order
+  DeviceBuffer d_span_off_, d_spans_;  // per k-step shells and cutoff radii
+  DeviceBuffer d_klist_all_;           // 0 .. nbasis_pad / 32 - 1: the k-steps of an unscreened tile
+  double screen_trunc_ = 0.0;          // 4 eps ||c||_2, c_j = ||L_j||_1: rho bound of the skipped blocks / sqrt(rho)
   DeviceBuffer d_ws_;      // AO + dots + staged points / outputs
   DeviceBuffer d_ws64_;    // mixed path: FP64 re-evaluation of flagged points
   DeviceBuffer d_flags_;   // mixed path: flagged point list of a whole call (device outputs)

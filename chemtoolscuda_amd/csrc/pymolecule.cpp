@@ -114,6 +114,7 @@ class Molecule {
     check(ct_mol_set_density_matrix(h_, P.data()));
   }
   void set_workspace_limit(int64_t bytes) { check(ct_mol_set_workspace_limit(h_, bytes)); }
+  void set_screening(double eps) { check(ct_mol_set_screening(h_, eps)); }
 
   // Generic multi-quantity evaluation: returns {name: array}.
   py::dict compute(const Points& pts, const std::vector<std::string>& names, const std::string& precision,
@@ -226,6 +227,7 @@ class Molecule {
     d["n_chunks"] = last_report_.n_chunks;
     d["max_error_estimate"] = last_report_.max_error_estimate;
     d["factor_rank"] = last_report_.factor_rank;
+    d["active_fraction"] = last_report_.active_fraction;
     return d;
   }
 
@@ -421,6 +423,9 @@ PYBIND11_MODULE(_chemtools_amd, m) {
       .def("set_coordinates", &Molecule::set_coordinates)
       .def("set_density_matrix", &Molecule::set_density_matrix)
       .def("set_workspace_limit", &Molecule::set_workspace_limit)
+      .def("set_screening", &Molecule::set_screening, py::arg("eps"),
+           "Spatial screening of the mixed factor route: basis-function blocks whose values and derivatives "
+           "are at most eps on a point tile are skipped (0 turns it off; default 1e-15).")
       .def("last_report", &Molecule::last_report)
       .def_property_readonly("coordinates", &Molecule::coordinates, "Cartesian coordinates of the atomic centres.")
       .def_property_readonly("numbers", &Molecule::numbers, "Atomic numbers of the atomic centres.")

@@ -1,0 +1,139 @@
+// Spatial screening of the mixed factor route, device side.
+//
+//   1. points are put in Morton order (30-bit keys on a lattice around the
+//      molecule; stable radix sort of (key, index) pairs), so that a point
+//      tile of the contraction is a compact box in space;
+//   2. one wave per point tile takes the tile's bounding box and marks the
+//      32-function k-steps holding a shell within reach of the box (distance
+//      below the shell's cutoff radius); the others are skipped by the AO
+//      kernel (no evaluation, no stores) and by the contraction (no loads, no
+//      MFMAs).  Per tile: the list of active k-steps in increasing order, its
+//      length and a bit mask.
+//
+// The reference evaluates every basis function at every point
+// (src/evaluate_density.cu:25-411); screening is this library's own.
+
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include "ct_internal.h"
+
+namespace ct {
+
+namespace {
+
+__device__ __forceinline__ uint32_t spread10(uint32_t v) {
+  v &= 0x3FFu;
+  v = (v | (v << 16)) & 0x030000FFu;
+  v = (v | (v << 8)) & 0x0300F00Fu;
+  v = (v | (v << 4)) & 0x030C30C3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+__global__ __launch_bounds__(256) void morton_keys_kernel(const double* __restrict__ pts, int64_t n, MortonFrame f,
+                                                          uint32_t* __restrict__ keys, int32_t* __restrict__ idx) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  uint32_t q[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const double t = (pts[3 * p + k] - f.origin[k]) * f.inv_cell;
+    q[k] = t > 0.0 ? (t < 1023.0 ? (uint32_t)t : 1023u) : 0u;  // NaN -> 0
+  }
+  keys[p] = (spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2]);
+  idx[p] = (int32_t)p;
+}
+
+// One wave per tile of `tp` points (tp <= 64), four tiles per workgroup.
+__global__ __launch_bounds__(256) void tile_activity_kernel(const double* __restrict__ pts, int64_t n, int tp,
+                                                            int64_t ntile, const int32_t* __restrict__ span_off,
+                                                            const ScreenSpan* __restrict__ spans, int nk,
+                                                            int32_t* __restrict__ klist, int32_t* __restrict__ kcount,
+                                                            uint32_t* __restrict__ kmask, int nwords,
+                                                            unsigned long long* __restrict__ active_total) {
+  const int lane = threadIdx.x & 63;
+  const int64_t tile = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (tile >= ntile) return;
+  const int64_t p = tile * tp + lane;
+  const bool valid = lane < tp && p < n;
+  double lo[3], hi[3];
+  bool finite = true;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const double v = valid ? pts[3 * p + k] : 0.0;
+    finite = finite && isfinite(v);
+    lo[k] = valid ? v : 1e300;
+    hi[k] = valid ? v : -1e300;
+  }
+  // a point with a non-finite coordinate has no place in space: its tile
+  // keeps every k-step (and its NaN results)
+  const bool keep_all = __any(!finite);
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = fmin(lo[k], __shfl_xor(lo[k], off));
+      hi[k] = fmax(hi[k], __shfl_xor(hi[k], off));
+    }
+  const bool any = lo[0] <= hi[0];  // the tile holds at least one point
+  int total = 0;
+  for (int kb0 = 0; kb0 < nk; kb0 += 64) {
+    const int kb = kb0 + lane;
+    bool act = keep_all && kb < nk && span_off[kb + 1] > span_off[kb];
+    if (any && !keep_all && kb < nk) {
+      for (int e = span_off[kb]; e < span_off[kb + 1] && !act; ++e) {
+        const ScreenSpan sp = spans[e];
+        const double dx = fmax(0.0, fmax(lo[0] - sp.cx, sp.cx - hi[0]));
+        const double dy = fmax(0.0, fmax(lo[1] - sp.cy, sp.cy - hi[1]));
+        const double dz = fmax(0.0, fmax(lo[2] - sp.cz, sp.cz - hi[2]));
+        act = dx * dx + dy * dy + dz * dz < sp.r2;
+      }
+    }
+    const uint64_t bal = __ballot(act);
+    const int before = __popcll(bal & ((1ull << lane) - 1ull));
+    if (act) klist[tile * nk + total + before] = kb;
+    if (lane == 0) {
+      const int w = kb0 >> 5;
+      kmask[tile * nwords + w] = (uint32_t)bal;
+      if (w + 1 < nwords) kmask[tile * nwords + w + 1] = (uint32_t)(bal >> 32);
+    }
+    total += __popcll(bal);
+  }
+  if (lane == 0) {
+    kcount[tile] = total;
+    atomicAdd(active_total, (unsigned long long)total);
+  }
+}
+
+}  // namespace
+
+size_t screen_sort_workspace(int64_t n) {
+  size_t bytes = 0;
+  CT_HIP_CHECK(rocprim::radix_sort_pairs(nullptr, bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                         (const int32_t*)nullptr, (int32_t*)nullptr, (size_t)n, 0, 30));
+  return bytes;
+}
+
+void launch_screen_sort(const double* pts, int64_t n, const MortonFrame& f, uint32_t* keys_in, uint32_t* keys_out,
+                        int32_t* idx_in, int32_t* perm, void* temp, size_t temp_bytes, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(morton_keys_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, pts, n, f, keys_in,
+                     idx_in);
+  CT_HIP_CHECK(hipGetLastError());
+  size_t bytes = temp_bytes;
+  CT_HIP_CHECK(rocprim::radix_sort_pairs(temp, bytes, (const uint32_t*)keys_in, keys_out, (const int32_t*)idx_in,
+                                         perm, (size_t)n, 0, 30, s));
+}
+
+void launch_tile_activity(const double* sorted_pts, int64_t n, int tile_points, int64_t ntile, const int32_t* span_off,
+                          const ScreenSpan* spans, int nk, int32_t* klist, int32_t* kcount, uint32_t* kmask,
+                          int nwords, unsigned long long* active_total, hipStream_t s) {
+  CT_REQUIRE(tile_points > 0 && tile_points <= 64, "tile of more than 64 points");
+  CT_REQUIRE(nwords == (nk + 31) / 32, "mask width does not match the number of k-steps");
+  if (ntile <= 0) return;
+  hipLaunchKernelGGL(tile_activity_kernel, dim3((unsigned)((ntile + 3) / 4)), dim3(256), 0, s, sorted_pts, n,
+                     tile_points, ntile, span_off, spans, nk, klist, kcount, kmask, nwords, active_total);
+  CT_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace ct

@@ -133,6 +133,8 @@ typedef struct {
   int64_t n_chunks;
   double max_error_estimate;
   int64_t factor_rank;     /* mixed path: rank of the factor if the FACTOR route ran, else -1 */
+  double active_fraction;  /* mixed FACTOR route with screening: fraction of the (point tile, k-step)
+                              blocks contracted; 1 otherwise */
 } ct_eval_report;
 
 /* Lifetime ---------------------------------------------------------------- */
@@ -171,6 +173,23 @@ int ct_mol_set_density_matrix(ct_mol* mol, const double* P);
 /* Device-memory budget for the per-chunk workspace (bytes; default 8 GiB). */
 int ct_mol_set_workspace_limit(ct_mol* mol, int64_t bytes);
 
+/* Spatial screening of the mixed FACTOR route (no counterpart in the
+ * reference, which evaluates every basis function at every point,
+ * src/evaluate_density.cu:25-411).  Points are taken in Morton order and,
+ * per tile of 32 or 64 of them, the 32-function blocks of the basis whose
+ * values, first and second derivatives are all at most eps on the tile are
+ * skipped; 4 eps ||c||_2 sqrt(rho) (c_j = ||L_j||_1, P = L L^T) is added to
+ * each point's error estimate, so mixed_tol still holds for rho.  eps = 0 turns it off; default
+ * 1e-15.  The other quantities change by at most about eps times their
+ * derivative order's AO scale. */
+int ct_mol_set_screening(ct_mol* mol, double eps);
+/* Distance from the centre beyond which every function of one shell (l,
+ * exponents and contraction coefficients as in ct_basis_desc) and its first
+ * and second derivatives are at most eps in magnitude: the cutoff the
+ * screening uses.  Host only. */
+int ct_shell_screening_radius(int32_t l, int32_t nprim, const double* exps, const double* coefs, double eps,
+                              double* radius);
+
 /* Evaluation --------------------------------------------------------------- */
 /* points: (n,3) row-major.  Output arrays named by the quantity bits must be
  * non-null; others are ignored.  report may be NULL. */
@@ -206,6 +225,8 @@ int ct_mol_synchronize(ct_mol* mol);
 typedef struct {
   double ao_ms, contract_ms, finalize_ms, esp_ms, backward_ms, reeval_ms;
   int64_t ao_calls, contract_calls, finalize_calls, esp_calls, backward_calls, reeval_calls;
+  double screen_ms;        /* point sort + tile activity of the screened factor route */
+  int64_t screen_calls;
 } ct_kernel_stats;
 int ct_mol_enable_profiling(ct_mol* mol, int enable);
 int ct_mol_kernel_stats(ct_mol* mol, ct_kernel_stats* out);

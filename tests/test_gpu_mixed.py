@@ -220,13 +220,16 @@ def test_factor_route_error_model_pointwise(ct, workload, npts):
     bound the kernel computes and the finaliser applies, 2 sum_j |psi_j|
     (|L|^T |phi|)_j, and the other quantities below 6u times their
     factor-route condition scales.  Also the rho-only and grad modes, and the
-    reported rank."""
+    reported rank.  Spatial screening off: this is the error model of the
+    split-bf16 engine at every point (screened tiles add a bound of their own,
+    tests/test_gpu_screening.py)."""
     from chemtoolscuda_amd import synthetic
 
     w = synthetic.large300(npts) if workload == "large300" else synthetic.WORKLOADS[workload]()
     rng = np.random.default_rng(9)
     pts = w.points[rng.choice(w.points.shape[0], size=min(npts, w.points.shape[0]), replace=False)]
     mol = ct.molecule_from_basis(w.basis, w.P, w.charges)
+    mol.set_screening(0.0)
     P = 0.5 * (w.P + w.P.T)
     L = _pivoted_cholesky(P)
     names = ["rho", "grad", "lap", "tau_pd"]
@@ -253,7 +256,8 @@ def test_factor_route_error_model_pointwise(ct, workload, npts):
           + ", ".join(f"{k} {v:.2f}" for k, v in ratios.items()))
     for k, v in ratios.items():
         assert v <= (3.0 if k.startswith("rho") else 6.0), (k, v)
-    # with re-evaluation: the 1e-6 target on rho
+    # with re-evaluation (and screening): the 1e-6 target on rho
+    mol.set_screening(1e-15)
     mix = mol.compute(pts, ["rho"], precision="mixed", route="factor")["rho"]
     m = f64["rho"] >= 1e-8
     assert np.max(np.abs(mix - f64["rho"])[m] / f64["rho"][m]) <= 1e-6
@@ -287,7 +291,8 @@ def test_device_resident_chunks_match_host_path(ct):
     points of all chunks re-evaluated in one FP64 pass at the end); host arrays
     re-evaluate chunk by chunk through pinned staging.  Per point both run the
     same kernels, so the results agree bit for bit, also with many small
-    chunks."""
+    chunks (spatial screening off: with it, the point tiles, and so the
+    skipped blocks, follow the chunking)."""
     import torch
     from chemtoolscuda_amd import capi, synthetic
 
@@ -297,6 +302,7 @@ def test_device_resident_chunks_match_host_path(ct):
     n = len(pts)
     names = ["rho", "grad", "lap", "tau_pd"]
     h = capi.Handle(w.basis.flat(), w.P, w.charges)
+    h.set_screening(0.0)
     host = {q: np.zeros((n, capi.WIDTH.get(q, 1))) for q in names}
     h.eval(pts.ctypes.data, n, {q: a.ctypes.data for q, a in host.items()}, "mixed")
     dev_pts = torch.from_numpy(pts).cuda()
@@ -335,13 +341,15 @@ def test_factor_route_hessian_pointwise(ct, workload, npts):
     """Hessian mode on the factor route (three launches: {phi, d phi} for rho,
     grad and t_kl; {phi, d_kl phi} twice for h_kl): without re-evaluation rho
     within 3u of its bound, the Hessian within 6u of its factor-route scale;
-    with re-evaluation rho to 1e-6 and rho ln rho to 1e-6 of |rho ln rho| + rho."""
+    with re-evaluation rho to 1e-6 and rho ln rho to 1e-6 of |rho ln rho| + rho.
+    Spatial screening off for the pointwise model, on for the 1e-6 targets."""
     from chemtoolscuda_amd import synthetic
 
     w = synthetic.large300(npts) if workload == "large300" else synthetic.WORKLOADS[workload]()
     rng = np.random.default_rng(13)
     pts = w.points[rng.choice(w.points.shape[0], size=min(npts, w.points.shape[0]), replace=False)]
     mol = ct.molecule_from_basis(w.basis, w.P, w.charges)
+    mol.set_screening(0.0)
     L = _pivoted_cholesky(0.5 * (w.P + w.P.T))
     names = ["rho", "grad", "hess"]
     raw = mol.compute(pts, names, precision="mixed", mixed_tol=1e30, route="factor")
@@ -356,6 +364,7 @@ def test_factor_route_hessian_pointwise(ct, workload, npts):
     print(f"{workload} (M={w.basis.nbasis}, rank {L.shape[1]}): factor route Hessian mode, max error / (u * scale): "
           f"rho {r_rho:.2f}, grad {r_grad:.2f}, hess {r_hess:.2f}")
     assert r_rho <= 3.0 and r_grad <= 6.0 and r_hess <= 6.0, (r_rho, r_grad, r_hess)
+    mol.set_screening(1e-15)
     mix = mol.compute(pts, ["rho", "hess", "rdg", "shannon"], precision="mixed", route="factor")
     m = f64["rho"] >= 1e-8
     rho = f64["rho"][m]
