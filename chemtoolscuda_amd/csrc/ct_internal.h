@@ -186,7 +186,7 @@ struct ScreenLayout {
 };
 ScreenLayout build_screen_layout(const std::vector<int>& shell_l, const std::vector<int>& shell_atom,
                                  const std::vector<double>& radius, const double* xyz, int nbasis_pad);
-// Lattice of the Morton keys: cell (floor((x - origin) * inv_cell)), 10 bits
+// Lattice of the Morton keys: cell (floor((x - origin) * inv_cell)), 8 bits
 // per axis, clamped.
 struct MortonFrame {
   double origin[3];
@@ -198,10 +198,12 @@ void launch_screen_sort(const double* pts, int64_t n, const MortonFrame& f, uint
                         int32_t* idx_in, int32_t* perm, void* temp, size_t temp_bytes, hipStream_t s);
 // Per tile of tile_points consecutive (sorted) points: active k-steps
 // klist[tile * nk ..] (kcount[tile] of them, increasing), mask
-// kmask[tile * nwords ..], nwords = ceil(nk / 32); active_total += the counts.
+// kmask[tile * nwords ..], nwords = ceil(nk / 32); the counts are added to
+// active_count[16 c], c = tile % kScreenCounters (sum them for the total).
+constexpr int kScreenCounters = 64;
 void launch_tile_activity(const double* sorted_pts, int64_t n, int tile_points, int64_t ntile, const int32_t* span_off,
                           const ScreenSpan* spans, int nk, int32_t* klist, int32_t* kcount, uint32_t* kmask,
-                          int nwords, unsigned long long* active_total, hipStream_t s);
+                          int nwords, unsigned long long* active_count, hipStream_t s);
 // Points per tile of the factor route's contraction in this mode (the
 // granularity of the activity lists).
 int factor_tile_points(int mode);

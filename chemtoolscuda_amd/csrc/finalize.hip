@@ -82,8 +82,11 @@ __global__ __launch_bounds__(256) void finalize_kernel(FinalizeArgs a) {
 }
 
 __global__ __launch_bounds__(256) void finalize_mixed_kernel(FinalizeArgs a, FlagArgs f) {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= a.n) return;
+  const int64_t p0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // lanes past the end run on the last point (no stores) so that the whole
+  // wave takes part in the reduction of the largest estimate
+  const bool live = p0 < a.n;
+  const int64_t p = live ? p0 : a.n - 1;
   const int nd = mode_ndot(a.mode);
   const int nd1 = nd + 1;
   double dot[16];
@@ -97,17 +100,22 @@ __global__ __launch_bounds__(256) void finalize_mixed_kernel(FinalizeArgs a, Fla
   }
   // index (screened layout): the chunk position of the p-th sorted point
   const int64_t pp = a.index ? (int64_t)a.index[p] : p;
-  emit_outputs(a, a.out_off + pp, dot);
+  if (live) emit_outputs(a, a.out_off + pp, dot);
   const double arho = fabs(dot[0]);
-  if (arho < f.rho_floor) return;
-  const double est = f.gamma * bound + f.trunc * sqrt(arho);
-  if (!(est <= f.tol * arho)) {  // also catches NaN
-    const int k = atomicAdd(f.count, 1);
-    f.index[k] = (int32_t)(f.index_base + pp);
-  } else {
-    const double rel = est / arho;
-    atomicMax(f.max_est, (unsigned long long)__double_as_longlong(rel));
+  double rel = 0.0;  // estimate / |rho| of an accepted point (non-negative: the bits order like the values)
+  if (live && !(arho < f.rho_floor)) {
+    const double est = f.gamma * bound + f.trunc * sqrt(arho);
+    if (!(est <= f.tol * arho)) {  // also catches NaN
+      const int k = atomicAdd(f.count, 1);
+      f.index[k] = (int32_t)(f.index_base + pp);
+    } else {
+      rel = est / arho;
+    }
   }
+  // one atomic per wave: the same address for every point would serialise
+  for (int off = 32; off >= 1; off >>= 1) rel = fmax(rel, __shfl_xor(rel, off));
+  if ((threadIdx.x & 63) == 0 && rel > 0.0)
+    atomicMax(f.max_est, (unsigned long long)__double_as_longlong(rel));
 }
 
 __global__ __launch_bounds__(256) void gather_points_kernel(const double* __restrict__ pts,

@@ -394,8 +394,10 @@ void Mol::build_screen() {
       radius[s] = shell_screen_radius(shell_l[s], m.nprim, a.data(), cn.data(), screen_eps_);
     }
     layout_ = build_screen_layout(shell_l, shell_atom, radius, xyz_.data(), Mp_);
-    // Morton lattice: the atoms' box with a margin of 12 bohr, 1024 cells per
-    // axis, cells of at least 0.25 bohr (points outside clamp to the border)
+    // Morton lattice: the atoms' box with a margin of 12 bohr, 256 cells per
+    // axis, cells of at least 0.25 bohr (points outside clamp to the border;
+    // order within a cell is the input order, which only matters below the
+    // size of a point tile)
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
     for (int a = 0; a < natom_; ++a)
       for (int k = 0; k < 3; ++k) {
@@ -407,7 +409,7 @@ void Mol::build_screen() {
       frame_.origin[k] = lo[k] - 12.0;
       span = std::max(span, hi[k] - lo[k] + 24.0);
     }
-    frame_.inv_cell = 1.0 / std::max(0.25, span / 1024.0);
+    frame_.inv_cell = 1.0 / std::max(0.25, span / 256.0);
   } else {
     layout_ = ScreenLayout{};
     layout_.order.resize(nshell_);
@@ -785,7 +787,9 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
   // point; k-step list (2 nk B), count and mask (4 + 4 nwords B) per tile
   const int64_t per_point_s = screened ? 40 + (4 * nk + 4 + 4 * nwords + tp - 1) / tp : 0;
   int64_t Nc = chunk_points(per_point + per_point_s, n);
-  const size_t head = 256;  // flag counter + max estimate + active k-step count
+  // flag counter, max estimate; from byte 256 the active k-step counters (one
+  // per 128-byte line)
+  const size_t head = 256 + 128 * kScreenCounters;
   size_t sort_bytes = 0;
   for (;;) {
     sort_bytes = screened ? (screen_sort_workspace(Nc) + 255) / 256 * 256 : 0;
@@ -796,7 +800,7 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
   char* ws = static_cast<char*>(d_ws_.get());
   int32_t* d_count = reinterpret_cast<int32_t*>(ws);
   unsigned long long* d_maxest = reinterpret_cast<unsigned long long*>(ws + 8);
-  unsigned long long* d_active = reinterpret_cast<unsigned long long*>(ws + 16);
+  unsigned long long* d_active = reinterpret_cast<unsigned long long*>(ws + 256);
   char* cur = ws + head;
   void* ao = cur;
   cur += (size_t)Nc * ao_bytes_pt;
@@ -833,7 +837,7 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
     CT_REQUIRE(cur <= ws + d_ws_.size(), "screening workspace overflow");
   }
   int64_t valid_tiles = 0;
-  h_small_.reserve(64);
+  h_small_.reserve(64 + 128 * kScreenCounters);
   auto* h_small = static_cast<int64_t*>(h_small_.get());
 
   const auto* meta = static_cast<const ShellMeta*>(d_meta_.get());
@@ -847,7 +851,7 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
   if (mixed) {
     CT_HIP_CHECK(hipMemsetAsync(d_maxest, 0, 8, stream_));
     CT_HIP_CHECK(hipMemsetAsync(d_count, 0, 4, stream_));
-    CT_HIP_CHECK(hipMemsetAsync(d_active, 0, 8, stream_));
+    if (screened) CT_HIP_CHECK(hipMemsetAsync(d_active, 0, 128 * kScreenCounters, stream_));
   }
 
   int64_t chunks = 0, reevaluated = 0;
@@ -985,7 +989,9 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
       reevaluated = count;
     }
   }
-  if (mixed) CT_HIP_CHECK(hipMemcpyAsync(h_small + 1, d_maxest, 16, hipMemcpyDeviceToHost, stream_));
+  if (mixed) CT_HIP_CHECK(hipMemcpyAsync(h_small + 1, d_maxest, 8, hipMemcpyDeviceToHost, stream_));
+  if (screened)
+    CT_HIP_CHECK(hipMemcpyAsync(h_small + 8, d_active, 128 * kScreenCounters, hipMemcpyDeviceToHost, stream_));
   CT_HIP_CHECK(hipStreamSynchronize(stream_));
   timer_.collect();
   if (rep) {
@@ -995,8 +1001,11 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
     rep->factor_rank = use_factor ? factor_rank_ : -1;
     // k-steps holding basis functions (the padding beyond them is never active)
     const int64_t nk_basis = (M_ + 31) / 32;
-    if (screened && valid_tiles > 0)
-      rep->active_fraction = (double)(uint64_t)h_small[2] / ((double)valid_tiles * (double)nk_basis);
+    if (screened && valid_tiles > 0) {
+      uint64_t active = 0;
+      for (int c = 0; c < kScreenCounters; ++c) active += (uint64_t)h_small[8 + 16 * c];
+      rep->active_fraction = (double)active / ((double)valid_tiles * (double)nk_basis);
+    }
   }
 }
 

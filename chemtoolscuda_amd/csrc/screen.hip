@@ -1,6 +1,6 @@
 // Spatial screening of the mixed factor route, device side.
 //
-//   1. points are put in Morton order (30-bit keys on a lattice around the
+//   1. points are put in Morton order (24-bit keys on a lattice around the
 //      molecule; stable radix sort of (key, index) pairs), so that a point
 //      tile of the contraction is a compact box in space;
 //   2. one wave per point tile takes the tile's bounding box and marks the
@@ -21,12 +21,12 @@ namespace ct {
 
 namespace {
 
-__device__ __forceinline__ uint32_t spread10(uint32_t v) {
-  v &= 0x3FFu;
-  v = (v | (v << 16)) & 0x030000FFu;
-  v = (v | (v << 8)) & 0x0300F00Fu;
-  v = (v | (v << 4)) & 0x030C30C3u;
-  v = (v | (v << 2)) & 0x09249249u;
+// bits of an 8-bit cell index spread to every third bit
+__device__ __forceinline__ uint32_t spread8(uint32_t v) {
+  v &= 0xFFu;
+  v = (v | (v << 8)) & 0x0000F00Fu;
+  v = (v | (v << 4)) & 0x000C30C3u;
+  v = (v | (v << 2)) & 0x00249249u;
   return v;
 }
 
@@ -38,9 +38,9 @@ __global__ __launch_bounds__(256) void morton_keys_kernel(const double* __restri
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     const double t = (pts[3 * p + k] - f.origin[k]) * f.inv_cell;
-    q[k] = t > 0.0 ? (t < 1023.0 ? (uint32_t)t : 1023u) : 0u;  // NaN -> 0
+    q[k] = t > 0.0 ? (t < 255.0 ? (uint32_t)t : 255u) : 0u;  // NaN -> 0
   }
-  keys[p] = (spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2]);
+  keys[p] = (spread8(q[0]) << 2) | (spread8(q[1]) << 1) | spread8(q[2]);
   idx[p] = (int32_t)p;
 }
 
@@ -50,7 +50,7 @@ __global__ __launch_bounds__(256) void tile_activity_kernel(const double* __rest
                                                             const ScreenSpan* __restrict__ spans, int nk,
                                                             int32_t* __restrict__ klist, int32_t* __restrict__ kcount,
                                                             uint32_t* __restrict__ kmask, int nwords,
-                                                            unsigned long long* __restrict__ active_total) {
+                                                            unsigned long long* __restrict__ active_count) {
   const int lane = threadIdx.x & 63;
   const int64_t tile = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (tile >= ntile) return;
@@ -101,7 +101,9 @@ __global__ __launch_bounds__(256) void tile_activity_kernel(const double* __rest
   }
   if (lane == 0) {
     kcount[tile] = total;
-    atomicAdd(active_total, (unsigned long long)total);
+    // kScreenCounters counters, one per 128-byte line: one address for all
+    // tiles would serialise the atomics
+    atomicAdd(active_count + (tile % kScreenCounters) * 16, (unsigned long long)total);
   }
 }
 
@@ -110,7 +112,7 @@ __global__ __launch_bounds__(256) void tile_activity_kernel(const double* __rest
 size_t screen_sort_workspace(int64_t n) {
   size_t bytes = 0;
   CT_HIP_CHECK(rocprim::radix_sort_pairs(nullptr, bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
-                                         (const int32_t*)nullptr, (int32_t*)nullptr, (size_t)n, 0, 30));
+                                         (const int32_t*)nullptr, (int32_t*)nullptr, (size_t)n, 0, 24));
   return bytes;
 }
 
@@ -122,17 +124,17 @@ void launch_screen_sort(const double* pts, int64_t n, const MortonFrame& f, uint
   CT_HIP_CHECK(hipGetLastError());
   size_t bytes = temp_bytes;
   CT_HIP_CHECK(rocprim::radix_sort_pairs(temp, bytes, (const uint32_t*)keys_in, keys_out, (const int32_t*)idx_in,
-                                         perm, (size_t)n, 0, 30, s));
+                                         perm, (size_t)n, 0, 24, s));
 }
 
 void launch_tile_activity(const double* sorted_pts, int64_t n, int tile_points, int64_t ntile, const int32_t* span_off,
                           const ScreenSpan* spans, int nk, int32_t* klist, int32_t* kcount, uint32_t* kmask,
-                          int nwords, unsigned long long* active_total, hipStream_t s) {
+                          int nwords, unsigned long long* active_count, hipStream_t s) {
   CT_REQUIRE(tile_points > 0 && tile_points <= 64, "tile of more than 64 points");
   CT_REQUIRE(nwords == (nk + 31) / 32, "mask width does not match the number of k-steps");
   if (ntile <= 0) return;
   hipLaunchKernelGGL(tile_activity_kernel, dim3((unsigned)((ntile + 3) / 4)), dim3(256), 0, s, sorted_pts, n,
-                     tile_points, ntile, span_off, spans, nk, klist, kcount, kmask, nwords, active_total);
+                     tile_points, ntile, span_off, spans, nk, klist, kcount, kmask, nwords, active_count);
   CT_HIP_CHECK(hipGetLastError());
 }
 
