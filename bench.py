@@ -37,8 +37,10 @@ PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16": 2.5e15}
 # profiles/contract_f64_organic60.md for the commands and raw counts.
 # A mixed call is several kernel dispatches (density route: phi columns, then
 # one per derivative type; factor route: two); its figure is their sum,
-# averaged over the chunks.
-MEASURED_TRAFFIC = {("organic60", "mixed", "factor"): 8.44e9 + 0.038e9,
+# averaged over the chunks.  The factor route with the library's default
+# screening reads only the active blocks; "factor-unscreened" is --screening 0.
+MEASURED_TRAFFIC = {("organic60", "mixed", "factor"): 4.72e9 + 0.038e9,
+                    ("organic60", "mixed", "factor-unscreened"): 8.44e9 + 0.038e9,
                     ("organic60", "mixed", "density"): 24.1e9 + 0.19e9,
                     ("organic60", "fp64", "density"): 53.7e9 + 0.021e9}
 
@@ -55,6 +57,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU-baseline work (0 disables)")
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64-vs-mixed comparison run")
     ap.add_argument("--workspace-mb", type=int, default=0, help="per-chunk workspace budget (0: library default)")
+    ap.add_argument("--screening", type=float, default=None, metavar="EPS",
+                    help="spatial screening threshold of the mixed factor route (0 turns it off; "
+                         "default: the library's, 1e-15)")
     ap.add_argument("--no-extra", action="store_true", help="skip the fwd+bwd and ESP secondary measurements")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="only run a device-to-device copy of MB MiB (calibrates FETCH_SIZE/WRITE_SIZE counters)")
@@ -239,6 +244,8 @@ def main():
     h = capi.Handle(w.basis.flat(), w.P, w.charges)
     if args.workspace_mb > 0:
         h.set_workspace_limit(args.workspace_mb << 20)
+    if args.screening is not None:
+        h.set_screening(args.screening)
     precision = args.precision
     if precision == "auto":
         precision = "mixed"
@@ -272,8 +279,9 @@ def main():
     peak_key = "fp64" if precision == "fp64" else "bf16"
     rank = rep.get("factor_rank", -1) if rep else -1
     used_route = "factor" if rank >= 0 else "density"
+    traffic_key = used_route + ("-unscreened" if used_route == "factor" and args.screening == 0 else "")
     roofline = {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK[peak_key] / 1e12, "unit": "TFLOP/s",
-                "frac": achieved / PEAK[peak_key], "traffic": MEASURED_TRAFFIC.get((w.name, precision, used_route)),
+                "frac": achieved / PEAK[peak_key], "traffic": MEASURED_TRAFFIC.get((w.name, precision, traffic_key)),
                 "kernel": "contract_" + ("f64" if precision == "fp64" else "mixed"),
                 "avg_launch_ms": contract_ms, "points_per_launch": pts_per_launch,
                 "flops_per_point": flops_pt, "route": used_route}
