@@ -382,8 +382,9 @@ void Mol::build_screen() {
   // shell order: the screened one, or the natural one when screening is off
   std::vector<int> shell_l(basis_.shell_l.begin(), basis_.shell_l.end());
   std::vector<int> shell_atom(basis_.shell_atom.begin(), basis_.shell_atom.end());
+  std::vector<double> radius;
   if (screen_eps_ > 0.0) {
-    std::vector<double> radius(nshell_);
+    radius.assign(nshell_, 0.0);
     for (int s = 0; s < nshell_; ++s) {
       const ShellMeta& m = meta_[s];
       std::vector<double> a(m.nprim), cn(m.nprim);
@@ -430,6 +431,12 @@ void Mol::build_screen() {
     const int a = basis_.shell_atom[s];
     double amin = 1e300;
     for (int k = 0; k < meta_[s].nprim; ++k) amin = std::min(amin, prim_[meta_[s].prim_off + k].a);
+    // screened: the AO kernel writes a shell as zeros, without its
+    // exponentials, when every point of a wave has amin r^2 > 50; with
+    // amin >= 50 / R_s^2 that holds beyond the cutoff radius too, where every
+    // value is at most eps (covered by the same truncation bound as a
+    // skipped k-step: each function is either computed or below eps)
+    if (screen_eps_ > 0.0) amin = std::max(amin, 50.0 / (radius[s] * radius[s]));
     sg[i] = ShellGeom{xyz_[3 * a], xyz_[3 * a + 1], xyz_[3 * a + 2], amin};
   }
   if (!d_smeta_.reserve(sizeof(ShellMeta) * nshell_) || !d_sgeom_.reserve(sizeof(ShellGeom) * nshell_))
