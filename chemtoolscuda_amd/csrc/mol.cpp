@@ -262,6 +262,7 @@ void Mol::upload_geometry() {
     geom[s] = ShellGeom{xyz_[3 * a], xyz_[3 * a + 1], xyz_[3 * a + 2], amin};
   }
   CT_HIP_CHECK(hipMemcpy(d_geom_.get(), geom.data(), sizeof(ShellGeom) * nshell_, hipMemcpyHostToDevice));
+  layout_stale_ = true;
   if (screen_built_) build_screen();  // the shell order follows the centres
 }
 
@@ -374,16 +375,19 @@ void Mol::set_screening(double eps) {
   CT_REQUIRE(eps >= 0.0 && std::isfinite(eps), "screening threshold must be finite and non-negative");
   CT_HIP_CHECK(hipStreamSynchronize(stream_));
   screen_eps_ = eps;
+  layout_stale_ = true;
   build_screen();
 }
 
 void Mol::build_screen() {
+  // the shell order and the radii depend on the centres and the threshold
+  // only: a new P keeps them
   screen_built_ = true;
   // shell order: the screened one, or the natural one when screening is off
   std::vector<int> shell_l(basis_.shell_l.begin(), basis_.shell_l.end());
   std::vector<int> shell_atom(basis_.shell_atom.begin(), basis_.shell_atom.end());
-  std::vector<double> radius;
-  if (screen_eps_ > 0.0) {
+  std::vector<double>& radius = shell_radius_;
+  if (screen_eps_ > 0.0 && layout_stale_) {
     radius.assign(nshell_, 0.0);
     for (int s = 0; s < nshell_; ++s) {
       const ShellMeta& m = meta_[s];
@@ -411,7 +415,7 @@ void Mol::build_screen() {
       span = std::max(span, hi[k] - lo[k] + 24.0);
     }
     frame_.inv_cell = 1.0 / std::max(0.25, span / 256.0);
-  } else {
+  } else if (screen_eps_ <= 0.0) {
     layout_ = ScreenLayout{};
     layout_.order.resize(nshell_);
     layout_.first.resize(nshell_);
@@ -422,6 +426,7 @@ void Mol::build_screen() {
     layout_.func_src.resize(M_);
     for (int i = 0; i < M_; ++i) layout_.func_src[i] = i;
   }
+  layout_stale_ = false;
   // shell table in that order
   std::vector<ShellMeta> sm(nshell_);
   std::vector<ShellGeom> sg(nshell_);

@@ -157,6 +157,8 @@ class Mol {
   // spatial screening of the mixed factor route
   double screen_eps_ = 1e-15;
   bool screen_built_ = false;
+  bool layout_stale_ = true;           // centres or threshold changed since the layout was built
+  std::vector<double> shell_radius_;   // cutoff radii at screen_eps_
   ScreenLayout layout_;    // identity order (and no spans) when screening is off
   MortonFrame frame_{};
   DeviceBuffer d_smeta_, d_sgeom_;     // shell table in the screened order

@@ -113,40 +113,169 @@ double shell_screen_radius(int l, int nprim, const double* a, const double* cn, 
   return hi;
 }
 
+namespace {
+
+// Reach of every shell on a lattice of cubic cells around the molecule: bit
+// (s, c) is set when cell c comes closer to the centre of shell s than its
+// cutoff radius.  A stand-in for the point tiles, which the layout cannot
+// know in advance.
+struct Reach {
+  int words = 0;
+  std::vector<uint64_t> bits;  // nshell x words
+  std::vector<int> count;      // cells reached per shell
+  const uint64_t* row(int s) const { return bits.data() + (size_t)s * words; }
+};
+
+Reach shell_reach(const std::vector<int>& shell_atom, const std::vector<double>& radius, const double* xyz) {
+  const int nshell = (int)shell_atom.size();
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int s = 0; s < nshell; ++s)
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = std::min(lo[k], xyz[3 * shell_atom[s] + k]);
+      hi[k] = std::max(hi[k], xyz[3 * shell_atom[s] + k]);
+    }
+  // the atoms' box and 6 bohr around it (where grids put their points), in
+  // cells of 1.5 bohr, at most 24 per axis for large molecules
+  constexpr double pad = 6.0;
+  double span = 0.0;
+  for (int k = 0; k < 3; ++k) {
+    lo[k] -= pad;
+    span = std::max(span, hi[k] + pad - lo[k]);
+  }
+  const double cell = std::max(1.5, span / 24.0);
+  int n[3];
+  for (int k = 0; k < 3; ++k) n[k] = std::max(1, (int)std::ceil((hi[k] + pad - lo[k]) / cell));
+  const int ncell = n[0] * n[1] * n[2];
+  Reach r;
+  r.words = (ncell + 63) / 64;
+  r.bits.assign((size_t)nshell * r.words, 0);
+  r.count.assign(nshell, 0);
+  for (int s = 0; s < nshell; ++s) {
+    const double* c = xyz + 3 * shell_atom[s];
+    const double r2 = radius[s] * radius[s];
+    uint64_t* row = r.bits.data() + (size_t)s * r.words;
+    int b0[3], b1[3];  // cells whose box can come within the radius
+    for (int k = 0; k < 3; ++k) {
+      b0[k] = std::max(0, (int)std::floor((c[k] - radius[s] - lo[k]) / cell));
+      b1[k] = std::min(n[k] - 1, (int)std::floor((c[k] + radius[s] - lo[k]) / cell));
+    }
+    for (int i = b0[0]; i <= b1[0]; ++i)
+      for (int j = b0[1]; j <= b1[1]; ++j)
+        for (int l = b0[2]; l <= b1[2]; ++l) {
+          const int idx[3] = {i, j, l};
+          double d2 = 0.0;
+          for (int k = 0; k < 3; ++k) {
+            const double a = lo[k] + idx[k] * cell, b = a + cell;
+            const double d = std::max(0.0, std::max(a - c[k], c[k] - b));
+            d2 += d * d;
+          }
+          if (d2 < r2) {
+            const int cid = (i * n[1] + j) * n[2] + l;
+            row[cid >> 6] |= 1ull << (cid & 63);
+            ++r.count[s];
+          }
+        }
+  }
+  return r;
+}
+
+}  // namespace
+
 ScreenLayout build_screen_layout(const std::vector<int>& shell_l, const std::vector<int>& shell_atom,
                                  const std::vector<double>& radius, const double* xyz, int nbasis_pad) {
   const int nshell = (int)shell_l.size();
   ScreenLayout out;
-  // Order: three classes of cutoff radius (largest first, split at the
-  // radius tertiles), within a class by the Morton code of the shell centre
-  // on a 2-bohr lattice, then by the original index.  K-steps then hold
-  // shells of similar reach from one region of space, so a compact point
-  // tile is out of reach of whole k-steps.
-  std::vector<double> sorted_r(radius);
-  std::sort(sorted_r.begin(), sorted_r.end());
-  const double t1 = sorted_r[nshell / 3], t2 = sorted_r[(2 * nshell) / 3];
-  double lo[3] = {1e300, 1e300, 1e300};
-  for (int s = 0; s < nshell; ++s)
-    for (int k = 0; k < 3; ++k) lo[k] = std::min(lo[k], xyz[3 * shell_atom[s] + k]);
-  auto spread = [](uint64_t v) {
-    uint64_t out = 0;
-    for (int b = 0; b < 21; ++b) out |= ((v >> b) & 1ull) << (3 * b);
-    return out;
-  };
-  std::vector<int> cls(nshell);
-  std::vector<uint64_t> key(nshell);
-  for (int s = 0; s < nshell; ++s) {
-    cls[s] = radius[s] > t2 ? 0 : radius[s] > t1 ? 1 : 2;
-    uint64_t q[3];
-    for (int k = 0; k < 3; ++k) q[k] = (uint64_t)std::floor((xyz[3 * shell_atom[s] + k] - lo[k]) / 2.0);
-    key[s] = (spread(q[0]) << 2) | (spread(q[1]) << 1) | spread(q[2]);
+  // Order.  A k-step of 32 functions is needed by a point tile when any of
+  // its shells reaches the tile, so the order should put shells with the
+  // same reach side by side.  On the lattice stand-in (shell_reach):
+  //   1. a greedy chain from the shell of widest reach, each next shell the
+  //      remaining one whose reach differs from the last one's in the fewest
+  //      cells;
+  //   2. a local search: swaps of two shells of the same size (the function
+  //      offsets stay put) in different k-steps, kept when the sum over
+  //      k-steps of the cells their shells reach does not grow.
+  // organic60 (288 shells): 52 % of the (tile, k-step) blocks of its grid
+  // active, against 57 % for classes of cutoff radius in Morton order.
+  const Reach reach = shell_reach(shell_atom, radius, xyz);
+  const int W = reach.words;
+  auto popc = [](uint64_t v) { return __builtin_popcountll(v); };
+  out.order.clear();
+  {
+    std::vector<char> used(nshell, 0);
+    int cur = (int)(std::max_element(reach.count.begin(), reach.count.end()) - reach.count.begin());
+    for (int step = 0; step < nshell; ++step) {
+      out.order.push_back(cur);
+      used[cur] = 1;
+      int best = -1;
+      long best_d = 0;
+      const uint64_t* a = reach.row(cur);
+      for (int s = 0; s < nshell; ++s) {
+        if (used[s]) continue;
+        const uint64_t* b = reach.row(s);
+        long inter = 0;
+        for (int w = 0; w < W; ++w) inter += popc(a[w] & b[w]);
+        const long d = reach.count[cur] + reach.count[s] - 2 * inter;
+        if (best < 0 || d < best_d) {
+          best = s;
+          best_d = d;
+        }
+      }
+      cur = best;
+    }
   }
-  out.order.resize(nshell);
-  std::iota(out.order.begin(), out.order.end(), 0);
-  std::stable_sort(out.order.begin(), out.order.end(), [&](int x, int y) {
-    if (cls[x] != cls[y]) return cls[x] < cls[y];
-    return key[x] < key[y];
-  });
+  {
+    // positions -> k-steps they touch, k-steps -> positions
+    std::vector<int> off(nshell + 1, 0);
+    for (int i = 0; i < nshell; ++i) off[i + 1] = off[i] + shell_nfunc(shell_l[out.order[i]]);
+    const int nkb = (off[nshell] + 31) / 32;
+    std::vector<std::vector<int>> at(nkb);
+    std::vector<int> k0(nshell), k1(nshell);
+    for (int i = 0; i < nshell; ++i) {
+      k0[i] = off[i] / 32;
+      k1[i] = (off[i + 1] - 1) / 32;
+      for (int k = k0[i]; k <= k1[i]; ++k) at[k].push_back(i);
+    }
+    std::vector<uint64_t> u(W);
+    auto cost = [&](int k) {
+      std::fill(u.begin(), u.end(), 0ull);
+      for (int i : at[k]) {
+        const uint64_t* r = reach.row(out.order[i]);
+        for (int w = 0; w < W; ++w) u[w] |= r[w];
+      }
+      long c = 0;
+      for (int w = 0; w < W; ++w) c += popc(u[w]);
+      return c;
+    };
+    std::vector<long> kc(nkb);
+    for (int k = 0; k < nkb; ++k) kc[k] = cost(k);
+    uint64_t seed = 0x9E3779B97F4A7C15ull;  // fixed: the same order on every run
+    auto next = [&]() {
+      seed ^= seed << 13;
+      seed ^= seed >> 7;
+      seed ^= seed << 17;
+      return seed;
+    };
+    const int iters = std::min(40000, 150 * nshell);
+    for (int it = 0; it < iters; ++it) {
+      const int i = (int)(next() % nshell), j = (int)(next() % nshell);
+      if (shell_nfunc(shell_l[out.order[i]]) != shell_nfunc(shell_l[out.order[j]])) continue;
+      if (k0[i] == k0[j] && k1[i] == k1[j]) continue;
+      int aff[4], na = 0;
+      for (int k : {k0[i], k1[i], k0[j], k1[j]})
+        if (std::find(aff, aff + na, k) == aff + na) aff[na++] = k;
+      std::swap(out.order[i], out.order[j]);
+      long delta = 0, nc[4];
+      for (int a = 0; a < na; ++a) {
+        nc[a] = cost(aff[a]);
+        delta += nc[a] - kc[aff[a]];
+      }
+      if (delta <= 0) {
+        for (int a = 0; a < na; ++a) kc[aff[a]] = nc[a];
+      } else {
+        std::swap(out.order[i], out.order[j]);
+      }
+    }
+  }
   // function map and per-k-step spans
   std::vector<int> nat_first(nshell, 0);
   for (int s = 1; s < nshell; ++s) nat_first[s] = nat_first[s - 1] + shell_nfunc(shell_l[s - 1]);
