@@ -191,11 +191,16 @@ ScreenLayout build_screen_layout(const std::vector<int>& shell_l, const std::vec
   //   1. a greedy chain from the shell of widest reach, each next shell the
   //      remaining one whose reach differs from the last one's in the fewest
   //      cells;
-  //   2. a local search: swaps of two shells of the same size (the function
-  //      offsets stay put) in different k-steps, kept when the sum over
-  //      k-steps of the cells their shells reach does not grow.
-  // organic60 (288 shells): 52 % of the (tile, k-step) blocks of its grid
-  // active, against 57 % for classes of cutoff radius in Morton order.
+  //   2. simulated annealing over swaps of two shells of the same size (the
+  //      function offsets stay put) in different k-steps, on the sum over
+  //      k-steps of the cells their shells reach: a swap is kept when that
+  //      sum does not grow, or with probability exp(-growth / T), T falling
+  //      linearly from 0.75 % of the lattice to 0 (fixed pseudo-random
+  //      sequence: the same order on every run).
+  // organic60 (288 shells): 46 % of the (tile, k-step) blocks of its grid
+  // active (46-49 % over other seeds of the sequence), against 51 % for the
+  // chain with a plain descent and 57 % for classes of cutoff radius in
+  // Morton order; 0.17 s on the host (large300, 1125 shells: 0.7 s).
   const Reach reach = shell_reach(shell_atom, radius, xyz);
   const int W = reach.words;
   auto popc = [](uint64_t v) { return __builtin_popcountll(v); };
@@ -255,7 +260,8 @@ ScreenLayout build_screen_layout(const std::vector<int>& shell_l, const std::vec
       seed ^= seed << 17;
       return seed;
     };
-    const int iters = std::min(40000, 150 * nshell);
+    const int iters = std::min(200000, 600 * nshell);
+    const double t0 = 0.0075 * 64.0 * W;
     for (int it = 0; it < iters; ++it) {
       const int i = (int)(next() % nshell), j = (int)(next() % nshell);
       if (shell_nfunc(shell_l[out.order[i]]) != shell_nfunc(shell_l[out.order[j]])) continue;
@@ -269,7 +275,8 @@ ScreenLayout build_screen_layout(const std::vector<int>& shell_l, const std::vec
         nc[a] = cost(aff[a]);
         delta += nc[a] - kc[aff[a]];
       }
-      if (delta <= 0) {
+      const double temp = t0 * (1.0 - (double)it / iters);
+      if (delta <= 0 || (temp > 0.0 && (double)(next() % 1000000) * 1e-6 < std::exp(-(double)delta / temp))) {
         for (int a = 0; a < na; ++a) kc[aff[a]] = nc[a];
       } else {
         std::swap(out.order[i], out.order[j]);
