@@ -226,6 +226,9 @@ void launch_modes(int mode, const double* P, int nbasis_pad, const double* ao, i
 //        mode) or t_kl (Hessian mode);
 //   F2:  {phi, lap phi} x 32 points: psi . lap psi;
 //   FH0, FH1: {phi, d_kl phi for kl = xx, xy, xz | yy, yz, zz} x 16 points: h_kl.
+// With gridDim.y > 1 (few points) each workgroup row y takes the 64 factor
+// columns of pass y only and writes its partial dots at dots + y * ndot * ld
+// (summed in order afterwards): more waves for the same work.
 enum VF64 : int { F64_F1, F64_F4, F64_F2, F64_FH0, F64_FH1 };
 
 template <int VF, int MODE>
@@ -244,6 +247,10 @@ __global__ __launch_bounds__(256) void contract_f64_factor_kernel(const double* 
   const int r = lane & 15, q = lane >> 4;
   const int64_t p0 = ((int64_t)blockIdx.x * 4 + wave) * PPW;
   const int64_t sd = (int64_t)mp * ld;
+  const bool split = gridDim.y > 1;
+  const int i_begin = split ? (int)blockIdx.y * 64 : 0;
+  const int i_end = split ? i_begin + 64 : rows;
+  dots += split ? (int64_t)blockIdx.y * mode_ndot(MODE) * ld : 0;
 
   // AO array of type t (0 = phi, 1..3 = d phi, 4 = lap phi | 4..9 = d_kl phi)
   auto arr = [](int t) {
@@ -259,7 +266,7 @@ __global__ __launch_bounds__(256) void contract_f64_factor_kernel(const double* 
 #pragma unroll
     for (int k = 0; k < NDV; ++k) dacc[pb][k] = 0.0;
 
-  for (int i0 = 0; i0 < rows; i0 += 64) {
+  for (int i0 = i_begin; i0 < i_end; i0 += 64) {
     d4 acc[4][4];
 #pragma unroll
     for (int rb = 0; rb < 4; ++rb)
@@ -355,35 +362,56 @@ __global__ __launch_bounds__(256) void contract_f64_factor_kernel(const double* 
 }
 
 template <int VF, int MODE>
-void launch_factor_var(const double* L, int rows, int mp, const double* ao, int64_t ld, double* dots, hipStream_t s) {
+void launch_factor_var(const double* L, int rows, int mp, const double* ao, int64_t ld, double* dots, hipStream_t s,
+                       bool split = false) {
   constexpr int PPW = VF == F64_F1 ? 64 : VF == F64_F2 ? 32 : 16;
-  const dim3 block(256), grid((unsigned)(ld / (4 * PPW)));  // ld is a multiple of 256
+  const dim3 block(256), grid((unsigned)(ld / (4 * PPW)), split ? (unsigned)(rows / 64) : 1u);  // ld: multiple of 256
   hipLaunchKernelGGL((contract_f64_factor_kernel<VF, MODE>), grid, block, 0, s, L, rows, mp, ao, ld, dots);
   CT_HIP_CHECK(hipGetLastError());
 }
 
 }  // namespace
 
-void launch_contract_f64_factor(int mode, const double* L, int rows_pad, int nbasis_pad, const double* ao,
-                                int64_t ld, double* dots, hipStream_t s) {
+namespace {
+void factor_modes(int mode, const double* L, int rows_pad, int nbasis_pad, const double* ao, int64_t ld, double* dots,
+                  hipStream_t s, bool split) {
   CT_REQUIRE(rows_pad % 64 == 0 && rows_pad > 0 && ld % 256 == 0, "factor padding / point stride");
   switch (mode) {
     case MODE_RHO:
-      launch_factor_var<F64_F1, MODE_RHO>(L, rows_pad, nbasis_pad, ao, ld, dots, s);
+      launch_factor_var<F64_F1, MODE_RHO>(L, rows_pad, nbasis_pad, ao, ld, dots, s, split);
       break;
     case MODE_GRAD:
-      launch_factor_var<F64_F4, MODE_GRAD>(L, rows_pad, nbasis_pad, ao, ld, dots, s);
+      launch_factor_var<F64_F4, MODE_GRAD>(L, rows_pad, nbasis_pad, ao, ld, dots, s, split);
       break;
     case MODE_LAP:
-      launch_factor_var<F64_F4, MODE_LAP>(L, rows_pad, nbasis_pad, ao, ld, dots, s);
-      launch_factor_var<F64_F2, MODE_LAP>(L, rows_pad, nbasis_pad, ao, ld, dots, s);
+      launch_factor_var<F64_F4, MODE_LAP>(L, rows_pad, nbasis_pad, ao, ld, dots, s, split);
+      launch_factor_var<F64_F2, MODE_LAP>(L, rows_pad, nbasis_pad, ao, ld, dots, s, split);
       break;
     default:
-      launch_factor_var<F64_F4, MODE_HESS>(L, rows_pad, nbasis_pad, ao, ld, dots, s);
-      launch_factor_var<F64_FH0, MODE_HESS>(L, rows_pad, nbasis_pad, ao, ld, dots, s);
-      launch_factor_var<F64_FH1, MODE_HESS>(L, rows_pad, nbasis_pad, ao, ld, dots, s);
+      launch_factor_var<F64_F4, MODE_HESS>(L, rows_pad, nbasis_pad, ao, ld, dots, s, split);
+      launch_factor_var<F64_FH0, MODE_HESS>(L, rows_pad, nbasis_pad, ao, ld, dots, s, split);
+      launch_factor_var<F64_FH1, MODE_HESS>(L, rows_pad, nbasis_pad, ao, ld, dots, s, split);
       break;
   }
+}
+}  // namespace
+
+void launch_contract_f64_factor(int mode, const double* L, int rows_pad, int nbasis_pad, const double* ao,
+                                int64_t ld, double* dots, hipStream_t s) {
+  factor_modes(mode, L, rows_pad, nbasis_pad, ao, ld, dots, s, false);
+}
+
+int64_t contract_f64_factor_split_workspace(int mode, int rows_pad, int64_t ld) {
+  return (int64_t)(rows_pad / 64) * mode_ndot(mode) * ld;
+}
+
+void launch_contract_f64_factor_split(int mode, const double* L, int rows_pad, int nbasis_pad, const double* ao,
+                                      int64_t ld, double* part, double* dots, hipStream_t s) {
+  factor_modes(mode, L, rows_pad, nbasis_pad, ao, ld, part, s, true);
+  const int64_t count = (int64_t)mode_ndot(mode) * ld;
+  hipLaunchKernelGGL(reduce_row_blocks_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, part,
+                     rows_pad / 64, count, dots);
+  CT_HIP_CHECK(hipGetLastError());
 }
 
 void launch_contract_f64(int mode, const double* P, int nbasis_pad, const double* ao, int64_t ld, double* dots,

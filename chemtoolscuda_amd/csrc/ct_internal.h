@@ -127,6 +127,12 @@ void launch_finalize(const FinalizeArgs& a, hipStream_t s);
 // k-major, rows_pad a multiple of 64): rank(P) columns instead of nbasis.
 void launch_contract_f64_factor(int mode, const double* L, int rows_pad, int nbasis_pad, const double* ao,
                                 int64_t ld, double* dots, hipStream_t s);
+// The same with each 64-column pass of the factor in its own workgroups and
+// an ordered sum of the passes' partial dots (for small point counts); part
+// holds contract_f64_factor_split_workspace() doubles.
+int64_t contract_f64_factor_split_workspace(int mode, int rows_pad, int64_t ld);
+void launch_contract_f64_factor_split(int mode, const double* L, int rows_pad, int nbasis_pad, const double* ao,
+                                      int64_t ld, double* part, double* dots, hipStream_t s);
 // The same contraction with one 64-row block of P per wave and an ordered
 // sum of the blocks' partial dots (for small point counts); part holds
 // contract_f64_split_workspace() doubles.
@@ -193,6 +199,9 @@ struct MortonFrame {
   double inv_cell;
 };
 size_t screen_sort_workspace(int64_t n);
+// ascending sort of n non-negative int32 (the flagged points of a call)
+size_t sort_indices_workspace(int64_t n);
+void launch_sort_indices(const int32_t* in, int32_t* out, int64_t n, void* temp, size_t temp_bytes, hipStream_t s);
 // perm[i] = index of the i-th point in Morton order (stable)
 void launch_screen_sort(const double* pts, int64_t n, const MortonFrame& f, uint32_t* keys_in, uint32_t* keys_out,
                         int32_t* idx_in, int32_t* perm, void* temp, size_t temp_bytes, hipStream_t s);

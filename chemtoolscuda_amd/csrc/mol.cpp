@@ -727,10 +727,13 @@ void Mol::reevaluate(int mode, const double* pts, int64_t count, const int32_t* 
                      bool factor) {
   const int nder = mode_nder(mode), ndot = mode_ndot(mode);
   const int64_t ld = round_up(count, kPointAlign);
-  // few points: one wave per (16 or 64 points, 64-row block of P) so the GPU
-  // fills; the factor route re-evaluates with the FP64 factor kernel
-  const bool split = !factor && ld < 65536;
-  const size_t split_doubles = split ? (size_t)contract_f64_split_workspace(mode, Mp_, ld) : 0;
+  // few points: one wave per (16 or 64 points, 64-row block of P, or
+  // 64-column pass of the factor) so the GPU fills
+  const bool split = ld < 65536;
+  const size_t split_doubles =
+      !split ? 0
+      : factor ? (size_t)contract_f64_factor_split_workspace(mode, factor_rows_, ld)
+               : (size_t)contract_f64_split_workspace(mode, Mp_, ld);
   const size_t bytes = (size_t)ld * ((size_t)nder * Mp_ * 8 + (size_t)ndot * 8 + 24) + split_doubles * 8;
   if (!d_ws64_.reserve(bytes)) throw Error(CT_E_NOMEM, "cannot allocate the FP64 re-evaluation workspace");
   double* ao = static_cast<double*>(d_ws64_.get());
@@ -740,7 +743,10 @@ void Mol::reevaluate(int mode, const double* pts, int64_t count, const int32_t* 
   launch_gather_points(pts, index, count, gp, stream_);
   launch_ao_f64(mode, lmax_, static_cast<const ShellMeta*>(d_meta_.get()), static_cast<const ShellGeom*>(d_geom_.get()),
                 static_cast<const PrimData*>(d_prim_.get()), nshell_, M_, Mp_, gp, count, ld, ao, 700.0, stream_);
-  if (factor)
+  if (factor && split)
+    launch_contract_f64_factor_split(mode, static_cast<const double*>(d_L64_.get()), factor_rows_, Mp_, ao, ld, part,
+                                     dots, stream_);
+  else if (factor)
     launch_contract_f64_factor(mode, static_cast<const double*>(d_L64_.get()), factor_rows_, Mp_, ao, ld, dots,
                                stream_);
   else if (split)
@@ -991,12 +997,21 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
       fa.par = par;
       for (size_t k = 0; k < NS; ++k)
         if (q & kSlots[k].bit) fa.out.*(kSlots[k].field) = out.*(kSlots[k].field);
-      // the FP64 re-evaluation runs in chunks sized like the FP64 path's
+      // the FP64 re-evaluation runs in chunks sized like the FP64 path's, on
+      // the flagged points in index order (appended in any order; in order,
+      // points given in a spatial order stay so for the AO kernel's waves)
       const int64_t per64 = (int64_t)nder * Mp_ * 8 + (int64_t)ndot * 8 + 24;
       const int64_t c64 = chunk_points(per64, count);
+      const size_t tmp = (sort_indices_workspace(count) + 255) / 256 * 256;
+      const int32_t* sorted = list;
+      if (d_flags_sorted_.reserve(tmp + sizeof(int32_t) * (size_t)count)) {
+        char* base = static_cast<char*>(d_flags_sorted_.get());
+        launch_sort_indices(list, reinterpret_cast<int32_t*>(base + tmp), count, base, tmp, stream_);
+        sorted = reinterpret_cast<const int32_t*>(base + tmp);
+      }
       timer_.begin(KernelTimer::REEVAL, stream_);
       for (int64_t s0 = 0; s0 < count; s0 += c64)
-        reevaluate(mode, points, std::min(c64, count - s0), list + s0, fa, use_factor);
+        reevaluate(mode, points, std::min(c64, count - s0), sorted + s0, fa, use_factor);
       timer_.end(stream_);
       reevaluated = count;
     }

@@ -168,6 +168,7 @@ class Mol {
   DeviceBuffer d_ws_;      // AO + dots + staged points / outputs
   DeviceBuffer d_ws64_;    // mixed path: FP64 re-evaluation of flagged points
   DeviceBuffer d_flags_;   // mixed path: flagged point list of a whole call (device outputs)
+  DeviceBuffer d_flags_sorted_;  // the same in index order, and the sort's scratch
   PinnedBuffer h_small_;   // pinned readback of the flag counter / error estimate
   PinnedBuffer h_stage_;   // host staging for pageable inputs/outputs
   int64_t ws_limit_ = int64_t(8) << 30;

@@ -127,6 +127,18 @@ void launch_screen_sort(const double* pts, int64_t n, const MortonFrame& f, uint
                                          perm, (size_t)n, 0, 24, s));
 }
 
+size_t sort_indices_workspace(int64_t n) {
+  size_t bytes = 0;
+  CT_HIP_CHECK(rocprim::radix_sort_keys(nullptr, bytes, (const int32_t*)nullptr, (int32_t*)nullptr, (size_t)n, 0, 31));
+  return bytes;
+}
+
+void launch_sort_indices(const int32_t* in, int32_t* out, int64_t n, void* temp, size_t temp_bytes, hipStream_t s) {
+  if (n <= 0) return;
+  size_t bytes = temp_bytes;
+  CT_HIP_CHECK(rocprim::radix_sort_keys(temp, bytes, in, out, (size_t)n, 0, 31, s));
+}
+
 void launch_tile_activity(const double* sorted_pts, int64_t n, int tile_points, int64_t ntile, const int32_t* span_off,
                           const ScreenSpan* spans, int nk, int32_t* klist, int32_t* kcount, uint32_t* kmask,
                           int nwords, unsigned long long* active_count, hipStream_t s) {
