@@ -79,15 +79,32 @@ __global__ __launch_bounds__(256) void ao_eval_kernel(const ShellMeta* __restric
     if constexpr (MODE == MODE_HESS) ao_shell_dispatch_hess<T, LMAX>(code, x, y, z, s0, s1, s2, st);
   };
 
-  // screened layout: the activity masks of the (one or two) point tiles of
-  // this wave's 64 points
+  // screened layout: the activity masks of the 64 / tile_points point tiles
+  // of this wave's 64 points
   const uint32_t* mask0 = nullptr;
-  const uint32_t* mask1 = nullptr;
+  int ntw = 0;
   if (kmask) {
     const int64_t p0 = (int64_t)blockIdx.x * blockDim.x + 64 * __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
     mask0 = kmask + (p0 / tile_points) * nwords;
-    mask1 = kmask + ((p0 + 63) / tile_points) * nwords;
+    ntw = 64 / tile_points;
   }
+  // whether k-step k is active for any tile of the wave; up to 64 k-steps
+  // (2048 functions) the wave's union mask is formed once, in SGPRs
+  uint32_t wm0 = 0, wm1 = 0;
+  if (kmask && nwords <= 2)
+    for (int t = 0; t < ntw; ++t) {
+      wm0 |= mask0[t * nwords];
+      if (nwords == 2) wm1 |= mask0[t * nwords + 1];
+    }
+  auto active = [&](int k) {
+    uint32_t m = 0;
+    if (nwords <= 2) {
+      m = (k >> 5) ? wm1 : wm0;
+    } else {
+      for (int t = 0; t < ntw; ++t) m |= mask0[t * nwords + (k >> 5)];
+    }
+    return (m >> (k & 31)) & 1u;
+  };
   constexpr int kNfunc[8] = {1, 3, 6, 10, 15, 5, 7, 9};  // per shell code
 
   for (int s = 0; s < nshell; ++s) {
@@ -97,9 +114,7 @@ __global__ __launch_bounds__(256) void ao_eval_kernel(const ShellMeta* __restric
       // for these points (a quad it shares with a neighbouring shell lies in
       // one of those k-steps, so storing it or not does not matter either)
       const int k0 = m.first >> 5, k1 = (m.first + kNfunc[m.code] - 1) >> 5;
-      const uint32_t b0 = 1u << (k0 & 31), b1 = 1u << (k1 & 31);
-      const bool need = ((mask0[k0 >> 5] | mask1[k0 >> 5]) & b0) || ((mask0[k1 >> 5] | mask1[k1 >> 5]) & b1);
-      if (!need) continue;
+      if (!active(k0) && !active(k1)) continue;
     }
     const ShellGeom g = geom[s];
     const double dx = px - g.cx, dy = py - g.cy, dz = pz - g.cz;

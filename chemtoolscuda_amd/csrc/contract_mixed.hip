@@ -41,7 +41,11 @@ using namespace mixed;
 //   V_F4<mode>:    32 points x {phi, d_x phi, d_y phi, d_z phi}: rho,
 //                  g_k = psi . d_k psi, the bound, and in the lap mode
 //                  t_kk = d_k psi . d_k psi;
-//   V_F2:          32 points x {phi, lap phi}: the lap mode's psi . lap psi;
+//   V_F2:          32 points x {phi, lap phi}: psi . lap psi (the lap
+//                  mode's second launch when V_F4<lap> computes the rest);
+//   V_F5:          16 points x {phi, d_x phi, d_y phi, d_z phi, lap phi}: the
+//                  whole lap mode in one launch (rho, g_k, t_kk, psi . lap psi
+//                  and the bound) -- psi computed once, phi read once;
 //   V_FH<part>:    32 points x {phi, d_kl phi for three kl}: the Hessian
 //                  mode's h_kl = psi . d_kl psi (part 0: xx, xy, xz; part 1:
 //                  yy, yz, zz); with V_F4<hess>, whose t_kl = d_k psi . d_l psi
@@ -50,7 +54,7 @@ using namespace mixed;
 // the first-order propagation of the per-column bound of psi into
 // rho = sum_j psi_j^2.
 // Launches of one mode write disjoint dot slots of the same partials array.
-enum Var : int { V_RHO, V_GRAD, V_LAPPHI, V_LAPDER, V_HESS, V_F1, V_F4, V_F2, V_FH };
+enum Var : int { V_RHO, V_GRAD, V_LAPPHI, V_LAPDER, V_HESS, V_F1, V_F4, V_F2, V_FH, V_F5 };
 
 // DT: the derivative type of a V_LAPDER launch, the mode of a V_F4 launch,
 // the part of a V_FH launch.
@@ -62,11 +66,14 @@ struct Cols {
                               : V == V_HESS || V == V_FH ? MODE_HESS
                               : V == V_F4             ? DT
                                                       : MODE_LAP;
-  static constexpr int NT = V == V_HESS || V == V_F4 || V == V_FH ? 4 : V == V_F2 ? 2 : 1;  // column types per tile
-  static constexpr int TC = V == V_HESS || V == V_LAPDER || V == V_F4 || V == V_FH ? 128 : 64;
+  static constexpr int NT = V == V_F5 ? 5 : V == V_HESS || V == V_F4 || V == V_FH ? 4 : V == V_F2 ? 2 : 1;  // column types
+  static constexpr int TC = V == V_F5 ? 80 : V == V_HESS || V == V_LAPDER || V == V_F4 || V == V_FH ? 128 : 64;
   static constexpr int NCB = TC / 16;                     // 16-column MFMA blocks
   static constexpr int PTS = TC / NT;                     // points per workgroup
-  static constexpr int NBC = V == V_LAPDER || V == V_F2 || V == V_FH ? 0 : V == V_HESS || V == V_F4 ? 2 : NCB;  // bound blocks
+  static constexpr int NBC = V == V_LAPDER || V == V_F2 || V == V_FH ? 0
+                             : V == V_F5                           ? 1
+                             : V == V_HESS || V == V_F4            ? 2
+                                                                   : NCB;  // bound blocks (the phi columns)
   static constexpr bool F64 = V != V_HESS;  // FP64 running sums (the Hessian tile lacks the registers)
   static constexpr bool TRI = V == V_RHO || V == V_LAPDER;  // quadratic forms only: triangle of P
   // AO array of column c: 0 = phi, 1..3 = d_x, d_y, d_z phi, then 4 = lap phi
@@ -74,7 +81,7 @@ struct Cols {
   __host__ __device__ static constexpr int type(int c) {
     return V == V_F2   ? (c < PTS ? 0 : 4)
            : V == V_FH ? (c < PTS ? 0 : 4 + 3 * DT + c / PTS - 1)
-           : NT == 4   ? c / PTS
+           : NT >= 4   ? c / PTS
            : V == V_LAPDER ? DT
                            : 0;
   }
@@ -123,8 +130,9 @@ __device__ __forceinline__ void factor_epilogue(const Tile& yt, uint16_t* lds, d
   constexpr bool LAP = C::MODE == MODE_LAP, HES = C::MODE == MODE_HESS;
   // dots per point: V_F1 [rho, bound]; V_F2 [psi.lap psi]; V_FH [psi.d_kl psi
   // x 3]; V_F4 [rho, g_x, g_y, g_z, (lap: t_xx, t_yy, t_zz | Hessian: t_xx,
-  // t_xy, t_yy, t_xz, t_yz, t_zz,) bound]
-  constexpr int NDF = V == V_F1 ? 2 : V == V_F2 ? 1 : V == V_FH ? 3 : (LAP ? 8 : HES ? 11 : 5);
+  // t_xy, t_yy, t_xz, t_yz, t_zz,) bound]; V_F5 [rho, g_x, g_y, g_z, t_xx,
+  // t_yy, t_zz, psi.lap psi, bound]
+  constexpr int NDF = V == V_F1 ? 2 : V == V_F2 ? 1 : V == V_FH ? 3 : V == V_F5 ? 9 : (LAP ? 8 : HES ? 11 : 5);
   const int lane = tid & 63, w = tid >> 6, r = lane & 15, q = lane >> 4;
   double* red = reinterpret_cast<double*>(lds);  // [NW][PTS][NDF]
 #pragma unroll
@@ -145,7 +153,7 @@ __device__ __forceinline__ void factor_epilogue(const Tile& yt, uint16_t* lds, d
         } else {
           d[0] += y0 * y0;
           d[NDF - 1] += 2.0 * fabs(y0) * (double)yt.bound[rb][pb][j];
-          if constexpr (V == V_F4) {
+          if constexpr (V == V_F4 || V == V_F5) {
             double yk[3];
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
@@ -153,6 +161,7 @@ __device__ __forceinline__ void factor_epilogue(const Tile& yt, uint16_t* lds, d
               d[1 + k] += y0 * yk[k];
               if constexpr (LAP) d[4 + k] += yk[k] * yk[k];
             }
+            if constexpr (V == V_F5) d[7] += y0 * yt.y(rb, 4 * NPB + pb, j);  // psi . lap psi
             if constexpr (HES) {  // t_xx, t_xy, t_yy, t_xz, t_yz, t_zz
               d[4] += yk[0] * yk[0];
               d[5] += yk[0] * yk[1];
@@ -187,9 +196,10 @@ __device__ __forceinline__ void factor_epilogue(const Tile& yt, uint16_t* lds, d
       // dot slots of the mode (mode_ndot): lap [rho, g, L0, t_xx, t_yy, t_zz],
       // Hessian [rho, g, h_xx..h_zz, t_xx, t_xy, t_yy, t_xz, t_yz, t_zz]; the
       // bound after them
-      const int slot_k = V == V_F2   ? 4
-                         : V == V_FH ? 4 + 3 * DT + k
-                         : k == NDF - 1 ? ND
+      const int slot_k = V == V_F2                 ? 4
+                         : V == V_FH               ? 4 + 3 * DT + k
+                         : V == V_F5 && k == 7     ? 4
+                         : k == NDF - 1            ? ND
                          : k < 4        ? k
                          : HES          ? k + 6
                                         : k + 1;
@@ -230,13 +240,15 @@ __global__ __launch_bounds__(256, 2) void contract_mixed_kernel(const u4* __rest
   const int64_t mq = mp / 4;  // row quads per AO array
 
   YTile<TC, NBC, C::F64> yt;
-  const int bc = tid & (TC - 1);
-  const float* bsrc = ao + ((int64_t)C::type(bc) * mq * ld + pbase + C::point(bc)) * 4;
+  // this thread's B column (TC a multiple of 64), or any column's (bcol)
+  auto bcol = [&](int c) { return ao + ((int64_t)C::type(c) * mq * ld + pbase + C::point(c)) * 4; };
+  const float* bsrc = TC % 64 == 0 ? bcol(tid & (TC - 1)) : nullptr;
   // factor route: the point tile's list of k-steps (screened: its own list
   // and length; else one shared list of all k-steps, kstride 0)
   const int32_t* kl = C::FAC ? klist + ptile * kstride : nullptr;
   const int kc = C::FAC ? (kcount ? kcount[ptile] : kcount_all) : 0;
-  y_tile<TC, NBC, C::F64, C::TRI, C::FAC>(Pf, rows, mp, ld, i0, tid, bsrc, Bs, yt, C::TRI ? i0 / KB : 0, kl, kc);
+  y_tile<TC, NBC, C::F64, C::TRI, C::FAC>(Pf, rows, mp, ld, i0, tid, bsrc, Bs, yt, C::TRI ? i0 / KB : 0, kl, kc,
+                                          bcol);
   if constexpr (C::FAC) {
     factor_epilogue<V, DT>(yt, &Bs[0][0][0][0], part, ld, rt, pbase, tid);
     return;
@@ -379,8 +391,7 @@ int factor_tile_points(int mode) {
     case MODE_RHO: return Cols<V_F1, 0>::PTS;
     case MODE_GRAD: return Cols<V_F4, MODE_GRAD>::PTS;
     case MODE_LAP:
-      static_assert(Cols<V_F4, MODE_LAP>::PTS == Cols<V_F2, 0>::PTS, "one tile size per mode");
-      return Cols<V_F4, MODE_LAP>::PTS;
+      return Cols<V_F5, 0>::PTS;
     default:
       static_assert(Cols<V_F4, MODE_HESS>::PTS == Cols<V_FH, 0>::PTS, "one tile size per mode");
       return Cols<V_F4, MODE_HESS>::PTS;
@@ -406,8 +417,7 @@ void launch_contract_mixed_factor(int mode, const uint16_t* Lf, int rows_pad, in
       launch_var<V_F4, MODE_GRAD>(L, rows_pad, nbasis_pad, ao, ld, part, s, klist, kstride, kcount, nk);
       break;
     case MODE_LAP:
-      launch_var<V_F4, MODE_LAP>(L, rows_pad, nbasis_pad, ao, ld, part, s, klist, kstride, kcount, nk);
-      launch_var<V_F2>(L, rows_pad, nbasis_pad, ao, ld, part, s, klist, kstride, kcount, nk);
+      launch_var<V_F5>(L, rows_pad, nbasis_pad, ao, ld, part, s, klist, kstride, kcount, nk);
       break;
     default:
       launch_var<V_F4, MODE_HESS>(L, rows_pad, nbasis_pad, ao, ld, part, s, klist, kstride, kcount, nk);

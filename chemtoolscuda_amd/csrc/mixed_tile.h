@@ -26,6 +26,8 @@
 //   A: A[r][8q + e], B: B[8q + e][r] (e = 0..7), D: D[4q + j][r] (j = 0..3).
 #pragma once
 
+#include <cstddef>
+
 #include "ct_internal.h"
 
 namespace ct {
@@ -101,13 +103,18 @@ struct YTile {
 // for about half the work.  The bound product is treated the same way
 // (|x|^T |P| |x| is a quadratic form too), its doubling applied exactly to
 // the |P_hi| operand (bf16 exponent + 1).
-template <int TC, int NBC, bool F64, bool TRI, bool LIST = false>
+//
+// TC a multiple of 64: bsrc is this thread's column (tid % TC).  Other TC
+// (a multiple of 16): bcol(c) gives column c's row quad 0, and the TC x 4
+// (column, octet) units of a k-step go round-robin to the 256 threads.
+template <int TC, int NBC, bool F64, bool TRI, bool LIST = false, class BCol = std::nullptr_t>
 __device__ __forceinline__ void y_tile(const u4* __restrict__ Pf, int rows, int mp, int64_t ld, int i0, int tid,
                                        const float* __restrict__ bsrc, uint16_t (&Bs)[2][NPL][TC][LROW],
                                        YTile<TC, NBC, F64>& t, int k_begin, const int32_t* klist = nullptr,
-                                       int kcount = 0) {
+                                       int kcount = 0, BCol bcol = nullptr) {
   static_assert(!TRI || F64, "the triangle needs the FP64 running sums");
   static_assert(!(TRI && LIST), "the triangle walks all k-steps from the diagonal");
+  static_assert(TC % 16 == 0, "column blocks of 16");
   constexpr int NCB = TC / 16;
   const int lane = tid & 63;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -127,22 +134,43 @@ __device__ __forceinline__ void y_tile(const u4* __restrict__ Pf, int rows, int 
       for (int rb = 0; rb < NRB; ++rb) areg[pl][rb] = abase[pl * aplane + ((int64_t)rb * nk + kb) * 64];
   };
 
-  // B: TC/64 (column, 8-row octet) units per thread, unit u = tid + 256 u
-  // -> column u % TC, octet u / TC; an octet is two row quads.
-  constexpr int NU = TC / 64;
-  const int bc = tid & (TC - 1), bo = tid / TC;
+  // B: (column, 8-row octet) units, unit u -> column u % TC, octet u / TC
+  // (an octet is two row quads); thread tid takes units tid + 256 i.
+  constexpr bool POW = TC % 64 == 0;
+  constexpr int NU = POW ? TC / 64 : (TC * 4 + 255) / 256;
+  const float* ubase[NU];
+  int ucol[NU], uoct[NU];
+  bool uok[NU];
+#pragma unroll
+  for (int u = 0; u < NU; ++u) {
+    if constexpr (POW) {
+      ucol[u] = tid & (TC - 1);
+      uoct[u] = tid / TC + (256 / TC) * u;
+      ubase[u] = bsrc;
+      uok[u] = true;
+    } else {
+      const int uu = tid + 256 * u;
+      uok[u] = uu < TC * 4;  // per wave: whole waves drop out of the last round
+      const int uc = uok[u] ? uu : tid;
+      ucol[u] = uc % TC;
+      uoct[u] = uc / TC;
+      ubase[u] = bcol(ucol[u]);
+    }
+  }
   f4 rbuf[NU][2];
   auto load_b = [&](int kb) {
 #pragma unroll
     for (int u = 0; u < NU; ++u)
+      if (uok[u]) {
 #pragma unroll
-      for (int h = 0; h < 2; ++h)
-        rbuf[u][h] =
-            *reinterpret_cast<const f4*>(bsrc + (int64_t)(8 * kb + 2 * (bo + (256 / TC) * u) + h) * ld * 4);
+        for (int h = 0; h < 2; ++h)
+          rbuf[u][h] = *reinterpret_cast<const f4*>(ubase[u] + (int64_t)(8 * kb + 2 * uoct[u] + h) * ld * 4);
+      }
   };
   auto store_b = [&](int s) {
 #pragma unroll
     for (int u = 0; u < NU; ++u) {
+      if (!uok[u]) continue;
       u4 vh, vm, vl;
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
@@ -151,10 +179,10 @@ __device__ __forceinline__ void y_tile(const u4* __restrict__ Pf, int rows, int 
         vm[e] = sp.m;
         vl[e] = sp.l;
       }
-      const int k = 8 * (bo + (256 / TC) * u);
-      *reinterpret_cast<u4*>(&Bs[s][0][bc][k]) = vh;
-      *reinterpret_cast<u4*>(&Bs[s][1][bc][k]) = vm;
-      *reinterpret_cast<u4*>(&Bs[s][2][bc][k]) = vl;
+      const int k = 8 * uoct[u];
+      *reinterpret_cast<u4*>(&Bs[s][0][ucol[u]][k]) = vh;
+      *reinterpret_cast<u4*>(&Bs[s][1][ucol[u]][k]) = vm;
+      *reinterpret_cast<u4*>(&Bs[s][2][ucol[u]][k]) = vl;
     }
   };
 
@@ -180,13 +208,15 @@ __device__ __forceinline__ void y_tile(const u4* __restrict__ Pf, int rows, int 
   // twice.
   auto compute = [&](int s, bool dbl) {
 #pragma unroll
-    for (int cp = 0; cp < NCB / 2; ++cp) {
+    for (int cp = 0; cp < (NCB + 1) / 2; ++cp) {
+      // an odd NCB leaves the second block of the last pair empty (c < nc)
+      const int nc = 2 * cp + 1 < NCB ? 2 : 1;
       bf16x8 bf[2][NPL];
 #pragma unroll
       for (int c = 0; c < 2; ++c)
 #pragma unroll
         for (int pl = 0; pl < NPL; ++pl)
-          bf[c][pl] = as_bf(*reinterpret_cast<const u4*>(&Bs[s][pl][16 * (2 * cp + c) + r][8 * q]));
+          if (c < nc) bf[c][pl] = as_bf(*reinterpret_cast<const u4*>(&Bs[s][pl][16 * (2 * cp + c) + r][8 * q]));
       if constexpr (F64) {
         constexpr int pa[6] = {2, 1, 0, 1, 0, 0};
         constexpr int pb[6] = {0, 1, 2, 0, 1, 0};
@@ -201,18 +231,22 @@ __device__ __forceinline__ void y_tile(const u4* __restrict__ Pf, int rows, int 
           for (int rb = 0; rb < NRB; ++rb)
 #pragma unroll
             for (int c = 0; c < 2; ++c)
-              tc[rb][c] =
-                  __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(areg[pa[pr]][rb]), bf[c][pb[pr]], tc[rb][c], 0, 0, 0);
+              if (c < nc)
+                tc[rb][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(areg[pa[pr]][rb]), bf[c][pb[pr]], tc[rb][c],
+                                                                    0, 0, 0);
         const double sc = (TRI && dbl) ? 2.0 : 1.0;
 #pragma unroll
         for (int rb = 0; rb < NRB; ++rb)
 #pragma unroll
           for (int c = 0; c < 2; ++c)
+            if (c < nc)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) t.yd[rb][2 * cp + c][j] = fma((double)tc[rb][c][j], sc, t.yd[rb][2 * cp + c][j]);
+              for (int j = 0; j < 4; ++j)
+                t.yd[rb][(2 * cp + c) % NCB][j] = fma((double)tc[rb][c][j], sc, t.yd[rb][(2 * cp + c) % NCB][j]);
       } else {
         constexpr int pa[5] = {2, 1, 0, 1, 0};
         constexpr int pb[5] = {0, 1, 2, 0, 1};
+        static_assert(F64 || NCB % 2 == 0, "FP32 accumulators: column blocks in pairs");
 #pragma unroll
         for (int pr = 0; pr < 5; ++pr)
 #pragma unroll
@@ -235,9 +269,10 @@ __device__ __forceinline__ void y_tile(const u4* __restrict__ Pf, int rows, int 
             const u4 pabs = (TRI && dbl) ? (areg[0][rb] & absmask) + exp1 : areg[0][rb] & absmask;
 #pragma unroll
             for (int c = 0; c < 2; ++c)
-              t.bound[rb][(2 * cp + c) % NBC] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                  as_bf(pabs), as_bf(__builtin_bit_cast(u4, bf[c][0]) & absmask), t.bound[rb][(2 * cp + c) % NBC], 0,
-                  0, 0);
+              if (2 * cp + c < NBC)
+                t.bound[rb][(2 * cp + c) % NBC] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                    as_bf(pabs), as_bf(__builtin_bit_cast(u4, bf[c][0]) & absmask), t.bound[rb][(2 * cp + c) % NBC],
+                    0, 0, 0);
           }
         }
       }
