@@ -39,13 +39,11 @@ using namespace mixed;
 // sum over those columns (phi^T P x = psi^T (L^T x)):
 //   V_F1:          64 points of phi: rho = psi . psi and the bound;
 //   V_F4<mode>:    32 points x {phi, d_x phi, d_y phi, d_z phi}: rho,
-//                  g_k = psi . d_k psi, the bound, and in the lap mode
-//                  t_kk = d_k psi . d_k psi;
-//   V_F2:          32 points x {phi, lap phi}: psi . lap psi (the lap
-//                  mode's second launch when V_F4<lap> computes the rest);
+//                  g_k = psi . d_k psi and the bound (grad mode), and
+//                  t_kl = d_k psi . d_l psi (Hessian mode);
 //   V_F5:          16 points x {phi, d_x phi, d_y phi, d_z phi, lap phi}: the
-//                  whole lap mode in one launch (rho, g_k, t_kk, psi . lap psi
-//                  and the bound) -- psi computed once, phi read once;
+//                  whole lap mode in one launch (rho, g_k, t_kk = d_k psi .
+//                  d_k psi, psi . lap psi and the bound);
 //   V_FH<part>:    32 points x {phi, d_kl phi for three kl}: the Hessian
 //                  mode's h_kl = psi . d_kl psi (part 0: xx, xy, xz; part 1:
 //                  yy, yz, zz); with V_F4<hess>, whose t_kl = d_k psi . d_l psi
@@ -54,7 +52,7 @@ using namespace mixed;
 // the first-order propagation of the per-column bound of psi into
 // rho = sum_j psi_j^2.
 // Launches of one mode write disjoint dot slots of the same partials array.
-enum Var : int { V_RHO, V_GRAD, V_LAPPHI, V_LAPDER, V_HESS, V_F1, V_F4, V_F2, V_FH, V_F5 };
+enum Var : int { V_RHO, V_GRAD, V_LAPPHI, V_LAPDER, V_HESS, V_F1, V_F4, V_FH, V_F5 };
 
 // DT: the derivative type of a V_LAPDER launch, the mode of a V_F4 launch,
 // the part of a V_FH launch.
@@ -66,11 +64,11 @@ struct Cols {
                               : V == V_HESS || V == V_FH ? MODE_HESS
                               : V == V_F4             ? DT
                                                       : MODE_LAP;
-  static constexpr int NT = V == V_F5 ? 5 : V == V_HESS || V == V_F4 || V == V_FH ? 4 : V == V_F2 ? 2 : 1;  // column types
+  static constexpr int NT = V == V_F5 ? 5 : V == V_HESS || V == V_F4 || V == V_FH ? 4 : 1;  // column types per tile
   static constexpr int TC = V == V_F5 ? 80 : V == V_HESS || V == V_LAPDER || V == V_F4 || V == V_FH ? 128 : 64;
   static constexpr int NCB = TC / 16;                     // 16-column MFMA blocks
   static constexpr int PTS = TC / NT;                     // points per workgroup
-  static constexpr int NBC = V == V_LAPDER || V == V_F2 || V == V_FH ? 0
+  static constexpr int NBC = V == V_LAPDER || V == V_FH ? 0
                              : V == V_F5                           ? 1
                              : V == V_HESS || V == V_F4            ? 2
                                                                    : NCB;  // bound blocks (the phi columns)
@@ -79,8 +77,7 @@ struct Cols {
   // AO array of column c: 0 = phi, 1..3 = d_x, d_y, d_z phi, then 4 = lap phi
   // (lap mode) or 4..9 = d_xx, d_xy, d_xz, d_yy, d_yz, d_zz phi (Hessian mode)
   __host__ __device__ static constexpr int type(int c) {
-    return V == V_F2   ? (c < PTS ? 0 : 4)
-           : V == V_FH ? (c < PTS ? 0 : 4 + 3 * DT + c / PTS - 1)
+    return V == V_FH   ? (c < PTS ? 0 : 4 + 3 * DT + c / PTS - 1)
            : NT >= 4   ? c / PTS
            : V == V_LAPDER ? DT
                            : 0;
@@ -119,8 +116,7 @@ constexpr int kMaxDots = 11;  // 10 Hessian-mode dots of the phi column + the bo
 // Epilogue of the factor route: per point, lane-local FP64 sums over the
 // tile's rows (columns of L) of products of the point's psi columns, then over
 // the four lane groups and the four waves in a fixed order; one partial per
-// (row block, dot, point).  Dots: V_F1 [rho, bound]; V_F4 [rho, gx, gy, gz,
-// (txx, tyy, tzz in the lap mode,) bound]; V_F2 [lap phi . P phi].
+// (row block, dot, point).
 template <int V, int DT, class Tile>
 __device__ __forceinline__ void factor_epilogue(const Tile& yt, uint16_t* lds, double* __restrict__ part,
                                                 int64_t ld, int rt, int64_t pbase, int tid) {
@@ -128,11 +124,11 @@ __device__ __forceinline__ void factor_epilogue(const Tile& yt, uint16_t* lds, d
   constexpr int ND = mode_ndot(C::MODE), ND1 = ND + 1;
   constexpr int NPB = C::PTS / 16;  // 16-point column blocks per column type
   constexpr bool LAP = C::MODE == MODE_LAP, HES = C::MODE == MODE_HESS;
-  // dots per point: V_F1 [rho, bound]; V_F2 [psi.lap psi]; V_FH [psi.d_kl psi
-  // x 3]; V_F4 [rho, g_x, g_y, g_z, (lap: t_xx, t_yy, t_zz | Hessian: t_xx,
-  // t_xy, t_yy, t_xz, t_yz, t_zz,) bound]; V_F5 [rho, g_x, g_y, g_z, t_xx,
-  // t_yy, t_zz, psi.lap psi, bound]
-  constexpr int NDF = V == V_F1 ? 2 : V == V_F2 ? 1 : V == V_FH ? 3 : V == V_F5 ? 9 : (LAP ? 8 : HES ? 11 : 5);
+  // dots per point: V_F1 [rho, bound]; V_FH [psi.d_kl psi x 3]; V_F4 [rho,
+  // g_x, g_y, g_z, (Hessian: t_xx, t_xy, t_yy, t_xz, t_yz, t_zz,) bound];
+  // V_F5 [rho, g_x, g_y, g_z, t_xx, t_yy, t_zz, psi.lap psi, bound]
+  static_assert(V != V_F4 || !LAP, "the lap mode runs as V_F5");
+  constexpr int NDF = V == V_F1 ? 2 : V == V_FH ? 3 : V == V_F5 ? 9 : HES ? 11 : 5;
   const int lane = tid & 63, w = tid >> 6, r = lane & 15, q = lane >> 4;
   double* red = reinterpret_cast<double*>(lds);  // [NW][PTS][NDF]
 #pragma unroll
@@ -145,9 +141,7 @@ __device__ __forceinline__ void factor_epilogue(const Tile& yt, uint16_t* lds, d
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const double y0 = yt.y(rb, pb, j);
-        if constexpr (V == V_F2) {
-          d[0] += y0 * yt.y(rb, NPB + pb, j);
-        } else if constexpr (V == V_FH) {
+        if constexpr (V == V_FH) {
 #pragma unroll
           for (int k = 0; k < 3; ++k) d[k] += y0 * yt.y(rb, (k + 1) * NPB + pb, j);
         } else {
@@ -196,13 +190,12 @@ __device__ __forceinline__ void factor_epilogue(const Tile& yt, uint16_t* lds, d
       // dot slots of the mode (mode_ndot): lap [rho, g, L0, t_xx, t_yy, t_zz],
       // Hessian [rho, g, h_xx..h_zz, t_xx, t_xy, t_yy, t_xz, t_yz, t_zz]; the
       // bound after them
-      const int slot_k = V == V_F2                 ? 4
-                         : V == V_FH               ? 4 + 3 * DT + k
-                         : V == V_F5 && k == 7     ? 4
-                         : k == NDF - 1            ? ND
-                         : k < 4        ? k
-                         : HES          ? k + 6
-                                        : k + 1;
+      const int slot_k = V == V_FH             ? 4 + 3 * DT + k
+                         : V == V_F5 && k == 7 ? 4
+                         : k == NDF - 1        ? ND
+                         : k < 4               ? k
+                         : HES                 ? k + 6
+                                               : k + 1;
       part[((int64_t)rt * ND1 + slot_k) * ld + pbase + tid] = sum;
     }
   }
