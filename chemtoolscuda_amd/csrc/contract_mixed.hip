@@ -113,6 +113,25 @@ using IC = std::integral_constant<int, N>;
 
 constexpr int kMaxDots = 11;  // 10 Hessian-mode dots of the phi column + the bound
 
+// Dots of a factor-route tile per point (see factor_epilogue) and the slot
+// of the k-th in the mode's partials layout (mode_ndot, the bound after the
+// dots).
+template <int V, int DT>
+__host__ __device__ constexpr int factor_ndf() {
+  using C = Cols<V, DT>;
+  return V == V_F1 ? 2 : V == V_FH ? 3 : V == V_F5 ? 9 : C::MODE == MODE_HESS ? 11 : 5;
+}
+template <int V, int DT>
+__host__ __device__ constexpr int factor_slot(int k) {
+  using C = Cols<V, DT>;
+  return V == V_FH                        ? 4 + 3 * DT + k
+         : V == V_F5 && k == 7            ? 4
+         : k == factor_ndf<V, DT>() - 1   ? mode_ndot(C::MODE)
+         : k < 4                          ? k
+         : C::MODE == MODE_HESS           ? k + 6
+                                          : k + 1;
+}
+
 // Epilogue of the factor route: per point, lane-local FP64 sums over the
 // tile's rows (columns of L) of products of the point's psi columns, then over
 // the four lane groups and the four waves in a fixed order; one partial per
@@ -128,7 +147,7 @@ __device__ __forceinline__ void factor_epilogue(const Tile& yt, uint16_t* lds, d
   // g_x, g_y, g_z, (Hessian: t_xx, t_xy, t_yy, t_xz, t_yz, t_zz,) bound];
   // V_F5 [rho, g_x, g_y, g_z, t_xx, t_yy, t_zz, psi.lap psi, bound]
   static_assert(V != V_F4 || !LAP, "the lap mode runs as V_F5");
-  constexpr int NDF = V == V_F1 ? 2 : V == V_FH ? 3 : V == V_F5 ? 9 : HES ? 11 : 5;
+  constexpr int NDF = factor_ndf<V, DT>();
   const int lane = tid & 63, w = tid >> 6, r = lane & 15, q = lane >> 4;
   double* red = reinterpret_cast<double*>(lds);  // [NW][PTS][NDF]
 #pragma unroll
@@ -190,13 +209,7 @@ __device__ __forceinline__ void factor_epilogue(const Tile& yt, uint16_t* lds, d
       // dot slots of the mode (mode_ndot): lap [rho, g, L0, t_xx, t_yy, t_zz],
       // Hessian [rho, g, h_xx..h_zz, t_xx, t_xy, t_yy, t_xz, t_yz, t_zz]; the
       // bound after them
-      const int slot_k = V == V_FH             ? 4 + 3 * DT + k
-                         : V == V_F5 && k == 7 ? 4
-                         : k == NDF - 1        ? ND
-                         : k < 4               ? k
-                         : HES                 ? k + 6
-                                               : k + 1;
-      part[((int64_t)rt * ND1 + slot_k) * ld + pbase + tid] = sum;
+      part[((int64_t)rt * ND1 + factor_slot<V, DT>(k)) * ld + pbase + tid] = sum;
     }
   }
 }

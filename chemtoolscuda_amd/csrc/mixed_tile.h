@@ -90,6 +90,132 @@ struct YTile {
   }
 };
 
+// One k-step of the tile engine: the six split products of A (areg) with
+// the LDS buffer s, column blocks in pairs (2 x NRB independent accumulators
+// per product, so dependent MFMAs are four issues apart), added to the
+// accumulators of t, and the bound product on the first NBC column blocks.
+// dbl (TRI only): add the products twice.
+template <int TC, int NBC, bool F64, bool TRI>
+__device__ __forceinline__ void tile_compute(const uint16_t (&Bs)[2][NPL][TC][LROW], int s, bool dbl,
+                                             const u4 (&areg)[NPL][NRB], YTile<TC, NBC, F64>& t, int r, int q) {
+  constexpr int NCB = TC / 16;
+  const u4 absmask = u4{0x7FFF7FFFu, 0x7FFF7FFFu, 0x7FFF7FFFu, 0x7FFF7FFFu};
+  const u4 exp1 = u4{0x00800080u, 0x00800080u, 0x00800080u, 0x00800080u};  // +1 in both bf16 exponents
+#pragma unroll
+  for (int cp = 0; cp < (NCB + 1) / 2; ++cp) {
+    // an odd NCB leaves the second block of the last pair empty (c < nc)
+    const int nc = 2 * cp + 1 < NCB ? 2 : 1;
+    bf16x8 bf[2][NPL];
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int pl = 0; pl < NPL; ++pl)
+        if (c < nc) bf[c][pl] = as_bf(*reinterpret_cast<const u4*>(&Bs[s][pl][16 * (2 * cp + c) + r][8 * q]));
+    if constexpr (F64) {
+      constexpr int pa[6] = {2, 1, 0, 1, 0, 0};
+      constexpr int pb[6] = {0, 1, 2, 0, 1, 0};
+      f4 tc[NRB][2];
+#pragma unroll
+      for (int rb = 0; rb < NRB; ++rb)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) tc[rb][c] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int pr = 0; pr < 6; ++pr)
+#pragma unroll
+        for (int rb = 0; rb < NRB; ++rb)
+#pragma unroll
+          for (int c = 0; c < 2; ++c)
+            if (c < nc)
+              tc[rb][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(areg[pa[pr]][rb]), bf[c][pb[pr]], tc[rb][c],
+                                                                  0, 0, 0);
+      const double sc = (TRI && dbl) ? 2.0 : 1.0;
+#pragma unroll
+      for (int rb = 0; rb < NRB; ++rb)
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+          if (c < nc)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+              t.yd[rb][(2 * cp + c) % NCB][j] = fma((double)tc[rb][c][j], sc, t.yd[rb][(2 * cp + c) % NCB][j]);
+    } else {
+      constexpr int pa[5] = {2, 1, 0, 1, 0};
+      constexpr int pb[5] = {0, 1, 2, 0, 1};
+      static_assert(F64 || NCB % 2 == 0, "FP32 accumulators: column blocks in pairs");
+#pragma unroll
+      for (int pr = 0; pr < 5; ++pr)
+#pragma unroll
+        for (int rb = 0; rb < NRB; ++rb)
+#pragma unroll
+          for (int c = 0; c < 2; ++c)
+            t.sm[rb][2 * cp + c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                as_bf(areg[pa[pr]][rb]), bf[c][pb[pr]], t.sm[rb][2 * cp + c], 0, 0, 0);
+#pragma unroll
+      for (int rb = 0; rb < NRB; ++rb)
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+          t.hh[rb][2 * cp + c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(areg[0][rb]), bf[c][0],
+                                                                          t.hh[rb][2 * cp + c], 0, 0, 0);
+    }
+    if constexpr (NBC > 0) {
+      if (2 * cp < NBC) {
+#pragma unroll
+        for (int rb = 0; rb < NRB; ++rb) {
+          const u4 pabs = (TRI && dbl) ? (areg[0][rb] & absmask) + exp1 : areg[0][rb] & absmask;
+#pragma unroll
+          for (int c = 0; c < 2; ++c)
+            if (2 * cp + c < NBC)
+              t.bound[rb][(2 * cp + c) % NBC] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                  as_bf(pabs), as_bf(__builtin_bit_cast(u4, bf[c][0]) & absmask), t.bound[rb][(2 * cp + c) % NBC],
+                  0, 0, 0);
+        }
+      }
+    }
+  }
+}
+
+// Zeroes the accumulators of a tile.
+template <int TC, int NBC, bool F64>
+__device__ __forceinline__ void tile_zero(YTile<TC, NBC, F64>& t) {
+  constexpr int NCB = TC / 16;
+#pragma unroll
+  for (int a = 0; a < NRB; ++a) {
+#pragma unroll
+    for (int b = 0; b < NCB; ++b) {
+      if constexpr (F64) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) t.yd[a][b][j] = 0.0;
+      } else {
+        t.hh[a][b] = t.sm[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+      }
+    }
+#pragma unroll
+    for (int b = 0; b < (NBC > 0 ? NBC : 1); ++b) t.bound[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+  }
+}
+
+// Splits this thread's B units (FP32 row-quad pairs in rbuf) into bf16
+// pieces and stores them into LDS buffer s.
+template <int TC, int NU>
+__device__ __forceinline__ void tile_store_b(uint16_t (&Bs)[2][NPL][TC][LROW], int s, const f4 (&rbuf)[NU][2],
+                                             const int (&ucol)[NU], const int (&uoct)[NU], const bool (&uok)[NU]) {
+#pragma unroll
+  for (int u = 0; u < NU; ++u) {
+    if (!uok[u]) continue;
+    u4 vh, vm, vl;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const Split2 sp = split2(rbuf[u][e >> 1][2 * (e & 1)], rbuf[u][e >> 1][2 * (e & 1) + 1]);
+      vh[e] = sp.h;
+      vm[e] = sp.m;
+      vl[e] = sp.l;
+    }
+    const int k = 8 * uoct[u];
+    *reinterpret_cast<u4*>(&Bs[s][0][ucol[u]][k]) = vh;
+    *reinterpret_cast<u4*>(&Bs[s][1][ucol[u]][k]) = vm;
+    *reinterpret_cast<u4*>(&Bs[s][2][ucol[u]][k]) = vl;
+  }
+}
+
 // Computes the tile.  bsrc: this thread's B column at row quad 0, i.e. the
 // column tid % TC; its row quad k lies at bsrc + k * ld * 4.  Bs: the
 // workgroup's LDS.  Ends with a barrier (Bs free for reuse).
@@ -167,117 +293,11 @@ __device__ __forceinline__ void y_tile(const u4* __restrict__ Pf, int rows, int 
           rbuf[u][h] = *reinterpret_cast<const f4*>(ubase[u] + (int64_t)(8 * kb + 2 * uoct[u] + h) * ld * 4);
       }
   };
-  auto store_b = [&](int s) {
-#pragma unroll
-    for (int u = 0; u < NU; ++u) {
-      if (!uok[u]) continue;
-      u4 vh, vm, vl;
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const Split2 sp = split2(rbuf[u][e >> 1][2 * (e & 1)], rbuf[u][e >> 1][2 * (e & 1) + 1]);
-        vh[e] = sp.h;
-        vm[e] = sp.m;
-        vl[e] = sp.l;
-      }
-      const int k = 8 * uoct[u];
-      *reinterpret_cast<u4*>(&Bs[s][0][ucol[u]][k]) = vh;
-      *reinterpret_cast<u4*>(&Bs[s][1][ucol[u]][k]) = vm;
-      *reinterpret_cast<u4*>(&Bs[s][2][ucol[u]][k]) = vl;
-    }
-  };
+  auto store_b = [&](int s) { tile_store_b<TC, NU>(Bs, s, rbuf, ucol, uoct, uok); };
 
-#pragma unroll
-  for (int a = 0; a < NRB; ++a) {
-#pragma unroll
-    for (int b = 0; b < NCB; ++b) {
-      if constexpr (F64) {
-#pragma unroll
-        for (int j = 0; j < 4; ++j) t.yd[a][b][j] = 0.0;
-      } else {
-        t.hh[a][b] = t.sm[a][b] = f4{0.f, 0.f, 0.f, 0.f};
-      }
-    }
-#pragma unroll
-    for (int b = 0; b < (NBC > 0 ? NBC : 1); ++b) t.bound[a][b] = f4{0.f, 0.f, 0.f, 0.f};
-  }
+  tile_zero(t);
 
-  const u4 absmask = u4{0x7FFF7FFFu, 0x7FFF7FFFu, 0x7FFF7FFFu, 0x7FFF7FFFu};
-  const u4 exp1 = u4{0x00800080u, 0x00800080u, 0x00800080u, 0x00800080u};  // +1 in both bf16 exponents
-  // Column blocks in pairs: 2 x NRB independent accumulators per product, so
-  // dependent MFMAs are four issues apart.  dbl (TRI only): add the products
-  // twice.
-  auto compute = [&](int s, bool dbl) {
-#pragma unroll
-    for (int cp = 0; cp < (NCB + 1) / 2; ++cp) {
-      // an odd NCB leaves the second block of the last pair empty (c < nc)
-      const int nc = 2 * cp + 1 < NCB ? 2 : 1;
-      bf16x8 bf[2][NPL];
-#pragma unroll
-      for (int c = 0; c < 2; ++c)
-#pragma unroll
-        for (int pl = 0; pl < NPL; ++pl)
-          if (c < nc) bf[c][pl] = as_bf(*reinterpret_cast<const u4*>(&Bs[s][pl][16 * (2 * cp + c) + r][8 * q]));
-      if constexpr (F64) {
-        constexpr int pa[6] = {2, 1, 0, 1, 0, 0};
-        constexpr int pb[6] = {0, 1, 2, 0, 1, 0};
-        f4 tc[NRB][2];
-#pragma unroll
-        for (int rb = 0; rb < NRB; ++rb)
-#pragma unroll
-          for (int c = 0; c < 2; ++c) tc[rb][c] = f4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int pr = 0; pr < 6; ++pr)
-#pragma unroll
-          for (int rb = 0; rb < NRB; ++rb)
-#pragma unroll
-            for (int c = 0; c < 2; ++c)
-              if (c < nc)
-                tc[rb][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(areg[pa[pr]][rb]), bf[c][pb[pr]], tc[rb][c],
-                                                                    0, 0, 0);
-        const double sc = (TRI && dbl) ? 2.0 : 1.0;
-#pragma unroll
-        for (int rb = 0; rb < NRB; ++rb)
-#pragma unroll
-          for (int c = 0; c < 2; ++c)
-            if (c < nc)
-#pragma unroll
-              for (int j = 0; j < 4; ++j)
-                t.yd[rb][(2 * cp + c) % NCB][j] = fma((double)tc[rb][c][j], sc, t.yd[rb][(2 * cp + c) % NCB][j]);
-      } else {
-        constexpr int pa[5] = {2, 1, 0, 1, 0};
-        constexpr int pb[5] = {0, 1, 2, 0, 1};
-        static_assert(F64 || NCB % 2 == 0, "FP32 accumulators: column blocks in pairs");
-#pragma unroll
-        for (int pr = 0; pr < 5; ++pr)
-#pragma unroll
-          for (int rb = 0; rb < NRB; ++rb)
-#pragma unroll
-            for (int c = 0; c < 2; ++c)
-              t.sm[rb][2 * cp + c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                  as_bf(areg[pa[pr]][rb]), bf[c][pb[pr]], t.sm[rb][2 * cp + c], 0, 0, 0);
-#pragma unroll
-        for (int rb = 0; rb < NRB; ++rb)
-#pragma unroll
-          for (int c = 0; c < 2; ++c)
-            t.hh[rb][2 * cp + c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(areg[0][rb]), bf[c][0],
-                                                                            t.hh[rb][2 * cp + c], 0, 0, 0);
-      }
-      if constexpr (NBC > 0) {
-        if (2 * cp < NBC) {
-#pragma unroll
-          for (int rb = 0; rb < NRB; ++rb) {
-            const u4 pabs = (TRI && dbl) ? (areg[0][rb] & absmask) + exp1 : areg[0][rb] & absmask;
-#pragma unroll
-            for (int c = 0; c < 2; ++c)
-              if (2 * cp + c < NBC)
-                t.bound[rb][(2 * cp + c) % NBC] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                    as_bf(pabs), as_bf(__builtin_bit_cast(u4, bf[c][0]) & absmask), t.bound[rb][(2 * cp + c) % NBC],
-                    0, 0, 0);
-          }
-        }
-      }
-    }
-  };
+  auto compute = [&](int s, bool dbl) { tile_compute<TC, NBC, F64, TRI>(Bs, s, dbl, areg, t, r, q); };
 
   // The loads run unconditionally, clamped to the last k-step on the final
   // iterations (those values are unused; the LDS buffer written then is the
@@ -316,6 +336,7 @@ __device__ __forceinline__ void y_tile(const u4* __restrict__ Pf, int rows, int 
     knext2 = k3;
   }
 }
+
 
 }  // namespace mixed
 }  // namespace ct
