@@ -122,7 +122,10 @@ class Handle:
 
     def __del__(self):
         if getattr(self, "ptr", None):
-            lib().ct_mol_destroy(self.ptr)
+            try:
+                lib().ct_mol_destroy(self.ptr)
+            except TypeError:  # interpreter shutdown: module globals already cleared
+                pass
             self.ptr = None
 
     def eval(self, points_ptr: int, n: int, outputs: dict, precision: str = "fp64", params: dict | None = None):

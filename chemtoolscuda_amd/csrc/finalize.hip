@@ -103,14 +103,24 @@ __global__ __launch_bounds__(256) void finalize_mixed_kernel(FinalizeArgs a, Fla
   if (live) emit_outputs(a, a.out_off + pp, dot);
   const double arho = fabs(dot[0]);
   double rel = 0.0;  // estimate / |rho| of an accepted point (non-negative: the bits order like the values)
+  bool flag = false;
   if (live && !(arho < f.rho_floor)) {
     const double est = f.gamma * bound + f.trunc * sqrt(arho);
     if (!(est <= f.tol * arho)) {  // also catches NaN
-      const int k = atomicAdd(f.count, 1);
-      f.index[k] = (int32_t)(f.index_base + pp);
+      flag = true;
     } else {
       rel = est / arho;
     }
+  }
+  // flagged points: one atomic per wave for its count, each lane at its rank
+  const uint64_t fm = __ballot(flag);
+  if (fm) {
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((unsigned long long)fm) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(f.count, __popcll(fm));
+    base = __shfl(base, leader);
+    if (flag) f.index[base + __popcll(fm & ((1ull << lane) - 1ull))] = (int32_t)(f.index_base + pp);
   }
   // one atomic per wave: the same address for every point would serialise
   for (int off = 32; off >= 1; off >>= 1) rel = fmax(rel, __shfl_xor(rel, off));

@@ -3,7 +3,7 @@
 //   1. points are put in Morton order (24-bit keys on a lattice around the
 //      molecule; stable radix sort of (key, index) pairs), so that a point
 //      tile of the contraction is a compact box in space;
-//   2. one wave per point tile takes the tile's bounding box and marks the
+//   2. per point tile (64 points per wave) the tile's bounding box marks the
 //      32-function k-steps holding a shell within reach of the box (distance
 //      below the shell's cutoff radius); the others are skipped by the AO
 //      kernel (no evaluation, no stores) and by the contraction (no loads, no
@@ -44,7 +44,9 @@ __global__ __launch_bounds__(256) void morton_keys_kernel(const double* __restri
   idx[p] = (int32_t)p;
 }
 
-// One wave per tile of `tp` points (tp <= 64), four tiles per workgroup.
+// One wave per 64 points = 64 / tp tiles of tp points (tp in 16, 32, 64):
+// the tiles' bounding boxes by reductions within groups of tp lanes, then
+// per tile the active k-steps, one lane per k-step.
 __global__ __launch_bounds__(256) void tile_activity_kernel(const double* __restrict__ pts, int64_t n, int tp,
                                                             int64_t ntile, const int32_t* __restrict__ span_off,
                                                             const ScreenSpan* __restrict__ spans, int nk,
@@ -52,10 +54,11 @@ __global__ __launch_bounds__(256) void tile_activity_kernel(const double* __rest
                                                             uint32_t* __restrict__ kmask, int nwords,
                                                             unsigned long long* __restrict__ active_count) {
   const int lane = threadIdx.x & 63;
-  const int64_t tile = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (tile >= ntile) return;
-  const int64_t p = tile * tp + lane;
-  const bool valid = lane < tp && p < n;
+  const int tpw = 64 / tp;
+  const int64_t t0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * tpw;  // the wave's first tile
+  if (t0 >= ntile) return;
+  const int64_t p = t0 * tp + lane;
+  const bool valid = t0 + lane / tp < ntile && p < n;
   double lo[3], hi[3];
   bool finite = true;
 #pragma unroll
@@ -67,43 +70,54 @@ __global__ __launch_bounds__(256) void tile_activity_kernel(const double* __rest
   }
   // a point with a non-finite coordinate has no place in space: its tile
   // keeps every k-step (and its NaN results)
-  const bool keep_all = __any(!finite);
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1)
+  const uint64_t nonfinite = __ballot(!finite);
+  for (int off = tp >> 1; off >= 1; off >>= 1)
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       lo[k] = fmin(lo[k], __shfl_xor(lo[k], off));
       hi[k] = fmax(hi[k], __shfl_xor(hi[k], off));
     }
-  const bool any = lo[0] <= hi[0];  // the tile holds at least one point
-  int total = 0;
-  for (int kb0 = 0; kb0 < nk; kb0 += 64) {
-    const int kb = kb0 + lane;
-    bool act = keep_all && kb < nk && span_off[kb + 1] > span_off[kb];
-    if (any && !keep_all && kb < nk) {
-      for (int e = span_off[kb]; e < span_off[kb + 1] && !act; ++e) {
-        const ScreenSpan sp = spans[e];
-        const double dx = fmax(0.0, fmax(lo[0] - sp.cx, sp.cx - hi[0]));
-        const double dy = fmax(0.0, fmax(lo[1] - sp.cy, sp.cy - hi[1]));
-        const double dz = fmax(0.0, fmax(lo[2] - sp.cz, sp.cz - hi[2]));
-        act = dx * dx + dy * dy + dz * dz < sp.r2;
+  const uint64_t gmask = tp == 64 ? ~0ull : ((1ull << tp) - 1ull);
+  for (int g = 0; g < tpw; ++g) {
+    const int64_t tile = t0 + g;
+    if (tile >= ntile) break;
+    double tlo[3], thi[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      tlo[k] = __shfl(lo[k], g * tp);
+      thi[k] = __shfl(hi[k], g * tp);
+    }
+    const bool keep_all = ((nonfinite >> (g * tp)) & gmask) != 0;
+    const bool any = tlo[0] <= thi[0];  // the tile holds at least one point
+    int total = 0;
+    for (int kb0 = 0; kb0 < nk; kb0 += 64) {
+      const int kb = kb0 + lane;
+      bool act = keep_all && kb < nk && span_off[kb + 1] > span_off[kb];
+      if (any && !keep_all && kb < nk) {
+        for (int e = span_off[kb]; e < span_off[kb + 1] && !act; ++e) {
+          const ScreenSpan sp = spans[e];
+          const double dx = fmax(0.0, fmax(tlo[0] - sp.cx, sp.cx - thi[0]));
+          const double dy = fmax(0.0, fmax(tlo[1] - sp.cy, sp.cy - thi[1]));
+          const double dz = fmax(0.0, fmax(tlo[2] - sp.cz, sp.cz - thi[2]));
+          act = dx * dx + dy * dy + dz * dz < sp.r2;
+        }
       }
+      const uint64_t bal = __ballot(act);
+      const int before = __popcll(bal & ((1ull << lane) - 1ull));
+      if (act) klist[tile * nk + total + before] = kb;
+      if (lane == 0) {
+        const int w = kb0 >> 5;
+        kmask[tile * nwords + w] = (uint32_t)bal;
+        if (w + 1 < nwords) kmask[tile * nwords + w + 1] = (uint32_t)(bal >> 32);
+      }
+      total += __popcll(bal);
     }
-    const uint64_t bal = __ballot(act);
-    const int before = __popcll(bal & ((1ull << lane) - 1ull));
-    if (act) klist[tile * nk + total + before] = kb;
     if (lane == 0) {
-      const int w = kb0 >> 5;
-      kmask[tile * nwords + w] = (uint32_t)bal;
-      if (w + 1 < nwords) kmask[tile * nwords + w + 1] = (uint32_t)(bal >> 32);
+      kcount[tile] = total;
+      // kScreenCounters counters, one per 128-byte line: one address for all
+      // tiles would serialise the atomics
+      atomicAdd(active_count + (tile % kScreenCounters) * 16, (unsigned long long)total);
     }
-    total += __popcll(bal);
-  }
-  if (lane == 0) {
-    kcount[tile] = total;
-    // kScreenCounters counters, one per 128-byte line: one address for all
-    // tiles would serialise the atomics
-    atomicAdd(active_count + (tile % kScreenCounters) * 16, (unsigned long long)total);
   }
 }
 
@@ -142,10 +156,11 @@ void launch_sort_indices(const int32_t* in, int32_t* out, int64_t n, void* temp,
 void launch_tile_activity(const double* sorted_pts, int64_t n, int tile_points, int64_t ntile, const int32_t* span_off,
                           const ScreenSpan* spans, int nk, int32_t* klist, int32_t* kcount, uint32_t* kmask,
                           int nwords, unsigned long long* active_count, hipStream_t s) {
-  CT_REQUIRE(tile_points > 0 && tile_points <= 64, "tile of more than 64 points");
+  CT_REQUIRE(tile_points == 16 || tile_points == 32 || tile_points == 64, "tiles of 16, 32 or 64 points");
   CT_REQUIRE(nwords == (nk + 31) / 32, "mask width does not match the number of k-steps");
   if (ntile <= 0) return;
-  hipLaunchKernelGGL(tile_activity_kernel, dim3((unsigned)((ntile + 3) / 4)), dim3(256), 0, s, sorted_pts, n,
+  const int64_t waves = (ntile + 64 / tile_points - 1) / (64 / tile_points);
+  hipLaunchKernelGGL(tile_activity_kernel, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, sorted_pts, n,
                      tile_points, ntile, span_off, spans, nk, klist, kcount, kmask, nwords, active_count);
   CT_HIP_CHECK(hipGetLastError());
 }

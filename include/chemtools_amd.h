@@ -105,7 +105,7 @@ typedef struct {
  * 2 psi^T grad psi, ...): rank(P) columns instead of nbasis, available when P
  * is positive semidefinite of rank at most ~half the basis (SCF densities).
  * AUTO takes FACTOR when it is available and needs fewer matrix-core
- * products (not for the Hessian).  Both routes meet the same mixed_tol through
+ * products (all modes).  Both routes meet the same mixed_tol through
  * an FP64 re-evaluation of flagged points (on the route that flagged them).
  * The FP64 path contracts with P unless FACTOR is asked for. */
 #define CT_ROUTE_AUTO 0
