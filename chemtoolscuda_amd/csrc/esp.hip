@@ -29,23 +29,26 @@ __device__ __forceinline__ void boys(double T, const double* __restrict__ tab, d
     const int g = (int)(T * (1.0 / kBoysStep) + 0.5);
     const double dT = T - g * kBoysStep;
     const double* row = tab + g * kBoysM + L;
+    // the divisions by small integers are multiplications by their
+    // reciprocals (constants after unrolling): no FP64 divide sequences
     double f = 0.0, term = 1.0;
 #pragma unroll
     for (int k = 0; k <= kBoysK; ++k) {
       f += row[k] * term;
-      term *= -dT / (k + 1);
+      term *= -dT * (1.0 / (k + 1));
     }
     F[L] = f;
     if constexpr (L > 0) {
       const double e = exp(-T);
 #pragma unroll
-      for (int n = L - 1; n >= 0; --n) F[n] = (2.0 * T * F[n + 1] + e) / (2 * n + 1);
+      for (int n = L - 1; n >= 0; --n) F[n] = (2.0 * T * F[n + 1] + e) * (1.0 / (2 * n + 1));
     }
   } else {
-    F[0] = 0.5 * sqrt(M_PI / T);
+    const double rT = 1.0 / T;
+    F[0] = 0.5 * sqrt(M_PI * rT);
     if constexpr (L > 0) {
       const double e = exp(-T);
-      const double inv2T = 0.5 / T;
+      const double inv2T = 0.5 * rT;
 #pragma unroll
       for (int n = 0; n < L; ++n) F[n + 1] = ((2 * n + 1) * F[n] - e) * inv2T;
     }
