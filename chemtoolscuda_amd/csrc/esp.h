@@ -35,6 +35,14 @@ constexpr double kBoysStep = 0.125;
 constexpr double kBoysTmax = 40.0;
 constexpr int kBoysNG = 321;
 
+// FP32 table of the mixed kernel, same grid: row g holds F_0..F_{kBoysM32-1}(T_g)
+// and e^{-T_g} at column kBoysM32; rows kBoysRow32 floats apart (odd, so that
+// lanes at different grid points read different LDS banks).  Six Taylor terms
+// (k = 0..5, |dT| <= 1/16) leave a truncation of (1/16)^6 / 6! < 1e-10 of F_L.
+constexpr int kBoysK32 = 5;
+constexpr int kBoysM32 = kEspLmax + kBoysK32 + 1;
+constexpr int kBoysRow32 = kBoysM32 + 1;
+
 // Number of g coefficients of a pair of total angular momentum L; the device
 // and the host enumerate them in the same nested order:
 //   for N = 0..L, for tot = max(0, 2N-L)..N, for i = tot..0, for j = tot-i..0, k = tot-i-j
@@ -62,15 +70,18 @@ struct EspData {
 EspData build_esp_pairs(const HostBasis& basis, const std::vector<double>& xyz, const std::vector<double>& P,
                         double screen);
 std::vector<double> build_boys_table();
+std::vector<float> build_boys_table_f32();
 
 // FP32 copy of the g coefficients (esp_ncoef(L) floats per pair, segment by
 // segment) for the mixed-precision kernel; fills EspSegment::off32.
 std::vector<float> esp_coefficients_f32(const std::vector<double>& records, std::vector<EspSegment>& segs);
 
-// records32 == nullptr: all-FP64 kernel.  Otherwise the mixed kernel: the
-// pair geometry (Q - C, T = p |Q - C|^2), the Boys function and the sums over
-// pairs in FP64, the Hermite polynomial sum_ijk g X^i Y^j Z^k in FP32.
+// records32 == nullptr: all-FP64 kernel (boys_tab).  Otherwise the mixed
+// kernel (boys_tab32): the pair geometry Q - C, T = p |Q - C|^2 and the sum
+// over pairs in FP64; the Boys function, the Hermite polynomial
+// sum_ijk g X^i Y^j Z^k and the pair's sum over N in FP32.
 void launch_esp(const double* pts, int64_t n, const double* records, const float* records32, const EspSegment* segs,
-                int nseg, const double* atoms, int natom, const double* boys_tab, double* out, hipStream_t s);
+                int nseg, const double* atoms, int natom, const double* boys_tab, const float* boys_tab32, double* out,
+                hipStream_t s);
 
 }  // namespace ct

@@ -14,8 +14,13 @@
 //
 // ESP(C) = sum_A Z_A / |C - R_A| - sum_pairs S_pair(C)
 // (reference src/evaluate_electrostatic.cu:10356-10371 for the nuclear term).
-// Two instantiations: all-FP64, and mixed (Boys function, pair geometry and
-// accumulation in FP64, Hermite polynomial in FP32).
+// Two instantiations: all-FP64, and mixed (pair geometry, T and the sum over
+// pairs in FP64; Boys function, Hermite polynomial and the pair's sum over N
+// in FP32).  The mixed Boys function uses an FP32 table that also holds
+// e^{-T_g}, six Taylor terms (truncation < 1e-10 of F_L at |dT| <= 1/16),
+// e^{-T} = e^{-T_g} e^{-dT} with the hardware exponential (no argument
+// reduction error at large T), and the hardware reciprocal square root
+// for T >= 40: F_N to a few FP32 ulps, no FP64 exp, divide or square root.
 
 #include "esp.h"
 
@@ -55,16 +60,48 @@ __device__ __forceinline__ void boys(double T, const double* __restrict__ tab, d
   }
 }
 
-// One primitive pair: F_0..F_L(T) in FP64, then sum_N F_N poly_N with the
-// polynomial in TP (double, or float for the mixed kernel: X, Y, Z and the
-// coefficients rounded to FP32 once, F_N * poly_N and the sums over pairs in
-// FP64).
+// FP32 Boys function of the mixed kernel; T arrives in FP64 so that the grid
+// point and dT are exact before rounding.
+template <int L>
+__device__ __forceinline__ void boys(double Td, const float* __restrict__ tab, float (&F)[L + 1]) {
+  const float T = (float)Td;
+  if (Td < kBoysTmax) {
+    const int g = (int)(Td * (1.0 / kBoysStep) + 0.5);
+    const float x = (float)(g * kBoysStep - Td);  // -dT
+    const float* row = tab + g * kBoysRow32;
+    // Horner form of sum_k F_{L+k}(T_g) x^k / k!
+    float f = row[L + kBoysK32];
+#pragma unroll
+    for (int k = kBoysK32 - 1; k >= 0; --k) f = row[L + k] + (x * (1.0f / (k + 1))) * f;
+    F[L] = f;
+    if constexpr (L > 0) {
+      const float e = row[kBoysM32] * __expf(x);
+      const float t2 = 2.0f * T;
+#pragma unroll
+      for (int n = L - 1; n >= 0; --n) F[n] = (t2 * F[n + 1] + e) * (1.0f / (2 * n + 1));
+    }
+  } else {
+    const float rs = __builtin_amdgcn_rsqf(T);
+    F[0] = 0.88622692545275801f * rs;  // sqrt(pi) / 2
+    if constexpr (L > 0) {
+      const float e = __expf(-T);
+      const float inv2T = 0.5f * rs * rs;
+#pragma unroll
+      for (int n = 0; n < L; ++n) F[n + 1] = ((2 * n + 1) * F[n] - e) * inv2T;
+    }
+  }
+}
+
+// One primitive pair: F_0..F_L(T), then sum_N F_N poly_N.  TP = double: all
+// in FP64.  TP = float (mixed kernel): Q - C and T in FP64, then X, Y, Z and
+// the coefficients rounded to FP32 once, the Boys function, polynomial and
+// sum over N in FP32; the sums over pairs stay FP64.
 template <int L, typename TP>
 __device__ __forceinline__ double pair_value(const double* __restrict__ rec, const TP* __restrict__ c, double cx,
-                                             double cy, double cz, const double* __restrict__ tab) {
+                                             double cy, double cz, const TP* __restrict__ tab) {
   const double X = rec[0] - cx, Y = rec[1] - cy, Z = rec[2] - cz, p = rec[3];
   const double T = p * (X * X + Y * Y + Z * Z);
-  double F[L + 1];
+  TP F[L + 1];
   boys<L>(T, tab, F);
   TP xp[L + 1], yp[L + 1], zp[L + 1];
   xp[0] = yp[0] = zp[0] = TP(1);
@@ -74,7 +111,7 @@ __device__ __forceinline__ double pair_value(const double* __restrict__ rec, con
     yp[i] = yp[i - 1] * (TP)Y;
     zp[i] = zp[i - 1] * (TP)Z;
   }
-  double s = 0.0;
+  TP s = TP(0);
   int idx = 0;
 #pragma unroll
   for (int N = 0; N <= L; ++N) {
@@ -85,14 +122,14 @@ __device__ __forceinline__ double pair_value(const double* __restrict__ rec, con
       for (int i = tot; i >= 0; --i)
 #pragma unroll
         for (int j = tot - i; j >= 0; --j) poly += c[idx++] * (xp[i] * yp[j] * zp[tot - i - j]);
-    s += F[N] * (double)poly;
+    s += F[N] * poly;
   }
-  return s;
+  return (double)s;
 }
 
 template <int L, typename TP>
 __device__ __forceinline__ double segment_sum(const double* __restrict__ rec, const TP* __restrict__ coef, int count,
-                                              double cx, double cy, double cz, const double* __restrict__ tab) {
+                                              double cx, double cy, double cz, const TP* __restrict__ tab) {
   constexpr int nc = esp_ncoef(L);
   constexpr int stride = 4 + nc;
   double s = 0.0;
@@ -114,9 +151,16 @@ __global__ __launch_bounds__(256) void esp_kernel(const double* __restrict__ pts
                                                   const float* __restrict__ records32,
                                                   const EspSegment* __restrict__ segs, int nseg,
                                                   const double* __restrict__ atoms, int natom,
-                                                  const double* __restrict__ boys_tab, double* __restrict__ out) {
-  __shared__ double tab[kBoysNG * kBoysM];
-  for (int k = threadIdx.x; k < kBoysNG * kBoysM; k += blockDim.x) tab[k] = boys_tab[k];
+                                                  const double* __restrict__ boys_tab,
+                                                  const float* __restrict__ boys_tab32, double* __restrict__ out) {
+  constexpr int ntab = sizeof(TP) == 8 ? kBoysNG * kBoysM : kBoysNG * kBoysRow32;
+  __shared__ TP tab[ntab];
+  const TP* src;
+  if constexpr (sizeof(TP) == 8)
+    src = boys_tab;
+  else
+    src = boys_tab32;
+  for (int k = threadIdx.x; k < ntab; k += blockDim.x) tab[k] = src[k];
   __syncthreads();
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t pc = p < n ? p : n - 1;
@@ -149,15 +193,16 @@ __global__ __launch_bounds__(256) void esp_kernel(const double* __restrict__ pts
 }  // namespace
 
 void launch_esp(const double* pts, int64_t n, const double* records, const float* records32, const EspSegment* segs,
-                int nseg, const double* atoms, int natom, const double* boys_tab, double* out, hipStream_t s) {
+                int nseg, const double* atoms, int natom, const double* boys_tab, const float* boys_tab32, double* out,
+                hipStream_t s) {
   if (n <= 0) return;
   const dim3 grid((unsigned)((n + 255) / 256)), block(256);
   if (records32 == nullptr)
     hipLaunchKernelGGL(esp_kernel<double>, grid, block, 0, s, pts, n, records, records32, segs, nseg, atoms, natom,
-                       boys_tab, out);
+                       boys_tab, boys_tab32, out);
   else
     hipLaunchKernelGGL(esp_kernel<float>, grid, block, 0, s, pts, n, records, records32, segs, nseg, atoms, natom,
-                       boys_tab, out);
+                       boys_tab, boys_tab32, out);
   CT_HIP_CHECK(hipGetLastError());
 }
 

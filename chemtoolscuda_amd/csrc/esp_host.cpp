@@ -59,25 +59,43 @@ std::vector<std::vector<double>> hermite_poly(int L) {
 
 }  // namespace
 
+namespace {
+
+// F_0..F_{mmax}(T) and e^{-T} in long double: the convergent series
+// F_m(T) = e^{-T} sum_i (2T)^i / ((2m+1)(2m+3)...(2m+2i+1)) for the top order,
+// then the downward recursion F_m = (2T F_{m+1} + e^{-T}) / (2m+1).
+long double boys_row(long double T, int mmax, long double* F) {
+  long double term = 1.0L / (2 * mmax + 1), sum = term;
+  for (int i = 1; i < 4000; ++i) {
+    term *= 2.0L * T / (2 * mmax + 2 * i + 1);
+    sum += term;
+    if (term < 1e-22L * sum) break;
+  }
+  const long double eT = std::exp(-T);
+  F[mmax] = eT * sum;
+  for (int m = mmax - 1; m >= 0; --m) F[m] = (2.0L * T * F[m + 1] + eT) / (2 * m + 1);
+  return eT;
+}
+
+}  // namespace
+
 std::vector<double> build_boys_table() {
   std::vector<double> tab((size_t)kBoysNG * kBoysM);
+  long double F[kBoysM];
   for (int g = 0; g < kBoysNG; ++g) {
-    const long double T = (long double)g * kBoysStep;
-    const int mmax = kBoysM - 1;
-    // F_mmax(T) = e^{-T} sum_i (2T)^i / ((2m+1)(2m+3)...(2m+2i+1)), then downward
-    long double term = 1.0L / (2 * mmax + 1), sum = term;
-    for (int i = 1; i < 4000; ++i) {
-      term *= 2.0L * T / (2 * mmax + 2 * i + 1);
-      sum += term;
-      if (term < 1e-22L * sum) break;
-    }
-    const long double eT = std::exp(-T);
-    long double F = eT * sum;
-    tab[(size_t)g * kBoysM + mmax] = (double)F;
-    for (int m = mmax - 1; m >= 0; --m) {
-      F = (2.0L * T * F + eT) / (2 * m + 1);
-      tab[(size_t)g * kBoysM + m] = (double)F;
-    }
+    boys_row((long double)g * kBoysStep, kBoysM - 1, F);
+    for (int m = 0; m < kBoysM; ++m) tab[(size_t)g * kBoysM + m] = (double)F[m];
+  }
+  return tab;
+}
+
+std::vector<float> build_boys_table_f32() {
+  std::vector<float> tab((size_t)kBoysNG * kBoysRow32, 0.0f);
+  long double F[kBoysM32];
+  for (int g = 0; g < kBoysNG; ++g) {
+    const long double eT = boys_row((long double)g * kBoysStep, kBoysM32 - 1, F);
+    for (int m = 0; m < kBoysM32; ++m) tab[(size_t)g * kBoysRow32 + m] = (float)F[m];
+    tab[(size_t)g * kBoysRow32 + kBoysM32] = (float)eT;
   }
   return tab;
 }

@@ -555,9 +555,15 @@ void Mol::prepare_esp() {
     atoms[4 * a + 3] = charges_[a];
   }
   if (!d_boys_.get()) {
+    // FP64 table, then the FP32 table of the mixed kernel
     const std::vector<double> tab = build_boys_table();
-    if (!d_boys_.reserve(sizeof(double) * tab.size())) throw Error(CT_E_NOMEM, "cannot allocate the Boys table");
-    CT_HIP_CHECK(hipMemcpy(d_boys_.get(), tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice));
+    const std::vector<float> tab32 = build_boys_table_f32();
+    const size_t b64 = sizeof(double) * tab.size();
+    if (!d_boys_.reserve(b64 + sizeof(float) * tab32.size()))
+      throw Error(CT_E_NOMEM, "cannot allocate the Boys table");
+    CT_HIP_CHECK(hipMemcpy(d_boys_.get(), tab.data(), b64, hipMemcpyHostToDevice));
+    CT_HIP_CHECK(hipMemcpy(static_cast<char*>(d_boys_.get()) + b64, tab32.data(), sizeof(float) * tab32.size(),
+                           hipMemcpyHostToDevice));
   }
   if (!d_esp_rec_.reserve(sizeof(double) * std::max<size_t>(1, d.records.size())) ||
       !d_esp_seg_.reserve(sizeof(EspSegment) * std::max<size_t>(1, d.segs.size())) ||
@@ -706,7 +712,9 @@ void Mol::esp(const double* points, int64_t n, int precision, double* out) {
   const float* rec32 = precision == CT_PREC_MIXED ? static_cast<const float*>(d_esp_rec32_.get()) : nullptr;
   launch_esp(dp, n, static_cast<const double*>(d_esp_rec_.get()), rec32,
              static_cast<const EspSegment*>(d_esp_seg_.get()), esp_nseg_, static_cast<const double*>(d_esp_atoms_.get()),
-             natom_, static_cast<const double*>(d_boys_.get()), dout, stream_);
+             natom_, static_cast<const double*>(d_boys_.get()),
+             reinterpret_cast<const float*>(static_cast<const double*>(d_boys_.get()) + (size_t)kBoysNG * kBoysM), dout,
+             stream_);
   timer_.end(stream_);
   if (!od) CT_HIP_CHECK(hipMemcpyAsync(out, dout, sizeof(double) * n, hipMemcpyDeviceToHost, stream_));
   CT_HIP_CHECK(hipStreamSynchronize(stream_));
