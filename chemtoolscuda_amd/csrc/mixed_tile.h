@@ -283,15 +283,19 @@ __device__ __forceinline__ void y_tile(const u4* __restrict__ Pf, int rows, int 
       ubase[u] = bcol(ucol[u]);
     }
   }
+  // Every thread loads all NU units, a unit past the last one (!uok) from its
+  // first unit's address (a cache hit; only the store skips it): with the
+  // loads unconditional the compiler counts them statically and waits for
+  // the next step's B only where it is split into LDS, a whole k-step after
+  // the load.  Masked loads make the count dynamic, and the wait for A at
+  // the top of each k-step then also drains the B prefetch.
   f4 rbuf[NU][2];
   auto load_b = [&](int kb) {
 #pragma unroll
     for (int u = 0; u < NU; ++u)
-      if (uok[u]) {
 #pragma unroll
-        for (int h = 0; h < 2; ++h)
-          rbuf[u][h] = *reinterpret_cast<const f4*>(ubase[u] + (int64_t)(8 * kb + 2 * uoct[u] + h) * ld * 4);
-      }
+      for (int h = 0; h < 2; ++h)
+        rbuf[u][h] = *reinterpret_cast<const f4*>(ubase[u] + (int64_t)(8 * kb + 2 * uoct[u] + h) * ld * 4);
   };
   auto store_b = [&](int s) { tile_store_b<TC, NU>(Bs, s, rbuf, ucol, uoct, uok); };
 
