@@ -49,20 +49,29 @@ constexpr int NRB = RW / 16;  // 16-row MFMA blocks per wave
 __device__ __forceinline__ bf16x8 as_bf(u4 v) { return __builtin_bit_cast(bf16x8, v); }
 
 // Exact three-piece split of two FP32 values (hardware RNE pack,
-// v_cvt_pk_bf16_f32): the packed hi, mid and lo pairs.
+// v_cvt_pk_bf16_f32): the packed hi, mid and lo pairs.  The pieces are
+// unpacked from the packed pair by a shift (low half) and a mask (high
+// half): a bf16 is the top half of the FP32 of the same value.  Three packs,
+// four bit operations and four subtractions per pair.
 struct Split2 {
   uint32_t h, m, l;
 };
+__device__ __forceinline__ uint32_t pack_bf16(float x0, float x1) {
+  bf16x2 v;
+  v[0] = (__bf16)x0;
+  v[1] = (__bf16)x1;
+  uint32_t u = __builtin_bit_cast(uint32_t, v);
+  // opaque to the optimiser, which would otherwise rebuild the low half as a
+  // second conversion instead of one shift
+  asm("" : "+v"(u));
+  return u;
+}
 __device__ __forceinline__ Split2 split2(float x0, float x1) {
-  bf16x2 hh, mm, ll;
-  hh[0] = (__bf16)x0;
-  hh[1] = (__bf16)x1;
-  const float r0 = x0 - (float)hh[0], r1 = x1 - (float)hh[1];
-  mm[0] = (__bf16)r0;
-  mm[1] = (__bf16)r1;
-  ll[0] = (__bf16)(r0 - (float)mm[0]);
-  ll[1] = (__bf16)(r1 - (float)mm[1]);
-  return Split2{__builtin_bit_cast(uint32_t, hh), __builtin_bit_cast(uint32_t, mm), __builtin_bit_cast(uint32_t, ll)};
+  const uint32_t h = pack_bf16(x0, x1);
+  const float r0 = x0 - __uint_as_float(h << 16), r1 = x1 - __uint_as_float(h & 0xFFFF0000u);
+  const uint32_t m = pack_bf16(r0, r1);
+  const float s0 = r0 - __uint_as_float(m << 16), s1 = r1 - __uint_as_float(m & 0xFFFF0000u);
+  return Split2{h, m, pack_bf16(s0, s1)};
 }
 
 // Accumulation of the six products, two schemes:
