@@ -61,8 +61,8 @@ __global__ __launch_bounds__(256, 2) void contract_adjoint_mixed_kernel(const u4
 
   for (int64_t pt = t0; pt < t1; ++pt) {
     const int64_t pbase = pt * kAdjTC;
-    YTile<kAdjTC, 0, true> yt;
-    y_tile<kAdjTC, 0, true, false>(Pf, mp, mp, ld, i0, tid, ao + (pbase + (tid & (kAdjTC - 1))) * 4, Bs, yt, 0);
+    YTile<kAdjTC, 0> yt;
+    y_tile<kAdjTC, 0, kAdjTC / 16, false>(Pf, mp, mp, ld, i0, tid, ao + (pbase + (tid & (kAdjTC - 1))) * 4, Bs, yt, 0);
 #pragma unroll
     for (int cb = 0; cb < kAdjTC / 16; ++cb) {
       const int64_t p = pbase + 16 * cb + r;

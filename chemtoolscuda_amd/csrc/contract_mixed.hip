@@ -72,7 +72,11 @@ struct Cols {
                              : V == V_F5                           ? 1
                              : V == V_HESS || V == V_F4            ? 2
                                                                    : NCB;  // bound blocks (the phi columns)
-  static constexpr bool F64 = V != V_HESS;  // FP64 running sums (the Hessian tile lacks the registers)
+  // column blocks with FP64 running sums: none in the Hessian tile (short of
+  // registers); in the lap tile only the phi block (psi, which rho and its
+  // error bound come from), the derivative blocks in FP32 pairs (no VALU
+  // work per k-step)
+  static constexpr int NF = V == V_HESS ? 0 : V == V_F5 ? 1 : NCB;
   static constexpr bool TRI = V == V_RHO || V == V_LAPDER;  // quadratic forms only: triangle of P
   // AO array of column c: 0 = phi, 1..3 = d_x, d_y, d_z phi, then 4 = lap phi
   // (lap mode) or 4..9 = d_xx, d_xy, d_xz, d_yy, d_yz, d_zz phi (Hessian mode)
@@ -245,7 +249,7 @@ __global__ __launch_bounds__(256, 2) void contract_mixed_kernel(const u4* __rest
   const int i0 = rt * TR;
   const int64_t mq = mp / 4;  // row quads per AO array
 
-  YTile<TC, NBC, C::F64> yt;
+  YTile<TC, NBC, C::NF> yt;
   // this thread's B column (TC a multiple of 64), or any column's (bcol)
   auto bcol = [&](int c) { return ao + ((int64_t)C::type(c) * mq * ld + pbase + C::point(c)) * 4; };
   const float* bsrc = TC % 64 == 0 ? bcol(tid & (TC - 1)) : nullptr;
@@ -253,7 +257,7 @@ __global__ __launch_bounds__(256, 2) void contract_mixed_kernel(const u4* __rest
   // and length; else one shared list of all k-steps, kstride 0)
   const int32_t* kl = C::FAC ? klist + ptile * kstride : nullptr;
   const int kc = C::FAC ? (kcount ? kcount[ptile] : kcount_all) : 0;
-  y_tile<TC, NBC, C::F64, C::TRI, C::FAC>(Pf, rows, mp, ld, i0, tid, bsrc, Bs, yt, C::TRI ? i0 / KB : 0, kl, kc,
+  y_tile<TC, NBC, C::NF, C::TRI, C::FAC>(Pf, rows, mp, ld, i0, tid, bsrc, Bs, yt, C::TRI ? i0 / KB : 0, kl, kc,
                                           bcol);
   if constexpr (C::FAC) {
     factor_epilogue<V, DT>(yt, &Bs[0][0][0][0], part, ld, rt, pbase, tid);

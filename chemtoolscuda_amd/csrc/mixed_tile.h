@@ -74,28 +74,31 @@ __device__ __forceinline__ Split2 split2(float x0, float x1) {
   return Split2{h, m, pack_bf16(s0, s1)};
 }
 
-// Accumulation of the six products, two schemes:
-//   F64 = true:  per k-step and column-block pair the six products are summed
-//     from zero in FP32 (their 32-term MFMA sums and five adds at the size of
-//     one k-step's contribution) and the chunk is added to FP64 running sums:
-//     the only FP32 rounding at full magnitude is gone (error ~2.4u |P||B|,
-//     close to the 2.1u floor of rounding the inputs to FP32); costs 2x the
-//     accumulator registers.
-//   F64 = false: hi*hi in one FP32 accumulator, the five smaller products in
-//     another (~3.7u |P||B|), for the Hessian mode's larger epilogue.
-template <int TC, int NBC, bool F64 = true>
+// Accumulation of the six products, per 16-column block, two schemes:
+//   FP64 sums (the first NF column blocks): per k-step and column block the
+//     six products are summed from zero in FP32 (their 32-term MFMA sums and
+//     five adds at the size of one k-step's contribution) and the chunk is
+//     added to FP64 running sums: the only FP32 rounding at full magnitude is
+//     gone (error ~2.4u |P||B|, close to the 2.1u floor of rounding the
+//     inputs to FP32); 80 VALU operations per k-step and 16-row x 16-column
+//     pair of blocks (conversion and FP64 add).
+//   FP32 pair (the other blocks): hi*hi in one FP32 accumulator, the five
+//     smaller products in another (~3.7u |P||B|), no VALU work per k-step.
+// NF = TC / 16: all blocks FP64; NF = 0: none (the Hessian mode's larger
+// epilogue).  Both cost the same registers.
+template <int TC, int NBC, int NF = TC / 16>
 struct YTile {
   static constexpr int NCB = TC / 16;
-  static constexpr bool kF64 = F64;
-  double yd[F64 ? NRB : 1][F64 ? NCB : 1][4];  // F64: running sums
-  f4 hh[F64 ? 1 : NRB][F64 ? 1 : NCB];         // !F64: hi*hi products
-  f4 sm[F64 ? 1 : NRB][F64 ? 1 : NCB];         // !F64: the five smaller products
-  f4 bound[NRB][NBC > 0 ? NBC : 1];            // |P_hi| |B_hi| for column blocks 0..NBC-1
+  static constexpr int N32 = NCB - NF;
+  static_assert(NF >= 0 && NF <= NCB, "FP64 blocks");
+  double yd[NRB][NF > 0 ? NF : 1][4];   // FP64 running sums of blocks 0..NF-1
+  f4 hh[NRB][N32 > 0 ? N32 : 1];        // FP32 pair of blocks NF..NCB-1: hi*hi
+  f4 sm[NRB][N32 > 0 ? N32 : 1];        //   and the five smaller products
+  f4 bound[NRB][NBC > 0 ? NBC : 1];     // |P_hi| |B_hi| for column blocks 0..NBC-1
   __device__ __forceinline__ double y(int rb, int cb, int j) const {
-    if constexpr (F64)
-      return yd[rb][cb][j];
-    else
-      return (double)hh[rb][cb][j] + (double)sm[rb][cb][j];
+    if (cb < NF) return yd[rb][cb < NF ? cb : 0][j];
+    const int c = cb >= NF ? cb - NF : 0;
+    return (double)hh[rb][c][j] + (double)sm[rb][c][j];
   }
 };
 
@@ -104,12 +107,15 @@ struct YTile {
 // per product, so dependent MFMAs are four issues apart), added to the
 // accumulators of t, and the bound product on the first NBC column blocks.
 // dbl (TRI only): add the products twice.
-template <int TC, int NBC, bool F64, bool TRI>
+template <int TC, int NBC, int NF, bool TRI>
 __device__ __forceinline__ void tile_compute(const uint16_t (&Bs)[2][NPL][TC][LROW], int s, bool dbl,
-                                             const u4 (&areg)[NPL][NRB], YTile<TC, NBC, F64>& t, int r, int q) {
+                                             const u4 (&areg)[NPL][NRB], YTile<TC, NBC, NF>& t, int r, int q) {
   constexpr int NCB = TC / 16;
   const u4 absmask = u4{0x7FFF7FFFu, 0x7FFF7FFFu, 0x7FFF7FFFu, 0x7FFF7FFFu};
   const u4 exp1 = u4{0x00800080u, 0x00800080u, 0x00800080u, 0x00800080u};  // +1 in both bf16 exponents
+  // products smallest first, hi*hi last
+  constexpr int pa[6] = {2, 1, 0, 1, 0, 0};
+  constexpr int pb[6] = {0, 1, 2, 0, 1, 0};
 #pragma unroll
   for (int cp = 0; cp < (NCB + 1) / 2; ++cp) {
     // an odd NCB leaves the second block of the last pair empty (c < nc)
@@ -120,51 +126,41 @@ __device__ __forceinline__ void tile_compute(const uint16_t (&Bs)[2][NPL][TC][LR
 #pragma unroll
       for (int pl = 0; pl < NPL; ++pl)
         if (c < nc) bf[c][pl] = as_bf(*reinterpret_cast<const u4*>(&Bs[s][pl][16 * (2 * cp + c) + r][8 * q]));
-    if constexpr (F64) {
-      constexpr int pa[6] = {2, 1, 0, 1, 0, 0};
-      constexpr int pb[6] = {0, 1, 2, 0, 1, 0};
-      f4 tc[NRB][2];
+    f4 tc[NRB][2];
+#pragma unroll
+    for (int rb = 0; rb < NRB; ++rb)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) tc[rb][c] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int pr = 0; pr < 6; ++pr)
 #pragma unroll
       for (int rb = 0; rb < NRB; ++rb)
 #pragma unroll
-        for (int c = 0; c < 2; ++c) tc[rb][c] = f4{0.f, 0.f, 0.f, 0.f};
+        for (int c = 0; c < 2; ++c) {
+          const int cb = 2 * cp + c;
+          const int c32 = cb >= NF ? cb - NF : 0;
+          if (c >= nc) continue;
+          if (cb < NF)
+            tc[rb][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(areg[pa[pr]][rb]), bf[c][pb[pr]], tc[rb][c], 0,
+                                                                0, 0);
+          else if (pr < 5)
+            t.sm[rb][c32] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(areg[pa[pr]][rb]), bf[c][pb[pr]],
+                                                                    t.sm[rb][c32], 0, 0, 0);
+          else
+            t.hh[rb][c32] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(areg[0][rb]), bf[c][0], t.hh[rb][c32], 0,
+                                                                    0, 0);
+        }
+    const double sc = (TRI && dbl) ? 2.0 : 1.0;
 #pragma unroll
-      for (int pr = 0; pr < 6; ++pr)
+    for (int rb = 0; rb < NRB; ++rb)
 #pragma unroll
-        for (int rb = 0; rb < NRB; ++rb)
+      for (int c = 0; c < 2; ++c) {
+        const int cb = 2 * cp + c;
+        if (c < nc && cb < NF)
 #pragma unroll
-          for (int c = 0; c < 2; ++c)
-            if (c < nc)
-              tc[rb][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(areg[pa[pr]][rb]), bf[c][pb[pr]], tc[rb][c],
-                                                                  0, 0, 0);
-      const double sc = (TRI && dbl) ? 2.0 : 1.0;
-#pragma unroll
-      for (int rb = 0; rb < NRB; ++rb)
-#pragma unroll
-        for (int c = 0; c < 2; ++c)
-          if (c < nc)
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-              t.yd[rb][(2 * cp + c) % NCB][j] = fma((double)tc[rb][c][j], sc, t.yd[rb][(2 * cp + c) % NCB][j]);
-    } else {
-      constexpr int pa[5] = {2, 1, 0, 1, 0};
-      constexpr int pb[5] = {0, 1, 2, 0, 1};
-      static_assert(F64 || NCB % 2 == 0, "FP32 accumulators: column blocks in pairs");
-#pragma unroll
-      for (int pr = 0; pr < 5; ++pr)
-#pragma unroll
-        for (int rb = 0; rb < NRB; ++rb)
-#pragma unroll
-          for (int c = 0; c < 2; ++c)
-            t.sm[rb][2 * cp + c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                as_bf(areg[pa[pr]][rb]), bf[c][pb[pr]], t.sm[rb][2 * cp + c], 0, 0, 0);
-#pragma unroll
-      for (int rb = 0; rb < NRB; ++rb)
-#pragma unroll
-        for (int c = 0; c < 2; ++c)
-          t.hh[rb][2 * cp + c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(areg[0][rb]), bf[c][0],
-                                                                          t.hh[rb][2 * cp + c], 0, 0, 0);
-    }
+          for (int j = 0; j < 4; ++j)
+            t.yd[rb][cb < NF ? cb : 0][j] = fma((double)tc[rb][c][j], sc, t.yd[rb][cb < NF ? cb : 0][j]);
+      }
     if constexpr (NBC > 0) {
       if (2 * cp < NBC) {
 #pragma unroll
@@ -183,20 +179,17 @@ __device__ __forceinline__ void tile_compute(const uint16_t (&Bs)[2][NPL][TC][LR
 }
 
 // Zeroes the accumulators of a tile.
-template <int TC, int NBC, bool F64>
-__device__ __forceinline__ void tile_zero(YTile<TC, NBC, F64>& t) {
-  constexpr int NCB = TC / 16;
+template <int TC, int NBC, int NF>
+__device__ __forceinline__ void tile_zero(YTile<TC, NBC, NF>& t) {
+  using T = YTile<TC, NBC, NF>;
 #pragma unroll
   for (int a = 0; a < NRB; ++a) {
 #pragma unroll
-    for (int b = 0; b < NCB; ++b) {
-      if constexpr (F64) {
+    for (int b = 0; b < (NF > 0 ? NF : 1); ++b)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) t.yd[a][b][j] = 0.0;
-      } else {
-        t.hh[a][b] = t.sm[a][b] = f4{0.f, 0.f, 0.f, 0.f};
-      }
-    }
+      for (int j = 0; j < 4; ++j) t.yd[a][b][j] = 0.0;
+#pragma unroll
+    for (int b = 0; b < (T::N32 > 0 ? T::N32 : 1); ++b) t.hh[a][b] = t.sm[a][b] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int b = 0; b < (NBC > 0 ? NBC : 1); ++b) t.bound[a][b] = f4{0.f, 0.f, 0.f, 0.f};
   }
@@ -242,12 +235,12 @@ __device__ __forceinline__ void tile_store_b(uint16_t (&Bs)[2][NPL][TC][LROW], i
 // TC a multiple of 64: bsrc is this thread's column (tid % TC).  Other TC
 // (a multiple of 16): bcol(c) gives column c's row quad 0, and the TC x 4
 // (column, octet) units of a k-step go round-robin to the 256 threads.
-template <int TC, int NBC, bool F64, bool TRI, bool LIST = false, class BCol = std::nullptr_t>
+template <int TC, int NBC, int NF, bool TRI, bool LIST = false, class BCol = std::nullptr_t>
 __device__ __forceinline__ void y_tile(const u4* __restrict__ Pf, int rows, int mp, int64_t ld, int i0, int tid,
                                        const float* __restrict__ bsrc, uint16_t (&Bs)[2][NPL][TC][LROW],
-                                       YTile<TC, NBC, F64>& t, int k_begin, const int32_t* klist = nullptr,
+                                       YTile<TC, NBC, NF>& t, int k_begin, const int32_t* klist = nullptr,
                                        int kcount = 0, BCol bcol = nullptr) {
-  static_assert(!TRI || F64, "the triangle needs the FP64 running sums");
+  static_assert(!TRI || NF == TC / 16, "the triangle needs the FP64 running sums");
   static_assert(!(TRI && LIST), "the triangle walks all k-steps from the diagonal");
   static_assert(TC % 16 == 0, "column blocks of 16");
   constexpr int NCB = TC / 16;
@@ -310,7 +303,7 @@ __device__ __forceinline__ void y_tile(const u4* __restrict__ Pf, int rows, int 
 
   tile_zero(t);
 
-  auto compute = [&](int s, bool dbl) { tile_compute<TC, NBC, F64, TRI>(Bs, s, dbl, areg, t, r, q); };
+  auto compute = [&](int s, bool dbl) { tile_compute<TC, NBC, NF, TRI>(Bs, s, dbl, areg, t, r, q); };
 
   // The loads run unconditionally, clamped to the last k-step on the final
   // iterations (those values are unused; the LDS buffer written then is the
