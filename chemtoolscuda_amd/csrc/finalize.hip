@@ -122,10 +122,15 @@ __global__ __launch_bounds__(256) void finalize_mixed_kernel(FinalizeArgs a, Fla
     base = __shfl(base, leader);
     if (flag) f.index[base + __popcll(fm & ((1ull << lane) - 1ull))] = (int32_t)(f.index_base + pp);
   }
-  // one atomic per wave: the same address for every point would serialise
+  // largest estimate: reduced per wave, and the wave's atomic only when it
+  // exceeds the value already stored (read first).  Atomics on one address
+  // serialise at the L2; most waves' maxima stay below the running maximum
+  // and end with the read.  A stale read only costs an unneeded atomic.
   for (int off = 32; off >= 1; off >>= 1) rel = fmax(rel, __shfl_xor(rel, off));
-  if ((threadIdx.x & 63) == 0 && rel > 0.0)
-    atomicMax(f.max_est, (unsigned long long)__double_as_longlong(rel));
+  if ((threadIdx.x & 63) == 0 && rel > 0.0) {
+    const unsigned long long bits = (unsigned long long)__double_as_longlong(rel);
+    if (bits > __hip_atomic_load(f.max_est, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(f.max_est, bits);
+  }
 }
 
 __global__ __launch_bounds__(256) void gather_points_kernel(const double* __restrict__ pts,
