@@ -199,9 +199,11 @@ struct MortonFrame {
   double inv_cell;
 };
 size_t screen_sort_workspace(int64_t n);
-// ascending sort of n non-negative int32 (the flagged points of a call)
-size_t sort_indices_workspace(int64_t n);
-void launch_sort_indices(const int32_t* in, int32_t* out, int64_t n, void* temp, size_t temp_bytes, hipStream_t s);
+// ascending sort of n non-negative int32 below 2^bits (the flagged points of
+// a call)
+size_t sort_indices_workspace(int64_t n, int bits);
+void launch_sort_indices(const int32_t* in, int32_t* out, int64_t n, int bits, void* temp, size_t temp_bytes,
+                         hipStream_t s);
 // perm[i] = index of the i-th point in Morton order (stable)
 void launch_screen_sort(const double* pts, int64_t n, const MortonFrame& f, uint32_t* keys_in, uint32_t* keys_out,
                         int32_t* idx_in, int32_t* perm, void* temp, size_t temp_bytes, hipStream_t s);

@@ -1010,11 +1010,13 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
       // points given in a spatial order stay so for the AO kernel's waves)
       const int64_t per64 = (int64_t)nder * Mp_ * 8 + (int64_t)ndot * 8 + 24;
       const int64_t c64 = chunk_points(per64, count);
-      const size_t tmp = (sort_indices_workspace(count) + 255) / 256 * 256;
+      int bits = 1;  // the indices lie below n
+      while (bits < 31 && (int64_t{1} << bits) < n) ++bits;
+      const size_t tmp = (sort_indices_workspace(count, bits) + 255) / 256 * 256;
       const int32_t* sorted = list;
       if (d_flags_sorted_.reserve(tmp + sizeof(int32_t) * (size_t)count)) {
         char* base = static_cast<char*>(d_flags_sorted_.get());
-        launch_sort_indices(list, reinterpret_cast<int32_t*>(base + tmp), count, base, tmp, stream_);
+        launch_sort_indices(list, reinterpret_cast<int32_t*>(base + tmp), count, bits, base, tmp, stream_);
         sorted = reinterpret_cast<const int32_t*>(base + tmp);
       }
       timer_.begin(KernelTimer::REEVAL, stream_);

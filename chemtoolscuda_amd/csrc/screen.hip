@@ -123,10 +123,17 @@ __global__ __launch_bounds__(256) void tile_activity_kernel(const double* __rest
 
 }  // namespace
 
+// rocPRIM sorts up to 2^20 items by a merge sort (a block sort and two
+// launches per merge pass: 17 launches for 663k pairs); this configuration
+// takes the radix passes for every size beyond one block instead: a
+// histogram, its scan and one pass per 8 key bits.
+using RadixOnly = rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config,
+                                             rocprim::default_config, 0>;
+
 size_t screen_sort_workspace(int64_t n) {
   size_t bytes = 0;
-  CT_HIP_CHECK(rocprim::radix_sort_pairs(nullptr, bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
-                                         (const int32_t*)nullptr, (int32_t*)nullptr, (size_t)n, 0, 24));
+  CT_HIP_CHECK(rocprim::radix_sort_pairs<RadixOnly>(nullptr, bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                                    (const int32_t*)nullptr, (int32_t*)nullptr, (size_t)n, 0, 24));
   return bytes;
 }
 
@@ -137,20 +144,24 @@ void launch_screen_sort(const double* pts, int64_t n, const MortonFrame& f, uint
                      idx_in);
   CT_HIP_CHECK(hipGetLastError());
   size_t bytes = temp_bytes;
-  CT_HIP_CHECK(rocprim::radix_sort_pairs(temp, bytes, (const uint32_t*)keys_in, keys_out, (const int32_t*)idx_in,
-                                         perm, (size_t)n, 0, 24, s));
+  CT_HIP_CHECK(rocprim::radix_sort_pairs<RadixOnly>(temp, bytes, (const uint32_t*)keys_in, keys_out,
+                                                    (const int32_t*)idx_in, perm, (size_t)n, 0, 24, s));
 }
 
-size_t sort_indices_workspace(int64_t n) {
+size_t sort_indices_workspace(int64_t n, int bits) {
   size_t bytes = 0;
-  CT_HIP_CHECK(rocprim::radix_sort_keys(nullptr, bytes, (const int32_t*)nullptr, (int32_t*)nullptr, (size_t)n, 0, 31));
+  CT_HIP_CHECK(rocprim::radix_sort_keys<RadixOnly>(nullptr, bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                                   (size_t)n, 0, (unsigned)bits));
   return bytes;
 }
 
-void launch_sort_indices(const int32_t* in, int32_t* out, int64_t n, void* temp, size_t temp_bytes, hipStream_t s) {
+void launch_sort_indices(const int32_t* in, int32_t* out, int64_t n, int bits, void* temp, size_t temp_bytes,
+                         hipStream_t s) {
   if (n <= 0) return;
   size_t bytes = temp_bytes;
-  CT_HIP_CHECK(rocprim::radix_sort_keys(temp, bytes, in, out, (size_t)n, 0, 31, s));
+  CT_HIP_CHECK(rocprim::radix_sort_keys<RadixOnly>(temp, bytes, reinterpret_cast<const uint32_t*>(in),
+                                                   reinterpret_cast<uint32_t*>(out), (size_t)n, 0, (unsigned)bits,
+                                                   s));
 }
 
 void launch_tile_activity(const double* sorted_pts, int64_t n, int tile_points, int64_t ntile, const int32_t* span_off,
