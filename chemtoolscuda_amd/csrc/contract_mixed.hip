@@ -73,10 +73,10 @@ struct Cols {
                              : V == V_HESS || V == V_F4            ? 2
                                                                    : NCB;  // bound blocks (the phi columns)
   // column blocks with FP64 running sums: none in the Hessian tile (short of
-  // registers); in the lap tile only the phi block (psi, which rho and its
-  // error bound come from), the derivative blocks in FP32 pairs (no VALU
-  // work per k-step)
-  static constexpr int NF = V == V_HESS ? 0 : V == V_F5 ? 1 : NCB;
+  // registers); in the factor route's lap and gradient/Hessian tiles only the
+  // phi blocks (psi, which rho and its error bound come from), the
+  // derivative blocks in FP32 pairs (no VALU work per k-step)
+  static constexpr int NF = V == V_HESS ? 0 : V == V_F5 || V == V_F4 ? PTS / 16 : NCB;
   static constexpr bool TRI = V == V_RHO || V == V_LAPDER;  // quadratic forms only: triangle of P
   // AO array of column c: 0 = phi, 1..3 = d_x, d_y, d_z phi, then 4 = lap phi
   // (lap mode) or 4..9 = d_xx, d_xy, d_xz, d_yy, d_yz, d_zz phi (Hessian mode)
