@@ -237,6 +237,15 @@ Mol::Mol(const HostBasis& basis, const double* atom_xyz, const double* charges, 
 
   CT_HIP_CHECK(hipGetDevice(&device_));
   CT_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  // default workspace budget: a quarter of the device's memory, at most
+  // 48 GiB (one chunk of 2.7M points for rho, grad, lap and tau of a
+  // 624-function basis on a 288 GB device; fewer, larger chunks run fewer
+  // sorts and launches per point)
+  {
+    size_t free_b = 0, total_b = 0;
+    CT_HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
+    ws_limit_ = std::max<int64_t>(int64_t(1) << 30, std::min<int64_t>(int64_t(48) << 30, (int64_t)(total_b / 4)));
+  }
   if (!d_meta_.reserve(sizeof(ShellMeta) * nshell_) || !d_prim_.reserve(sizeof(PrimData) * prim_.size()) ||
       !d_geom_.reserve(sizeof(ShellGeom) * nshell_) || !d_P_.reserve(sizeof(double) * (size_t)Mp_ * Mp_))
     throw Error(CT_E_NOMEM, "cannot allocate the device shell table / density matrix");

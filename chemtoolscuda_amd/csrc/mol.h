@@ -171,7 +171,7 @@ class Mol {
   DeviceBuffer d_flags_sorted_;  // the same in index order, and the sort's scratch
   PinnedBuffer h_small_;   // pinned readback of the flag counter / error estimate
   PinnedBuffer h_stage_;   // host staging for pageable inputs/outputs
-  int64_t ws_limit_ = int64_t(8) << 30;
+  int64_t ws_limit_ = int64_t(8) << 30;  // set from the device's memory at construction
   KernelTimer timer_;
   // ESP: Hermite pre-contraction of P, rebuilt when P or the centres change
   bool esp_valid_ = false;

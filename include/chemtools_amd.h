@@ -170,7 +170,8 @@ int ct_mol_get_density_matrix(const ct_mol* mol, double* P /* nbasis^2 */);
 /* Replace centres / P (host or device pointers). */
 int ct_mol_set_coordinates(ct_mol* mol, const double* atom_xyz);
 int ct_mol_set_density_matrix(ct_mol* mol, const double* P);
-/* Device-memory budget for the per-chunk workspace (bytes; default 8 GiB). */
+/* Device-memory budget for the per-chunk workspace (bytes; default a quarter
+ * of the device's memory, at most 48 GiB). */
 int ct_mol_set_workspace_limit(ct_mol* mol, int64_t bytes);
 
 /* Spatial screening of the mixed FACTOR route (no counterpart in the
