@@ -31,18 +31,25 @@ if ROOT not in sys.path:
 
 PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16": 2.5e15}
 
-# HBM bytes per contraction call (FETCH_SIZE + WRITE_SIZE, calibrated against
-# a 1 GiB device copy), measured with rocprofv3 --pmc on this workload; see
+# HBM bytes per point of a contraction call (FETCH_SIZE + WRITE_SIZE,
+# calibrated against a 1 GiB device copy: FETCH x 2.0, WRITE x 1.0), measured
+# with rocprofv3 --pmc on this workload; see
 # profiles/contract_mixed_{factor,density}_organic60.md and
-# profiles/contract_f64_organic60.md for the commands and raw counts.
-# A mixed call is several kernel dispatches (density route: phi columns, then
-# one per derivative type; factor route: two); its figure is their sum,
-# averaged over the chunks.  The factor route with the library's default
-# screening reads only the active blocks; "factor-unscreened" is --screening 0.
-MEASURED_TRAFFIC = {("organic60", "mixed", "factor"): 3.152e9 + 0.038e9,
-                    ("organic60", "mixed", "factor-unscreened"): 8.44e9 + 0.038e9,
-                    ("organic60", "mixed", "density"): 24.1e9 + 0.19e9,
-                    ("organic60", "fp64", "density"): 53.7e9 + 0.021e9}
+# profiles/contract_f64_organic60.md for the commands and raw counts.  The
+# traffic of a call is this figure times its points (the AO reads dominate
+# and scale with the points).  Default (screened factor route, one call over
+# all 2 666 298 points): 7.68e6 + 0.1875e6 KiB calibrated = 15.92 GB per call.
+# The other routes were measured per call over 8 GiB chunks (mean 533 260
+# points mixed, 333 287 FP64) and are given per point the same way.
+MEASURED_TRAFFIC_PER_POINT = {("organic60", "mixed", "factor"): (2 * 7.68e6 + 0.1875e6) * 1024 / 2666298,
+                              ("organic60", "mixed", "factor-unscreened"): (8.44e9 + 0.038e9) / 533260,
+                              ("organic60", "mixed", "density"): (24.1e9 + 0.19e9) / 533260,
+                              ("organic60", "fp64", "density"): (53.7e9 + 0.021e9) / 333287}
+
+
+def _traffic(workload, precision, key, points):
+    per_point = MEASURED_TRAFFIC_PER_POINT.get((workload, precision, key))
+    return None if per_point is None else per_point * points
 
 
 def parse():
@@ -281,7 +288,7 @@ def main():
     used_route = "factor" if rank >= 0 else "density"
     traffic_key = used_route + ("-unscreened" if used_route == "factor" and args.screening == 0 else "")
     roofline = {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK[peak_key] / 1e12, "unit": "TFLOP/s",
-                "frac": achieved / PEAK[peak_key], "traffic": MEASURED_TRAFFIC.get((w.name, precision, traffic_key)),
+                "frac": achieved / PEAK[peak_key], "traffic": _traffic(w.name, precision, traffic_key, pts_per_launch),
                 "kernel": "contract_" + ("f64" if precision == "fp64" else "mixed"),
                 "avg_launch_ms": contract_ms, "points_per_launch": pts_per_launch,
                 "flops_per_point": flops_pt, "route": used_route}
