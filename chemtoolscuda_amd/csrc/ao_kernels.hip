@@ -68,9 +68,24 @@ __global__ __launch_bounds__(256) void ao_eval_kernel(const ShellMeta* __restric
   for (int d = 0; d < (quads ? nder : 1); ++d)
 #pragma unroll
     for (int e = 0; e < 4; ++e) qb[d][e] = TS(0);
+  // screened layout, up to 64 k-steps: this lane's own tile mask; a quad is
+  // stored only where the lane's tile reads its k-step (the wave evaluates
+  // the union of its tiles' k-steps, ~10 % more blocks than any one tile
+  // reads with 16-point tiles).  Tiles past the chunk's last point hold no
+  // mask, and the padding lanes in them store everything.
+  uint32_t lm0 = 0xFFFFFFFFu, lm1 = 0xFFFFFFFFu;
+  if (kmask && nwords <= 2 && p < npts) {
+    const uint32_t* lmask = kmask + (p / tile_points) * nwords;
+    lm0 = lmask[0];
+    if (nwords == 2) lm1 = lmask[1];
+  }
   auto store_quad = [&](int d, int quad) {
-    if constexpr (quads)
-      *reinterpret_cast<f4*>(ao + ((int64_t)d * mq + quad) * ld * 4 + p * 4) = f4{qb[d][0], qb[d][1], qb[d][2], qb[d][3]};
+    if constexpr (quads) {
+      const int k = quad >> 3;  // 32-function k-step of the quad
+      if ((((k >> 5) ? lm1 : lm0) >> (k & 31)) & 1u)
+        *reinterpret_cast<f4*>(ao + ((int64_t)d * mq + quad) * ld * 4 + p * 4) =
+            f4{qb[d][0], qb[d][1], qb[d][2], qb[d][3]};
+    }
   };
   auto dispatch = [&](int code, T x, T y, T z, T s0, T s1, T s2, auto&& st) {
     if constexpr (MODE == MODE_RHO) ao_shell_dispatch_val<T, LMAX>(code, x, y, z, s0, s1, s2, st);
