@@ -46,7 +46,9 @@ __global__ __launch_bounds__(256) void morton_keys_kernel(const double* __restri
 
 // One wave per 64 points = 64 / tp tiles of tp points (tp in 16, 32, 64):
 // the tiles' bounding boxes by reductions within groups of tp lanes, then
-// per tile the active k-steps, one lane per k-step.
+// each group the active k-steps of its tile, one lane per k-step, tp k-steps
+// per round (all lanes busy: one tile at a time left three quarters of them
+// idle with 16-point tiles and 20 k-steps).
 __global__ __launch_bounds__(256) void tile_activity_kernel(const double* __restrict__ pts, int64_t n, int tp,
                                                             int64_t ntile, const int32_t* __restrict__ span_off,
                                                             const ScreenSpan* __restrict__ spans, int nk,
@@ -78,47 +80,53 @@ __global__ __launch_bounds__(256) void tile_activity_kernel(const double* __rest
       hi[k] = fmax(hi[k], __shfl_xor(hi[k], off));
     }
   const uint64_t gmask = tp == 64 ? ~0ull : ((1ull << tp) - 1ull);
-  for (int g = 0; g < tpw; ++g) {
-    const int64_t tile = t0 + g;
-    if (tile >= ntile) break;
-    double tlo[3], thi[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      tlo[k] = __shfl(lo[k], g * tp);
-      thi[k] = __shfl(hi[k], g * tp);
+  // the wave's tiles side by side: lane group g = lane / tp holds tile t0 + g
+  // (its box is already in every lane of the group) and tests tp k-steps per
+  // round, one per lane
+  const int g = lane / tp, j = lane % tp;
+  const int64_t tile = t0 + g;
+  const bool tvalid = tile < ntile;
+  const bool keep_all = ((nonfinite >> (g * tp)) & gmask) != 0;
+  const bool any = lo[0] <= hi[0];  // the tile holds at least one point
+  int total = 0;                    // the tile's active k-steps so far (same in its group)
+  uint32_t word = 0;                // mask word in assembly (group leader)
+  for (int kb0 = 0; kb0 < nk; kb0 += tp) {
+    const int kb = kb0 + j;
+    bool act = tvalid && keep_all && kb < nk && span_off[kb + 1] > span_off[kb];
+    if (tvalid && any && !keep_all && kb < nk) {
+      for (int e = span_off[kb]; e < span_off[kb + 1] && !act; ++e) {
+        const ScreenSpan sp = spans[e];
+        const double dx = fmax(0.0, fmax(lo[0] - sp.cx, sp.cx - hi[0]));
+        const double dy = fmax(0.0, fmax(lo[1] - sp.cy, sp.cy - hi[1]));
+        const double dz = fmax(0.0, fmax(lo[2] - sp.cz, sp.cz - hi[2]));
+        act = dx * dx + dy * dy + dz * dz < sp.r2;
+      }
     }
-    const bool keep_all = ((nonfinite >> (g * tp)) & gmask) != 0;
-    const bool any = tlo[0] <= thi[0];  // the tile holds at least one point
-    int total = 0;
-    for (int kb0 = 0; kb0 < nk; kb0 += 64) {
-      const int kb = kb0 + lane;
-      bool act = keep_all && kb < nk && span_off[kb + 1] > span_off[kb];
-      if (any && !keep_all && kb < nk) {
-        for (int e = span_off[kb]; e < span_off[kb + 1] && !act; ++e) {
-          const ScreenSpan sp = spans[e];
-          const double dx = fmax(0.0, fmax(tlo[0] - sp.cx, sp.cx - thi[0]));
-          const double dy = fmax(0.0, fmax(tlo[1] - sp.cy, sp.cy - thi[1]));
-          const double dz = fmax(0.0, fmax(tlo[2] - sp.cz, sp.cz - thi[2]));
-          act = dx * dx + dy * dy + dz * dz < sp.r2;
+    const uint64_t gbits = (__ballot(act) >> (g * tp)) & gmask;
+    if (act) klist[tile * nk + total + __popcll(gbits & ((1ull << j) - 1ull))] = kb;
+    if (j == 0 && tvalid) {
+      // this round's tp mask bits start at bit kb0 of the tile's mask
+      if (tp == 64) {
+        kmask[tile * nwords + (kb0 >> 5)] = (uint32_t)gbits;
+        if ((kb0 >> 5) + 1 < nwords) kmask[tile * nwords + (kb0 >> 5) + 1] = (uint32_t)(gbits >> 32);
+      } else {
+        word |= (uint32_t)gbits << (kb0 & 31);
+        if (((kb0 + tp) & 31) == 0 || kb0 + tp >= nk) {
+          kmask[tile * nwords + (kb0 >> 5)] = word;
+          word = 0;
         }
       }
-      const uint64_t bal = __ballot(act);
-      const int before = __popcll(bal & ((1ull << lane) - 1ull));
-      if (act) klist[tile * nk + total + before] = kb;
-      if (lane == 0) {
-        const int w = kb0 >> 5;
-        kmask[tile * nwords + w] = (uint32_t)bal;
-        if (w + 1 < nwords) kmask[tile * nwords + w + 1] = (uint32_t)(bal >> 32);
-      }
-      total += __popcll(bal);
     }
-    if (lane == 0) {
-      kcount[tile] = total;
-      // kScreenCounters counters, one per 128-byte line: one address for all
-      // tiles would serialise the atomics
-      atomicAdd(active_count + (tile % kScreenCounters) * 16, (unsigned long long)total);
-    }
+    total += __popcll(gbits);
   }
+  if (j == 0 && tvalid) kcount[tile] = total;
+  // the wave's active blocks, one atomic per wave over kScreenCounters
+  // counters on separate 128-byte lines (one address for every wave would
+  // serialise the atomics)
+  int wsum = 0;
+  for (int t = 0; t < 64 / tp; ++t) wsum += __shfl(total, t * tp);  // 0 for tiles past the last
+  if (lane == 0)
+    atomicAdd(active_count + ((t0 / (64 / tp)) % kScreenCounters) * 16, (unsigned long long)wsum);
 }
 
 }  // namespace
