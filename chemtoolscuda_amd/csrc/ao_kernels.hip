@@ -34,6 +34,8 @@
 
 #include <type_traits>
 
+#include <algorithm>
+
 #include "ct_internal.h"
 #include "generated/ao_poly.h"
 
@@ -122,7 +124,12 @@ __global__ __launch_bounds__(256) void ao_eval_kernel(const ShellMeta* __restric
   };
   constexpr int kNfunc[8] = {1, 3, 6, 10, 15, 5, 7, 9};  // per shell code
 
-  for (int s = 0; s < nshell; ++s) {
+  // FP64 rows (no quads shared between shells): gridDim.y > 1 splits the
+  // shells into ranges, one per workgroup row, so that few points still fill
+  // the GPU (the re-evaluation of a mixed call's flagged points)
+  const int s_begin = (int)((int64_t)blockIdx.y * nshell / gridDim.y);
+  const int s_end = (int)((int64_t)(blockIdx.y + 1) * nshell / gridDim.y);
+  for (int s = s_begin; s < s_end; ++s) {
     const ShellMeta m = meta[s];
     if (kmask) {
       // skipped: no k-step that the shell touches is read by the contraction
@@ -178,8 +185,9 @@ __global__ __launch_bounds__(256) void ao_eval_kernel(const ShellMeta* __restric
   }
   // zero rows of the padded basis so the contraction never reads garbage
   if constexpr (!quads) {
-    for (int d = 0; d < nder; ++d)
-      for (int i = nbasis; i < nbasis_pad; ++i) ao[((int64_t)d * nbasis_pad + i) * ld + p] = TS(0);
+    if (blockIdx.y + 1 == gridDim.y)
+      for (int d = 0; d < nder; ++d)
+        for (int i = nbasis; i < nbasis_pad; ++i) ao[((int64_t)d * nbasis_pad + i) * ld + p] = TS(0);
   } else {
     const int a = nbasis & 3;
     if (a != 0) {
@@ -204,7 +212,13 @@ template <typename TS, int MODE>
 void launch_lmax(int lmax, const ShellMeta* meta, const ShellGeom* geom, const PrimData* prim, int nshell, int nbasis,
                  int nbasis_pad, const double* pts, int64_t npts, int64_t ld, TS* ao, double screen,
                  const uint32_t* kmask, int nwords, int tile_points, hipStream_t s) {
-  const dim3 block(256), grid((unsigned)(ld / 256));
+  // FP64 rows: shell ranges over grid.y until ~2048 waves (a few per SIMD)
+  // are in flight; the FP32 row quads carry partial quads from shell to shell
+  const int64_t waves = ld / 64;
+  const int nsplit = sizeof(TS) == 8 ? (int)std::max<int64_t>(1, std::min<int64_t>({(2048 + waves - 1) / waves, 8,
+                                                                                      (int64_t)nshell}))
+                                     : 1;
+  const dim3 block(256), grid((unsigned)(ld / 256), (unsigned)nsplit);
   if (lmax <= 2)
     hipLaunchKernelGGL((ao_eval_kernel<TS, MODE, 2>), grid, block, 0, s, meta, geom, prim, nshell, nbasis, nbasis_pad,
                        pts, npts, ld, ao, screen, kmask, nwords, tile_points);
