@@ -106,7 +106,10 @@ __host__ __device__ inline int64_t ao_quad_index(int d, int i, int64_t p, int mp
 struct FinalizeArgs {
   const double* dots;     // FP64 path: [ndot][ld];  mixed path: partials [nrt][ndot+1][ld]
   const int32_t* index;   // optional: output index of point p is out_off + index[p] (FP64: the
-                          // re-evaluated points; mixed: the points in Morton order)
+                          // re-evaluated points)
+  const int32_t* gather;  // mixed, optional (index null): the dots of output point p are at
+                          // position gather[p] (the points in Morton order: the inverse
+                          // of their permutation; outputs written in order)
   int nrt;                // mixed path: number of 128-row blocks to sum
   int64_t ld;
   int64_t n;        // points in this chunk
@@ -230,6 +233,8 @@ struct FlagArgs {
 };
 void launch_finalize_mixed(const FinalizeArgs& a, const FlagArgs& f, hipStream_t s);
 void launch_gather_points(const double* pts, const int32_t* index, int64_t n, double* out, hipStream_t s);
+// inv[perm[i]] = i for i < n
+void launch_invert_permutation(const int32_t* perm, int64_t n, int32_t* inv, hipStream_t s);
 
 // Uniform (possibly skewed) grid: point (i, j, k) = origin + i a0 + j a1 + k a2,
 // a_r = axes[3r .. 3r+2]; linear index (i * n1 + j) * n2 + k (k fastest).

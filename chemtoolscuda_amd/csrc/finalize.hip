@@ -92,14 +92,18 @@ __global__ __launch_bounds__(256) void finalize_mixed_kernel(FinalizeArgs a, Fla
   double dot[16];
   for (int d = 0; d < 16; ++d) dot[d] = 0.0;
   double bound = 0.0;
+  // gather: p is an output point (coalesced stores), its dots lie at the
+  // sorted position gather[p] (reads from the contraction's fresh output);
+  // else the dots of p go to output index[p]
+  const int64_t src = a.gather ? (int64_t)a.gather[p] : p;
   for (int rt = 0; rt < a.nrt; ++rt) {
-    const double* base = a.dots + (int64_t)rt * nd1 * a.ld + p;
+    const double* base = a.dots + (int64_t)rt * nd1 * a.ld + src;
     for (int d = 0; d < 16; ++d)
       if (d < nd) dot[d] += base[(int64_t)d * a.ld];
     bound += base[(int64_t)nd * a.ld];
   }
-  // index (screened layout): the chunk position of the p-th sorted point
-  const int64_t pp = a.index ? (int64_t)a.index[p] : p;
+  // the chunk position of the output
+  const int64_t pp = a.gather ? p : a.index ? (int64_t)a.index[p] : p;
   if (live) emit_outputs(a, a.out_off + pp, dot);
   const double arho = fabs(dot[0]);
   double rel = 0.0;  // estimate / |rho| of an accepted point (non-negative: the bits order like the values)
@@ -131,6 +135,12 @@ __global__ __launch_bounds__(256) void finalize_mixed_kernel(FinalizeArgs a, Fla
     const unsigned long long bits = (unsigned long long)__double_as_longlong(rel);
     if (bits > __hip_atomic_load(f.max_est, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(f.max_est, bits);
   }
+}
+
+__global__ __launch_bounds__(256) void invert_permutation_kernel(const int32_t* __restrict__ perm, int64_t n,
+                                                                 int32_t* __restrict__ inv) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) inv[perm[i]] = (int32_t)i;
 }
 
 __global__ __launch_bounds__(256) void gather_points_kernel(const double* __restrict__ pts,
@@ -181,6 +191,12 @@ void launch_finalize_mixed(const FinalizeArgs& a, const FlagArgs& f, hipStream_t
 void launch_gather_points(const double* pts, const int32_t* index, int64_t n, double* out, hipStream_t s) {
   if (n <= 0) return;
   hipLaunchKernelGGL(gather_points_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, pts, index, n, out);
+  CT_HIP_CHECK(hipGetLastError());
+}
+
+void launch_invert_permutation(const int32_t* perm, int64_t n, int32_t* inv, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(invert_permutation_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, perm, n, inv);
   CT_HIP_CHECK(hipGetLastError());
 }
 

@@ -946,12 +946,16 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
         timer_.begin(KernelTimer::SCREEN, stream_);
         launch_screen_sort(pts, nc, frame_, keys_in, keys_out, idx_in, perm, sort_tmp, sort_bytes, stream_);
         launch_gather_points(pts, perm, nc, spts, stream_);
+        // the sort's output keys are spent: their space holds the inverse permutation
+        launch_invert_permutation(perm, nc, reinterpret_cast<int32_t*>(keys_out), stream_);
         launch_tile_activity(spts, nc, tp, ld / tp, static_cast<const int32_t*>(d_span_off_.get()),
                              static_cast<const ScreenSpan*>(d_spans_.get()), nk, klist, kcount, kmask, nwords,
                              d_active, stream_);
         timer_.end(stream_);
         apts = spts;
-        fa.index = perm;  // results of sorted point p belong to chunk point perm[p]
+        // the finaliser runs over the chunk's points in their own order and
+        // reads each one's results at its sorted position
+        fa.gather = reinterpret_cast<const int32_t*>(keys_out);
         valid_tiles += (nc + tp - 1) / tp;
       }
       // the factor route reads the AOs in the order of its factor (screened
