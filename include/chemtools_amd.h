@@ -177,7 +177,8 @@ int ct_mol_set_workspace_limit(ct_mol* mol, int64_t bytes);
 /* Spatial screening of the mixed FACTOR route (no counterpart in the
  * reference, which evaluates every basis function at every point,
  * src/evaluate_density.cu:25-411).  Points are taken in Morton order and,
- * per tile of 32 or 64 of them, the 32-function blocks of the basis whose
+ * per tile of 16 (rho+grad+lap+tau), 32 (grad, Hessian) or 64 (rho) of them,
+ * the 32-function blocks of the basis whose
  * values, first and second derivatives are all at most eps on the tile are
  * skipped; 4 eps ||c||_2 sqrt(rho) (c_j = ||L_j||_1, P = L L^T) is added to
  * each point's error estimate, so mixed_tol still holds for rho.  eps = 0 turns it off; default
