@@ -42,8 +42,12 @@ constexpr int kIBlock = 64;  // rows of P per wave pass (4 MFMA row blocks); the
 // ISPLIT: one 64-row block of P per wave (blockIdx.y), each writing its own
 // partial dots [block][dot][point] that reduce_row_blocks adds in order --
 // enough waves for small point counts (the mixed path's re-evaluation).
+// Two waves per SIMD: left to itself the compiler keeps the 64 x 64 FP64
+// accumulator tile in 128 AGPRs beside ~190 VGPRs (one wave per SIMD); with
+// the bound it fits both in 256 registers without spills (organic60 FP64
+// contraction 183 -> 129 ms per step).
 template <int MODE, bool ISPLIT>
-__global__ __launch_bounds__(256) void contract_f64_kernel(const double* __restrict__ P, int mp,
+__global__ __launch_bounds__(256, 2) void contract_f64_kernel(const double* __restrict__ P, int mp,
                                                            const double* __restrict__ ao, int64_t ld,
                                                            double* __restrict__ dots) {
   constexpr int NT = (MODE == MODE_RHO || MODE == MODE_GRAD) ? 1 : 4;
