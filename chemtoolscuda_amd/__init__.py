@@ -24,7 +24,7 @@ try:
     from . import _chemtools_amd as _native  # noqa: F401
 except ImportError as exc:  # pragma: no cover - exercised only on an unbuilt tree
     raise ImportError(
-        "chemtoolscuda_amd: native extension not built; run `python -m chemtoolscuda_amd.build_ext` "
+        "chemtoolscuda_amd: native extension not built; run `python chemtoolscuda_amd/build_ext.py` "
         f"(or __graft_entry__.build()) first ({exc})") from exc
 
 from .basis import BasisSet, Shell, n_functions  # noqa: E402

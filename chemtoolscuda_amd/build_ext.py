@@ -4,7 +4,7 @@ Outputs (next to this file):
   * ``libchemtools_amd.so``  -- HIP kernels + C ABI (include/chemtools_amd.h), gfx950
   * ``_chemtools_amd<ext>``  -- pybind11 module linking the library via rpath $ORIGIN
 
-Run:  python -m chemtoolscuda_amd.build_ext
+Run:  python chemtoolscuda_amd/build_ext.py
 """
 from __future__ import annotations
 

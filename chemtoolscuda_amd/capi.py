@@ -60,7 +60,7 @@ def lib():
         except ImportError:
             pass
         if not os.path.exists(LIB_PATH):
-            raise ImportError(f"{LIB_PATH} missing: build with `python -m chemtoolscuda_amd.build_ext`")
+            raise ImportError(f"{LIB_PATH} missing: build with `python chemtoolscuda_amd/build_ext.py`")
         L = ctypes.CDLL(LIB_PATH)
         vp, i32, i64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64
         L.ct_mol_create.argtypes = [ctypes.POINTER(BasisDesc), vp, vp, i32, vp, i32, ctypes.POINTER(vp)]

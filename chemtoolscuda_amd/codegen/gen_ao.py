@@ -29,7 +29,7 @@ the explicit table instead, so the two are checked against each other).
 Output: ``csrc/generated/ao_poly.h`` plus a JSON file with static flop counts
 per (shell type, derivative set), used for the AO roofline.
 
-Run:  python -m chemtoolscuda_amd.codegen.gen_ao
+Run:  python chemtoolscuda_amd/codegen/gen_ao.py
 """
 from __future__ import annotations
 
