@@ -119,24 +119,46 @@ def fwd_bwd(torch, capi, synthetic, precision, reps=3):
     return n * reps / (time.perf_counter() - t0)
 
 
+def esp_ncoef(L):
+    """Polynomial coefficients of a primitive pair of total angular momentum L
+    (N = 0..L, total degree max(0, 2N - L) .. N)."""
+    return sum((t + 1) * (t + 2) // 2 for N in range(L + 1) for t in range(max(0, 2 * N - L), N + 1))
+
+
+def esp_pair_flops(L):
+    """Arithmetic of one (point, primitive pair) evaluation of the ESP kernel on
+    its table branch (FMA = 2): geometry and T 9, Boys table index and offset
+    4, Taylor sum 3 K = 24 (K = 8), for L > 0 e^{-dT} 18 + 2 and the downward
+    recursion 3 L, monomials 3 L, polynomial 4 per coefficient, sum over N
+    2 (L + 1).  The far branch (T >= 50 or 60) needs fewer."""
+    boys = 4 + 24 + (20 + 3 * L if L > 0 else 0)
+    return 9 + boys + 3 * L + 4 * esp_ncoef(L) + 2 * (L + 1)
+
+
 def esp_rate(torch, capi, synthetic, reps=3):
     """ESP workload: 30 atoms, M = 312, 2e5 points on the 1.4 x vdW surface;
-    points/s of the FP64 and mixed kernels and the mixed kernel's largest error
-    relative to |V_nuc| + |V_el| (the ESP crosses zero where they cancel)."""
+    points/s of the FP64 and mixed kernels (mixed: pair geometry, Boys function
+    and pair sums in FP64, each pair's Hermite polynomial in FP32), the mixed
+    kernel's largest error relative to |V_nuc| + |V_el| (the ESP crosses zero
+    where they cancel) and the FP64 kernel's VALU roofline."""
     w = synthetic.esp30()
     pts = torch.from_numpy(np.ascontiguousarray(w.points)).cuda()
     n = pts.shape[0]
     h = capi.Handle(w.basis.flat(), w.P, w.charges)
-    res, outs = {}, {}
+    res, outs, kms = {}, {}, {}
     for prec in ("fp64", "mixed"):
         out = torch.empty(n, dtype=torch.float64, device="cuda")
         h.esp(pts.data_ptr(), n, out.data_ptr(), prec)
         torch.cuda.synchronize()
+        h.enable_profiling(True)
         t0 = time.perf_counter()
         for _ in range(reps):
             h.esp(pts.data_ptr(), n, out.data_ptr(), prec)
         torch.cuda.synchronize()
         res[f"{prec}_points_per_s"] = n * reps / (time.perf_counter() - t0)
+        st = h.kernel_stats()
+        h.enable_profiling(False)
+        kms[prec] = st["esp_ms"] / max(st["esp_calls"], 1)
         outs[prec] = out
     xyz = torch.from_numpy(w.basis.atom_xyz).cuda()
     Z = torch.from_numpy(np.asarray(w.charges, dtype=np.float64)).cuda()
@@ -144,6 +166,16 @@ def esp_rate(torch, capi, synthetic, reps=3):
     scale = nuc.abs() + (nuc - outs["fp64"]).abs()
     res["mixed_max_rel_err"] = float(((outs["mixed"] - outs["fp64"]).abs() / scale).max())
     res["mixed_over_fp64"] = res["mixed_points_per_s"] / res["fp64_points_per_s"]
+    res["precision"] = {"fp64": "all FP64",
+                        "mixed": "Boys function F_0..F_L, pair geometry and pair sums FP64; Hermite polynomial FP32"}
+    pairs = h.esp_pairs_by_L()
+    flops_pt = float(sum(int(pairs[L]) * esp_pair_flops(L) for L in range(9)))
+    res["pairs_by_L"] = [int(c) for c in pairs]
+    ach = flops_pt * n / (kms["fp64"] * 1e-3)
+    res["roofline"] = {"bound": "valu", "achieved": ach / 1e12, "peak": PEAK["fp64"] / 1e12, "unit": "TFLOP/s",
+                       "frac": ach / PEAK["fp64"], "traffic": None, "kernel": "esp_kernel<double, 4, 512>",
+                       "avg_launch_ms": kms["fp64"], "flops_per_point": flops_pt,
+                       "mixed_avg_launch_ms": kms["mixed"]}
     res["workload"] = f"esp30: 30 atoms, M={w.basis.nbasis}, {n} points on the 1.4 x vdW surface"
     return res
 

@@ -74,6 +74,8 @@ def lib():
         L.ct_eval.argtypes = [vp, vp, i64, ctypes.c_uint32, i32, ctypes.POINTER(EvalParams), ctypes.POINTER(Outputs),
                               ctypes.POINTER(EvalReport)]
         L.ct_esp.argtypes = [vp, vp, i64, i32, vp]
+        L.ct_boys.argtypes = [vp, i64, i32, i32, vp]
+        L.ct_mol_esp_pairs.argtypes = [vp, vp]
         L.ct_backward.argtypes = [vp, vp, i64, vp, i32, vp, vp]
         L.ct_mol_set_coordinates.argtypes = [vp, vp]
         L.ct_mol_set_density_matrix.argtypes = [vp, vp]
@@ -144,6 +146,14 @@ class Handle:
     def esp(self, points_ptr: int, n: int, out_ptr: int, precision: str = "fp64"):
         check(lib().ct_esp(self.ptr, points_ptr, n, PREC[precision], out_ptr))
 
+    def esp_pairs_by_L(self):
+        """Screened primitive pairs of the ESP pre-contraction by total L (0..8)."""
+        import numpy as np
+
+        out = np.zeros(9, dtype=np.int64)
+        check(lib().ct_mol_esp_pairs(self.ptr, out.ctypes.data))
+        return out
+
     def backward(self, points_ptr: int, n: int, w_ptr: int, dP_ptr, dR_ptr, precision: str = "fp64"):
         check(lib().ct_backward(self.ptr, points_ptr, n, w_ptr, PREC[precision], dP_ptr, dR_ptr))
 
@@ -197,6 +207,17 @@ def read_wavefunction(path: str) -> dict:
     finally:
         L.ct_wavefunction_destroy(h)
     return out
+
+
+def boys(T, m: int, lmax_pairs: int):
+    """F_0..F_m(T) of the device Boys function as the ESP kernels evaluate it
+    for primitive pairs up to total angular momentum lmax_pairs: (len(T), m+1)."""
+    import numpy as np
+
+    t = np.ascontiguousarray(T, dtype=np.float64)
+    F = np.zeros((t.size, m + 1))
+    check(lib().ct_boys(t.ctypes.data, t.size, int(m), int(lmax_pairs), F.ctypes.data))
+    return F
 
 
 def shell_screening_radius(l: int, exps, coefs, eps: float) -> float:

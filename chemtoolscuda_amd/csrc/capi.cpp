@@ -9,6 +9,7 @@
 #include <cstring>
 #include <string>
 
+#include "esp.h"
 #include "mol.h"
 
 struct ct_mol {
@@ -303,6 +304,39 @@ int ct_norm_rows(const double* v, int64_t n, double* out) {
 
 int ct_esp(ct_mol* mol, const double* points, int64_t n, int32_t precision, double* out) {
   return guard([&] { M(mol).esp(points, n, precision, out); });
+}
+int ct_mol_esp_pairs(ct_mol* mol, int64_t* pairs_by_L) {
+  return guard([&] {
+    CT_REQUIRE(pairs_by_L, "null output");
+    M(mol).esp_pairs_by_L(pairs_by_L);
+  });
+}
+int ct_boys(const double* T, int64_t n, int32_t m, int32_t lmax_pairs, double* F) {
+  return guard([&] {
+    CT_REQUIRE(n >= 0, "negative size");
+    CT_REQUIRE(m >= 0 && m <= lmax_pairs && lmax_pairs <= ct::kEspLmax, "need 0 <= m <= lmax_pairs <= 8");
+    if (n == 0) return;
+    CT_REQUIRE(T && F, "null pointer");
+    const bool td = ct::is_device_pointer(T), fd = ct::is_device_pointer(F);
+    const std::vector<double> tab = ct::build_boys_tables();
+    const size_t nf = (size_t)n * (m + 1);
+    ct::DeviceBuffer buf;
+    if (!buf.reserve(sizeof(double) * (tab.size() + (td ? 0 : n) + (fd ? 0 : nf))))
+      throw ct::Error(CT_E_NOMEM, "cannot allocate");
+    double* dtab = static_cast<double*>(buf.get());
+    double* cur = dtab + tab.size();
+    CT_HIP_CHECK(hipMemcpy(dtab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice));
+    const double* dT = T;
+    if (!td) {
+      CT_HIP_CHECK(hipMemcpy(cur, T, sizeof(double) * n, hipMemcpyHostToDevice));
+      dT = cur;
+      cur += n;
+    }
+    double* dF = fd ? F : cur;
+    ct::launch_boys_probe(dT, n, m, lmax_pairs, dtab, dF, nullptr);
+    if (!fd) CT_HIP_CHECK(hipMemcpy(F, dF, sizeof(double) * nf, hipMemcpyDeviceToHost));
+    CT_HIP_CHECK(hipDeviceSynchronize());
+  });
 }
 
 int ct_backward(ct_mol* mol, const double* points, int64_t n, const double* dL_drho, int32_t precision,

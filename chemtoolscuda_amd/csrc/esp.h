@@ -26,22 +26,27 @@
 
 namespace ct {
 
-// Boys-function Taylor table: F_m(T_g) for T_g = g * kBoysStep, g = 0..kBoysNG-1
-// (T < kBoysTmax), m = 0..kBoysM-1.  F_L(T) = sum_{k<=kBoysK} F_{L+k}(T_g) (-dT)^k / k!.
+// Boys-function tables.  F_m(T_g) for T_g = g * step, g = 0 .. ng - 1, and
+// e^{-T_g}; a row holds F_0 .. F_{lmax + K} and then e^{-T_g}.  Below tmax,
+// F_L(T) = sum_{k <= K} F_{L+k}(T_g) (-dT)^k / k! with |dT| <= step / 2: the
+// remainder is at most e^{1/16} (1/16)^9 / 9! < 5e-17 of F_L (|F_m'| < F_m).
+// From tmax on, F_0 = sqrt(pi / T) / 2 and the upward recursion without the
+// e^{-T} term: the dropped terms stay below 2e-17 of every F_n, n <= lmax, at
+// tmax = 50 for lmax = 4 and tmax = 60 for lmax = 8.  Two grids: pairs up to
+// L = 4 (shells up to d) and up to L = 8 (up to g).
+template <int LMAX>
+struct BoysGrid {
+  static constexpr double step = 0.125;
+  static constexpr int K = 8;
+  static constexpr int ncol = LMAX + K + 1;  // F_0 .. F_{LMAX+K}
+  static constexpr int row = ncol + 1;       // + e^{-T_g}
+  static constexpr double tmax = LMAX <= 4 ? 50.0 : 60.0;
+  static constexpr int ng = (int)(tmax / step) + 1;
+  static constexpr int size = ng * row;  // doubles
+};
 constexpr int kEspLmax = 8;
-constexpr int kBoysK = 10;
-constexpr int kBoysM = kEspLmax + kBoysK + 1;
-constexpr double kBoysStep = 0.125;
-constexpr double kBoysTmax = 40.0;
-constexpr int kBoysNG = 321;
-
-// FP32 table of the mixed kernel, same grid: row g holds F_0..F_{kBoysM32-1}(T_g)
-// and e^{-T_g} at column kBoysM32; rows kBoysRow32 floats apart (odd, so that
-// lanes at different grid points read different LDS banks).  Six Taylor terms
-// (k = 0..5, |dT| <= 1/16) leave a truncation of (1/16)^6 / 6! < 1e-10 of F_L.
-constexpr int kBoysK32 = 5;
-constexpr int kBoysM32 = kEspLmax + kBoysK32 + 1;
-constexpr int kBoysRow32 = kBoysM32 + 1;
+// the two tables, one after the other: BoysGrid<4> then BoysGrid<8>
+constexpr int64_t kBoysTableDoubles = BoysGrid<4>::size + BoysGrid<8>::size;
 
 // Number of g coefficients of a pair of total angular momentum L; the device
 // and the host enumerate them in the same nested order:
@@ -69,19 +74,28 @@ struct EspData {
 
 EspData build_esp_pairs(const HostBasis& basis, const std::vector<double>& xyz, const std::vector<double>& P,
                         double screen);
-std::vector<double> build_boys_table();
-std::vector<float> build_boys_table_f32();
+// Both Boys tables (kBoysTableDoubles doubles).
+std::vector<double> build_boys_tables();
 
 // FP32 copy of the g coefficients (esp_ncoef(L) floats per pair, segment by
 // segment) for the mixed-precision kernel; fills EspSegment::off32.
 std::vector<float> esp_coefficients_f32(const std::vector<double>& records, std::vector<EspSegment>& segs);
 
-// records32 == nullptr: all-FP64 kernel (boys_tab).  Otherwise the mixed
-// kernel (boys_tab32): the pair geometry Q - C, T = p |Q - C|^2 and the sum
-// over pairs in FP64; the Boys function, the Hermite polynomial
-// sum_ijk g X^i Y^j Z^k and the pair's sum over N in FP32.
+// records32 == nullptr: all-FP64 kernel.  Otherwise the mixed kernel: the
+// pair geometry Q - C, T = p |Q - C|^2, the Boys function F_0..F_L(T) and the
+// sum over pairs in FP64; the Hermite polynomial sum_ijk g X^i Y^j Z^k and the
+// pair's sum over N in FP32.  lmax: largest pair L of the segments (selects
+// the Boys grid); boys_tabs: build_boys_tables() in device memory; npairs:
+// the pairs of all segments.  nslice > 1 splits the pair list over that many
+// workgroups per point block (part: nslice * n doubles of partial sums, added
+// in slice order); block: 256 or 512 threads.
 void launch_esp(const double* pts, int64_t n, const double* records, const float* records32, const EspSegment* segs,
-                int nseg, const double* atoms, int natom, const double* boys_tab, const float* boys_tab32, double* out,
-                hipStream_t s);
+                int nseg, int64_t npairs, int lmax, const double* atoms, int natom, const double* boys_tabs,
+                double* out, double* part, int nslice, int block, hipStream_t s);
+// F_0..F_m(T) at n values of T (device pointers; F is (n, m + 1) row-major)
+// as the ESP kernels evaluate them for pairs up to lmax_pairs (which selects
+// the grid); 0 <= m <= lmax_pairs <= 8.
+void launch_boys_probe(const double* T, int64_t n, int m, int lmax_pairs, const double* boys_tabs, double* F,
+                       hipStream_t s);
 
 }  // namespace ct

@@ -79,24 +79,25 @@ long double boys_row(long double T, int mmax, long double* F) {
 
 }  // namespace
 
-std::vector<double> build_boys_table() {
-  std::vector<double> tab((size_t)kBoysNG * kBoysM);
-  long double F[kBoysM];
-  for (int g = 0; g < kBoysNG; ++g) {
-    boys_row((long double)g * kBoysStep, kBoysM - 1, F);
-    for (int m = 0; m < kBoysM; ++m) tab[(size_t)g * kBoysM + m] = (double)F[m];
+namespace {
+
+template <int LMAX>
+void fill_boys_grid(double* tab) {
+  using G = BoysGrid<LMAX>;
+  long double F[G::ncol];
+  for (int g = 0; g < G::ng; ++g) {
+    const long double eT = boys_row((long double)g * G::step, G::ncol - 1, F);
+    for (int m = 0; m < G::ncol; ++m) tab[(size_t)g * G::row + m] = (double)F[m];
+    tab[(size_t)g * G::row + G::ncol] = (double)eT;
   }
-  return tab;
 }
 
-std::vector<float> build_boys_table_f32() {
-  std::vector<float> tab((size_t)kBoysNG * kBoysRow32, 0.0f);
-  long double F[kBoysM32];
-  for (int g = 0; g < kBoysNG; ++g) {
-    const long double eT = boys_row((long double)g * kBoysStep, kBoysM32 - 1, F);
-    for (int m = 0; m < kBoysM32; ++m) tab[(size_t)g * kBoysRow32 + m] = (float)F[m];
-    tab[(size_t)g * kBoysRow32 + kBoysM32] = (float)eT;
-  }
+}  // namespace
+
+std::vector<double> build_boys_tables() {
+  std::vector<double> tab((size_t)kBoysTableDoubles);
+  fill_boys_grid<4>(tab.data());
+  fill_boys_grid<8>(tab.data() + BoysGrid<4>::size);
   return tab;
 }
 

@@ -564,15 +564,9 @@ void Mol::prepare_esp() {
     atoms[4 * a + 3] = charges_[a];
   }
   if (!d_boys_.get()) {
-    // FP64 table, then the FP32 table of the mixed kernel
-    const std::vector<double> tab = build_boys_table();
-    const std::vector<float> tab32 = build_boys_table_f32();
-    const size_t b64 = sizeof(double) * tab.size();
-    if (!d_boys_.reserve(b64 + sizeof(float) * tab32.size()))
-      throw Error(CT_E_NOMEM, "cannot allocate the Boys table");
-    CT_HIP_CHECK(hipMemcpy(d_boys_.get(), tab.data(), b64, hipMemcpyHostToDevice));
-    CT_HIP_CHECK(hipMemcpy(static_cast<char*>(d_boys_.get()) + b64, tab32.data(), sizeof(float) * tab32.size(),
-                           hipMemcpyHostToDevice));
+    const std::vector<double> tab = build_boys_tables();
+    if (!d_boys_.reserve(sizeof(double) * tab.size())) throw Error(CT_E_NOMEM, "cannot allocate the Boys table");
+    CT_HIP_CHECK(hipMemcpy(d_boys_.get(), tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice));
   }
   if (!d_esp_rec_.reserve(sizeof(double) * std::max<size_t>(1, d.records.size())) ||
       !d_esp_seg_.reserve(sizeof(EspSegment) * std::max<size_t>(1, d.segs.size())) ||
@@ -587,6 +581,12 @@ void Mol::prepare_esp() {
     CT_HIP_CHECK(hipMemcpy(d_esp_seg_.get(), d.segs.data(), sizeof(EspSegment) * d.segs.size(), hipMemcpyHostToDevice));
   CT_HIP_CHECK(hipMemcpy(d_esp_atoms_.get(), atoms.data(), sizeof(double) * atoms.size(), hipMemcpyHostToDevice));
   esp_nseg_ = (int)d.segs.size();
+  esp_lmax_ = 0;
+  for (int64_t& c : esp_pairs_by_L_) c = 0;
+  for (const EspSegment& sg : d.segs) {
+    esp_lmax_ = std::max(esp_lmax_, (int)sg.L);
+    esp_pairs_by_L_[sg.L] += sg.count;
+  }
   esp_npairs_ = d.npairs - d.screened;
   esp_valid_ = true;
 }
@@ -717,17 +717,36 @@ void Mol::esp(const double* points, int64_t n, int precision, double* out) {
     }
     if (!od) dout = io + 3 * n;
   }
+  // 512-thread workgroups (the Boys table, 45 or 69 KB of LDS, allows three
+  // or two per CU) and the pair list split over enough slices for about
+  // 12 288 workgroups, at least 256 pairs per slice, at most 32 slices:
+  // esp30 (2e5 points, 512-point blocks) 1.30e7 points/s unsplit at 256
+  // threads, 2.27e7 with 8 slices, 2.64e7 with 32
+  const int block = 512;
+  const int64_t nbx = (n + block - 1) / block;
+  const int nslice = (int)std::max<int64_t>(
+      1, std::min<int64_t>({32, (12288 + nbx - 1) / nbx, std::max<int64_t>(1, esp_npairs_ / 256)}));
+  double* part = nullptr;
+  if (nslice > 1) {
+    if (!d_esp_part_.reserve(sizeof(double) * (size_t)nslice * n)) throw Error(CT_E_NOMEM, "cannot allocate ESP partials");
+    part = static_cast<double*>(d_esp_part_.get());
+  }
   timer_.begin(KernelTimer::ESP, stream_);
   const float* rec32 = precision == CT_PREC_MIXED ? static_cast<const float*>(d_esp_rec32_.get()) : nullptr;
   launch_esp(dp, n, static_cast<const double*>(d_esp_rec_.get()), rec32,
-             static_cast<const EspSegment*>(d_esp_seg_.get()), esp_nseg_, static_cast<const double*>(d_esp_atoms_.get()),
-             natom_, static_cast<const double*>(d_boys_.get()),
-             reinterpret_cast<const float*>(static_cast<const double*>(d_boys_.get()) + (size_t)kBoysNG * kBoysM), dout,
-             stream_);
+             static_cast<const EspSegment*>(d_esp_seg_.get()), esp_nseg_, esp_npairs_, esp_lmax_,
+             static_cast<const double*>(d_esp_atoms_.get()), natom_, static_cast<const double*>(d_boys_.get()), dout,
+             part, nslice, block, stream_);
   timer_.end(stream_);
   if (!od) CT_HIP_CHECK(hipMemcpyAsync(out, dout, sizeof(double) * n, hipMemcpyDeviceToHost, stream_));
   CT_HIP_CHECK(hipStreamSynchronize(stream_));
   timer_.collect();
+}
+
+void Mol::esp_pairs_by_L(int64_t* out) {
+  CT_HIP_CHECK(hipStreamSynchronize(stream_));
+  prepare_esp();
+  for (int L = 0; L <= kEspLmax; ++L) out[L] = esp_pairs_by_L_[L];
 }
 
 void Mol::synchronize() { CT_HIP_CHECK(hipStreamSynchronize(stream_)); }

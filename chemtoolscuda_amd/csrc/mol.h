@@ -123,6 +123,9 @@ class Mol {
   void esp(const double* points, int64_t n, int precision, double* out);
   void backward(const double* points, int64_t n, const double* w, int precision, double* dP, double* dR);
   int64_t esp_pairs() const { return esp_npairs_; }
+  // primitive pairs of the ESP pre-contraction by total angular momentum 0..8
+  // (after screening; builds the pre-contraction if needed)
+  void esp_pairs_by_L(int64_t* out);
 
  private:
   void upload_geometry();
@@ -176,9 +179,11 @@ class Mol {
   // ESP: Hermite pre-contraction of P, rebuilt when P or the centres change
   bool esp_valid_ = false;
   int esp_nseg_ = 0;
+  int esp_lmax_ = 0;  // largest pair L of the ESP segments (selects the Boys grid)
+  int64_t esp_pairs_by_L_[9] = {0};
   int64_t esp_npairs_ = 0;
   DeviceBuffer d_tiles_;    // mixed adjoint: upper-triangle Gram tile list
-  DeviceBuffer d_esp_rec_, d_esp_rec32_, d_esp_seg_, d_esp_atoms_, d_boys_, d_esp_io_;
+  DeviceBuffer d_esp_rec_, d_esp_rec32_, d_esp_seg_, d_esp_atoms_, d_boys_, d_esp_io_, d_esp_part_;
   DeviceBuffer d_adj_;     // adjoint: G, T, slice partials
   void prepare_esp();
 };

@@ -34,6 +34,7 @@
  *                            Molecule::compute_electron_density_cubic (src/pymolecule.cu:59-71)
  *   ct_norm_rows             compute_norm_of_3d_vector (src/evaluate_densbased.cu:10-41)
  *   ct_esp                   compute_electrostatic_potential_over_points (src/evaluate_electrostatic.cu:10161-10372)
+ *   ct_boys                  boys_function<M> (include/boys_functions.cuh:54-72)
  *   ct_backward              (no reference counterpart: adjoint of rho w.r.t. P and the centres)
  */
 #ifndef CHEMTOOLS_AMD_H_
@@ -210,8 +211,22 @@ int ct_eval_cubic(ct_mol* mol, const double* origin, const double* axes, const i
 /* |v| of each row of an (n,3) array. */
 int ct_norm_rows(const double* v, int64_t n, double* out);
 
-/* Molecular electrostatic potential at (n,3) row-major points. */
+/* Molecular electrostatic potential at (n,3) row-major points.  CT_PREC_FP64:
+ * all in FP64.  CT_PREC_MIXED: the pair geometry, the Boys function and the
+ * sums over primitive pairs in FP64, each pair's Hermite polynomial in FP32. */
 int ct_esp(ct_mol* mol, const double* points, int64_t n, int32_t precision, double* out);
+
+/* Primitive pairs of the molecule's ESP pre-contraction, by total angular
+ * momentum L = 0..8 (pairs_by_L: 9 entries), after screening: the units of
+ * work of ct_esp per point.  (No reference counterpart: the reference
+ * evaluates every shell pair's integrals per point.) */
+int ct_mol_esp_pairs(ct_mol* mol, int64_t* pairs_by_L);
+
+/* The device Boys function F_0..F_m(T) at n values of T, as the ESP kernels
+ * evaluate it for primitive pairs of total angular momentum up to lmax_pairs
+ * (which selects the table grid); 0 <= m <= lmax_pairs <= 8; F is (n, m+1)
+ * row-major.  T >= 0; host or device pointers. */
+int ct_boys(const double* T, int64_t n, int32_t m, int32_t lmax_pairs, double* F);
 
 /* Adjoint of rho: given dL/drho at the points, dL/dP (nbasis x nbasis,
  * row-major, accumulated as sum_g w_g phi_i phi_j) and dL/dR (natom x 3).

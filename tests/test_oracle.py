@@ -46,6 +46,15 @@ def test_boys_against_hyp1f1_golden():
     assert np.allclose(oracle.boys(8, [0.0])[0], 1.0 / (2 * np.arange(9) + 1), rtol=0, atol=1e-15)
 
 
+def test_boys_against_mpmath_golden():
+    """40-digit mpmath vectors (tests/golden/make_boys_mp.py): on and between
+    the 0.125 grid points, around T = 50 and 60, tiny and large T."""
+    g = np.load(os.path.join(GOLD, "boys_mp.npz"))
+    F = oracle.boys(8, g["T"])
+    err = np.abs(F - g["F"]) / g["F"]
+    assert err.max() < 4e-15, err.max()
+
+
 def test_ao_closed_forms_s_and_p():
     a, c = 0.83, 0.6
     basis = BasisSet([Shell(0, 0, [a], [c]), Shell(0, 1, [a], [c])], np.array([[0.1, -0.2, 0.3]]))
