@@ -19,6 +19,7 @@ Q = {
 }
 WIDTH = {"grad": 3, "hess": 9}
 PREC = {"fp64": 0, "mixed": 1}
+LAYOUT = {"row": 0, "col": 1}
 
 
 class BasisDesc(ctypes.Structure):
@@ -30,7 +31,7 @@ class BasisDesc(ctypes.Structure):
 class EvalParams(ctypes.Structure):
     _fields_ = [("gge_a", ctypes.c_double), ("gge_b", ctypes.c_double), ("ked_a", ctypes.c_double),
                 ("ked_alpha", ctypes.c_double), ("mixed_tol", ctypes.c_double), ("rho_floor", ctypes.c_double),
-                ("route", ctypes.c_int32)]
+                ("route", ctypes.c_int32), ("out_layout", ctypes.c_int32)]
 
 
 class Outputs(ctypes.Structure):
@@ -71,12 +72,12 @@ def lib():
         L.ct_wavefunction_copy.argtypes = [vp] * 11
         L.ct_wavefunction_destroy.argtypes = [vp]
         L.ct_mol_destroy.argtypes = [vp]
-        L.ct_eval.argtypes = [vp, vp, i64, ctypes.c_uint32, i32, ctypes.POINTER(EvalParams), ctypes.POINTER(Outputs),
-                              ctypes.POINTER(EvalReport)]
-        L.ct_esp.argtypes = [vp, vp, i64, i32, vp]
+        L.ct_eval.argtypes = [vp, vp, i64, i32, ctypes.c_uint32, i32, ctypes.POINTER(EvalParams),
+                              ctypes.POINTER(Outputs), ctypes.POINTER(EvalReport)]
+        L.ct_esp.argtypes = [vp, vp, i64, i32, i32, vp]
         L.ct_boys.argtypes = [vp, i64, i32, i32, vp]
         L.ct_mol_esp_pairs.argtypes = [vp, vp]
-        L.ct_backward.argtypes = [vp, vp, i64, vp, i32, vp, vp]
+        L.ct_backward.argtypes = [vp, vp, i64, i32, vp, i32, vp, vp]
         L.ct_mol_set_coordinates.argtypes = [vp, vp]
         L.ct_mol_set_density_matrix.argtypes = [vp, vp]
         L.ct_mol_set_workspace_limit.argtypes = [vp, i64]
@@ -130,8 +131,11 @@ class Handle:
                 pass
             self.ptr = None
 
-    def eval(self, points_ptr: int, n: int, outputs: dict, precision: str = "fp64", params: dict | None = None):
-        """outputs: {quantity: pointer}; pointers may be host or device."""
+    def eval(self, points_ptr: int, n: int, outputs: dict, precision: str = "fp64", params: dict | None = None,
+             layout: str = "row"):
+        """outputs: {quantity: pointer}; pointers may be host or device.  layout:
+        points (n, 3) "row" or (3, n) "col"; params["out_layout"] = 1 gives grad
+        (3, n) and hess (9, n)."""
         q = 0
         o = Outputs()
         for name, ptr in outputs.items():
@@ -139,12 +143,12 @@ class Handle:
             setattr(o, name, ptr)
         par = EvalParams(**(params or {}))
         rep = EvalReport()
-        check(lib().ct_eval(self.ptr, points_ptr, n, q, PREC[precision], ctypes.byref(par), ctypes.byref(o),
-                            ctypes.byref(rep)))
+        check(lib().ct_eval(self.ptr, points_ptr, n, LAYOUT[layout], q, PREC[precision], ctypes.byref(par),
+                            ctypes.byref(o), ctypes.byref(rep)))
         return {k: getattr(rep, k) for k, _ in EvalReport._fields_}
 
-    def esp(self, points_ptr: int, n: int, out_ptr: int, precision: str = "fp64"):
-        check(lib().ct_esp(self.ptr, points_ptr, n, PREC[precision], out_ptr))
+    def esp(self, points_ptr: int, n: int, out_ptr: int, precision: str = "fp64", layout: str = "row"):
+        check(lib().ct_esp(self.ptr, points_ptr, n, LAYOUT[layout], PREC[precision], out_ptr))
 
     def esp_pairs_by_L(self):
         """Screened primitive pairs of the ESP pre-contraction by total L (0..8)."""
@@ -154,8 +158,9 @@ class Handle:
         check(lib().ct_mol_esp_pairs(self.ptr, out.ctypes.data))
         return out
 
-    def backward(self, points_ptr: int, n: int, w_ptr: int, dP_ptr, dR_ptr, precision: str = "fp64"):
-        check(lib().ct_backward(self.ptr, points_ptr, n, w_ptr, PREC[precision], dP_ptr, dR_ptr))
+    def backward(self, points_ptr: int, n: int, w_ptr: int, dP_ptr, dR_ptr, precision: str = "fp64",
+                 layout: str = "row"):
+        check(lib().ct_backward(self.ptr, points_ptr, n, LAYOUT[layout], w_ptr, PREC[precision], dP_ptr, dR_ptr))
 
     def set_coordinates(self, ptr: int):
         check(lib().ct_mol_set_coordinates(self.ptr, ptr))

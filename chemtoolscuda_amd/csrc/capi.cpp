@@ -248,7 +248,7 @@ int ct_shell_screening_radius(int32_t l, int32_t nprim, const double* exps, cons
   });
 }
 
-int ct_eval(ct_mol* mol, const double* points, int64_t n, uint32_t quantities, int32_t precision,
+int ct_eval(ct_mol* mol, const double* points, int64_t n, int32_t layout, uint32_t quantities, int32_t precision,
             const ct_eval_params* params, const ct_outputs* out, ct_eval_report* report) {
   return guard([&] {
     CT_REQUIRE(out != nullptr, "null outputs");
@@ -256,7 +256,7 @@ int ct_eval(ct_mol* mol, const double* points, int64_t n, uint32_t quantities, i
     if (params) p = *params;
     if (p.mixed_tol <= 0) p.mixed_tol = 1e-6;
     if (p.rho_floor <= 0) p.rho_floor = 1e-10;
-    M(mol).eval(points, n, quantities, precision, p, *out, report);
+    M(mol).eval(points, n, layout, quantities, precision, p, *out, report);
   });
 }
 
@@ -276,7 +276,7 @@ int ct_eval_cubic(ct_mol* mol, const double* origin, const double* axes, const i
     if (params) p = *params;
     if (p.mixed_tol <= 0) p.mixed_tol = 1e-6;
     if (p.rho_floor <= 0) p.rho_floor = 1e-10;
-    M(mol).eval(nullptr, shape[0] * shape[1] * shape[2], quantities, precision, p, *out, report, &g);
+    M(mol).eval(nullptr, shape[0] * shape[1] * shape[2], CT_LAYOUT_ROW, quantities, precision, p, *out, report, &g);
   });
 }
 
@@ -302,8 +302,8 @@ int ct_norm_rows(const double* v, int64_t n, double* out) {
   });
 }
 
-int ct_esp(ct_mol* mol, const double* points, int64_t n, int32_t precision, double* out) {
-  return guard([&] { M(mol).esp(points, n, precision, out); });
+int ct_esp(ct_mol* mol, const double* points, int64_t n, int32_t layout, int32_t precision, double* out) {
+  return guard([&] { M(mol).esp(points, n, layout, precision, out); });
 }
 int ct_mol_esp_pairs(ct_mol* mol, int64_t* pairs_by_L) {
   return guard([&] {
@@ -339,9 +339,9 @@ int ct_boys(const double* T, int64_t n, int32_t m, int32_t lmax_pairs, double* F
   });
 }
 
-int ct_backward(ct_mol* mol, const double* points, int64_t n, const double* dL_drho, int32_t precision,
-                double* dL_dP, double* dL_dR) {
-  return guard([&] { M(mol).backward(points, n, dL_drho, precision, dL_dP, dL_dR); });
+int ct_backward(ct_mol* mol, const double* points, int64_t n, int32_t layout, const double* dL_drho,
+                int32_t precision, double* dL_dP, double* dL_dR) {
+  return guard([&] { M(mol).backward(points, n, layout, dL_drho, precision, dL_dP, dL_dR); });
 }
 
 int ct_mol_synchronize(ct_mol* mol) {

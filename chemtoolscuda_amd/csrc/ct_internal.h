@@ -118,6 +118,8 @@ struct FinalizeArgs {
   uint32_t q;
   ct_eval_params par;
   ct_outputs out;   // device pointers (full-length arrays, indexed with out_off)
+  int64_t grad_ld;  // 0: grad (n, 3) rows; else component k of output point o at grad[k * grad_ld + o]
+  int64_t hess_ld;  // 0: hess (n, 3, 3) rows; else component 3k+l at hess[(3k + l) * hess_ld + o]
 };
 
 void launch_ao_f64(int mode, int lmax, const ShellMeta* meta, const ShellGeom* geom, const PrimData* prim, int nshell,
@@ -232,7 +234,11 @@ struct FlagArgs {
   double trunc;          // screened factor route: + trunc * sqrt|rho| (bound of the skipped AO blocks)
 };
 void launch_finalize_mixed(const FinalizeArgs& a, const FlagArgs& f, hipStream_t s);
-void launch_gather_points(const double* pts, const int32_t* index, int64_t n, double* out, hipStream_t s);
+// out (n, 3) row-major: the points index[k] (index == nullptr: k) of pts,
+// which is (m, 3) row-major (col_stride = 0) or column-major with x, y, z of
+// point p at pts[p], pts[col_stride + p], pts[2 col_stride + p]
+void launch_gather_points(const double* pts, const int32_t* index, int64_t n, double* out, hipStream_t s,
+                          int64_t col_stride = 0);
 // inv[perm[i]] = i for i < n
 void launch_invert_permutation(const int32_t* perm, int64_t n, int32_t* inv, hipStream_t s);
 

@@ -42,14 +42,18 @@ __device__ void emit_outputs(const FinalizeArgs& a, int64_t o, const double* dot
   }
   const uint32_t q = a.q;
   if (q & CT_Q_RHO) a.out.rho[o] = rho;
+  // (n, 3) / (n, 3, 3) rows, or components grad_ld / hess_ld apart
   if (q & CT_Q_GRAD)
-    for (int k = 0; k < 3; ++k) a.out.grad[3 * o + k] = g[k];
+    for (int k = 0; k < 3; ++k) a.out.grad[a.grad_ld ? k * a.grad_ld + o : 3 * o + k] = g[k];
   if (q & CT_Q_HESS) {
     // h_kl slots 4..9 = xx,xy,xz,yy,yz,zz ; t_kl slots 10..15 = xx,xy,yy,xz,yz,zz
     const int hs[3][3] = {{4, 5, 6}, {5, 7, 8}, {6, 8, 9}};
     const int ts[3][3] = {{10, 11, 13}, {11, 12, 14}, {13, 14, 15}};
     for (int k = 0; k < 3; ++k)
-      for (int l = 0; l < 3; ++l) a.out.hess[9 * o + 3 * k + l] = 2.0 * (dot[hs[k][l]] + dot[ts[k][l]]);
+      for (int l = 0; l < 3; ++l) {
+        const int c = 3 * k + l;
+        a.out.hess[a.hess_ld ? c * a.hess_ld + o : 9 * o + c] = 2.0 * (dot[hs[k][l]] + dot[ts[k][l]]);
+      }
   }
   if (q & CT_Q_LAP) a.out.lap[o] = lap;
   if (q & CT_Q_TAU_PD) a.out.tau_pd[o] = tau;
@@ -143,15 +147,16 @@ __global__ __launch_bounds__(256) void invert_permutation_kernel(const int32_t* 
   if (i < n) inv[perm[i]] = (int32_t)i;
 }
 
-__global__ __launch_bounds__(256) void gather_points_kernel(const double* __restrict__ pts,
+// col_stride = 0: pts is (m, 3) row-major; else x, y, z of point p are at
+// pts[p], pts[col_stride + p], pts[2 col_stride + p]
+__global__ __launch_bounds__(256) void gather_points_kernel(const double* __restrict__ pts, int64_t col_stride,
                                                             const int32_t* __restrict__ index, int64_t n,
                                                             double* __restrict__ out) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
-  const int64_t p = index[k];
-  out[3 * k] = pts[3 * p];
-  out[3 * k + 1] = pts[3 * p + 1];
-  out[3 * k + 2] = pts[3 * p + 2];
+  const int64_t p = index ? (int64_t)index[k] : k;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) out[3 * k + c] = col_stride ? pts[c * col_stride + p] : pts[3 * p + c];
 }
 
 __global__ __launch_bounds__(256) void cubic_points_kernel(CubicGrid g, int64_t start, int64_t n,
@@ -188,9 +193,11 @@ void launch_finalize_mixed(const FinalizeArgs& a, const FlagArgs& f, hipStream_t
   CT_HIP_CHECK(hipGetLastError());
 }
 
-void launch_gather_points(const double* pts, const int32_t* index, int64_t n, double* out, hipStream_t s) {
+void launch_gather_points(const double* pts, const int32_t* index, int64_t n, double* out, hipStream_t s,
+                          int64_t col_stride) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(gather_points_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, pts, index, n, out);
+  hipLaunchKernelGGL(gather_points_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, pts, col_stride, index,
+                     n, out);
   CT_HIP_CHECK(hipGetLastError());
 }
 

@@ -591,15 +591,19 @@ void Mol::prepare_esp() {
   esp_valid_ = true;
 }
 
-void Mol::backward(const double* points, int64_t n, const double* w, int precision, double* dP, double* dR) {
+void Mol::backward(const double* points, int64_t n, int layout, const double* w, int precision, double* dP,
+                   double* dR) {
   CT_REQUIRE(n >= 0, "negative number of points");
+  CT_REQUIRE(layout == CT_LAYOUT_ROW || layout == CT_LAYOUT_COL, "unknown points layout");
   CT_REQUIRE(precision == CT_PREC_FP64 || precision == CT_PREC_MIXED, "unknown precision");
   CT_REQUIRE(n == 0 || (points != nullptr && w != nullptr), "null points or weights");
   CT_HIP_CHECK(hipStreamSynchronize(stream_));
   const int mode = MODE_GRAD;
   const int nder = mode_nder(mode);
   const bool mixed = precision == CT_PREC_MIXED;
-  const int64_t per_point = (int64_t)nder * Mp_ * (mixed ? 4 : 8) + 24 + 8;
+  const bool col = layout == CT_LAYOUT_COL;
+  // AO block, staged points, staged weights (+ host columns)
+  const int64_t per_point = (int64_t)nder * Mp_ * (mixed ? 4 : 8) + 24 + 8 + 24;
   int64_t Nc = n > 0 ? chunk_points(per_point, n) : kPointAlign;
   while (!d_ws_.reserve((size_t)Nc * per_point)) {
     if (Nc <= kPointAlign) throw Error(CT_E_NOMEM, "cannot allocate the adjoint workspace");
@@ -632,6 +636,7 @@ void Mol::backward(const double* points, int64_t n, const double* w, int precisi
   char* ws = static_cast<char*>(d_ws_.get());
   double* pts_stage = reinterpret_cast<double*>(ws + (size_t)nder * Mp_ * Nc * (mixed ? 4 : 8));
   double* w_stage = pts_stage + 3 * (size_t)Nc;
+  double* col_stage = w_stage + (size_t)Nc;
   const bool pd = is_device_pointer(points), wd = is_device_pointer(w);
   const auto* meta = static_cast<const ShellMeta*>(d_meta_.get());
   const auto* geom = static_cast<const ShellGeom*>(d_geom_.get());
@@ -641,7 +646,18 @@ void Mol::backward(const double* points, int64_t n, const double* w, int precisi
     const int64_t ld = round_up(nc, kPointAlign);
     const double* pts = points + 3 * start;
     const double* wc = w + start;
-    if (!pd) {
+    if (col) {
+      const double* cols = points + start;
+      int64_t stride = n;
+      if (!pd) {
+        CT_HIP_CHECK(hipMemcpy2DAsync(col_stage, sizeof(double) * nc, points + start, sizeof(double) * n,
+                                      sizeof(double) * nc, 3, hipMemcpyHostToDevice, stream_));
+        cols = col_stage;
+        stride = nc;
+      }
+      launch_gather_points(cols, nullptr, nc, pts_stage, stream_, stride);
+      pts = pts_stage;
+    } else if (!pd) {
       CT_HIP_CHECK(hipMemcpyAsync(pts_stage, pts, sizeof(double) * 3 * nc, hipMemcpyHostToDevice, stream_));
       pts = pts_stage;
     }
@@ -698,20 +714,32 @@ void Mol::backward(const double* points, int64_t n, const double* w, int precisi
   }
 }
 
-void Mol::esp(const double* points, int64_t n, int precision, double* out) {
+void Mol::esp(const double* points, int64_t n, int layout, int precision, double* out) {
   CT_REQUIRE(n >= 0, "negative number of points");
   CT_REQUIRE(precision == CT_PREC_FP64 || precision == CT_PREC_MIXED, "unknown precision");
+  CT_REQUIRE(layout == CT_LAYOUT_ROW || layout == CT_LAYOUT_COL, "unknown points layout");
   if (n == 0) return;
   CT_REQUIRE(points != nullptr && out != nullptr, "null pointer");
   CT_HIP_CHECK(hipStreamSynchronize(stream_));
   prepare_esp();
   const bool pd = is_device_pointer(points), od = is_device_pointer(out);
+  const bool col = layout == CT_LAYOUT_COL;
   const double* dp = points;
   double* dout = out;
-  if (!pd || !od) {
-    if (!d_esp_io_.reserve(sizeof(double) * 4 * (size_t)n)) throw Error(CT_E_NOMEM, "cannot allocate ESP staging");
+  if (!pd || !od || col) {
+    // [0, 3n) row-major points, [3n, 4n) output, [4n, 7n) host columns
+    const size_t nd = (size_t)n * (col && !pd ? 7 : 4);
+    if (!d_esp_io_.reserve(sizeof(double) * nd)) throw Error(CT_E_NOMEM, "cannot allocate ESP staging");
     double* io = static_cast<double*>(d_esp_io_.get());
-    if (!pd) {
+    if (col) {
+      const double* cols = points;
+      if (!pd) {
+        CT_HIP_CHECK(hipMemcpyAsync(io + 4 * n, points, sizeof(double) * 3 * n, hipMemcpyHostToDevice, stream_));
+        cols = io + 4 * n;
+      }
+      launch_gather_points(cols, nullptr, n, io, stream_, n);
+      dp = io;
+    } else if (!pd) {
       CT_HIP_CHECK(hipMemcpyAsync(io, points, sizeof(double) * 3 * n, hipMemcpyHostToDevice, stream_));
       dp = io;
     }
@@ -759,8 +787,8 @@ int64_t Mol::chunk_points(int64_t per_point, int64_t n) const {
 
 // FP64 evaluation of `count` points of the current chunk (index = chunk-
 // relative positions), written into the chunk's outputs through the index map.
-void Mol::reevaluate(int mode, const double* pts, int64_t count, const int32_t* index, const FinalizeArgs& fin,
-                     bool factor) {
+void Mol::reevaluate(int mode, const double* pts, int64_t col_stride, int64_t count, const int32_t* index,
+                     const FinalizeArgs& fin, bool factor) {
   const int nder = mode_nder(mode), ndot = mode_ndot(mode);
   const int64_t ld = round_up(count, kPointAlign);
   // few points: one wave per (16 or 64 points, 64-row block of P, or
@@ -776,7 +804,7 @@ void Mol::reevaluate(int mode, const double* pts, int64_t count, const int32_t* 
   double* dots = ao + (size_t)nder * Mp_ * ld;
   double* gp = dots + (size_t)ndot * ld;
   double* part = gp + 3 * (size_t)ld;
-  launch_gather_points(pts, index, count, gp, stream_);
+  launch_gather_points(pts, index, count, gp, stream_, col_stride);
   launch_ao_f64(mode, lmax_, static_cast<const ShellMeta*>(d_meta_.get()), static_cast<const ShellGeom*>(d_geom_.get()),
                 static_cast<const PrimData*>(d_prim_.get()), nshell_, M_, Mp_, gp, count, ld, ao, 700.0, stream_);
   if (factor && split)
@@ -798,11 +826,13 @@ void Mol::reevaluate(int mode, const double* pts, int64_t count, const int32_t* 
   launch_finalize(f, stream_);
 }
 
-void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const ct_eval_params& par,
+void Mol::eval(const double* points, int64_t n, int layout, uint32_t q, int precision, const ct_eval_params& par,
                const ct_outputs& out, ct_eval_report* rep, const CubicGrid* cube) {
   CT_REQUIRE(n >= 0, "negative number of points");
   CT_REQUIRE(q != 0, "no quantity requested");
   CT_REQUIRE(precision == CT_PREC_FP64 || precision == CT_PREC_MIXED, "unknown precision");
+  CT_REQUIRE(layout == CT_LAYOUT_ROW || layout == CT_LAYOUT_COL, "unknown points layout");
+  CT_REQUIRE(par.out_layout == CT_LAYOUT_ROW || par.out_layout == CT_LAYOUT_COL, "unknown output layout");
   for (const OutSlot& s : kSlots)
     if ((q & s.bit) && out.*(s.field) == nullptr) throw Error(CT_E_INVALID, "output array missing for a requested quantity");
   if (rep) *rep = ct_eval_report{n, 0, 0, 0.0, -1, 1.0};
@@ -820,6 +850,11 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
   const int nk = Mp_ / 32, nwords = (nk + 31) / 32;
   const int tp = screened ? factor_tile_points(mode) : 64;
   const bool pts_dev = cube == nullptr && is_device_pointer(points);
+  // column-major points: (3, n), x, y, z of point p at points[p], points[n + p],
+  // points[2n + p]; each chunk is gathered into row-major staging (host
+  // arrays through a column staging buffer of the chunk's size)
+  const bool col_in = cube == nullptr && layout == CT_LAYOUT_COL;
+  const bool col_out = par.out_layout == CT_LAYOUT_COL;
   constexpr size_t NS = sizeof(kSlots) / sizeof(kSlots[0]);
   bool out_dev[NS];
   int stage_width = 0;
@@ -835,8 +870,10 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
   for (size_t k = 0; k < NS; ++k) all_dev = all_dev && out_dev[k];
   const bool batch = mixed && all_dev;
   const int64_t ao_bytes_pt = (int64_t)nder * Mp_ * (mixed ? 4 : 8);
-  const int64_t per_point = mixed ? ao_bytes_pt + (int64_t)nrt * (ndot + 1) * 8 + 4 + 24 + stage_width * 8
-                                  : ao_bytes_pt + (int64_t)ndot * 8 + 24 + stage_width * 8;
+  const int64_t col_stage_pt = col_in && !pts_dev ? 24 : 0;
+  const int64_t per_point = (mixed ? ao_bytes_pt + (int64_t)nrt * (ndot + 1) * 8 + 4 + 24 + stage_width * 8
+                                   : ao_bytes_pt + (int64_t)ndot * 8 + 24 + stage_width * 8) +
+                            col_stage_pt;
   // screened: keys (2 x 4 B), indices (2 x 4 B), sorted points (24 B) per
   // point; k-step list (2 nk B), count and mask (4 + 4 nwords B) per tile
   const int64_t per_point_s = screened ? 40 + (4 * nk + 4 + 4 * nwords + tp - 1) / tp : 0;
@@ -866,6 +903,8 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
   cur += (size_t)Nc * stage_width * 8;
   int32_t* flag_idx = reinterpret_cast<int32_t*>(cur);
   cur += (size_t)Nc * 4;
+  double* col_stage = reinterpret_cast<double*>(cur);
+  cur += (size_t)Nc * col_stage_pt;
   // screened layout buffers (256-byte aligned pieces)
   auto take = [&](size_t bytes) {
     char* p = cur;
@@ -916,6 +955,17 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
     if (cube) {
       launch_cubic_points(*cube, start, nc, pts_stage, stream_);
       pts = pts_stage;
+    } else if (col_in) {
+      const double* cols = points + start;
+      int64_t stride = n;
+      if (!pts_dev) {
+        CT_HIP_CHECK(hipMemcpy2DAsync(col_stage, sizeof(double) * nc, points + start, sizeof(double) * n,
+                                      sizeof(double) * nc, 3, hipMemcpyHostToDevice, stream_));
+        cols = col_stage;
+        stride = nc;
+      }
+      launch_gather_points(cols, nullptr, nc, pts_stage, stream_, stride);
+      pts = pts_stage;
     } else if (!pts_dev) {
       CT_HIP_CHECK(hipMemcpyAsync(pts_stage, pts, sizeof(double) * 3 * nc, hipMemcpyHostToDevice, stream_));
       pts = pts_stage;
@@ -935,12 +985,17 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
     for (size_t k = 0; k < NS; ++k) {
       if (!(q & kSlots[k].bit)) continue;
       double* user = out.*(kSlots[k].field);
+      const int w = kSlots[k].width;
+      // column layout of grad / hess: component c of the chunk's point p at
+      // c * ld + p (ld = n in the caller's array, nc in staging)
+      const bool cols = col_out && w > 1;
       if (out_dev[k]) {
-        fa.out.*(kSlots[k].field) = user + start * kSlots[k].width;
+        fa.out.*(kSlots[k].field) = user + start * (cols ? 1 : w);
       } else {
         fa.out.*(kSlots[k].field) = stage_cursor;
-        stage_cursor += (size_t)kSlots[k].width * nc;
+        stage_cursor += (size_t)w * nc;
       }
+      if (cols) (kSlots[k].bit == CT_Q_GRAD ? fa.grad_ld : fa.hess_ld) = out_dev[k] ? n : nc;
     }
 
     if (!mixed) {
@@ -1011,7 +1066,7 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
         const int64_t count = (int64_t) * reinterpret_cast<int32_t*>(h_small);
         if (count > 0) {
           timer_.begin(KernelTimer::REEVAL, stream_);
-          reevaluate(mode, pts, count, flag_idx, fa, use_factor);
+          reevaluate(mode, pts, 0, count, flag_idx, fa, use_factor);
           timer_.end(stream_);
           reevaluated += count;
         }
@@ -1020,8 +1075,13 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
     }
     for (size_t k = 0; k < NS; ++k) {
       if (!(q & kSlots[k].bit) || out_dev[k]) continue;
-      CT_HIP_CHECK(hipMemcpyAsync(out.*(kSlots[k].field) + start * kSlots[k].width, fa.out.*(kSlots[k].field),
-                                  sizeof(double) * kSlots[k].width * nc, hipMemcpyDeviceToHost, stream_));
+      const int w = kSlots[k].width;
+      if (col_out && w > 1)
+        CT_HIP_CHECK(hipMemcpy2DAsync(out.*(kSlots[k].field) + start, sizeof(double) * n, fa.out.*(kSlots[k].field),
+                                      sizeof(double) * nc, sizeof(double) * nc, w, hipMemcpyDeviceToHost, stream_));
+      else
+        CT_HIP_CHECK(hipMemcpyAsync(out.*(kSlots[k].field) + start * w, fa.out.*(kSlots[k].field),
+                                    sizeof(double) * w * nc, hipMemcpyDeviceToHost, stream_));
     }
     ++chunks;
   }
@@ -1037,6 +1097,7 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
       fa.par = par;
       for (size_t k = 0; k < NS; ++k)
         if (q & kSlots[k].bit) fa.out.*(kSlots[k].field) = out.*(kSlots[k].field);
+      if (col_out) fa.grad_ld = fa.hess_ld = n;
       // the FP64 re-evaluation runs in chunks sized like the FP64 path's, on
       // the flagged points in index order (appended in any order; in order,
       // points given in a spatial order stay so for the AO kernel's waves)
@@ -1053,7 +1114,7 @@ void Mol::eval(const double* points, int64_t n, uint32_t q, int precision, const
       }
       timer_.begin(KernelTimer::REEVAL, stream_);
       for (int64_t s0 = 0; s0 < count; s0 += c64)
-        reevaluate(mode, points, std::min(c64, count - s0), sorted + s0, fa, use_factor);
+        reevaluate(mode, points, col_in ? n : 0, std::min(c64, count - s0), sorted + s0, fa, use_factor);
       timer_.end(stream_);
       reevaluated = count;
     }

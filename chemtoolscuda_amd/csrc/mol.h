@@ -116,12 +116,13 @@ class Mol {
   void synchronize();
   KernelTimer& timer() { return timer_; }
 
-  // points: (n, 3) host or device; or points == nullptr and cube != nullptr:
-  // the points of a uniform grid, generated on the device chunk by chunk.
-  void eval(const double* points, int64_t n, uint32_t q, int precision, const ct_eval_params& par,
+  // points: host or device, (n, 3) row-major (layout CT_LAYOUT_ROW) or (3, n)
+  // (CT_LAYOUT_COL); or points == nullptr and cube != nullptr: the points of
+  // a uniform grid, generated on the device chunk by chunk.
+  void eval(const double* points, int64_t n, int layout, uint32_t q, int precision, const ct_eval_params& par,
             const ct_outputs& out, ct_eval_report* rep, const CubicGrid* cube = nullptr);
-  void esp(const double* points, int64_t n, int precision, double* out);
-  void backward(const double* points, int64_t n, const double* w, int precision, double* dP, double* dR);
+  void esp(const double* points, int64_t n, int layout, int precision, double* out);
+  void backward(const double* points, int64_t n, int layout, const double* w, int precision, double* dP, double* dR);
   int64_t esp_pairs() const { return esp_npairs_; }
   // primitive pairs of the ESP pre-contraction by total angular momentum 0..8
   // (after screening; builds the pre-contraction if needed)
@@ -135,8 +136,9 @@ class Mol {
   // the threshold change
   void build_screen();
   int64_t chunk_points(int64_t per_point, int64_t n) const;
-  void reevaluate(int mode, const double* pts, int64_t count, const int32_t* index, const FinalizeArgs& fin,
-                  bool factor);
+  // pts: (m, 3) row-major (col_stride 0) or column-major with that stride
+  void reevaluate(int mode, const double* pts, int64_t col_stride, int64_t count, const int32_t* index,
+                  const FinalizeArgs& fin, bool factor);
   // whether this mode runs on the factor of P (ct_eval_params.route; the FP64
   // path only when asked)
   bool factor_route(int mode, int route, bool mixed) const;

@@ -184,7 +184,7 @@ class Molecule {
     int st;
     {
       py::gil_scoped_release nogil;
-      st = ct_eval(h_, pts.data(), n, bit, prec, &par, &out, &rep);
+      st = ct_eval(h_, pts.data(), n, CT_LAYOUT_ROW, bit, prec, &par, &out, &rep);
     }
     check(st);
     last_report_ = rep;
@@ -198,7 +198,7 @@ class Molecule {
     int st;
     {
       py::gil_scoped_release nogil;
-      st = ct_esp(h_, pts.data(), n, prec, arr.mutable_data());
+      st = ct_esp(h_, pts.data(), n, CT_LAYOUT_ROW, prec, arr.mutable_data());
     }
     check(st);
     return arr;
@@ -214,7 +214,7 @@ class Molecule {
     int st;
     {
       py::gil_scoped_release nogil;
-      st = ct_backward(h_, pts.data(), n, w.data(), prec, dP.mutable_data(), dR.mutable_data());
+      st = ct_backward(h_, pts.data(), n, CT_LAYOUT_ROW, w.data(), prec, dP.mutable_data(), dR.mutable_data());
     }
     check(st);
     return py::make_tuple(dP, dR);
@@ -303,7 +303,7 @@ class Molecule {
       if (cube)
         st = ct_eval_cubic(h_, cube->origin, cube->axes, cube->shape, q, prec, &par, &out, &rep);
       else
-        st = ct_eval(h_, pts, n, q, prec, &par, &out, &rep);
+        st = ct_eval(h_, pts, n, CT_LAYOUT_ROW, q, prec, &par, &out, &rep);
     }
     check(st);
     last_report_ = rep;

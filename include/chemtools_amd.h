@@ -84,6 +84,18 @@ typedef struct ct_mol ct_mol;
 #define CT_Q_GENERAL_KED (1u << 10)  /* tau_PD + (a-1)/4 lap          out->general_ked (a)   */
 #define CT_Q_KED_ALPHA (1u << 11)    /* tau_PD + alpha lap            out->ked_alpha (alpha) */
 
+/* Layouts of points (ct_eval, ct_esp, ct_backward) and of the grad / hess
+ * outputs (ct_eval_params.out_layout).  ROW: points (n,3) row-major, x y z of
+ * a point together (the reference's Molecule methods, include/pymolecule.cuh:20,
+ * and compute_electrostatic_potential_over_points); grad (n,3), hess (n,3,3).
+ * COL: points (3,n), x[0..n) then y then z (the reference's free functions,
+ * e.g. evaluate_electron_density_on_any_grid, src/evaluate_density.cu:663-664,
+ * 745-752); grad (3,n) and hess (9,n) with d_k d_l rho at row 3k+l (the
+ * reference's return_row = false, src/evaluate_gradient.cu:1399-1413,
+ * src/evaluate_hessian.cu:2244-2279). */
+#define CT_LAYOUT_ROW 0
+#define CT_LAYOUT_COL 1
+
 /* Precision. */
 #define CT_PREC_FP64 0        /* everything in FP64 (parity mode) */
 #define CT_PREC_MIXED 1       /* FP32 AOs, split-BF16 MFMA contraction with FP32 accumulation,
@@ -97,6 +109,7 @@ typedef struct {
   double mixed_tol;        /* relative error target of the mixed path (0 => 1e-6) */
   double rho_floor;        /* points with rho below this are not re-evaluated (0 => 1e-10) */
   int32_t route;           /* mixed path: CT_ROUTE_AUTO (0), CT_ROUTE_DENSITY, CT_ROUTE_FACTOR */
+  int32_t out_layout;      /* grad / hess outputs: CT_LAYOUT_ROW (0) or CT_LAYOUT_COL */
 } ct_eval_params;
 
 /* Routes of the mixed contraction (rho, grad and lap quantities).  DENSITY
@@ -194,9 +207,10 @@ int ct_shell_screening_radius(int32_t l, int32_t nprim, const double* exps, cons
                               double* radius);
 
 /* Evaluation --------------------------------------------------------------- */
-/* points: (n,3) row-major.  Output arrays named by the quantity bits must be
- * non-null; others are ignored.  report may be NULL. */
-int ct_eval(ct_mol* mol, const double* points, int64_t n, uint32_t quantities, int32_t precision,
+/* points: (n,3) row-major (layout CT_LAYOUT_ROW) or (3,n) (CT_LAYOUT_COL).
+ * Output arrays named by the quantity bits must be non-null; others are
+ * ignored.  params and report may be NULL. */
+int ct_eval(ct_mol* mol, const double* points, int64_t n, int32_t layout, uint32_t quantities, int32_t precision,
             const ct_eval_params* params, const ct_outputs* out, ct_eval_report* report);
 
 /* The same quantities on a uniform grid generated on the device (no point
@@ -211,10 +225,11 @@ int ct_eval_cubic(ct_mol* mol, const double* origin, const double* axes, const i
 /* |v| of each row of an (n,3) array. */
 int ct_norm_rows(const double* v, int64_t n, double* out);
 
-/* Molecular electrostatic potential at (n,3) row-major points.  CT_PREC_FP64:
+/* Molecular electrostatic potential at (n,3) row-major (CT_LAYOUT_ROW) or
+ * (3,n) (CT_LAYOUT_COL) points.  CT_PREC_FP64:
  * all in FP64.  CT_PREC_MIXED: the pair geometry, the Boys function and the
  * sums over primitive pairs in FP64, each pair's Hermite polynomial in FP32. */
-int ct_esp(ct_mol* mol, const double* points, int64_t n, int32_t precision, double* out);
+int ct_esp(ct_mol* mol, const double* points, int64_t n, int32_t layout, int32_t precision, double* out);
 
 /* Primitive pairs of the molecule's ESP pre-contraction, by total angular
  * momentum L = 0..8 (pairs_by_L: 9 entries), after screening: the units of
@@ -228,11 +243,13 @@ int ct_mol_esp_pairs(ct_mol* mol, int64_t* pairs_by_L);
  * row-major.  T >= 0; host or device pointers. */
 int ct_boys(const double* T, int64_t n, int32_t m, int32_t lmax_pairs, double* F);
 
-/* Adjoint of rho: given dL/drho at the points, dL/dP (nbasis x nbasis,
- * row-major, accumulated as sum_g w_g phi_i phi_j) and dL/dR (natom x 3).
- * Either output may be NULL. */
-int ct_backward(ct_mol* mol, const double* points, int64_t n, const double* dL_drho, int32_t precision,
-                double* dL_dP, double* dL_dR);
+/* Adjoint of rho: given dL/drho at the points ((n,3) or (3,n) by layout),
+ * dL/dP (nbasis x nbasis, row-major, accumulated as sum_g w_g phi_i phi_j)
+ * and dL/dR (natom x 3).  Either output may be NULL.  dL/dP is the same bits
+ * on every call; dL/dR is summed with FP64 atomics and agrees from call to
+ * call to the last bits. */
+int ct_backward(ct_mol* mol, const double* points, int64_t n, int32_t layout, const double* dL_drho,
+                int32_t precision, double* dL_dP, double* dL_dR);
 
 /* Synchronise the handle's stream. */
 int ct_mol_synchronize(ct_mol* mol);
