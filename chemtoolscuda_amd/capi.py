@@ -82,6 +82,8 @@ def lib():
         L.ct_mol_set_density_matrix.argtypes = [vp, vp]
         L.ct_mol_set_workspace_limit.argtypes = [vp, i64]
         L.ct_mol_set_screening.argtypes = [vp, ctypes.c_double]
+        L.ct_set_device_allocation_limit.argtypes = [i64]
+        L.ct_device_allocation_held.argtypes = [ctypes.POINTER(i64)]
         L.ct_shell_screening_radius.argtypes = [i32, i32, vp, vp, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
         L.ct_mol_synchronize.argtypes = [vp]
         L.ct_mol_enable_profiling.argtypes = [vp, ctypes.c_int]
@@ -212,6 +214,19 @@ def read_wavefunction(path: str) -> dict:
     finally:
         L.ct_wavefunction_destroy(h)
     return out
+
+
+def set_device_allocation_limit(nbytes: int) -> None:
+    """Process-wide cap on the library's device buffers (-1: none); requests
+    beyond it fail like a failed hipMalloc (the chunker halves and retries)."""
+    check(lib().ct_set_device_allocation_limit(int(nbytes)))
+
+
+def device_allocation_held() -> int:
+    """Device memory the library's buffers hold now (all handles; bytes)."""
+    v = ctypes.c_int64()
+    check(lib().ct_device_allocation_held(ctypes.byref(v)))
+    return v.value
 
 
 def boys(T, m: int, lmax_pairs: int):

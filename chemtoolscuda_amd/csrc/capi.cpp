@@ -226,6 +226,18 @@ int ct_mol_set_workspace_limit(ct_mol* mol, int64_t bytes) {
   });
 }
 
+int ct_set_device_allocation_limit(int64_t bytes) {
+  return guard([&] {
+    CT_REQUIRE(bytes >= -1, "allocation limit must be -1 (none) or a byte count");
+    ct::set_device_allocation_limit(bytes);
+  });
+}
+int ct_device_allocation_held(int64_t* bytes) {
+  return guard([&] {
+    CT_REQUIRE(bytes, "null output");
+    *bytes = ct::device_allocation_held();
+  });
+}
 int ct_mol_set_screening(ct_mol* mol, double eps) {
   return guard([&] {
     CT_REQUIRE(eps >= 0.0 && std::isfinite(eps), "screening threshold must be finite and non-negative");

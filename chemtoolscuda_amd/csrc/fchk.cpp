@@ -12,6 +12,7 @@
 //     alpha/beta electron counts (src/iodata.cpp:139-160); restricted files
 //     use the alpha orbitals for both channels.
 
+#include <cmath>
 #include <cstdlib>
 #include <fstream>
 #include <map>
@@ -83,8 +84,16 @@ std::map<std::string, Field> parse(const std::string& path) {
 
 const Field& need(const std::map<std::string, Field>& m, const std::string& key, const std::string& path) {
   auto it = m.find(key);
-  if (it == m.end()) throw Error(CT_E_IO, "fchk field '" + key + "' missing in " + path);
+  if (it == m.end() || it->second.values.empty())
+    throw Error(CT_E_IO, "fchk field '" + key + "' missing or empty in " + path);
   return it->second;
+}
+
+// an integer-valued entry of a field, range-checked
+int as_int(double v, int lo, int hi, const std::string& what, const std::string& path) {
+  if (!(v >= lo && v <= hi) || v != (double)(long)v)
+    throw Error(CT_E_IO, "fchk: " + what + " out of range (" + std::to_string(v) + ") in " + path);
+  return (int)v;
 }
 
 }  // namespace
@@ -92,16 +101,17 @@ const Field& need(const std::map<std::string, Field>& m, const std::string& key,
 FchkData load_fchk(const std::string& path) {
   const auto m = parse(path);
   FchkData d;
-  const int nbasis = (int)need(m, "Number of basis functions", path).values.at(0);
-  const int na = (int)need(m, "Number of alpha electrons", path).values.at(0);
-  const int nb = (int)need(m, "Number of beta electrons", path).values.at(0);
+  const int nbasis = as_int(need(m, "Number of basis functions", path).values[0], 1, 1 << 20, "basis size", path);
+  const int na = as_int(need(m, "Number of alpha electrons", path).values[0], 0, 1 << 20, "electron count", path);
+  const int nb = as_int(need(m, "Number of beta electrons", path).values[0], 0, 1 << 20, "electron count", path);
   const auto& nums = need(m, "Atomic numbers", path).values;
   const int natom = (int)nums.size();
   d.atom_xyz = need(m, "Current cartesian coordinates", path).values;
   if ((int)d.atom_xyz.size() != 3 * natom) throw Error(CT_E_IO, "coordinate count mismatch in " + path);
   d.numbers.resize(natom);
-  for (int a = 0; a < natom; ++a) d.numbers[a] = (int32_t)nums[a];
+  for (int a = 0; a < natom; ++a) d.numbers[a] = (int32_t)as_int(nums[a], 0, 200, "atomic number", path);
   d.charges = need(m, "Nuclear charges", path).values;
+  if ((int)d.charges.size() != natom) throw Error(CT_E_IO, "nuclear charge count mismatch in " + path);
 
   const auto& types = need(m, "Shell types", path).values;
   const auto& nprims = need(m, "Number of primitives per shell", path).values;
@@ -111,13 +121,19 @@ FchkData load_fchk(const std::string& path) {
   const std::vector<double>* spc = nullptr;
   auto it = m.find("P(S=P) Contraction coefficients");
   if (it != m.end()) spc = &it->second.values;
+  if (nprims.size() != types.size() || s2a.size() != types.size())
+    throw Error(CT_E_IO, "shell array length mismatch in " + path);
+  if (cc.size() != exps.size() || (spc && spc->size() != exps.size()))
+    throw Error(CT_E_IO, "contraction coefficient count mismatch in " + path);
+  for (double a : exps)
+    if (!(a > 0.0) || !std::isfinite(a)) throw Error(CT_E_IO, "non-positive primitive exponent in " + path);
 
   HostBasis& b = d.basis;
   size_t off = 0;
   for (size_t s = 0; s < types.size(); ++s) {
-    const int t = (int)types[s];
-    const int np = (int)nprims[s];
-    const int atom = (int)s2a[s] - 1;
+    const int t = as_int(types[s], -1000, 1000, "shell type", path);
+    const int np = as_int(nprims[s], 1, 1 << 20, "primitives per shell", path);
+    const int atom = as_int(s2a[s], 1, natom, "shell centre", path) - 1;
     if (off + np > exps.size()) throw Error(CT_E_IO, "primitive count mismatch in " + path);
     auto add = [&](int l, const std::vector<double>& coef) {
       b.shell_atom.push_back(atom);
@@ -151,7 +167,11 @@ FchkData load_fchk(const std::string& path) {
   const std::vector<double>* cb = &ca;
   auto itb = m.find("Beta MO coefficients");
   const bool unrestricted = itb != m.end();
-  if (unrestricted) cb = &itb->second.values;
+  if (unrestricted) {
+    cb = &itb->second.values;
+    if (cb->size() % nbasis != 0 || (int)(cb->size() / nbasis) < nb)
+      throw Error(CT_E_IO, "beta MO coefficient count mismatch in " + path);
+  }
   d.P.assign((size_t)nbasis * nbasis, 0.0);
   // MO k occupies ca[k*nbasis .. (k+1)*nbasis)
   auto accumulate = [&](const std::vector<double>& C, int nocc, double occ) {

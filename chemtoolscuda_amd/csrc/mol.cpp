@@ -13,6 +13,7 @@
 #include "esp.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 
@@ -92,20 +93,36 @@ bool is_device_pointer(const void* ptr) {
   return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
 }
 
+namespace {
+// Process-wide cap on the device memory held by the library's buffers
+// (ct_set_device_allocation_limit; -1: none) and what they hold now.
+std::atomic<int64_t> g_alloc_cap{-1};
+std::atomic<int64_t> g_alloc_held{0};
+}  // namespace
+
+void set_device_allocation_limit(int64_t bytes) { g_alloc_cap.store(bytes); }
+int64_t device_allocation_held() { return g_alloc_held.load(); }
+
 bool DeviceBuffer::reserve(size_t bytes) {
   if (bytes <= bytes_) return true;
   release();
+  const int64_t cap = g_alloc_cap.load();
+  if (cap >= 0 && g_alloc_held.load() + (int64_t)bytes > cap) return false;  // as if hipMalloc had failed
   if (hipMalloc(&ptr_, bytes) != hipSuccess) {
     (void)hipGetLastError();
     ptr_ = nullptr;
     return false;
   }
   bytes_ = bytes;
+  g_alloc_held += (int64_t)bytes;
   return true;
 }
 
 void DeviceBuffer::release() {
-  if (ptr_) (void)hipFree(ptr_);
+  if (ptr_) {
+    (void)hipFree(ptr_);
+    g_alloc_held -= (int64_t)bytes_;
+  }
   ptr_ = nullptr;
   bytes_ = 0;
 }
@@ -785,45 +802,71 @@ int64_t Mol::chunk_points(int64_t per_point, int64_t n) const {
   return std::min<int64_t>(nc, round_up(n, kPointAlign));
 }
 
-// FP64 evaluation of `count` points of the current chunk (index = chunk-
-// relative positions), written into the chunk's outputs through the index map.
-void Mol::reevaluate(int mode, const double* pts, int64_t col_stride, int64_t count, const int32_t* index,
-                     const FinalizeArgs& fin, bool factor) {
+size_t Mol::reeval_bytes(int mode, int64_t count, bool factor) const {
+  // AO arrays, dots, gathered points and the split contraction's partials
+  // (an upper bound for every slice of at most count points)
   const int nder = mode_nder(mode), ndot = mode_ndot(mode);
   const int64_t ld = round_up(count, kPointAlign);
-  // few points: one wave per (16 or 64 points, 64-row block of P, or
-  // 64-column pass of the factor) so the GPU fills
-  const bool split = ld < 65536;
-  const size_t split_doubles =
-      !split ? 0
-      : factor ? (size_t)contract_f64_factor_split_workspace(mode, factor_rows_, ld)
-               : (size_t)contract_f64_split_workspace(mode, Mp_, ld);
-  const size_t bytes = (size_t)ld * ((size_t)nder * Mp_ * 8 + (size_t)ndot * 8 + 24) + split_doubles * 8;
-  if (!d_ws64_.reserve(bytes)) throw Error(CT_E_NOMEM, "cannot allocate the FP64 re-evaluation workspace");
-  double* ao = static_cast<double*>(d_ws64_.get());
-  double* dots = ao + (size_t)nder * Mp_ * ld;
-  double* gp = dots + (size_t)ndot * ld;
-  double* part = gp + 3 * (size_t)ld;
-  launch_gather_points(pts, index, count, gp, stream_, col_stride);
-  launch_ao_f64(mode, lmax_, static_cast<const ShellMeta*>(d_meta_.get()), static_cast<const ShellGeom*>(d_geom_.get()),
-                static_cast<const PrimData*>(d_prim_.get()), nshell_, M_, Mp_, gp, count, ld, ao, 700.0, stream_);
-  if (factor && split)
-    launch_contract_f64_factor_split(mode, static_cast<const double*>(d_L64_.get()), factor_rows_, Mp_, ao, ld, part,
-                                     dots, stream_);
-  else if (factor)
-    launch_contract_f64_factor(mode, static_cast<const double*>(d_L64_.get()), factor_rows_, Mp_, ao, ld, dots,
-                               stream_);
-  else if (split)
-    launch_contract_f64_split(mode, static_cast<const double*>(d_P_.get()), Mp_, ao, ld, part, dots, stream_);
-  else
-    launch_contract_f64(mode, static_cast<const double*>(d_P_.get()), Mp_, ao, ld, dots, stream_);
-  FinalizeArgs f = fin;
-  f.dots = dots;
-  f.index = index;
-  f.nrt = 0;
-  f.ld = ld;
-  f.n = count;
-  launch_finalize(f, stream_);
+  const int blocks = factor ? factor_rows_ / 64 : Mp_ / 64;
+  return (size_t)ld * ((size_t)nder * Mp_ * 8 + (size_t)ndot * 8 + 24 + (size_t)blocks * ndot * 8);
+}
+
+// FP64 evaluation of `count` points (index: their positions in pts), written
+// into fin's outputs through the index map, in slices that fit the FP64
+// workspace: the caller's region ws when given, else d_ws64_, grown to a
+// slice sized like an FP64 chunk.
+void Mol::reevaluate(int mode, const double* pts, int64_t col_stride, int64_t count, const int32_t* index,
+                     const FinalizeArgs& fin, bool factor, char* ws, size_t ws_bytes) {
+  if (count <= 0) return;
+  const int nder = mode_nder(mode), ndot = mode_ndot(mode);
+  int64_t S = round_up(count, kPointAlign);
+  if (ws) {
+    while (S > kPointAlign && reeval_bytes(mode, S, factor) > ws_bytes)
+      S = std::max<int64_t>(kPointAlign, S / 2 / kPointAlign * kPointAlign);
+    if (reeval_bytes(mode, S, factor) > ws_bytes)
+      throw Error(CT_E_NOMEM, "the FP64 re-evaluation needs " + std::to_string(reeval_bytes(mode, S, factor)) +
+                                  " bytes for one slice of 256 points; the workspace holds " + std::to_string(ws_bytes));
+  } else {
+    S = chunk_points((int64_t)(reeval_bytes(mode, kPointAlign, factor) / kPointAlign), count);
+    while (!d_ws64_.reserve(reeval_bytes(mode, S, factor))) {
+      if (S <= kPointAlign) throw Error(CT_E_NOMEM, "cannot allocate the FP64 re-evaluation workspace");
+      S = std::max<int64_t>(kPointAlign, S / 2 / kPointAlign * kPointAlign);
+    }
+    ws = static_cast<char*>(d_ws64_.get());
+  }
+  for (int64_t s0 = 0; s0 < count; s0 += S) {
+    const int64_t c = std::min(S, count - s0);
+    const int32_t* idx = index + s0;
+    const int64_t ld = round_up(c, kPointAlign);
+    // few points: one wave per (16 or 64 points, 64-row block of P, or
+    // 64-column pass of the factor) so the GPU fills
+    const bool split = ld < 65536;
+    double* ao = reinterpret_cast<double*>(ws);
+    double* dots = ao + (size_t)nder * Mp_ * ld;
+    double* gp = dots + (size_t)ndot * ld;
+    double* part = gp + 3 * (size_t)ld;
+    launch_gather_points(pts, idx, c, gp, stream_, col_stride);
+    launch_ao_f64(mode, lmax_, static_cast<const ShellMeta*>(d_meta_.get()),
+                  static_cast<const ShellGeom*>(d_geom_.get()), static_cast<const PrimData*>(d_prim_.get()), nshell_,
+                  M_, Mp_, gp, c, ld, ao, 700.0, stream_);
+    if (factor && split)
+      launch_contract_f64_factor_split(mode, static_cast<const double*>(d_L64_.get()), factor_rows_, Mp_, ao, ld,
+                                       part, dots, stream_);
+    else if (factor)
+      launch_contract_f64_factor(mode, static_cast<const double*>(d_L64_.get()), factor_rows_, Mp_, ao, ld, dots,
+                                 stream_);
+    else if (split)
+      launch_contract_f64_split(mode, static_cast<const double*>(d_P_.get()), Mp_, ao, ld, part, dots, stream_);
+    else
+      launch_contract_f64(mode, static_cast<const double*>(d_P_.get()), Mp_, ao, ld, dots, stream_);
+    FinalizeArgs f = fin;
+    f.dots = dots;
+    f.index = idx;
+    f.nrt = 0;
+    f.ld = ld;
+    f.n = c;
+    launch_finalize(f, stream_);
+  }
 }
 
 void Mol::eval(const double* points, int64_t n, int layout, uint32_t q, int precision, const ct_eval_params& par,
@@ -1098,11 +1141,10 @@ void Mol::eval(const double* points, int64_t n, int layout, uint32_t q, int prec
       for (size_t k = 0; k < NS; ++k)
         if (q & kSlots[k].bit) fa.out.*(kSlots[k].field) = out.*(kSlots[k].field);
       if (col_out) fa.grad_ld = fa.hess_ld = n;
-      // the FP64 re-evaluation runs in chunks sized like the FP64 path's, on
-      // the flagged points in index order (appended in any order; in order,
-      // points given in a spatial order stay so for the AO kernel's waves)
-      const int64_t per64 = (int64_t)nder * Mp_ * 8 + (int64_t)ndot * 8 + 24;
-      const int64_t c64 = chunk_points(per64, count);
+      // the FP64 re-evaluation runs in slices sized like the FP64 path's
+      // chunks, on the flagged points in index order (appended in any order;
+      // in order, points given in a spatial order stay so for the AO kernel's
+      // waves)
       int bits = 1;  // the indices lie below n
       while (bits < 31 && (int64_t{1} << bits) < n) ++bits;
       const size_t tmp = (sort_indices_workspace(count, bits) + 255) / 256 * 256;
@@ -1113,8 +1155,7 @@ void Mol::eval(const double* points, int64_t n, int layout, uint32_t q, int prec
         sorted = reinterpret_cast<const int32_t*>(base + tmp);
       }
       timer_.begin(KernelTimer::REEVAL, stream_);
-      for (int64_t s0 = 0; s0 < count; s0 += c64)
-        reevaluate(mode, points, col_in ? n : 0, std::min(c64, count - s0), sorted + s0, fa, use_factor);
+      reevaluate(mode, points, col_in ? n : 0, count, sorted, fa, use_factor);
       timer_.end(stream_);
       reevaluated = count;
     }

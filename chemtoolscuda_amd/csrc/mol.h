@@ -31,6 +31,11 @@ FchkData load_wfx(const std::string& path);
 FchkData load_molden(const std::string& path);
 FchkData load_wavefunction(const std::string& path);
 
+// Caps the device memory all DeviceBuffers together may hold (bytes; -1:
+// no cap); a reserve() beyond it fails as a failed hipMalloc does.
+void set_device_allocation_limit(int64_t bytes);
+int64_t device_allocation_held();
+
 class DeviceBuffer {
  public:
   DeviceBuffer() = default;
@@ -136,9 +141,13 @@ class Mol {
   // the threshold change
   void build_screen();
   int64_t chunk_points(int64_t per_point, int64_t n) const;
-  // pts: (m, 3) row-major (col_stride 0) or column-major with that stride
+  // FP64 re-evaluation of `count` points (pts: (m, 3) row-major, col_stride
+  // 0, or column-major with that stride; index: positions in pts), in slices
+  // that fit ws (ws_bytes) or, ws null, the handle's growing FP64 workspace
   void reevaluate(int mode, const double* pts, int64_t col_stride, int64_t count, const int32_t* index,
-                  const FinalizeArgs& fin, bool factor);
+                  const FinalizeArgs& fin, bool factor, char* ws = nullptr, size_t ws_bytes = 0);
+  // workspace bytes of one re-evaluation slice of at most count points
+  size_t reeval_bytes(int mode, int64_t count, bool factor) const;
   // whether this mode runs on the factor of P (ct_eval_params.route; the FP64
   // path only when asked)
   bool factor_route(int mode, int route, bool mixed) const;

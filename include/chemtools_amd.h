@@ -187,6 +187,14 @@ int ct_mol_set_density_matrix(ct_mol* mol, const double* P);
 /* Device-memory budget for the per-chunk workspace (bytes; default a quarter
  * of the device's memory, at most 48 GiB). */
 int ct_mol_set_workspace_limit(ct_mol* mol, int64_t bytes);
+/* Process-wide cap on the device memory the library's own buffers hold (all
+ * handles together; bytes, -1 = no cap, the default).  A workspace that would
+ * exceed it fails as a failed hipMalloc does: evaluation halves its chunk and
+ * retries, down to 256 points, then returns CT_E_NOMEM.  (The reference
+ * asserts instead when its chunk would be empty, src/evaluate_density.cu:690-705.) */
+int ct_set_device_allocation_limit(int64_t bytes);
+/* Device memory the library's buffers hold now (all handles; bytes). */
+int ct_device_allocation_held(int64_t* bytes);
 
 /* Spatial screening of the mixed FACTOR route (no counterpart in the
  * reference, which evaluates every basis function at every point,

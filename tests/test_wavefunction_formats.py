@@ -176,3 +176,70 @@ def test_reader_errors(ct, tmp_path):
         ct.read_wavefunction(cut)
     fchk = ct.read_wavefunction(os.path.join(fchk_py.FCHK_DIR, "atom_he.fchk"))
     assert np.allclose(fchk["P"], d["P"], rtol=0, atol=1e-14)
+
+
+def _fchk_lines(name):
+    with open(os.path.join(fchk_py.FCHK_DIR, name)) as f:
+        return f.read().splitlines()
+
+
+def _field_span(lines, key):
+    """(first, end) line indices of a field's header and values."""
+    i = next(k for k, ln in enumerate(lines) if ln.startswith(key))
+    j = i + 1
+    while j < len(lines) and lines[j].startswith(" "):
+        j += 1
+    return i, j
+
+
+@pytest.mark.parametrize("case", ["truncated", "missing_shell_types", "short_shell_map", "zero_basis",
+                                  "bad_centre", "short_beta", "shell_type_7", "negative_exponent",
+                                  "bad_scalar"])
+def test_malformed_fchk_is_rejected(ct, tmp_path, case):
+    """Malformed .fchk files end in a readable error (CT_E_IO / CT_E_UNSUPPORTED
+    through the host-only reader; ct_mol_create_from_fchk parses the same way
+    before any device work), never in a crash or an out-of-range read."""
+    lines = _fchk_lines("ch4.fchk" if case != "short_beta" else "atom_08_O_N09_M2_ub3lyp_ccpvtz_g09.fchk")
+    if case == "truncated":
+        i, j = _field_span(lines, "Alpha MO coefficients")
+        lines = lines[: i + 3]
+        match = "truncated array"
+    elif case == "missing_shell_types":
+        i, j = _field_span(lines, "Shell types")
+        lines = lines[:i] + lines[j:]
+        match = "Shell types"
+    elif case == "short_shell_map":
+        i, j = _field_span(lines, "Shell to atom map")
+        n = int(lines[i].split("N=")[1])
+        vals = " ".join(lines[i + 1:j]).split()[:-1]
+        lines = lines[:i] + [lines[i].split("N=")[0] + f"N={n - 1:12d}", " ".join(vals)] + lines[j:]
+        match = "length mismatch"
+    elif case == "zero_basis":
+        i, _ = _field_span(lines, "Number of basis functions")
+        lines[i] = lines[i][:43] + "I" + " " * 15 + "0"
+        match = "basis size"
+    elif case == "bad_centre":
+        i, j = _field_span(lines, "Shell to atom map")
+        lines[i + 1] = " ".join(["99"] + lines[i + 1].split()[1:])
+        match = "shell centre"
+    elif case == "short_beta":
+        i, j = _field_span(lines, "Beta MO coefficients")
+        lines = lines[:i] + [lines[i].split("N=")[0] + "N=           5", " ".join(["1.0"] * 5)] + lines[j:]
+        match = "beta MO coefficient count"
+    elif case == "shell_type_7":
+        i, j = _field_span(lines, "Shell types")
+        lines[i + 1] = " ".join(["7"] + lines[i + 1].split()[1:])
+        match = "unsupported"
+    elif case == "negative_exponent":
+        i, j = _field_span(lines, "Primitive exponents")
+        lines[i + 1] = " ".join(["-1.0"] + lines[i + 1].split()[1:])
+        match = "exponent"
+    else:  # bad_scalar: a non-integer electron count
+        i, _ = _field_span(lines, "Number of alpha electrons")
+        lines[i] = lines[i][:43] + "I" + " " * 13 + "2.5"
+        match = "electron count"
+    p = os.path.join(tmp_path, f"{case}.fchk")
+    with open(p, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    with pytest.raises(RuntimeError, match=match):
+        ct.read_wavefunction(p)
