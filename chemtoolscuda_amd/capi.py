@@ -75,6 +75,8 @@ def lib():
         L.ct_eval.argtypes = [vp, vp, i64, i32, ctypes.c_uint32, i32, ctypes.POINTER(EvalParams),
                               ctypes.POINTER(Outputs), ctypes.POINTER(EvalReport)]
         L.ct_esp.argtypes = [vp, vp, i64, i32, i32, vp]
+        L.ct_eval_noalloc.argtypes = [vp, vp, i64, i32, ctypes.c_uint32, i32, ctypes.POINTER(EvalParams),
+                                      ctypes.POINTER(Outputs), ctypes.POINTER(EvalReport)]
         L.ct_boys.argtypes = [vp, i64, i32, i32, vp]
         L.ct_mol_esp_pairs.argtypes = [vp, vp]
         L.ct_backward.argtypes = [vp, vp, i64, i32, vp, i32, vp, vp]
@@ -147,6 +149,20 @@ class Handle:
         rep = EvalReport()
         check(lib().ct_eval(self.ptr, points_ptr, n, LAYOUT[layout], q, PREC[precision], ctypes.byref(par),
                             ctypes.byref(o), ctypes.byref(rep)))
+        return {k: getattr(rep, k) for k, _ in EvalReport._fields_}
+
+    def eval_noalloc(self, points_ptr: int, n: int, outputs: dict, precision: str = "fp64",
+                     params: dict | None = None, layout: str = "row"):
+        """ct_eval_noalloc: like eval(), device pointers only, no allocation."""
+        q = 0
+        o = Outputs()
+        for name, ptr in outputs.items():
+            q |= Q[name]
+            setattr(o, name, ptr)
+        par = EvalParams(**(params or {}))
+        rep = EvalReport()
+        check(lib().ct_eval_noalloc(self.ptr, points_ptr, n, LAYOUT[layout], q, PREC[precision], ctypes.byref(par),
+                                    ctypes.byref(o), ctypes.byref(rep)))
         return {k: getattr(rep, k) for k, _ in EvalReport._fields_}
 
     def esp(self, points_ptr: int, n: int, out_ptr: int, precision: str = "fp64", layout: str = "row"):

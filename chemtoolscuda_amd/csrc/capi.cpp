@@ -272,6 +272,19 @@ int ct_eval(ct_mol* mol, const double* points, int64_t n, int32_t layout, uint32
   });
 }
 
+int ct_eval_noalloc(ct_mol* mol, const double* points, int64_t n, int32_t layout, uint32_t quantities,
+                    int32_t precision, const ct_eval_params* params, const ct_outputs* out, ct_eval_report* report) {
+  return guard([&] {
+    CT_REQUIRE(out != nullptr, "null outputs");
+    ct_eval_params p{};
+    if (params) p = *params;
+    if (p.mixed_tol <= 0) p.mixed_tol = 1e-6;
+    if (p.rho_floor <= 0) p.rho_floor = 1e-10;
+    ct::Mol& m = M(mol);
+    const ct::Workspace w{static_cast<char*>(ct::noalloc_scratch(m.device())), (size_t)CT_NOALLOC_SCRATCH_BYTES};
+    m.eval(points, n, layout, quantities, precision, p, *out, report, nullptr, &w);
+  });
+}
 int ct_eval_cubic(ct_mol* mol, const double* origin, const double* axes, const int64_t* shape, uint32_t quantities,
                   int32_t precision, const ct_eval_params* params, const ct_outputs* out, ct_eval_report* report) {
   return guard([&] {

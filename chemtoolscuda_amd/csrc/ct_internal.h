@@ -252,6 +252,8 @@ struct CubicGrid {
 // Writes the points of linear indices start .. start+n-1 as (n, 3) row-major.
 void launch_cubic_points(const CubicGrid& g, int64_t start, int64_t n, double* out, hipStream_t s);
 void launch_norm_rows(const double* v, int64_t n, double* out, hipStream_t s);
+// zeroes n 64-bit words (a kernel: no memset nodes in work a caller may capture)
+void launch_fill_zero(unsigned long long* p, int64_t n, hipStream_t s);
 
 // Adjoint.
 int gram_slices(int64_t npts);

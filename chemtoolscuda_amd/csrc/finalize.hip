@@ -179,7 +179,18 @@ __global__ __launch_bounds__(256) void norm_rows_kernel(const double* __restrict
   out[p] = sqrt(x * x + y * y + z * z);
 }
 
+__global__ __launch_bounds__(256) void fill_zero_kernel(unsigned long long* __restrict__ p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
 }  // namespace
+
+void launch_fill_zero(unsigned long long* p, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n);
+  CT_HIP_CHECK(hipGetLastError());
+}
 
 void launch_finalize(const FinalizeArgs& a, hipStream_t s) {
   if (a.n <= 0) return;

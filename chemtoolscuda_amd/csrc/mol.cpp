@@ -100,11 +100,22 @@ std::atomic<int64_t> g_alloc_cap{-1};
 std::atomic<int64_t> g_alloc_held{0};
 }  // namespace
 
+// Set while an evaluation runs inside a caller's region (ct_eval_noalloc):
+// any buffer growth then is a bug and fails loudly instead of allocating.
+thread_local bool t_no_alloc = false;
+
+struct NoAllocScope {
+  bool prev;
+  explicit NoAllocScope(bool on) : prev(t_no_alloc) { t_no_alloc = prev || on; }
+  ~NoAllocScope() { t_no_alloc = prev; }
+};
+
 void set_device_allocation_limit(int64_t bytes) { g_alloc_cap.store(bytes); }
 int64_t device_allocation_held() { return g_alloc_held.load(); }
 
 bool DeviceBuffer::reserve(size_t bytes) {
   if (bytes <= bytes_) return true;
+  if (t_no_alloc) throw Error(CT_E_INVALID, "internal error: device allocation inside the allocation-free path");
   release();
   const int64_t cap = g_alloc_cap.load();
   if (cap >= 0 && g_alloc_held.load() + (int64_t)bytes > cap) return false;  // as if hipMalloc had failed
@@ -129,6 +140,7 @@ void DeviceBuffer::release() {
 
 void PinnedBuffer::reserve(size_t bytes) {
   if (bytes <= bytes_) return;
+  if (t_no_alloc) throw Error(CT_E_INVALID, "internal error: host allocation inside the allocation-free path");
   release();
   CT_HIP_CHECK(hipHostMalloc(&ptr_, bytes, hipHostMallocDefault));
   bytes_ = bytes;
@@ -870,7 +882,7 @@ void Mol::reevaluate(int mode, const double* pts, int64_t col_stride, int64_t co
 }
 
 void Mol::eval(const double* points, int64_t n, int layout, uint32_t q, int precision, const ct_eval_params& par,
-               const ct_outputs& out, ct_eval_report* rep, const CubicGrid* cube) {
+               const ct_outputs& out, ct_eval_report* rep, const CubicGrid* cube, const Workspace* fixed) {
   CT_REQUIRE(n >= 0, "negative number of points");
   CT_REQUIRE(q != 0, "no quantity requested");
   CT_REQUIRE(precision == CT_PREC_FP64 || precision == CT_PREC_MIXED, "unknown precision");
@@ -881,6 +893,7 @@ void Mol::eval(const double* points, int64_t n, int layout, uint32_t q, int prec
   if (rep) *rep = ct_eval_report{n, 0, 0, 0.0, -1, 1.0};
   if (n == 0) return;
   CT_REQUIRE(points != nullptr || cube != nullptr, "null points");
+  const NoAllocScope no_alloc(fixed != nullptr);
 
   const bool mixed = precision == CT_PREC_MIXED;
   const int mode = select_mode(q);
@@ -911,7 +924,12 @@ void Mol::eval(const double* points, int64_t n, int layout, uint32_t q, int prec
   // (one large launch instead of one small launch per chunk).
   bool all_dev = pts_dev && n < (int64_t(1) << 31);  // (grid points are not materialised: per-chunk re-evaluation)
   for (size_t k = 0; k < NS; ++k) all_dev = all_dev && out_dev[k];
-  const bool batch = mixed && all_dev;
+  if (fixed)
+    CT_REQUIRE(cube == nullptr && all_dev,
+               "allocation-free evaluation: points and outputs must be device memory, fewer than 2^31 points");
+  // (the call-wide flag list is an allocation: the allocation-free path
+  // re-evaluates chunk by chunk)
+  const bool batch = mixed && all_dev && !fixed;
   const int64_t ao_bytes_pt = (int64_t)nder * Mp_ * (mixed ? 4 : 8);
   const int64_t col_stage_pt = col_in && !pts_dev ? 24 : 0;
   const int64_t per_point = (mixed ? ao_bytes_pt + (int64_t)nrt * (ndot + 1) * 8 + 4 + 24 + stage_width * 8
@@ -920,18 +938,51 @@ void Mol::eval(const double* points, int64_t n, int layout, uint32_t q, int prec
   // screened: keys (2 x 4 B), indices (2 x 4 B), sorted points (24 B) per
   // point; k-step list (2 nk B), count and mask (4 + 4 nwords B) per tile
   const int64_t per_point_s = screened ? 40 + (4 * nk + 4 + 4 * nwords + tp - 1) / tp : 0;
-  int64_t Nc = chunk_points(per_point + per_point_s, n);
   // flag counter, max estimate; from byte 256 the active k-step counters (one
   // per 128-byte line)
   const size_t head = 256 + 128 * kScreenCounters;
   size_t sort_bytes = 0;
-  for (;;) {
-    sort_bytes = screened ? (screen_sort_workspace(Nc) + 255) / 256 * 256 : 0;
-    if (d_ws_.reserve(head + (size_t)Nc * (per_point + per_point_s) + sort_bytes + 4096)) break;
-    if (Nc <= kPointAlign) throw Error(CT_E_NOMEM, "cannot allocate the evaluation workspace");
-    Nc = std::max<int64_t>(kPointAlign, Nc / 2 / kPointAlign * kPointAlign);
+  int64_t Nc = 0;
+  char* ws = nullptr;
+  size_t ws_size = 0;
+  char* ws64 = nullptr;  // caller's region: the FP64 re-evaluation slice
+  size_t ws64_bytes = 0;
+  if (!fixed) {
+    Nc = chunk_points(per_point + per_point_s, n);
+    for (;;) {
+      sort_bytes = screened ? (screen_sort_workspace(Nc) + 255) / 256 * 256 : 0;
+      if (d_ws_.reserve(head + (size_t)Nc * (per_point + per_point_s) + sort_bytes + 4096)) break;
+      if (Nc <= kPointAlign) throw Error(CT_E_NOMEM, "cannot allocate the evaluation workspace");
+      Nc = std::max<int64_t>(kPointAlign, Nc / 2 / kPointAlign * kPointAlign);
+    }
+    ws = static_cast<char*>(d_ws_.get());
+    ws_size = d_ws_.size();
+  } else {
+    // the re-evaluation slice (mixed) at the end of the region, at most a
+    // quarter of it (4096 points or fewer), the chunk workspace before it
+    if (mixed) {
+      int64_t S = 4096;
+      while (S > kPointAlign && reeval_bytes(mode, S, use_factor) > fixed->bytes / 4) S /= 2;
+      ws64_bytes = (reeval_bytes(mode, S, use_factor) + 255) / 256 * 256;
+    }
+    const size_t main = fixed->bytes > ws64_bytes ? fixed->bytes - ws64_bytes : 0;
+    const size_t base_cost = head + 4096;
+    Nc = main > base_cost ? (int64_t)((main - base_cost) / (size_t)(per_point + per_point_s)) : 0;
+    Nc = std::min<int64_t>(Nc / kPointAlign * kPointAlign, round_up(n, kPointAlign));
+    for (;;) {
+      if (Nc < kPointAlign || ws64_bytes > fixed->bytes)
+        throw Error(CT_E_NOMEM, "allocation-free evaluation: the scratch (" + std::to_string(fixed->bytes) +
+                                    " bytes) cannot hold one 256-point chunk of this call (" +
+                                    std::to_string(per_point + per_point_s) + " bytes per point, " +
+                                    std::to_string(ws64_bytes) + " for the FP64 re-evaluation); use ct_eval");
+      sort_bytes = screened ? (screen_sort_workspace(Nc) + 255) / 256 * 256 : 0;
+      if (base_cost + (size_t)Nc * (per_point + per_point_s) + sort_bytes <= main) break;
+      Nc = (Nc - Nc / 8) / kPointAlign * kPointAlign;
+    }
+    ws = fixed->base;
+    ws_size = main;
+    ws64 = fixed->base + main;
   }
-  char* ws = static_cast<char*>(d_ws_.get());
   int32_t* d_count = reinterpret_cast<int32_t*>(ws);
   unsigned long long* d_maxest = reinterpret_cast<unsigned long long*>(ws + 8);
   unsigned long long* d_active = reinterpret_cast<unsigned long long*>(ws + 256);
@@ -970,11 +1021,17 @@ void Mol::eval(const double* points, int64_t n, int layout, uint32_t q, int prec
     kcount = reinterpret_cast<int32_t*>(take((size_t)ntile_max * 4));
     kmask = reinterpret_cast<uint32_t*>(take((size_t)ntile_max * nwords * 4));
     sort_tmp = take(sort_bytes);
-    CT_REQUIRE(cur <= ws + d_ws_.size(), "screening workspace overflow");
+    CT_REQUIRE(cur <= ws + ws_size, "screening workspace overflow");
   }
   int64_t valid_tiles = 0;
-  h_small_.reserve(64 + 128 * kScreenCounters);
-  auto* h_small = static_cast<int64_t*>(h_small_.get());
+  // readbacks of the counters: pinned, or (allocation-free) pageable memory
+  // on this stack, read after the stream synchronises
+  int64_t h_local[8 + 16 * kScreenCounters];
+  int64_t* h_small = h_local;
+  if (!fixed) {
+    h_small_.reserve(sizeof(h_local));
+    h_small = static_cast<int64_t*>(h_small_.get());
+  }
 
   const auto* meta = static_cast<const ShellMeta*>(d_meta_.get());
   const auto* geom = static_cast<const ShellGeom*>(d_geom_.get());
@@ -984,11 +1041,9 @@ void Mol::eval(const double* points, int64_t n, int layout, uint32_t q, int prec
     if (!d_flags_.reserve(sizeof(int32_t) * (size_t)n)) throw Error(CT_E_NOMEM, "cannot allocate the flag list");
     list = static_cast<int32_t*>(d_flags_.get());
   }
-  if (mixed) {
-    CT_HIP_CHECK(hipMemsetAsync(d_maxest, 0, 8, stream_));
-    CT_HIP_CHECK(hipMemsetAsync(d_count, 0, 4, stream_));
-    if (screened) CT_HIP_CHECK(hipMemsetAsync(d_active, 0, 128 * kScreenCounters, stream_));
-  }
+  // the flag counter (bytes 0-3), the largest estimate (8-15) and the
+  // active k-step counters start at zero
+  if (mixed) launch_fill_zero(reinterpret_cast<unsigned long long*>(ws), (int64_t)(head / 8), stream_);
 
   int64_t chunks = 0, reevaluated = 0;
   for (int64_t start = 0; start < n; start += Nc) {
@@ -1109,11 +1164,11 @@ void Mol::eval(const double* points, int64_t n, int layout, uint32_t q, int prec
         const int64_t count = (int64_t) * reinterpret_cast<int32_t*>(h_small);
         if (count > 0) {
           timer_.begin(KernelTimer::REEVAL, stream_);
-          reevaluate(mode, pts, 0, count, flag_idx, fa, use_factor);
+          reevaluate(mode, pts, 0, count, flag_idx, fa, use_factor, ws64, ws64_bytes);
           timer_.end(stream_);
           reevaluated += count;
         }
-        CT_HIP_CHECK(hipMemsetAsync(d_count, 0, 4, stream_));
+        launch_fill_zero(reinterpret_cast<unsigned long long*>(d_count), 1, stream_);
       }
     }
     for (size_t k = 0; k < NS; ++k) {

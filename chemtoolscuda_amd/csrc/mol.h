@@ -36,6 +36,17 @@ FchkData load_wavefunction(const std::string& path);
 void set_device_allocation_limit(int64_t bytes);
 int64_t device_allocation_held();
 
+// Device address of the allocation-free entry point's scratch on a device
+// (CT_NOALLOC_SCRATCH_BYTES; looked up on the first call per device).
+void* noalloc_scratch(int device);
+
+// A caller-provided device region that an evaluation uses for all of its
+// scratch instead of the handle's growing buffers.
+struct Workspace {
+  char* base;
+  size_t bytes;
+};
+
 class DeviceBuffer {
  public:
   DeviceBuffer() = default;
@@ -101,6 +112,7 @@ class Mol {
   ~Mol();
 
   int nbasis() const { return M_; }
+  int device() const { return device_; }
   int nbasis_pad() const { return Mp_; }
   int natom() const { return natom_; }
   const std::vector<double>& coordinates() const { return xyz_; }
@@ -124,8 +136,11 @@ class Mol {
   // points: host or device, (n, 3) row-major (layout CT_LAYOUT_ROW) or (3, n)
   // (CT_LAYOUT_COL); or points == nullptr and cube != nullptr: the points of
   // a uniform grid, generated on the device chunk by chunk.
+  // fixed: evaluate inside that region only, allocating nothing (device
+  // points and outputs; flagged points re-evaluated chunk by chunk).
   void eval(const double* points, int64_t n, int layout, uint32_t q, int precision, const ct_eval_params& par,
-            const ct_outputs& out, ct_eval_report* rep, const CubicGrid* cube = nullptr);
+            const ct_outputs& out, ct_eval_report* rep, const CubicGrid* cube = nullptr,
+            const Workspace* fixed = nullptr);
   void esp(const double* points, int64_t n, int layout, int precision, double* out);
   void backward(const double* points, int64_t n, int layout, const double* w, int precision, double* dP, double* dR);
   int64_t esp_pairs() const { return esp_npairs_; }

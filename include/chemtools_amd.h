@@ -36,6 +36,7 @@
  *   ct_esp                   compute_electrostatic_potential_over_points (src/evaluate_electrostatic.cu:10161-10372)
  *   ct_boys                  boys_function<M> (include/boys_functions.cuh:54-72)
  *   ct_backward              (no reference counterpart: adjoint of rho w.r.t. P and the centres)
+ *   ct_eval_noalloc          (no reference counterpart: ct_eval for callers that cannot allocate)
  */
 #ifndef CHEMTOOLS_AMD_H_
 #define CHEMTOOLS_AMD_H_
@@ -220,6 +221,23 @@ int ct_shell_screening_radius(int32_t l, int32_t nprim, const double* exps, cons
  * ignored.  params and report may be NULL. */
 int ct_eval(ct_mol* mol, const double* points, int64_t n, int32_t layout, uint32_t quantities, int32_t precision,
             const ct_eval_params* params, const ct_outputs* out, ct_eval_report* report);
+
+/* ct_eval without any allocation, for callers that cannot allocate device or
+ * pinned memory (work under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  No reference counterpart.  Points and outputs must
+ * be device memory, n < 2^31; the handle (shell table, P, its factor and the
+ * screening tables) is made beforehand.  All scratch lives in one
+ * file-scope device array of CT_NOALLOC_SCRATCH_BYTES per GPU, held by that
+ * GPU until the process ends: points are processed in chunks that fit it, and
+ * flagged points of the mixed path are re-evaluated chunk by chunk.  A call
+ * whose smallest chunk (256 points) does not fit returns CT_E_NOMEM with the
+ * sizes in the message.  Every call on one GPU shares that GPU's array: calls
+ * that overlap on one GPU (other threads, other handles) race and must be
+ * serialised by the caller.  Results: FP64 the same bits as ct_eval; mixed
+ * within mixed_tol (the screened route groups points by chunk). */
+#define CT_NOALLOC_SCRATCH_BYTES (256u << 20)
+int ct_eval_noalloc(ct_mol* mol, const double* points, int64_t n, int32_t layout, uint32_t quantities,
+                    int32_t precision, const ct_eval_params* params, const ct_outputs* out, ct_eval_report* report);
 
 /* The same quantities on a uniform grid generated on the device (no point
  * upload): point (i, j, k) = origin + i axes[0..2] + j axes[3..5] + k axes[6..8]
