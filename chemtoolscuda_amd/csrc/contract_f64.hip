@@ -440,4 +440,49 @@ void launch_contract_f64_split(int mode, const double* P, int nbasis_pad, const 
   CT_HIP_CHECK(hipGetLastError());
 }
 
+namespace {
+
+// Largest violation of |R_ij| <= sqrt(R_ii R_jj) over i, j < m, R = A - L L^T
+// (A: mp x mp row-major; L: row i of the factor at L[i * rows ..], rank
+// columns; diag[i] = R_ii).  A positive semidefinite remainder satisfies it
+// to rounding; an indefinite one does not, even with a zero diagonal.  NaN
+// counts as an infinite violation.  16 x 16 threads per block, one (i, j)
+// each; the block's largest value goes to *out with one atomic (non-negative
+// doubles order like their bit patterns).
+__global__ __launch_bounds__(256) void factor_residual_kernel(const double* __restrict__ A, int mp,
+                                                              const double* __restrict__ L, int rows, int rank,
+                                                              int m, const double* __restrict__ diag,
+                                                              unsigned long long* __restrict__ out) {
+  const int i = blockIdx.y * 16 + threadIdx.y;
+  const int j = blockIdx.x * 16 + threadIdx.x;
+  double v = 0.0;
+  if (i < m && j < m) {
+    double r = A[(int64_t)i * mp + j];
+    const double* li = L + (int64_t)i * rows;
+    const double* lj = L + (int64_t)j * rows;
+    for (int k = 0; k < rank; ++k) r = fma(-li[k], lj[k], r);
+    v = fabs(r) - sqrt(fmax(diag[i], 0.0) * fmax(diag[j], 0.0));
+    v = v <= 1e300 ? fmax(v, 0.0) : 1e300;
+  }
+  for (int off = 32; off >= 1; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+  __shared__ double wmax[4];
+  const int t = threadIdx.y * 16 + threadIdx.x;
+  if ((t & 63) == 0) wmax[t >> 6] = v;
+  __syncthreads();
+  if (t == 0) {
+    const double b = fmax(fmax(wmax[0], wmax[1]), fmax(wmax[2], wmax[3]));
+    if (b > 0.0) atomicMax(out, (unsigned long long)__double_as_longlong(b));
+  }
+}
+
+}  // namespace
+
+void launch_factor_residual(const double* A, int mp, const double* L, int rows, int rank, int m, const double* diag,
+                            unsigned long long* out, hipStream_t s) {
+  if (m <= 0) return;
+  const dim3 grid((unsigned)((m + 15) / 16), (unsigned)((m + 15) / 16)), block(16, 16);
+  hipLaunchKernelGGL(factor_residual_kernel, grid, block, 0, s, A, mp, L, rows, rank, m, diag, out);
+  CT_HIP_CHECK(hipGetLastError());
+}
+
 }  // namespace ct

@@ -172,9 +172,17 @@ void launch_contract_mixed_factor(int mode, const uint16_t* Lf, int rows_pad, in
 // lda) on the host: A = L L^T with L (n x rank, column k at L[k * n ...]),
 // columns in pivot order, stopping when the largest remaining diagonal
 // element is <= rel_tol times the largest diagonal element of A.  Returns
-// false when A is not positive semidefinite to that tolerance or its rank
-// exceeds rmax.
+// false when the remaining diagonal is negative beyond that tolerance or the
+// rank exceeds rmax.  A remainder with a small diagonal can still be
+// indefinite (e.g. a zero diagonal with non-zero off-diagonal elements): the
+// caller checks the off-diagonal residual (launch_factor_residual).
 bool pivoted_cholesky(const double* A, int n, int lda, int rmax, double rel_tol, std::vector<double>& L, int& rank);
+// *out = max(*out, largest violation of |R_ij| <= sqrt(R_ii R_jj) over
+// i, j < m), R = A - L L^T (A: mp x mp row-major; row i of L at L[i * rows]
+// with rank columns; diag[i] = R_ii), as the bits of a non-negative double
+// (NaN -> 1e300).
+void launch_factor_residual(const double* A, int mp, const double* L, int rows, int rank, int m, const double* diag,
+                            unsigned long long* out, hipStream_t s);
 
 // ---------------------------------------------------------------------------
 // Spatial screening of the mixed factor route (screen_host.cpp, screen.hip).

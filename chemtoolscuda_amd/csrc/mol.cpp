@@ -393,16 +393,44 @@ void Mol::upload_P() {
   std::vector<double> L;
   int rank = 0;
   if (pivoted_cholesky(Pp.data(), M_, Mp_, Mp_ / 2 + 1, 1e-12, L, rank)) {
-    factor_rank_ = rank;
-    factor_rows_ = std::max(kRowBlock, (rank + kRowBlock - 1) / kRowBlock * kRowBlock);
+    const int rows = std::max(kRowBlock, (rank + kRowBlock - 1) / kRowBlock * kRowBlock);
     // FP64 copy for the FP64 factor route (re-evaluation of the mixed factor
     // route's flagged points; FP64 evaluation on request): L k-major,
     // L64[k * rows + j], padded with zeros
-    std::vector<double> L64((size_t)Mp_ * factor_rows_, 0.0);
+    std::vector<double> L64((size_t)Mp_ * rows, 0.0);
     for (int j = 0; j < rank; ++j)
-      for (int i = 0; i < M_; ++i) L64[(size_t)i * factor_rows_ + j] = L[(size_t)j * M_ + i];
+      for (int i = 0; i < M_; ++i) L64[(size_t)i * rows + j] = L[(size_t)j * M_ + i];
     if (!d_L64_.reserve(sizeof(double) * L64.size())) throw Error(CT_E_NOMEM, "cannot allocate the factor of P");
     CT_HIP_CHECK(hipMemcpy(d_L64_.get(), L64.data(), sizeof(double) * L64.size(), hipMemcpyHostToDevice));
+    // the remainder R = P - L L^T must itself be positive semidefinite: with
+    // its diagonal at most tol that means |R_ij| <= sqrt(R_ii R_jj) for all
+    // i, j, up to rounding (the pivoted Cholesky only looked at the diagonal)
+    double dmax = 0.0;
+    std::vector<double> rdiag(M_);
+    for (int i = 0; i < M_; ++i) {
+      double r = Pp[(size_t)i * Mp_ + i];
+      dmax = std::max(dmax, std::fabs(r));
+      for (int j = 0; j < rank; ++j) r -= L[(size_t)j * M_ + i] * L[(size_t)j * M_ + i];
+      rdiag[i] = r;
+    }
+    DeviceBuffer chk;
+    if (!chk.reserve(sizeof(double) * (M_ + 1))) throw Error(CT_E_NOMEM, "cannot allocate the factor check");
+    double* d_diag = static_cast<double*>(chk.get());
+    auto* d_viol = reinterpret_cast<unsigned long long*>(d_diag + M_);
+    CT_HIP_CHECK(hipMemcpyAsync(d_diag, rdiag.data(), sizeof(double) * M_, hipMemcpyHostToDevice, stream_));
+    launch_fill_zero(d_viol, 1, stream_);
+    launch_factor_residual(static_cast<const double*>(d_P_.get()), Mp_, static_cast<const double*>(d_L64_.get()),
+                           rows, rank, M_, d_diag, d_viol, stream_);
+    unsigned long long vbits = 0;
+    CT_HIP_CHECK(hipMemcpyAsync(&vbits, d_viol, sizeof(vbits), hipMemcpyDeviceToHost, stream_));
+    CT_HIP_CHECK(hipStreamSynchronize(stream_));
+    double viol;
+    std::memcpy(&viol, &vbits, sizeof(viol));
+    const double slack = 1e-12 * dmax + 4.0 * (rank + 2) * 2.220446049250313e-16 * dmax;
+    if (viol <= slack) {
+      factor_rank_ = rank;
+      factor_rows_ = rows;
+    }
   }
   L_.swap(L);
   if (factor_rank_ < 0) L_.clear();

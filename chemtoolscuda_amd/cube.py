@@ -18,16 +18,20 @@ from dataclasses import dataclass
 
 import numpy as np
 
-# Padding around an atom of nuclear charge Z (bohr), src/utils.cpp:20-84.
-_PAD = [(1, 1, 2.6), (2, 6, 2.8), (7, 7, 3.0), (8, 8, 3.3), (9, 9, 2.9), (10, 15, 3.3), (16, 16, 3.4),
-        (17, 35, 3.5), (36, 52, 3.7), (53, 53, 3.8)]
+# Padding around an atom of nuclear charge Z (bohr), the branch chain of
+# get_grid_from_coordinates_charges (src/utils.cpp:163-188): upper bounds of
+# Z, tested in order; Z above 53 gets no padding.
+_PAD = [(1, 1, 2.6), (None, 6, 2.8), (7, 7, 3.0), (8, 8, 3.3), (9, 9, 2.9), (None, 15, 3.3), (16, 16, 3.4),
+        (None, 35, 3.5), (None, 52, 3.7), (53, 53, 3.8)]
 
 
-def atom_padding(z: int) -> float:
-    for lo, hi, pad in _PAD:
-        if lo <= z <= hi:
+def atom_padding(z) -> float:
+    """Padding of an atom of nuclear charge z: the first matching branch
+    (equal to z, or z at most the bound), 0 beyond 53."""
+    for eq, hi, pad in _PAD:
+        if (eq is not None and z == eq) or (eq is None and z <= hi):
             return pad
-    raise ValueError(f"no grid padding known for nuclear charge {z}")
+    return 0.0
 
 
 @dataclass
@@ -70,7 +74,7 @@ class UniformGrid:
             itensor += z * (np.dot(xyz, xyz) * np.eye(3) - np.outer(xyz, xyz))
         _, v = np.linalg.eigh(itensor)
         rc = R @ v
-        pad = np.array([atom_padding(int(round(z))) for z in Z])
+        pad = np.array([atom_padding(z) for z in Z])
         lo = np.min(rc - pad[:, None], axis=0) - extension
         hi = np.max(rc + pad[:, None], axis=0) + extension
         shape = np.ceil(np.abs(hi - lo) / spacing).astype(int)

@@ -36,6 +36,12 @@ def test_cube_file_round_trip(tmp_path):
     assert list(nums) == [8, 1] and np.allclose(xyz[1], [0, 0, 1.8])
 
 
-def test_unknown_element_padding_raises():
-    with pytest.raises(ValueError):
-        atom_padding(92)
+def test_padding_follows_the_reference_branch_chain():
+    """src/utils.cpp:163-188: Z == 1 2.6, Z <= 6 2.8 (ghost atoms, Z = 0,
+    too), 7 3.0, 8 3.3, 9 2.9, <= 15 3.3, 16 3.4, <= 35 3.5, <= 52 3.7, 53 3.8,
+    and no padding above 53 (the chain has no branch there)."""
+    expect = {0: 2.8, 1: 2.6, 2: 2.8, 6: 2.8, 7: 3.0, 8: 3.3, 9: 2.9, 10: 3.3, 15: 3.3, 16: 3.4, 17: 3.5,
+              35: 3.5, 36: 3.7, 51: 3.7, 52: 3.7, 53: 3.8, 54: 0.0, 92: 0.0}
+    for z, pad in expect.items():
+        assert atom_padding(z) == pad, z
+    assert atom_padding(7.5) == 3.3  # a non-integer core charge takes the first bound it meets

@@ -69,9 +69,19 @@ def test_mixed_matches_fp64_and_oracle(ct, workload, npts):
     _check(mix["grad"][sub], f64["grad"][sub], s_grad, 1e-6, m)
     _check(mix["lap"][sub], f64["lap"][sub], s_lap, 1e-6, m)
     _check(mix["tau_pd"][sub], f64["tau_pd"][sub], s_tau, 1e-6, m)
-    # and the FP64 path still agrees with the oracle on the same points
-    ref = oracle.density(w.basis.flat(), w.P, pts[sub], rho=True)
+    # and the FP64 path agrees with the oracle on the same points, every quantity
+    ref = oracle.density(w.basis.flat(), w.P, pts[sub], rho=True, grad=True, lap=True, tau=True)
     assert np.max(np.abs(f64["rho"][sub] - ref["rho"]) / np.abs(ref["rho"])) < 1e-12
+    # (condition scales: the Laplacian's two terms cancel, both sides round
+    # relative to their size |lap - 4 tau| + 4 tau; grad against
+    # max(|grad|, sqrt(8 rho tau)))
+    tau = np.abs(ref["tau"])
+    scales = {"grad": np.maximum(np.linalg.norm(ref["grad"], axis=1), np.sqrt(8 * np.abs(ref["rho"]) * tau)),
+              "lap": np.abs(ref["lap"] - 4 * tau) + 4 * tau, "tau_pd": tau}
+    for k, kr in (("grad", "grad"), ("lap", "lap"), ("tau_pd", "tau")):
+        d = np.abs(f64[k][sub] - ref[kr]).reshape(len(sub), -1).max(axis=1)
+        r = d / (scales[k] + 1e-10 * scales[k].max())
+        assert r.max() < 1e-12, (workload, k, r.max())
     print(f"{workload}: max rel err rho {rel[mask].max():.2e}, re-evaluated {rep['n_reevaluated']} "
           f"of {pts.shape[0]}, max estimate {rep['max_error_estimate']:.2e}")
 
@@ -284,6 +294,38 @@ def test_route_selection_and_errors(ct):
     b = mol.compute(w.points, ["rho"], precision="fp64")["rho"]
     m = np.abs(b) >= 1e-8
     assert np.max(np.abs(a - b)[m] / np.abs(b)[m]) <= 1e-6
+
+
+def test_indefinite_P_with_zero_diagonal_has_no_factor(ct):
+    """A pivoted Cholesky sees only the diagonal: a symmetric P with a zero
+    diagonal and non-zero off-diagonal elements (indefinite: trace 0) would
+    pass as "rank 0" and give rho = 0.  The handle checks the residual
+    P - L L^T as well, so such a P has no factor: the factor route is refused
+    in both precisions and the mixed AUTO route contracts with P."""
+    from chemtoolscuda_amd import synthetic
+
+    w = synthetic.water(300)
+    P = np.zeros_like(w.P)
+    P[0, 1] = P[1, 0] = 0.6
+    P[1, 2] = P[2, 1] = 0.3
+    mol = ct.molecule_from_basis(w.basis, P, w.charges)
+    for prec in ("fp64", "mixed"):
+        with pytest.raises(RuntimeError, match="not positive semidefinite"):
+            mol.compute(w.points, ["rho"], precision=prec, route="factor")
+    o = synthetic.organic60()
+    rng = np.random.default_rng(51)
+    E = rng.normal(0.0, 1e-2, size=o.P.shape)
+    P = E + E.T
+    np.fill_diagonal(P, 0.0)
+    mol = ct.molecule_from_basis(o.basis, P, o.charges)
+    pts = o.points[rng.choice(o.points.shape[0], 4000, replace=False)]
+    auto = mol.compute(pts, ["rho", "grad"], precision="mixed")
+    assert mol.last_report()["factor_rank"] == -1  # (a rank-0 factor would have been the cheaper route)
+    dens = mol.compute(pts, ["rho", "grad"], precision="mixed", route="density")
+    for q in ("rho", "grad"):
+        assert np.array_equal(auto[q], dens[q])
+    ref = mol.compute(pts, ["rho"], precision="fp64")["rho"]
+    assert np.max(np.abs(ref)) > 1e-3  # rho is far from the zero a rank-0 factor gives
 
 
 def test_device_resident_chunks_match_host_path(ct):
