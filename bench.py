@@ -89,13 +89,27 @@ def cpu_baseline(w, target_s):
     oracle.density(flat, w.P, sample, rho=True, grad=True, lap=True, tau=True)
     dt = time.perf_counter() - t0
     return {"value": n / dt, "unit": "points/s", "cores": oracle.max_threads(), "kind": "port",
+            "cpu_model": _cpu_model(),
             "sample": f"{n} seeded random points of the {w.name} grid, rho+grad+lap+tau_PD in FP64 "
                       f"({dt:.1f} s, OpenMP, {oracle.max_threads()} threads)"}
 
 
+def _cpu_model():
+    """The host CPU's model name (what lscpu reports), from /proc/cpuinfo."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def fwd_bwd(torch, capi, synthetic, precision, reps=3):
     """Differentiation workload: 40 atoms, M = 416, 5e5 points; rho forward
-    plus adjoint (dL/dP, dL/dR) per step, inputs resident in HBM."""
+    plus adjoint (dL/dP, dL/dR) per step through the C ABI, inputs resident in
+    HBM.  Also the adjoint kernels' time per call (in-library HIP events)."""
     w = synthetic.diff40()
     order = synthetic.morton_order(w.points)
     pts = torch.from_numpy(np.ascontiguousarray(w.points[order])).cuda()
@@ -112,11 +126,59 @@ def fwd_bwd(torch, capi, synthetic, precision, reps=3):
 
     step()
     torch.cuda.synchronize()
+    h.enable_profiling(True)
     t0 = time.perf_counter()
     for _ in range(reps):
         step()
     torch.cuda.synchronize()
-    return n * reps / (time.perf_counter() - t0)
+    rate = n * reps / (time.perf_counter() - t0)
+    st = h.kernel_stats()
+    h.enable_profiling(False)
+    return rate, st["backward_ms"] / max(st["backward_calls"], 1)
+
+
+def fwd_bwd_autograd(torch, synthetic, precision, reps=3, move=False):
+    """The differentiation workload through torch.autograd: L = sum_g w_g
+    (rho_g(P, R) - rho_t,g)^2 with rho_t from the perturbed P_target (FP64,
+    before timing), L.backward() for dL/dP and dL/dR per step.  Mixed: the
+    forward holds its error to 1e-6 of |rho - rho_t| (rho_target), so the
+    gradients meet 1e-6.  move=True changes P and R every step (an optimiser's
+    loop: the handle refactorises P and refreshes the screening tables)."""
+    from chemtoolscuda_amd.autograd import DensityModel
+
+    w = synthetic.diff40()
+    order = synthetic.morton_order(w.points)
+    pts = torch.from_numpy(np.ascontiguousarray(w.points[order])).cuda()
+    wts = torch.from_numpy(np.ascontiguousarray(w.extra["weights"][order])).cuda()
+    n = pts.shape[0]
+    ref = DensityModel(w.basis, w.charges, precision="fp64")
+    model = DensityModel(w.basis, w.charges, precision=precision)
+    R0 = torch.from_numpy(w.basis.atom_xyz.copy()).cuda()
+    with torch.no_grad():
+        rho_t = ref.density(torch.from_numpy(w.extra["P_target"]).cuda(), R0, pts)
+    P = torch.from_numpy(w.P.copy()).cuda().requires_grad_(True)
+    R = R0.clone().requires_grad_(True)
+    tgt = rho_t if precision == "mixed" else None
+
+    def step(k):
+        if move:
+            with torch.no_grad():
+                P.add_(1e-6 * (k + 1))
+                R.add_(1e-4)
+        P.grad = None
+        R.grad = None
+        loss = torch.sum(wts * (model.density(P, R, pts, rho_target=tgt) - rho_t) ** 2)
+        loss.backward()
+
+    step(0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(reps):
+        step(k + 1)
+    torch.cuda.synchronize()
+    rate = n * reps / (time.perf_counter() - t0)
+    rep = model.last_report or {}
+    return rate, rep.get("n_reevaluated", 0) / n
 
 
 def esp_ncoef(L):
@@ -180,6 +242,24 @@ def esp_rate(torch, capi, synthetic, reps=3):
     return res
 
 
+def host_api_rate(w, pts_host, quantities, reps=2):
+    """The drop-in API as the reference's users call it: Molecule.compute on
+    numpy arrays, host points in and host outputs out (copies included), in
+    both precisions."""
+    import chemtoolscuda_amd as ct
+
+    mol = ct.molecule_from_basis(w.basis, w.P, w.charges)
+    res = {}
+    for prec in ("mixed", "fp64"):
+        mol.compute(pts_host, quantities, precision=prec)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            mol.compute(pts_host, quantities, precision=prec)
+        res[f"{prec}_points_per_s"] = pts_host.shape[0] * reps / (time.perf_counter() - t0)
+    res["api"] = "Molecule.compute(numpy (N,3), ['rho','grad','lap','tau_pd']) -> numpy, points and outputs in host memory"
+    return res
+
+
 def _err_scales(torch, ref):
     """Per-point error scales: rho, tau pointwise; grad against max(|grad|, sqrt(8 rho tau))
     (the Cauchy-Schwarz scale of d_k phi.P phi for a PSD P: grad vanishes at critical
@@ -217,7 +297,7 @@ def _max_errors(torch, outs, ref, names):
     return res
 
 
-def large_rate(torch, capi, synthetic, n_points=200_000):
+def large_rate(torch, capi, synthetic, n_points=1_000_000):
     """300-atom workload (M = 5400 Cartesian, f on H, f+g on heavy atoms):
     Hessian, RDG and Shannon (with rho, grad, lap, tau_PD) in the mixed and
     FP64 paths, points/s of each and the mixed path's per-quantity errors."""
@@ -231,7 +311,8 @@ def large_rate(torch, capi, synthetic, n_points=200_000):
     for prec in ("mixed", "fp64"):
         o = {q: torch.empty((n, capi.WIDTH.get(q, 1)), dtype=torch.float64, device="cuda") for q in qs}
         ptrs = {q: t.data_ptr() for q, t in o.items()}
-        h.eval(pts.data_ptr(), n, ptrs, prec)
+        if prec == "mixed":  # (FP64: ~6 s per call at 10^6 points; its one call is the timed one)
+            h.eval(pts.data_ptr(), n, ptrs, prec)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         rep = h.eval(pts.data_ptr(), n, ptrs, prec)
@@ -364,8 +445,22 @@ def main():
         extra["n_reevaluated"] = rep.get("n_reevaluated") if rep else None
 
     if not args.no_extra:
-        extra["fwd_bwd_points_per_s"] = {p: fwd_bwd(torch, capi, synthetic, p) for p in ("fp64", "mixed")}
-        extra["fwd_bwd_workload"] = "diff40: 40 atoms, M=416, 500000 points, rho + dL/dP + dL/dR per step"
+        extra["host_arrays"] = host_api_rate(w, pts_host, quantities)
+        fb, bw_ms = {}, {}
+        for p in ("fp64", "mixed"):
+            fb[p], bw_ms[p] = fwd_bwd(torch, capi, synthetic, p)
+        extra["fwd_bwd_points_per_s"] = fb
+        extra["fwd_bwd_backward_ms_per_call"] = bw_ms
+        ag = {}
+        for p in ("fp64", "mixed"):
+            rate, fr = fwd_bwd_autograd(torch, synthetic, p)
+            rate_mv, _ = fwd_bwd_autograd(torch, synthetic, p, move=True)
+            ag[p] = {"points_per_s": rate, "points_per_s_moving_P_R": rate_mv, "reevaluated_fraction": fr}
+        ag["mixed_over_fp64"] = ag["mixed"]["points_per_s"] / ag["fp64"]["points_per_s"]
+        extra["fwd_bwd_autograd"] = ag
+        extra["fwd_bwd_workload"] = ("diff40: 40 atoms, M=416, 500000 points, rho + dL/dP + dL/dR per step; "
+                                     "autograd: L = sum w (rho - rho_t)^2, mixed forward held to 1e-6 of "
+                                     "|rho - rho_t| (rho_target)")
         extra["esp"] = esp_rate(torch, capi, synthetic)
         extra["large300"] = large_rate(torch, capi, synthetic)
 

@@ -11,11 +11,15 @@ pointers.
 
 (The reference has no differentiation; these are new entry points.)
 
-Losses on differences of densities, e.g. sum_g w_g (rho_g - rho_t,g)^2 with
-rho_t from P_t, are best written on ``density(P - P_t, R, points)``: rho is
-linear in P, so this is the same function, and the contraction of the
-difference keeps the mixed path's FP32-level accuracy relative to the residual
-itself rather than to rho.
+Losses on residuals, e.g. sum_g w_g (rho_g - rho_t,g)^2: their gradients
+need rho - rho_t, not rho, to the target accuracy.  Pass the target as
+``density(P, R, points, rho_target=rho_t)``: the mixed forward then holds its
+error to 1e-6 (``DensityModel.tol``) of |rho - rho_t| per point, re-evaluating
+in FP64 the points whose error bound misses that (``ct_eval_params.rho_target``).
+
+The handle keeps P and the centres between calls: setting values equal to the
+current ones (the backward after its forward, a loop that passes the same
+tensors) costs a copy and a comparison, not the factorisation of P.
 """
 from __future__ import annotations
 
@@ -29,18 +33,22 @@ from .basis import BasisSet
 class DensityModel:
     """Owns one C-ABI handle for a basis set; call ``density(P, R, points)``."""
 
-    def __init__(self, basis: BasisSet, charges=None, precision: str = "fp64"):
+    def __init__(self, basis: BasisSet, charges=None, precision: str = "fp64", tol: float = 1e-6):
         if precision not in capi.PREC:
             raise ValueError("precision must be 'fp64' or 'mixed'")
         self.basis = basis
         self.precision = precision
+        self.tol = float(tol)
+        self.last_report = None  # ct_eval_report of the last forward (points re-evaluated in FP64, ...)
         M = basis.nbasis
         Z = np.zeros(basis.natom) if charges is None else np.asarray(charges, dtype=np.float64)
         self.handle = capi.Handle(basis.flat(), np.zeros((M, M)), Z)
 
     def _upload(self, P: torch.Tensor, R: torch.Tensor) -> None:
-        """Makes the handle's P and centres those of this call (always: a tensor
-        address can be reused by the caching allocator for different values)."""
+        """Makes the handle's P and centres those of this call (the library
+        compares them with what it holds and skips the work when they are
+        equal; a tensor address alone proves nothing, the caching allocator
+        reuses them)."""
         if P.shape != (self.basis.nbasis, self.basis.nbasis) or R.shape != (self.basis.natom, 3):
             raise ValueError("P must be (nbasis, nbasis) and R (natom, 3)")
         if P.is_cuda or R.is_cuda:
@@ -48,8 +56,11 @@ class DensityModel:
         self.handle.set_density_matrix(P.data_ptr())
         self.handle.set_coordinates(R.data_ptr())
 
-    def density(self, P: torch.Tensor, R: torch.Tensor, points: torch.Tensor) -> torch.Tensor:
-        return _Density.apply(P, R, points, self)
+    def density(self, P: torch.Tensor, R: torch.Tensor, points: torch.Tensor,
+                rho_target: torch.Tensor | None = None) -> torch.Tensor:
+        """rho at the points, differentiable in P and R.  rho_target (mixed
+        precision): the error is held to tol * |rho - rho_target| per point."""
+        return _Density.apply(P, R, points, self, rho_target)
 
 
 def _as_f64(t: torch.Tensor) -> torch.Tensor:
@@ -58,16 +69,22 @@ def _as_f64(t: torch.Tensor) -> torch.Tensor:
 
 class _Density(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, P, R, points, model: DensityModel):
+    def forward(ctx, P, R, points, model: DensityModel, rho_target=None):
         P64, R64, pts = _as_f64(P), _as_f64(R), _as_f64(points)
         if pts.dim() != 2 or pts.shape[1] != 3:
             raise ValueError("points must have shape (N, 3)")
         model._upload(P64, R64)
         n = pts.shape[0]
         rho = torch.empty(n, dtype=torch.float64, device=pts.device)
+        params = {"mixed_tol": model.tol}
+        if rho_target is not None:
+            tgt = _as_f64(rho_target).to(pts.device)
+            if tgt.shape != (n,):
+                raise ValueError("rho_target must have one entry per point")
+            params["rho_target"] = tgt.data_ptr()
         if pts.is_cuda:
             torch.cuda.current_stream().synchronize()
-        model.handle.eval(pts.data_ptr(), n, {"rho": rho.data_ptr()}, model.precision)
+        model.last_report = model.handle.eval(pts.data_ptr(), n, {"rho": rho.data_ptr()}, model.precision, params)
         ctx.model = model
         ctx.save_for_backward(P64, R64, pts)
         return rho
@@ -86,9 +103,10 @@ class _Density(torch.autograd.Function):
             torch.cuda.current_stream().synchronize()
         model.handle.backward(pts.data_ptr(), pts.shape[0], g.data_ptr(), dP.data_ptr(), dR.data_ptr(),
                               model.precision)
-        return dP, dR, None, None
+        return dP, dR, None, None, None
 
 
-def density(model: DensityModel, P: torch.Tensor, R: torch.Tensor, points: torch.Tensor) -> torch.Tensor:
+def density(model: DensityModel, P: torch.Tensor, R: torch.Tensor, points: torch.Tensor,
+            rho_target: torch.Tensor | None = None) -> torch.Tensor:
     """Functional form of ``DensityModel.density``."""
-    return _Density.apply(P, R, points, model)
+    return _Density.apply(P, R, points, model, rho_target)

@@ -31,7 +31,7 @@ class BasisDesc(ctypes.Structure):
 class EvalParams(ctypes.Structure):
     _fields_ = [("gge_a", ctypes.c_double), ("gge_b", ctypes.c_double), ("ked_a", ctypes.c_double),
                 ("ked_alpha", ctypes.c_double), ("mixed_tol", ctypes.c_double), ("rho_floor", ctypes.c_double),
-                ("route", ctypes.c_int32), ("out_layout", ctypes.c_int32)]
+                ("route", ctypes.c_int32), ("out_layout", ctypes.c_int32), ("rho_target", ctypes.c_void_p)]
 
 
 class Outputs(ctypes.Structure):

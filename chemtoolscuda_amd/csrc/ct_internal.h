@@ -205,6 +205,11 @@ struct ScreenLayout {
 };
 ScreenLayout build_screen_layout(const std::vector<int>& shell_l, const std::vector<int>& shell_atom,
                                  const std::vector<double>& radius, const double* xyz, int nbasis_pad);
+// Refills span_off / spans of a layout (its order and function offsets kept)
+// from the given centres and radii: after the centres move by less than what
+// warrants a new order.
+void screen_layout_spans(ScreenLayout& out, const std::vector<int>& shell_l, const std::vector<int>& shell_atom,
+                         const std::vector<double>& radius, const double* xyz, int nbasis_pad);
 // Lattice of the Morton keys: cell (floor((x - origin) * inv_cell)), 8 bits
 // per axis, clamped.
 struct MortonFrame {
@@ -240,6 +245,7 @@ struct FlagArgs {
   int64_t index_base;    // added to the chunk-relative index of a flagged point
   unsigned long long* max_est;  // max estimate / |rho| over accepted points (bit pattern of a double)
   double trunc;          // screened factor route: + trunc * sqrt|rho| (bound of the skipped AO blocks)
+  const double* target;  // optional, chunk positions: the estimate is held against tol * |rho - target|
 };
 void launch_finalize_mixed(const FinalizeArgs& a, const FlagArgs& f, hipStream_t s);
 // out (n, 3) row-major: the points index[k] (index == nullptr: k) of pts,

@@ -110,14 +110,16 @@ __global__ __launch_bounds__(256) void finalize_mixed_kernel(FinalizeArgs a, Fla
   const int64_t pp = a.gather ? p : a.index ? (int64_t)a.index[p] : p;
   if (live) emit_outputs(a, a.out_off + pp, dot);
   const double arho = fabs(dot[0]);
-  double rel = 0.0;  // estimate / |rho| of an accepted point (non-negative: the bits order like the values)
+  // what the error is held against: |rho|, or the residual |rho - target|
+  const double ref = f.target ? fabs(dot[0] - f.target[pp]) : arho;
+  double rel = 0.0;  // estimate / ref of an accepted point (non-negative: the bits order like the values)
   bool flag = false;
   if (live && !(arho < f.rho_floor)) {
     const double est = f.gamma * bound + f.trunc * sqrt(arho);
-    if (!(est <= f.tol * arho)) {  // also catches NaN
+    if (!(est <= f.tol * ref)) {  // also catches NaN
       flag = true;
-    } else {
-      rel = est / arho;
+    } else if (ref > 0.0) {
+      rel = est / ref;
     }
   }
   // flagged points: one atomic per wave for its count, each lane at its rank

@@ -300,8 +300,21 @@ void Mol::upload_geometry() {
     geom[s] = ShellGeom{xyz_[3 * a], xyz_[3 * a + 1], xyz_[3 * a + 2], amin};
   }
   CT_HIP_CHECK(hipMemcpy(d_geom_.get(), geom.data(), sizeof(ShellGeom) * nshell_, hipMemcpyHostToDevice));
-  layout_stale_ = true;
-  if (screen_built_) build_screen();  // the shell order follows the centres
+  // the screened shell order follows the centres: rebuilt (annealing, a
+  // fraction of a second) once some centre has moved by more than
+  // kLayoutMove since it was built; smaller moves keep the order and refresh
+  // the k-step tables with the new centres
+  double moved = 1e300;
+  if (xyz_layout_.size() == xyz_.size()) {
+    moved = 0.0;
+    for (int a = 0; a < natom_; ++a) {
+      double d2 = 0.0;
+      for (int k = 0; k < 3; ++k) d2 += (xyz_[3 * a + k] - xyz_layout_[3 * a + k]) * (xyz_[3 * a + k] - xyz_layout_[3 * a + k]);
+      moved = std::max(moved, std::sqrt(d2));
+    }
+  }
+  if (moved > kLayoutMove) layout_stale_ = true;
+  if (screen_built_) build_screen();
 }
 
 namespace {
@@ -445,6 +458,25 @@ void Mol::set_screening(double eps) {
   build_screen();
 }
 
+void Mol::set_morton_frame() {
+  // Morton lattice: the atoms' box with a margin of 12 bohr, 256 cells per
+  // axis, cells of at least 0.25 bohr (points outside clamp to the border;
+  // order within a cell is the input order, which only matters below the
+  // size of a point tile)
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int a = 0; a < natom_; ++a)
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = std::min(lo[k], xyz_[3 * a + k]);
+      hi[k] = std::max(hi[k], xyz_[3 * a + k]);
+    }
+  double span = 0.0;
+  for (int k = 0; k < 3; ++k) {
+    frame_.origin[k] = lo[k] - 12.0;
+    span = std::max(span, hi[k] - lo[k] + 24.0);
+  }
+  frame_.inv_cell = 1.0 / std::max(0.25, span / 256.0);
+}
+
 void Mol::build_screen() {
   // the shell order and the radii depend on the centres and the threshold
   // only: a new P keeps them
@@ -465,23 +497,13 @@ void Mol::build_screen() {
       radius[s] = shell_screen_radius(shell_l[s], m.nprim, a.data(), cn.data(), screen_eps_);
     }
     layout_ = build_screen_layout(shell_l, shell_atom, radius, xyz_.data(), Mp_);
-    // Morton lattice: the atoms' box with a margin of 12 bohr, 256 cells per
-    // axis, cells of at least 0.25 bohr (points outside clamp to the border;
-    // order within a cell is the input order, which only matters below the
-    // size of a point tile)
-    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-    for (int a = 0; a < natom_; ++a)
-      for (int k = 0; k < 3; ++k) {
-        lo[k] = std::min(lo[k], xyz_[3 * a + k]);
-        hi[k] = std::max(hi[k], xyz_[3 * a + k]);
-      }
-    double span = 0.0;
-    for (int k = 0; k < 3; ++k) {
-      frame_.origin[k] = lo[k] - 12.0;
-      span = std::max(span, hi[k] - lo[k] + 24.0);
-    }
-    frame_.inv_cell = 1.0 / std::max(0.25, span / 256.0);
-  } else if (screen_eps_ <= 0.0) {
+    xyz_layout_ = xyz_;
+    set_morton_frame();
+  } else if (screen_eps_ > 0.0) {
+    // same order, current centres (small moves since the order was built)
+    screen_layout_spans(layout_, shell_l, shell_atom, radius, xyz_.data(), Mp_);
+    set_morton_frame();
+  } else {
     layout_ = ScreenLayout{};
     layout_.order.resize(nshell_);
     layout_.first.resize(nshell_);
@@ -589,13 +611,21 @@ bool Mol::factor_route(int mode, int route, bool mixed) const {
   }
 }
 
+// Both setters compare the new values with the current ones first: setting
+// the same centres or the same P again (an autograd backward after its
+// forward, a loop that resets them) costs a copy and a comparison, not the
+// factorisation, the bf16 splits and the screening tables.
 void Mol::set_coordinates(const double* xyz) {
   CT_REQUIRE(xyz != nullptr, "null coordinates");
   CT_HIP_CHECK(hipStreamSynchronize(stream_));
+  std::vector<double> v(3 * (size_t)natom_);
   if (is_device_pointer(xyz))
-    CT_HIP_CHECK(hipMemcpy(xyz_.data(), xyz, sizeof(double) * 3 * natom_, hipMemcpyDeviceToHost));
+    CT_HIP_CHECK(hipMemcpy(v.data(), xyz, sizeof(double) * v.size(), hipMemcpyDeviceToHost));
   else
-    std::memcpy(xyz_.data(), xyz, sizeof(double) * 3 * natom_);
+    std::memcpy(v.data(), xyz, sizeof(double) * v.size());
+  if (std::memcmp(v.data(), xyz_.data(), sizeof(double) * v.size()) == 0) return;
+  for (double c : v) CT_REQUIRE(std::isfinite(c), "coordinates must be finite");
+  xyz_.swap(v);
   upload_geometry();
   esp_valid_ = false;
 }
@@ -603,10 +633,13 @@ void Mol::set_coordinates(const double* xyz) {
 void Mol::set_density_matrix(const double* P) {
   CT_REQUIRE(P != nullptr, "null density matrix");
   CT_HIP_CHECK(hipStreamSynchronize(stream_));
+  std::vector<double> v((size_t)M_ * M_);
   if (is_device_pointer(P))
-    CT_HIP_CHECK(hipMemcpy(P_.data(), P, sizeof(double) * (size_t)M_ * M_, hipMemcpyDeviceToHost));
+    CT_HIP_CHECK(hipMemcpy(v.data(), P, sizeof(double) * v.size(), hipMemcpyDeviceToHost));
   else
-    std::memcpy(P_.data(), P, sizeof(double) * (size_t)M_ * M_);
+    std::memcpy(v.data(), P, sizeof(double) * v.size());
+  if (std::memcmp(v.data(), P_.data(), sizeof(double) * v.size()) == 0) return;
+  P_.swap(v);
   upload_P();
   esp_valid_ = false;
 }
@@ -952,17 +985,23 @@ void Mol::eval(const double* points, int64_t n, int layout, uint32_t q, int prec
   // (one large launch instead of one small launch per chunk).
   bool all_dev = pts_dev && n < (int64_t(1) << 31);  // (grid points are not materialised: per-chunk re-evaluation)
   for (size_t k = 0; k < NS; ++k) all_dev = all_dev && out_dev[k];
+  // a per-point target density (mixed path): chunk slices of the caller's
+  // device array, or staged from host memory
+  const bool has_target = mixed && par.rho_target != nullptr;
+  const bool tgt_dev = has_target && is_device_pointer(par.rho_target);
   if (fixed)
-    CT_REQUIRE(cube == nullptr && all_dev,
-               "allocation-free evaluation: points and outputs must be device memory, fewer than 2^31 points");
+    CT_REQUIRE(cube == nullptr && all_dev && (!has_target || tgt_dev),
+               "allocation-free evaluation: points, outputs and targets must be device memory, fewer than 2^31 "
+               "points");
   // (the call-wide flag list is an allocation: the allocation-free path
   // re-evaluates chunk by chunk)
   const bool batch = mixed && all_dev && !fixed;
   const int64_t ao_bytes_pt = (int64_t)nder * Mp_ * (mixed ? 4 : 8);
   const int64_t col_stage_pt = col_in && !pts_dev ? 24 : 0;
+  const int64_t tgt_stage_pt = has_target && !tgt_dev ? 8 : 0;
   const int64_t per_point = (mixed ? ao_bytes_pt + (int64_t)nrt * (ndot + 1) * 8 + 4 + 24 + stage_width * 8
                                    : ao_bytes_pt + (int64_t)ndot * 8 + 24 + stage_width * 8) +
-                            col_stage_pt;
+                            col_stage_pt + tgt_stage_pt;
   // screened: keys (2 x 4 B), indices (2 x 4 B), sorted points (24 B) per
   // point; k-step list (2 nk B), count and mask (4 + 4 nwords B) per tile
   const int64_t per_point_s = screened ? 40 + (4 * nk + 4 + 4 * nwords + tp - 1) / tp : 0;
@@ -1027,6 +1066,8 @@ void Mol::eval(const double* points, int64_t n, int layout, uint32_t q, int prec
   cur += (size_t)Nc * 4;
   double* col_stage = reinterpret_cast<double*>(cur);
   cur += (size_t)Nc * col_stage_pt;
+  double* tgt_stage = reinterpret_cast<double*>(cur);
+  cur += (size_t)Nc * tgt_stage_pt;
   // screened layout buffers (256-byte aligned pieces)
   auto take = [&](size_t bytes) {
     char* p = cur;
@@ -1183,6 +1224,16 @@ void Mol::eval(const double* points, int64_t n, int layout, uint32_t q, int prec
       fl.index_base = batch ? start : 0;
       fl.max_est = d_maxest;
       fl.trunc = screened ? screen_trunc_ : 0.0;
+      fl.target = nullptr;
+      if (has_target) {
+        if (tgt_dev) {
+          fl.target = par.rho_target + start;
+        } else {
+          CT_HIP_CHECK(hipMemcpyAsync(tgt_stage, par.rho_target + start, sizeof(double) * nc, hipMemcpyHostToDevice,
+                                      stream_));
+          fl.target = tgt_stage;
+        }
+      }
       timer_.begin(KernelTimer::FINALIZE, stream_);
       launch_finalize_mixed(fa, fl, stream_);
       timer_.end(stream_);

@@ -155,6 +155,7 @@ class Mol {
   // that order, k-step spans, truncation bounds); after P, the geometry or
   // the threshold change
   void build_screen();
+  void set_morton_frame();  // the Morton lattice of the point sort, from the centres
   int64_t chunk_points(int64_t per_point, int64_t n) const;
   // FP64 re-evaluation of `count` points (pts: (m, 3) row-major, col_stride
   // 0, or column-major with that stride; index: positions in pts), in slices
@@ -186,7 +187,9 @@ class Mol {
   // spatial screening of the mixed factor route
   double screen_eps_ = 1e-15;
   bool screen_built_ = false;
-  bool layout_stale_ = true;           // centres or threshold changed since the layout was built
+  bool layout_stale_ = true;           // the shell order must be rebuilt (threshold changed, centres moved far)
+  std::vector<double> xyz_layout_;     // centres the screened shell order was built for
+  static constexpr double kLayoutMove = 1.0;  // bohr: a larger move of any centre rebuilds the order
   std::vector<double> shell_radius_;   // cutoff radii at screen_eps_
   ScreenLayout layout_;    // identity order (and no spans) when screening is off
   MortonFrame frame_{};

@@ -294,8 +294,16 @@ ScreenLayout build_screen_layout(const std::vector<int>& shell_l, const std::vec
     for (int c = 0; c < shell_nfunc(shell_l[s]); ++c) out.func_src.push_back(nat_first[s] + c);
     f += shell_nfunc(shell_l[s]);
   }
+  screen_layout_spans(out, shell_l, shell_atom, radius, xyz, nbasis_pad);
+  return out;
+}
+
+void screen_layout_spans(ScreenLayout& out, const std::vector<int>& shell_l, const std::vector<int>& shell_atom,
+                         const std::vector<double>& radius, const double* xyz, int nbasis_pad) {
+  const int nshell = (int)shell_l.size();
   const int nk = nbasis_pad / 32;
   out.span_off.assign(nk + 1, 0);
+  out.spans.clear();
   std::vector<std::vector<int>> per_k(nk);
   for (int s = 0; s < nshell; ++s) {
     const int k0 = out.first[s] / 32, k1 = (out.first[s] + shell_nfunc(shell_l[s]) - 1) / 32;
@@ -308,7 +316,6 @@ ScreenLayout build_screen_layout(const std::vector<int>& shell_l, const std::vec
       out.spans.push_back(ScreenSpan{c[0], c[1], c[2], radius[s] * radius[s]});
     }
   }
-  return out;
 }
 
 }  // namespace ct

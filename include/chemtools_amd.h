@@ -111,6 +111,13 @@ typedef struct {
   double rho_floor;        /* points with rho below this are not re-evaluated (0 => 1e-10) */
   int32_t route;           /* mixed path: CT_ROUTE_AUTO (0), CT_ROUTE_DENSITY, CT_ROUTE_FACTOR */
   int32_t out_layout;      /* grad / hess outputs: CT_LAYOUT_ROW (0) or CT_LAYOUT_COL */
+  /* Mixed path, optional (NULL: off): a target density per point (n
+   * entries, host or device).  The error of rho is then bounded by
+   * mixed_tol * |rho - rho_target| instead of mixed_tol * |rho| (points that
+   * miss it are re-evaluated in FP64): for losses on the residual
+   * rho - rho_target, whose gradients need the residual, not rho, to
+   * mixed_tol.  Points with |rho| below rho_floor are still accepted. */
+  const double* rho_target;
 } ct_eval_params;
 
 /* Routes of the mixed contraction (rho, grad and lap quantities).  DENSITY
