@@ -68,6 +68,8 @@ def parse():
                     help="spatial screening threshold of the mixed factor route (0 turns it off; "
                          "default: the library's, 1e-15)")
     ap.add_argument("--no-extra", action="store_true", help="skip the fwd+bwd and ESP secondary measurements")
+    ap.add_argument("--part", default=None, choices=["esp", "fwd_bwd", "autograd", "large300", "host"],
+                    help="run only this secondary measurement and print its JSON (for profiling)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="only run a device-to-device copy of MB MiB (calibrates FETCH_SIZE/WRITE_SIZE counters)")
     return ap.parse_args()
@@ -346,6 +348,21 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes": a.numel()}))
         return
+    if args.part:
+        if args.part == "esp":
+            out = esp_rate(torch, capi, synthetic)
+        elif args.part == "fwd_bwd":
+            out = {p: fwd_bwd(torch, capi, synthetic, p) for p in ("fp64", "mixed")}
+        elif args.part == "autograd":
+            out = {p: fwd_bwd_autograd(torch, synthetic, p) for p in ("fp64", "mixed")}
+        elif args.part == "large300":
+            out = large_rate(torch, capi, synthetic)
+        else:
+            w = synthetic.organic60()
+            pts_host = np.ascontiguousarray(w.points[synthetic.morton_order(w.points)])
+            out = host_api_rate(w, pts_host, ["rho", "grad", "lap", "tau_pd"])
+        print(json.dumps({"part": args.part, **({"result": out} if not isinstance(out, dict) else out)}))
+        return
 
     meta = json.load(open(os.path.join(ROOT, "BASELINE.json")))
     w = synthetic.WORKLOADS[args.workload]()
@@ -451,6 +468,18 @@ def main():
             fb[p], bw_ms[p] = fwd_bwd(torch, capi, synthetic, p)
         extra["fwd_bwd_points_per_s"] = fb
         extra["fwd_bwd_backward_ms_per_call"] = bw_ms
+        # the backward call (AO + Gram + Y = P phi per chunk): 4 M^2 flop per
+        # point (dL/dP: 2 M^2; dL/dR: 2 M^2), 5e5 points per call
+        m_d = synthetic.diff40(1).basis.nbasis
+        f_bw = 4.0 * m_d * m_d * 500_000
+        a_bw = f_bw / (bw_ms["mixed"] * 1e-3)
+        extra["fwd_bwd_roofline"] = {"bound": "mfma", "achieved": a_bw / 1e12, "peak": PEAK["bf16"] / 1e12,
+                                     "unit": "TFLOP/s", "frac": a_bw / PEAK["bf16"], "traffic": None,
+                                     "kernel": "backward (gram_mixed + contract_adjoint_mixed + AO)",
+                                     "avg_launch_ms": bw_ms["mixed"], "flops_per_point": 4.0 * m_d * m_d,
+                                     "fp64": {"achieved": f_bw / (bw_ms["fp64"] * 1e-3) / 1e12,
+                                              "peak": PEAK["fp64"] / 1e12,
+                                              "frac": f_bw / (bw_ms["fp64"] * 1e-3) / PEAK["fp64"]}}
         ag = {}
         for p in ("fp64", "mixed"):
             rate, fr = fwd_bwd_autograd(torch, synthetic, p)
