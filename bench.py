@@ -440,26 +440,32 @@ def main():
 
     extra = {}
     if precision == "mixed" and not args.no_compare:
+        # FP64 on the same route as the mixed run (AUTO: the screened factor
+        # route when P has a low-rank factor): the like-for-like ratio
         ref = alloc()
-        run(ref, "fp64")  # warm-up (workspace growth) and the reference values
+        rep64 = run(ref, "fp64")  # warm-up (workspace growth)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
-        run(ref, "fp64")
+        rep64 = run(ref, "fp64")
         torch.cuda.synchronize()
         extra["fp64_points_per_s"] = n / (time.perf_counter() - t1)
+        extra["fp64_route"] = "factor (screened)" if rep64.get("factor_rank", -1) >= 0 else "density"
         extra["mixed_over_fp64"] = value / extra["fp64_points_per_s"]
-        if rank >= 0:  # FP64 on the factor route (on request only: the FP64 default contracts with P)
-            tmp = alloc()
-            h.eval(pts.data_ptr(), n, {q: t.data_ptr() for q, t in tmp.items()}, "fp64", {"route": 2})
-            torch.cuda.synchronize()
-            t2 = time.perf_counter()
-            h.eval(pts.data_ptr(), n, {q: t.data_ptr() for q, t in tmp.items()}, "fp64", {"route": 2})
-            torch.cuda.synchronize()
-            extra["fp64_factor_route_points_per_s"] = n / (time.perf_counter() - t2)
-            del tmp
-        for q, v in _max_errors(torch, outs, ref, quantities).items():
+        # FP64 contracting with P itself (the reference's formulation): the
+        # reference values for the error figures, and the ratio against them
+        dens = alloc()
+        h.eval(pts.data_ptr(), n, {q: t.data_ptr() for q, t in dens.items()}, "fp64", {"route": 1})
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        h.eval(pts.data_ptr(), n, {q: t.data_ptr() for q, t in dens.items()}, "fp64", {"route": 1})
+        torch.cuda.synchronize()
+        extra["fp64_density_route_points_per_s"] = n / (time.perf_counter() - t2)
+        extra["mixed_over_fp64_density_route"] = value / extra["fp64_density_route_points_per_s"]
+        for q, v in _max_errors(torch, outs, dens, quantities).items():
             extra[f"max_rel_err_{q}"] = v
+        extra["fp64_factor_vs_density_max_rel_err"] = max(_max_errors(torch, ref, dens, quantities).values())
         extra["n_reevaluated"] = rep.get("n_reevaluated") if rep else None
+        del ref, dens
 
     if not args.no_extra:
         extra["host_arrays"] = host_api_rate(w, pts_host, quantities)

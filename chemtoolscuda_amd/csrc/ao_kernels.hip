@@ -259,9 +259,11 @@ void launch_mode(int mode, int lmax, const ShellMeta* meta, const ShellGeom* geo
 
 void launch_ao_f64(int mode, int lmax, const ShellMeta* meta, const ShellGeom* geom, const PrimData* prim, int nshell,
                    int nbasis, int nbasis_pad, const double* pts, int64_t npts, int64_t ld, double* ao,
-                   double screen, hipStream_t s) {
-  launch_mode<double>(mode, lmax, meta, geom, prim, nshell, nbasis, nbasis_pad, pts, npts, ld, ao, screen, nullptr, 0,
-                      64, s);
+                   double screen, hipStream_t s, const uint32_t* kmask, int nwords, int tile_points) {
+  CT_REQUIRE(!kmask || (tile_points > 0 && 64 % tile_points == 0 && nwords == (nbasis_pad / 32 + 31) / 32),
+             "activity masks do not match the AO kernel's waves");
+  launch_mode<double>(mode, lmax, meta, geom, prim, nshell, nbasis, nbasis_pad, pts, npts, ld, ao, screen, kmask,
+                      nwords, tile_points, s);
 }
 
 void launch_ao_f32(int mode, int lmax, const ShellMeta* meta, const ShellGeom* geom, const PrimData* prim, int nshell,

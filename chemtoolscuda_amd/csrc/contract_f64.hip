@@ -239,10 +239,20 @@ enum VF64 : int { F64_F1, F64_F4, F64_F2, F64_FH0, F64_FH1 };
 // lap and Hessian F4 tiles keep two (their 3-wave builds spill).  Few-point
 // launches (the re-evaluation of a mixed call's flagged points) are latency
 // bound: 1.14 -> 0.79 ms per step for 25 000 points.
-template <int VF, int MODE>
-__global__ __launch_bounds__(256, VF == F64_F4 && MODE != MODE_GRAD ? 2 : 3) void contract_f64_factor_kernel(const double* __restrict__ L, int rows, int mp,
+// kmask (optional, the screened layout): per tile of tp points, the 32-function
+// k-steps holding a shell within its cutoff radius (nwords 32-bit words per
+// tile); a wave contracts the union of its tiles' k-steps only (a wave's
+// points span at most two tiles).  kmask null: every k-step.
+
+// SCR: the screened layout (kmask given), at two waves per SIMD (the masked
+// k-loop needs a few more registers than three waves leave); unscreened
+// launches keep the plain k-loop and three waves where the variant allows.
+template <int VF, int MODE, bool SCR>
+__global__ __launch_bounds__(256, SCR || (VF == F64_F4 && MODE != MODE_GRAD) ? 2 : 3) void contract_f64_factor_kernel(const double* __restrict__ L, int rows, int mp,
                                                                   const double* __restrict__ ao, int64_t ld,
-                                                                  double* __restrict__ dots) {
+                                                                  double* __restrict__ dots,
+                                                                  const uint32_t* __restrict__ kmask, int nwords,
+                                                                  int tp) {
   constexpr int NT = VF == F64_F1 ? 1 : VF == F64_F2 ? 2 : 4;
   constexpr int PPW = 64 / NT;   // points per wave
   constexpr int NPB = PPW / 16;  // 16-point column blocks per type
@@ -251,11 +261,32 @@ __global__ __launch_bounds__(256, VF == F64_F4 && MODE != MODE_GRAD ? 2 : 3) voi
   // (lap: t_xx, t_yy, t_zz | Hessian: t_xx, t_xy, t_yy, t_xz, t_yz, t_zz)]
   constexpr int NDV = VF == F64_F1 || VF == F64_F2 ? 1 : VF == F64_F4 ? (LAP ? 7 : HES ? 10 : 4) : 3;
   const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
   const int r = lane & 15, q = lane >> 4;
   const int64_t p0 = ((int64_t)blockIdx.x * 4 + wave) * PPW;
   const int64_t sd = (int64_t)mp * ld;
   const bool split = gridDim.y > 1;
+  // the wave's active k-steps: the union of its (one or two) tiles' masks,
+  // read word by word when the k-loop needs the next active k-step (once per
+  // eight MFMA steps; wave-uniform, scalar loads)
+  const int nk = mp / 32;
+  const int64_t t0 = kmask ? p0 / tp : 0, t1 = kmask ? (p0 + PPW - 1) / tp : 0;
+  // the first active k-step after kb (nk: none)
+  auto next_active = [&](int kb) {
+    int k = kb + 1;
+    while (k < nk) {
+      const int w = k >> 5;
+      uint32_t m = 0xFFFFFFFFu;
+      if (kmask) {
+        m = kmask[t0 * nwords + w];
+        if (t1 != t0) m |= kmask[t1 * nwords + w];
+      }
+      m >>= (k & 31);
+      if (m) return k + __builtin_ctz(m);
+      k = (w + 1) << 5;
+    }
+    return nk;
+  };
   const int i_begin = split ? (int)blockIdx.y * 64 : 0;
   const int i_end = split ? i_begin + 64 : rows;
   dots += split ? (int64_t)blockIdx.y * mode_ndot(MODE) * ld : 0;
@@ -280,28 +311,70 @@ __global__ __launch_bounds__(256, VF == F64_F4 && MODE != MODE_GRAD ? 2 : 3) voi
     for (int rb = 0; rb < 4; ++rb)
 #pragma unroll
       for (int cb = 0; cb < 4; ++cb) acc[rb][cb] = d4{0.0, 0.0, 0.0, 0.0};
-    // A[r][q] = L^T[i0 + 16 rb + r][k0 + q] = L[k0 + q][i0 + 16 rb + r]
+    // A[r][q] = L^T[i0 + 16 rb + r][k0 + q] = L[k0 + q][i0 + 16 rb + r];
+    // the k-loop walks the active 32-function k-steps, eight MFMA steps each,
+    // the operands of the next step prefetched (within a k-step, or the
+    // first of the next active one)
     const double* aptr = L + (int64_t)q * rows + i0 + r;
-    double a[4], b[4];
+    if constexpr (SCR) {
+      int kb = next_active(-1);
+      if (kb < nk) {
+        int kb_next = next_active(kb);
+        int k0 = 32 * kb;
+        double a[4], b[4];
 #pragma unroll
-    for (int rb = 0; rb < 4; ++rb) a[rb] = aptr[16 * rb];
+        for (int rb = 0; rb < 4; ++rb) a[rb] = aptr[(int64_t)k0 * rows + 16 * rb];
 #pragma unroll
-    for (int cb = 0; cb < 4; ++cb) b[cb] = bptr[cb][0];
-    for (int k0 = 0; k0 < mp; k0 += 4) {
-      const int kn = (k0 + 4 < mp) ? k0 + 4 : k0;  // prefetch (clamped on the last step; unused)
-      double an[4], bn[4];
+        for (int cb = 0; cb < 4; ++cb) b[cb] = bptr[cb][(int64_t)k0 * ld];
+        for (;;) {
+          const bool step_end = ((k0 + 4) & 31) == 0;  // last MFMA step of the k-step
+          // prefetch (clamped on the very last step; unused)
+          const int kn = !step_end ? k0 + 4 : (kb_next < nk ? 32 * kb_next : k0);
+          double an[4], bn[4];
 #pragma unroll
-      for (int rb = 0; rb < 4; ++rb) an[rb] = aptr[(int64_t)kn * rows + 16 * rb];
+          for (int rb = 0; rb < 4; ++rb) an[rb] = aptr[(int64_t)kn * rows + 16 * rb];
 #pragma unroll
-      for (int cb = 0; cb < 4; ++cb) bn[cb] = bptr[cb][(int64_t)kn * ld];
+          for (int cb = 0; cb < 4; ++cb) bn[cb] = bptr[cb][(int64_t)kn * ld];
 #pragma unroll
-      for (int rb = 0; rb < 4; ++rb)
+          for (int rb = 0; rb < 4; ++rb)
 #pragma unroll
-        for (int cb = 0; cb < 4; ++cb) acc[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb], b[cb], acc[rb][cb], 0, 0, 0);
+            for (int cb = 0; cb < 4; ++cb)
+              acc[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb], b[cb], acc[rb][cb], 0, 0, 0);
 #pragma unroll
-      for (int rb = 0; rb < 4; ++rb) a[rb] = an[rb];
+          for (int rb = 0; rb < 4; ++rb) a[rb] = an[rb];
 #pragma unroll
-      for (int cb = 0; cb < 4; ++cb) b[cb] = bn[cb];
+          for (int cb = 0; cb < 4; ++cb) b[cb] = bn[cb];
+          if (step_end) {
+            if (kb_next >= nk) break;
+            kb = kb_next;
+            kb_next = next_active(kb);
+          }
+          k0 = kn;
+        }
+      }
+    } else {
+      double a[4], b[4];
+#pragma unroll
+      for (int rb = 0; rb < 4; ++rb) a[rb] = aptr[16 * rb];
+#pragma unroll
+      for (int cb = 0; cb < 4; ++cb) b[cb] = bptr[cb][0];
+      for (int k0 = 0; k0 < mp; k0 += 4) {
+        const int kn = (k0 + 4 < mp) ? k0 + 4 : k0;  // prefetch (clamped on the last step; unused)
+        double an[4], bn[4];
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb) an[rb] = aptr[(int64_t)kn * rows + 16 * rb];
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb) bn[cb] = bptr[cb][(int64_t)kn * ld];
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+          for (int cb = 0; cb < 4; ++cb)
+            acc[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb], b[cb], acc[rb][cb], 0, 0, 0);
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb) a[rb] = an[rb];
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb) b[cb] = bn[cb];
+      }
     }
     // lane (r, q) holds rows q + 4 j of every 16-row block rb, column r of
     // every column block: all column types of its point(s)
@@ -371,10 +444,15 @@ __global__ __launch_bounds__(256, VF == F64_F4 && MODE != MODE_GRAD ? 2 : 3) voi
 
 template <int VF, int MODE>
 void launch_factor_var(const double* L, int rows, int mp, const double* ao, int64_t ld, double* dots, hipStream_t s,
-                       bool split = false) {
+                       bool split = false, const uint32_t* kmask = nullptr, int nwords = 0, int tp = 64) {
   constexpr int PPW = VF == F64_F1 ? 64 : VF == F64_F2 ? 32 : 16;
   const dim3 block(256), grid((unsigned)(ld / (4 * PPW)), split ? (unsigned)(rows / 64) : 1u);  // ld: multiple of 256
-  hipLaunchKernelGGL((contract_f64_factor_kernel<VF, MODE>), grid, block, 0, s, L, rows, mp, ao, ld, dots);
+  if (kmask)
+    hipLaunchKernelGGL((contract_f64_factor_kernel<VF, MODE, true>), grid, block, 0, s, L, rows, mp, ao, ld, dots,
+                       kmask, nwords, tp);
+  else
+    hipLaunchKernelGGL((contract_f64_factor_kernel<VF, MODE, false>), grid, block, 0, s, L, rows, mp, ao, ld, dots,
+                       kmask, nwords, tp);
   CT_HIP_CHECK(hipGetLastError());
 }
 
@@ -382,31 +460,33 @@ void launch_factor_var(const double* L, int rows, int mp, const double* ao, int6
 
 namespace {
 void factor_modes(int mode, const double* L, int rows_pad, int nbasis_pad, const double* ao, int64_t ld, double* dots,
-                  hipStream_t s, bool split) {
+                  hipStream_t s, bool split, const uint32_t* kmask = nullptr, int nwords = 0, int tp = 64) {
   CT_REQUIRE(rows_pad % 64 == 0 && rows_pad > 0 && ld % 256 == 0, "factor padding / point stride");
+  CT_REQUIRE(!kmask || (nwords == (nbasis_pad / 32 + 31) / 32 && tp >= 16 && ld % tp == 0),
+             "activity masks do not match the FP64 factor contraction");
   switch (mode) {
     case MODE_RHO:
-      launch_factor_var<F64_F1, MODE_RHO>(L, rows_pad, nbasis_pad, ao, ld, dots, s, split);
+      launch_factor_var<F64_F1, MODE_RHO>(L, rows_pad, nbasis_pad, ao, ld, dots, s, split, kmask, nwords, tp);
       break;
     case MODE_GRAD:
-      launch_factor_var<F64_F4, MODE_GRAD>(L, rows_pad, nbasis_pad, ao, ld, dots, s, split);
+      launch_factor_var<F64_F4, MODE_GRAD>(L, rows_pad, nbasis_pad, ao, ld, dots, s, split, kmask, nwords, tp);
       break;
     case MODE_LAP:
-      launch_factor_var<F64_F4, MODE_LAP>(L, rows_pad, nbasis_pad, ao, ld, dots, s, split);
-      launch_factor_var<F64_F2, MODE_LAP>(L, rows_pad, nbasis_pad, ao, ld, dots, s, split);
+      launch_factor_var<F64_F4, MODE_LAP>(L, rows_pad, nbasis_pad, ao, ld, dots, s, split, kmask, nwords, tp);
+      launch_factor_var<F64_F2, MODE_LAP>(L, rows_pad, nbasis_pad, ao, ld, dots, s, split, kmask, nwords, tp);
       break;
     default:
-      launch_factor_var<F64_F4, MODE_HESS>(L, rows_pad, nbasis_pad, ao, ld, dots, s, split);
-      launch_factor_var<F64_FH0, MODE_HESS>(L, rows_pad, nbasis_pad, ao, ld, dots, s, split);
-      launch_factor_var<F64_FH1, MODE_HESS>(L, rows_pad, nbasis_pad, ao, ld, dots, s, split);
+      launch_factor_var<F64_F4, MODE_HESS>(L, rows_pad, nbasis_pad, ao, ld, dots, s, split, kmask, nwords, tp);
+      launch_factor_var<F64_FH0, MODE_HESS>(L, rows_pad, nbasis_pad, ao, ld, dots, s, split, kmask, nwords, tp);
+      launch_factor_var<F64_FH1, MODE_HESS>(L, rows_pad, nbasis_pad, ao, ld, dots, s, split, kmask, nwords, tp);
       break;
   }
 }
 }  // namespace
 
 void launch_contract_f64_factor(int mode, const double* L, int rows_pad, int nbasis_pad, const double* ao,
-                                int64_t ld, double* dots, hipStream_t s) {
-  factor_modes(mode, L, rows_pad, nbasis_pad, ao, ld, dots, s, false);
+                                int64_t ld, double* dots, hipStream_t s, const uint32_t* kmask, int nwords, int tp) {
+  factor_modes(mode, L, rows_pad, nbasis_pad, ao, ld, dots, s, false, kmask, nwords, tp);
 }
 
 int64_t contract_f64_factor_split_workspace(int mode, int rows_pad, int64_t ld) {

@@ -122,16 +122,23 @@ struct FinalizeArgs {
   int64_t hess_ld;  // 0: hess (n, 3, 3) rows; else component 3k+l at hess[(3k + l) * hess_ld + o]
 };
 
+// kmask (optional): as for launch_ao_f32 (a wave skips, stores included, the
+// shells no tile of its points reads).
 void launch_ao_f64(int mode, int lmax, const ShellMeta* meta, const ShellGeom* geom, const PrimData* prim, int nshell,
                    int nbasis, int nbasis_pad, const double* pts, int64_t npts, int64_t ld, double* ao,
-                   double screen, hipStream_t s);
+                   double screen, hipStream_t s, const uint32_t* kmask = nullptr, int nwords = 0,
+                   int tile_points = 64);
 void launch_contract_f64(int mode, const double* P, int nbasis_pad, const double* ao, int64_t ld, double* dots,
                          hipStream_t s);
 void launch_finalize(const FinalizeArgs& a, hipStream_t s);
 // The same FP64 dots from a factor P = L L^T (L: nbasis_pad x rows_pad,
 // k-major, rows_pad a multiple of 64): rank(P) columns instead of nbasis.
+// kmask (optional, screened layout): per tile of
+// tile_points points the active 32-function k-steps (launch_tile_activity);
+// only those are contracted.
 void launch_contract_f64_factor(int mode, const double* L, int rows_pad, int nbasis_pad, const double* ao,
-                                int64_t ld, double* dots, hipStream_t s);
+                                int64_t ld, double* dots, hipStream_t s, const uint32_t* kmask = nullptr,
+                                int nwords = 0, int tile_points = 64);
 // The same with each 64-column pass of the factor in its own workgroups and
 // an ordered sum of the passes' partial dots (for small point counts); part
 // holds contract_f64_factor_split_workspace() doubles.

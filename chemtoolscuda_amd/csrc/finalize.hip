@@ -79,9 +79,12 @@ __global__ __launch_bounds__(256) void finalize_kernel(FinalizeArgs a) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= a.n) return;
   const int nd = mode_ndot(a.mode);
+  // gather (screened layout): output point p reads its dots at its sorted
+  // position; else the dots of p go to output index[p] (or p)
+  const int64_t src = a.gather ? (int64_t)a.gather[p] : p;
   double dot[16];
-  for (int d = 0; d < 16; ++d) dot[d] = d < nd ? a.dots[(int64_t)d * a.ld + p] : 0.0;
-  const int64_t o = a.out_off + (a.index ? (int64_t)a.index[p] : p);
+  for (int d = 0; d < 16; ++d) dot[d] = d < nd ? a.dots[(int64_t)d * a.ld + src] : 0.0;
+  const int64_t o = a.out_off + (a.gather ? p : a.index ? (int64_t)a.index[p] : p);
   emit_outputs(a, o, dot);
 }
 

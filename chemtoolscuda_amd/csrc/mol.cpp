@@ -499,6 +499,17 @@ void Mol::build_screen() {
     layout_ = build_screen_layout(shell_l, shell_atom, radius, xyz_.data(), Mp_);
     xyz_layout_ = xyz_;
     set_morton_frame();
+    // the FP64 path's cutoff radii, at its tighter threshold
+    shell_radius64_.assign(nshell_, 0.0);
+    for (int s = 0; s < nshell_; ++s) {
+      const ShellMeta& m = meta_[s];
+      std::vector<double> a(m.nprim), cn(m.nprim);
+      for (int k = 0; k < m.nprim; ++k) {
+        a[k] = prim_[m.prim_off + k].a;
+        cn[k] = prim_[m.prim_off + k].c;
+      }
+      shell_radius64_[s] = shell_screen_radius(shell_l[s], m.nprim, a.data(), cn.data(), screen_eps_ * kFp64ScreenFactor);
+    }
   } else if (screen_eps_ > 0.0) {
     // same order, current centres (small moves since the order was built)
     screen_layout_spans(layout_, shell_l, shell_atom, radius, xyz_.data(), Mp_);
@@ -544,6 +555,17 @@ void Mol::build_screen() {
                            hipMemcpyHostToDevice));
     CT_HIP_CHECK(hipMemcpy(d_spans_.get(), layout_.spans.data(), sizeof(ScreenSpan) * layout_.spans.size(),
                            hipMemcpyHostToDevice));
+    // the same k-steps with the FP64 path's radii
+    ScreenLayout l64;
+    l64.first = layout_.first;
+    screen_layout_spans(l64, shell_l, shell_atom, shell_radius64_, xyz_.data(), Mp_);
+    if (!d_span_off64_.reserve(sizeof(int32_t) * l64.span_off.size()) ||
+        !d_spans64_.reserve(sizeof(ScreenSpan) * std::max<size_t>(1, l64.spans.size())))
+      throw Error(CT_E_NOMEM, "cannot allocate the screening tables");
+    CT_HIP_CHECK(hipMemcpy(d_span_off64_.get(), l64.span_off.data(), sizeof(int32_t) * l64.span_off.size(),
+                           hipMemcpyHostToDevice));
+    CT_HIP_CHECK(hipMemcpy(d_spans64_.get(), l64.spans.data(), sizeof(ScreenSpan) * l64.spans.size(),
+                           hipMemcpyHostToDevice));
   }
   if (factor_rank_ < 0) return;
   // L^T with its columns (basis functions) in that order, FP32 split into
@@ -567,6 +589,14 @@ void Mol::build_screen() {
   mixed_fragment_order(Lb.data(), factor_rows_, Mp_, Lf.data());
   if (!d_Lb_.reserve(sizeof(uint16_t) * Lf.size())) throw Error(CT_E_NOMEM, "cannot allocate the bf16 factor of P");
   CT_HIP_CHECK(hipMemcpy(d_Lb_.get(), Lf.data(), sizeof(uint16_t) * Lf.size(), hipMemcpyHostToDevice));
+  // FP64 copy in the same function order, k-major (L64s[i * rows + j] =
+  // L_ij for screened function i): the FP64 factor route and the
+  // re-evaluation of the mixed factor route's flagged points
+  std::vector<double> L64s((size_t)Mp_ * factor_rows_, 0.0);
+  for (int j = 0; j < rank; ++j)
+    for (int i = 0; i < M_; ++i) L64s[(size_t)i * factor_rows_ + j] = L_[(size_t)j * M_ + layout_.func_src[i]];
+  if (!d_L64s_.reserve(sizeof(double) * L64s.size())) throw Error(CT_E_NOMEM, "cannot allocate the factor of P");
+  CT_HIP_CHECK(hipMemcpy(d_L64s_.get(), L64s.data(), sizeof(double) * L64s.size(), hipMemcpyHostToDevice));
   // Truncation bound: a skipped AO value is at most eps, so the skipped part
   // of psi_j is at most t_j = eps ||L_j||_1 and rho moves by at most
   // sum_j t_j (2 |psi_j| + t_j) <= 2 eps ||c||_2 sqrt(rho) + O(eps^2)
@@ -585,8 +615,8 @@ void Mol::build_screen() {
   CT_HIP_CHECK(hipMemcpy(d_klist_all_.get(), all.data(), sizeof(int32_t) * all.size(), hipMemcpyHostToDevice));
 }
 
-bool Mol::factor_route(int mode, int route, bool mixed) const {
-  if (route == CT_ROUTE_DENSITY || (!mixed && route == CT_ROUTE_AUTO)) return false;  // FP64: P unless asked
+bool Mol::factor_route(int mode, int route, bool /*mixed*/) const {
+  if (route == CT_ROUTE_DENSITY) return false;
   const bool avail = factor_rank_ >= 0;
   if (route == CT_ROUTE_FACTOR) {
     if (!avail)
@@ -595,6 +625,7 @@ bool Mol::factor_route(int mode, int route, bool mixed) const {
     return true;
   }
   if (!avail) return false;
+  // both precisions (the FP64 kernels follow the same column counts):
   // matrix-core products per point, in units of Mp^2 x (the six split
   // products): density route rho 0.525 (triangle), grad 1, lap 1 + 3 x 0.525,
   // Hessian 4; factor route 1, 4, 5 and 4 + 2 x 4 columns of rows / Mp.  The
@@ -919,14 +950,16 @@ void Mol::reevaluate(int mode, const double* pts, int64_t col_stride, int64_t co
     double* gp = dots + (size_t)ndot * ld;
     double* part = gp + 3 * (size_t)ld;
     launch_gather_points(pts, idx, c, gp, stream_, col_stride);
-    launch_ao_f64(mode, lmax_, static_cast<const ShellMeta*>(d_meta_.get()),
-                  static_cast<const ShellGeom*>(d_geom_.get()), static_cast<const PrimData*>(d_prim_.get()), nshell_,
-                  M_, Mp_, gp, c, ld, ao, 700.0, stream_);
+    // the factor route in the order of its factor: the screened shell order
+    // (every shell evaluated here)
+    launch_ao_f64(mode, lmax_, static_cast<const ShellMeta*>(factor ? d_smeta_.get() : d_meta_.get()),
+                  static_cast<const ShellGeom*>(factor ? d_sgeom_.get() : d_geom_.get()),
+                  static_cast<const PrimData*>(d_prim_.get()), nshell_, M_, Mp_, gp, c, ld, ao, 700.0, stream_);
     if (factor && split)
-      launch_contract_f64_factor_split(mode, static_cast<const double*>(d_L64_.get()), factor_rows_, Mp_, ao, ld,
+      launch_contract_f64_factor_split(mode, static_cast<const double*>(d_L64s_.get()), factor_rows_, Mp_, ao, ld,
                                        part, dots, stream_);
     else if (factor)
-      launch_contract_f64_factor(mode, static_cast<const double*>(d_L64_.get()), factor_rows_, Mp_, ao, ld, dots,
+      launch_contract_f64_factor(mode, static_cast<const double*>(d_L64s_.get()), factor_rows_, Mp_, ao, ld, dots,
                                  stream_);
     else if (split)
       launch_contract_f64_split(mode, static_cast<const double*>(d_P_.get()), Mp_, ao, ld, part, dots, stream_);
@@ -935,6 +968,7 @@ void Mol::reevaluate(int mode, const double* pts, int64_t col_stride, int64_t co
     FinalizeArgs f = fin;
     f.dots = dots;
     f.index = idx;
+    f.gather = nullptr;  // the slice's dots are in index order; outputs go to index[p]
     f.nrt = 0;
     f.ld = ld;
     f.n = c;
@@ -963,8 +997,8 @@ void Mol::eval(const double* points, int64_t n, int layout, uint32_t q, int prec
   const int nrt = (use_factor ? factor_rows_ : Mp_) / kRowBlock;  // row blocks of the mixed partials
   // screened layout (mixed factor route): points in Morton order, per-tile
   // lists of the active k-steps
-  const bool screened = mixed && use_factor && screen_eps_ > 0.0;
   const int nk = Mp_ / 32, nwords = (nk + 31) / 32;
+  const bool screened = use_factor && screen_eps_ > 0.0;
   const int tp = screened ? factor_tile_points(mode) : 64;
   const bool pts_dev = cube == nullptr && is_device_pointer(points);
   // column-major points: (3, n), x, y, z of point p at points[p], points[n + p],
@@ -999,9 +1033,11 @@ void Mol::eval(const double* points, int64_t n, int layout, uint32_t q, int prec
   const int64_t ao_bytes_pt = (int64_t)nder * Mp_ * (mixed ? 4 : 8);
   const int64_t col_stage_pt = col_in && !pts_dev ? 24 : 0;
   const int64_t tgt_stage_pt = has_target && !tgt_dev ? 8 : 0;
-  const int64_t per_point = (mixed ? ao_bytes_pt + (int64_t)nrt * (ndot + 1) * 8 + 4 + 24 + stage_width * 8
-                                   : ao_bytes_pt + (int64_t)ndot * 8 + 24 + stage_width * 8) +
-                            col_stage_pt + tgt_stage_pt;
+  // AO block, dots (mixed: per row block, with the bound), staged points,
+  // staged outputs, flag index, staged columns, staged targets: the pieces
+  // carved below, in that order
+  const int64_t per_point = (mixed ? ao_bytes_pt + (int64_t)nrt * (ndot + 1) * 8 : ao_bytes_pt + (int64_t)ndot * 8) +
+                            24 + stage_width * 8 + 4 + col_stage_pt + tgt_stage_pt;
   // screened: keys (2 x 4 B), indices (2 x 4 B), sorted points (24 B) per
   // point; k-step list (2 nk B), count and mask (4 + 4 nwords B) per tile
   const int64_t per_point_s = screened ? 40 + (4 * nk + 4 + 4 * nwords + tp - 1) / tp : 0;
@@ -1090,8 +1126,8 @@ void Mol::eval(const double* points, int64_t n, int layout, uint32_t q, int prec
     kcount = reinterpret_cast<int32_t*>(take((size_t)ntile_max * 4));
     kmask = reinterpret_cast<uint32_t*>(take((size_t)ntile_max * nwords * 4));
     sort_tmp = take(sort_bytes);
-    CT_REQUIRE(cur <= ws + ws_size, "screening workspace overflow");
   }
+  CT_REQUIRE(cur <= ws + ws_size, "internal error: evaluation workspace overflow");
   int64_t valid_tiles = 0;
   // readbacks of the counters: pinned, or (allocation-free) pageable memory
   // on this stack, read after the stream synchronises
@@ -1165,14 +1201,41 @@ void Mol::eval(const double* points, int64_t n, int layout, uint32_t q, int prec
       if (cols) (kSlots[k].bit == CT_Q_GRAD ? fa.grad_ld : fa.hess_ld) = out_dev[k] ? n : nc;
     }
 
+    // the factor route reads the AOs in the order of its factor (screened
+    // shell order, or the natural one when screening is off)
+    const auto* ameta = use_factor ? static_cast<const ShellMeta*>(d_smeta_.get()) : meta;
+    const auto* ageom = use_factor ? static_cast<const ShellGeom*>(d_sgeom_.get()) : geom;
+    const double* apts = pts;  // the points the AO kernel and the contraction see
+    if (screened) {
+      timer_.begin(KernelTimer::SCREEN, stream_);
+      launch_screen_sort(pts, nc, frame_, keys_in, keys_out, idx_in, perm, sort_tmp, sort_bytes, stream_);
+      launch_gather_points(pts, perm, nc, spts, stream_);
+      // the sort's output keys are spent: their space holds the inverse permutation
+      launch_invert_permutation(perm, nc, reinterpret_cast<int32_t*>(keys_out), stream_);
+      // the FP64 path's k-step tables hold the cutoff radii of its tighter
+      // threshold
+      launch_tile_activity(spts, nc, tp, ld / tp,
+                           static_cast<const int32_t*>(mixed ? d_span_off_.get() : d_span_off64_.get()),
+                           static_cast<const ScreenSpan*>(mixed ? d_spans_.get() : d_spans64_.get()), nk, klist,
+                           kcount, kmask, nwords, d_active, stream_);
+      timer_.end(stream_);
+      apts = spts;
+      // the finaliser runs over the chunk's points in their own order and
+      // reads each one's results at its sorted position
+      fa.gather = reinterpret_cast<const int32_t*>(keys_out);
+      valid_tiles += (nc + tp - 1) / tp;
+    }
+
     if (!mixed) {
       timer_.begin(KernelTimer::AO, stream_);
-      launch_ao_f64(mode, lmax_, meta, geom, prim, nshell_, M_, Mp_, pts, nc, ld, static_cast<double*>(ao), 700.0, stream_);
+      launch_ao_f64(mode, lmax_, ameta, ageom, prim, nshell_, M_, Mp_, apts, nc, ld, static_cast<double*>(ao), 700.0,
+                    stream_, screened ? kmask : nullptr, nwords, tp);
       timer_.end(stream_);
       timer_.begin(KernelTimer::CONTRACT, stream_);
       if (use_factor)
-        launch_contract_f64_factor(mode, static_cast<const double*>(d_L64_.get()), factor_rows_, Mp_,
-                                   static_cast<const double*>(ao), ld, dots, stream_);
+        launch_contract_f64_factor(mode, static_cast<const double*>(d_L64s_.get()), factor_rows_, Mp_,
+                                   static_cast<const double*>(ao), ld, dots, stream_, screened ? kmask : nullptr,
+                                   nwords, tp);
       else
         launch_contract_f64(mode, static_cast<const double*>(d_P_.get()), Mp_, static_cast<const double*>(ao), ld,
                             dots, stream_);
@@ -1182,27 +1245,6 @@ void Mol::eval(const double* points, int64_t n, int layout, uint32_t q, int prec
       timer_.end(stream_);
     } else {
       float* aoc = static_cast<float*>(ao);
-      const double* apts = pts;  // the points the AO kernel and the contraction see
-      if (screened) {
-        timer_.begin(KernelTimer::SCREEN, stream_);
-        launch_screen_sort(pts, nc, frame_, keys_in, keys_out, idx_in, perm, sort_tmp, sort_bytes, stream_);
-        launch_gather_points(pts, perm, nc, spts, stream_);
-        // the sort's output keys are spent: their space holds the inverse permutation
-        launch_invert_permutation(perm, nc, reinterpret_cast<int32_t*>(keys_out), stream_);
-        launch_tile_activity(spts, nc, tp, ld / tp, static_cast<const int32_t*>(d_span_off_.get()),
-                             static_cast<const ScreenSpan*>(d_spans_.get()), nk, klist, kcount, kmask, nwords,
-                             d_active, stream_);
-        timer_.end(stream_);
-        apts = spts;
-        // the finaliser runs over the chunk's points in their own order and
-        // reads each one's results at its sorted position
-        fa.gather = reinterpret_cast<const int32_t*>(keys_out);
-        valid_tiles += (nc + tp - 1) / tp;
-      }
-      // the factor route reads the AOs in the order of its factor (screened
-      // shell order, or the natural one when screening is off)
-      const auto* ameta = use_factor ? static_cast<const ShellMeta*>(d_smeta_.get()) : meta;
-      const auto* ageom = use_factor ? static_cast<const ShellGeom*>(d_sgeom_.get()) : geom;
       timer_.begin(KernelTimer::AO, stream_);
       launch_ao_f32(mode, lmax_, ameta, ageom, prim, nshell_, M_, Mp_, apts, nc, ld, aoc, 50.0,
                     screened ? kmask : nullptr, nwords, tp, stream_);

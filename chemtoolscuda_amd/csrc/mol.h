@@ -180,7 +180,8 @@ class Mol {
   DeviceBuffer d_meta_, d_geom_, d_prim_, d_P_;
   DeviceBuffer d_Pb_;      // mixed path: bf16 pieces of P (hi, mid, lo) in MFMA fragment order
   DeviceBuffer d_Lb_;      // mixed path: bf16 pieces of L^T (P = L L^T) in MFMA fragment order
-  DeviceBuffer d_L64_;     // L in FP64, k-major (nbasis_pad x factor rows)
+  DeviceBuffer d_L64_;     // L in FP64, k-major (nbasis_pad x factor rows), natural order (residual check)
+  DeviceBuffer d_L64s_;    // the same in the screened function order (FP64 factor route, re-evaluation)
   int factor_rank_ = -1;   // rank of the factor, -1 when P has none (not PSD or rank too high)
   int factor_rows_ = 0;    // rows of L^T padded to a multiple of 128
   std::vector<double> L_;  // the factor, rank x M (row j = column j of L), natural order
@@ -195,6 +196,13 @@ class Mol {
   MortonFrame frame_{};
   DeviceBuffer d_smeta_, d_sgeom_;     // shell table in the screened order
   DeviceBuffer d_span_off_, d_spans_;  // per k-step shells and cutoff radii
+  // The FP64 path screens the factor route at screen_eps_ * kFp64ScreenFactor
+  // (1e-22 by default): a skipped AO value is at most that, and rho moves by
+  // at most 2 eps64 ||c||_2 sqrt(rho) (c_j = ||L_j||_1), below rounding for
+  // FP64 parity wherever rho is above ~1e-12 (||c||_2 of a few thousand).
+  static constexpr double kFp64ScreenFactor = 1e-7;
+  std::vector<double> shell_radius64_;  // cutoff radii at the FP64 threshold
+  DeviceBuffer d_span_off64_, d_spans64_;  // per k-step shells and the FP64 cutoff radii
   DeviceBuffer d_klist_all_;           // 0 .. nbasis_pad / 32 - 1: the k-steps of an unscreened tile
   double screen_trunc_ = 0.0;          // 4 eps ||c||_2, c_j = ||L_j||_1: rho bound of the skipped blocks / sqrt(rho)
   DeviceBuffer d_ws_;      // AO + dots + staged points / outputs

@@ -4,8 +4,11 @@
   the library's buffers may hold; a chunk workspace beyond the cap fails as a
   failed hipMalloc does, and evaluation halves the chunk and retries (the
   reference asserts instead, src/evaluate_density.cu:690-705).  The results
-  must not change (FP64: the same bits), and a cap too small for even the
-  smallest chunk must end in CT_E_NOMEM with a message.
+  must not change: the FP64 density route (the reference's formulation) the
+  same bits; the screened FP64 factor route, whose point tiles follow the
+  chunks, within 1e-13 of each quantity's scale; mixed within its bar.  A
+  cap too small for even the smallest chunk must end in CT_E_NOMEM with a
+  message.
 * FP64 re-evaluation in slices: with host arrays, a small workspace and a
   mixed tolerance no point meets, every point of every chunk is re-evaluated
   in FP64 in workspace-sized slices; the results are the FP64 path's.
@@ -39,7 +42,8 @@ def test_allocation_failure_halves_the_chunk(capped):
     mol = ct.molecule_from_basis(w.basis, w.P, w.charges)
     qs = ["rho", "grad", "lap", "tau_pd"]
     ref = mol.compute(pts, qs)
-    assert mol.last_report()["n_chunks"] == 1
+    assert mol.last_report()["n_chunks"] == 1 and mol.last_report()["factor_rank"] > 0
+    ref_dens = mol.compute(pts, qs, route="density")
     ref_mixed = mol.compute(pts, qs, precision="mixed")
     # fresh handle (its buffers have not grown yet); the cap (on everything
     # the process's handles hold) leaves 192 MiB more: room for chunks of a
@@ -50,7 +54,13 @@ def test_allocation_failure_halves_the_chunk(capped):
     rep = mol2.last_report()
     assert rep["n_chunks"] > 1, rep
     for q in qs:
-        assert np.array_equal(got[q], ref[q]), q
+        scale = np.maximum(np.abs(ref[q]).reshape(len(pts), -1).max(axis=1), 1e-8)
+        d = np.abs(got[q] - ref[q]).reshape(len(pts), -1).max(axis=1)
+        assert (d / scale).max() < 1e-13, q
+    got_dens = mol2.compute(pts, qs, route="density")
+    assert mol2.last_report()["n_chunks"] > 1
+    for q in qs:
+        assert np.array_equal(got_dens[q], ref_dens[q]), q
     got_mixed = mol2.compute(pts, qs, precision="mixed")
     assert mol2.last_report()["n_chunks"] > 1
     for q in qs:
@@ -63,7 +73,7 @@ def test_allocation_failure_halves_the_chunk(capped):
     with pytest.raises(RuntimeError, match="cannot allocate"):
         mol3.compute(pts, qs)
     capped.set_device_allocation_limit(-1)
-    assert np.array_equal(mol3.compute(pts, qs)["rho"], ref["rho"])
+    assert np.array_equal(mol3.compute(pts, qs, route="density")["rho"], ref_dens["rho"])
 
 
 def test_host_arrays_reevaluate_every_point_in_slices():

@@ -2,9 +2,11 @@
 
 Through the C ABI (ctypes) on device arrays: two calls in a row, the second
 with fewer points, FP64 and mixed (screened factor route, flagged points
-re-evaluated chunk by chunk in the scratch), rows and columns.  FP64 must be
-the same bits as ct_eval; mixed must meet the mixed path's bar (1e-6 of each
-quantity's scale against FP64 where rho >= 1e-8).  The library's own device
+re-evaluated chunk by chunk in the scratch), rows and columns.  FP64 on the
+density route (the reference's formulation) must be the same bits as
+ct_eval; on the screened factor route, whose point tiles follow the chunks,
+within 1e-13 of each quantity's scale; mixed must meet the mixed path's bar
+(1e-6 of each quantity's scale against FP64 where rho >= 1e-8).  The library's own device
 buffers must not grow during the calls, and host pointers, too many points
 and unknown layouts are rejected with a message.
 """
@@ -62,10 +64,17 @@ def test_noalloc_matches_ct_eval_on_two_calls(torch, case):
     qs = ["rho", "grad", "lap", "tau_pd"]
     for n in (pts_all.shape[0], pts_all.shape[0] // 3 + 7):  # the second call smaller
         pts = pts_all[:n].contiguous()
+        dens = {"route": 1}
+        refd, _ = _run(torch, h, h.eval, pts, qs, "fp64", dens)
+        gotd, _ = _run(torch, h, h.eval_noalloc, pts, qs, "fp64", dens)
+        for q in qs:
+            assert torch.equal(gotd[q], refd[q]), (case, n, q)
         ref64, _ = _run(torch, h, h.eval, pts, qs, "fp64")
         got64, rep = _run(torch, h, h.eval_noalloc, pts, qs, "fp64")
         for q in qs:
-            assert torch.equal(got64[q], ref64[q]), (case, n, q)
+            sc = _scale(torch, q, ref64).clamp(min=1e-300)
+            d = (torch.linalg.vector_norm(got64[q] - ref64[q], dim=1) / sc)[ref64["rho"][:, 0] >= 1e-8].max().item()
+            assert d < 1e-13, (case, n, q, d)
         gotm = _outputs(torch, qs, n)
         ptrs = {q: t.data_ptr() for q, t in gotm.items()}
         torch.cuda.synchronize()
