@@ -24,7 +24,8 @@ ARCH = os.environ.get("CT_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["ao_kernels.hip", "contract_f64.hip", "contract_mixed.hip", "finalize.hip", "esp.hip", "adjoint.hip",
                "adjoint_mixed.hip", "screen.hip", "noalloc.hip"]
-HOST_SOURCES = ["mol.cpp", "capi.cpp", "fchk.cpp", "wavefunction.cpp", "esp_host.cpp", "screen_host.cpp"]
+HOST_SOURCES = ["mol.cpp", "capi.cpp", "fchk.cpp", "wavefunction.cpp", "esp_host.cpp", "screen_host.cpp",
+                "host_copy.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # Packed-FP32 instructions are disabled for every HIP translation unit.

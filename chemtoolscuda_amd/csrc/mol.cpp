@@ -285,10 +285,23 @@ Mol::Mol(const HostBasis& basis, const double* atom_xyz, const double* charges, 
 }
 
 Mol::~Mol() {
-  if (stream_) {
-    (void)hipStreamSynchronize(stream_);
-    (void)hipStreamDestroy(stream_);
-  }
+  for (hipStream_t s : {stream_, h2d_stream_, d2h_stream_, re_stream_})
+    if (s) {
+      (void)hipStreamSynchronize(s);
+      (void)hipStreamDestroy(s);
+    }
+  for (auto& slot : pipe_ev_)
+    for (hipEvent_t e : slot)
+      if (e) (void)hipEventDestroy(e);
+}
+
+void Mol::ensure_pipeline() {
+  if (!h2d_stream_) CT_HIP_CHECK(hipStreamCreateWithFlags(&h2d_stream_, hipStreamNonBlocking));
+  if (!d2h_stream_) CT_HIP_CHECK(hipStreamCreateWithFlags(&d2h_stream_, hipStreamNonBlocking));
+  if (!re_stream_) CT_HIP_CHECK(hipStreamCreateWithFlags(&re_stream_, hipStreamNonBlocking));
+  for (auto& slot : pipe_ev_)
+    for (hipEvent_t& e : slot)
+      if (!e) CT_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
 }
 
 void Mol::upload_geometry() {
@@ -900,6 +913,39 @@ void Mol::esp_pairs_by_L(int64_t* out) {
 
 void Mol::synchronize() { CT_HIP_CHECK(hipStreamSynchronize(stream_)); }
 
+// Chunk sizes of the host-array pipeline for n points, at most C each
+// (C a multiple of kPointAlign): sizes growing fourfold from C/16 at the
+// start and shrinking to C/16 at the end, so that the first upload and the
+// last download, which nothing overlaps, are short; the middle in near-equal
+// chunks of at most C.  Every size but the last is a multiple of kPointAlign
+// (the chunks start at aligned points).
+static std::vector<int64_t> pipeline_chunks(int64_t n, int64_t C) {
+  std::vector<int64_t> head, mid, tail;
+  const int64_t N = round_up(n, kPointAlign);
+  int64_t rem = N;  // a multiple of kPointAlign throughout
+  for (int64_t c = std::max<int64_t>(kPointAlign, C / 16 / kPointAlign * kPointAlign);; c = std::min(C, 4 * c)) {
+    if (rem <= 2 * c) {
+      const int64_t a = round_up(rem / 2, kPointAlign);
+      head.push_back(a);
+      if (rem > a) tail.push_back(rem - a);
+      break;
+    }
+    head.push_back(c);
+    tail.push_back(c);
+    rem -= 2 * c;
+    if (c == C) {
+      const int64_t k = (rem + C - 1) / C;
+      const int64_t each = round_up((rem + k - 1) / k, kPointAlign);
+      for (; rem > 0; rem -= std::min(each, rem)) mid.push_back(std::min(each, rem));
+      break;
+    }
+  }
+  head.insert(head.end(), mid.begin(), mid.end());
+  head.insert(head.end(), tail.rbegin(), tail.rend());
+  head.back() -= N - n;  // the last chunk >= kPointAlign > N - n
+  return head;
+}
+
 int64_t Mol::chunk_points(int64_t per_point, int64_t n) const {
   int64_t nc = ws_limit_ / per_point / kPointAlign * kPointAlign;
   nc = std::max<int64_t>(nc, kPointAlign);
@@ -920,8 +966,9 @@ size_t Mol::reeval_bytes(int mode, int64_t count, bool factor) const {
 // workspace: the caller's region ws when given, else d_ws64_, grown to a
 // slice sized like an FP64 chunk.
 void Mol::reevaluate(int mode, const double* pts, int64_t col_stride, int64_t count, const int32_t* index,
-                     const FinalizeArgs& fin, bool factor, char* ws, size_t ws_bytes) {
+                     const FinalizeArgs& fin, bool factor, char* ws, size_t ws_bytes, hipStream_t rs) {
   if (count <= 0) return;
+  if (rs == nullptr) rs = stream_;
   const int nder = mode_nder(mode), ndot = mode_ndot(mode);
   int64_t S = round_up(count, kPointAlign);
   if (ws) {
@@ -949,22 +996,22 @@ void Mol::reevaluate(int mode, const double* pts, int64_t col_stride, int64_t co
     double* dots = ao + (size_t)nder * Mp_ * ld;
     double* gp = dots + (size_t)ndot * ld;
     double* part = gp + 3 * (size_t)ld;
-    launch_gather_points(pts, idx, c, gp, stream_, col_stride);
+    launch_gather_points(pts, idx, c, gp, rs, col_stride);
     // the factor route in the order of its factor: the screened shell order
     // (every shell evaluated here)
     launch_ao_f64(mode, lmax_, static_cast<const ShellMeta*>(factor ? d_smeta_.get() : d_meta_.get()),
                   static_cast<const ShellGeom*>(factor ? d_sgeom_.get() : d_geom_.get()),
-                  static_cast<const PrimData*>(d_prim_.get()), nshell_, M_, Mp_, gp, c, ld, ao, 700.0, stream_);
+                  static_cast<const PrimData*>(d_prim_.get()), nshell_, M_, Mp_, gp, c, ld, ao, 700.0, rs);
     if (factor && split)
       launch_contract_f64_factor_split(mode, static_cast<const double*>(d_L64s_.get()), factor_rows_, Mp_, ao, ld,
-                                       part, dots, stream_);
+                                       part, dots, rs);
     else if (factor)
       launch_contract_f64_factor(mode, static_cast<const double*>(d_L64s_.get()), factor_rows_, Mp_, ao, ld, dots,
-                                 stream_);
+                                 rs);
     else if (split)
-      launch_contract_f64_split(mode, static_cast<const double*>(d_P_.get()), Mp_, ao, ld, part, dots, stream_);
+      launch_contract_f64_split(mode, static_cast<const double*>(d_P_.get()), Mp_, ao, ld, part, dots, rs);
     else
-      launch_contract_f64(mode, static_cast<const double*>(d_P_.get()), Mp_, ao, ld, dots, stream_);
+      launch_contract_f64(mode, static_cast<const double*>(d_P_.get()), Mp_, ao, ld, dots, rs);
     FinalizeArgs f = fin;
     f.dots = dots;
     f.index = idx;
@@ -972,7 +1019,7 @@ void Mol::reevaluate(int mode, const double* pts, int64_t col_stride, int64_t co
     f.nrt = 0;
     f.ld = ld;
     f.n = c;
-    launch_finalize(f, stream_);
+    launch_finalize(f, rs);
   }
 }
 
@@ -1002,8 +1049,7 @@ void Mol::eval(const double* points, int64_t n, int layout, uint32_t q, int prec
   const int tp = screened ? factor_tile_points(mode) : 64;
   const bool pts_dev = cube == nullptr && is_device_pointer(points);
   // column-major points: (3, n), x, y, z of point p at points[p], points[n + p],
-  // points[2n + p]; each chunk is gathered into row-major staging (host
-  // arrays through a column staging buffer of the chunk's size)
+  // points[2n + p]; each chunk is gathered into row-major staging
   const bool col_in = cube == nullptr && layout == CT_LAYOUT_COL;
   const bool col_out = par.out_layout == CT_LAYOUT_COL;
   constexpr size_t NS = sizeof(kSlots) / sizeof(kSlots[0]);
@@ -1029,20 +1075,36 @@ void Mol::eval(const double* points, int64_t n, int layout, uint32_t q, int prec
                "points");
   // (the call-wide flag list is an allocation: the allocation-free path
   // re-evaluates chunk by chunk)
-  const bool batch = mixed && all_dev && !fixed;
+  // Host arrays (points, outputs or targets in host memory): the chunks run as
+  // a pipeline through kPipeSlots slots of pinned and device staging.  Per
+  // chunk i, in slot i % kPipeSlots: the caller's inputs are copied into the
+  // slot's pinned region (host threads) and uploaded on the upload stream;
+  // the kernels run on the handle's stream; once chunk i + 1 is queued behind
+  // them, chunk i's flag count is read, its flagged points re-evaluated and
+  // its outputs downloaded on the download stream; the host copies them into
+  // the caller's arrays when the slot comes round again.  The uploads and
+  // downloads of neighbouring chunks overlap the kernels.
+  const bool host_in = cube == nullptr && !pts_dev;
+  const bool host_tgt = has_target && !tgt_dev;
+  const bool piped = host_in || host_tgt || stage_width > 0;
+  const int nslot = piped ? kPipeSlots : 1;
+  // (the call-wide flag list is an allocation: the allocation-free path
+  // re-evaluates chunk by chunk)
+  const bool batch = mixed && all_dev && !fixed && !piped;
   const int64_t ao_bytes_pt = (int64_t)nder * Mp_ * (mixed ? 4 : 8);
-  const int64_t col_stage_pt = col_in && !pts_dev ? 24 : 0;
-  const int64_t tgt_stage_pt = has_target && !tgt_dev ? 8 : 0;
-  // AO block, dots (mixed: per row block, with the bound), staged points,
-  // staged outputs, flag index, staged columns, staged targets: the pieces
-  // carved below, in that order
-  const int64_t per_point = (mixed ? ao_bytes_pt + (int64_t)nrt * (ndot + 1) * 8 : ao_bytes_pt + (int64_t)ndot * 8) +
-                            24 + stage_width * 8 + 4 + col_stage_pt + tgt_stage_pt;
+  // per slot: points in row layout (24 B), the uploaded inputs (points as
+  // given, 24 B; targets, 8 B), staged outputs, flag index (4 B)
+  const int64_t in_pt = (host_in ? 24 : 0) + (host_tgt ? 8 : 0);
+  const int64_t slot_pt = 24 + in_pt + stage_width * 8 + 4;
+  // AO block, dots (mixed: per row block, with the bound), the slots: the
+  // pieces carved below, in that order
+  const int64_t per_point =
+      (mixed ? ao_bytes_pt + (int64_t)nrt * (ndot + 1) * 8 : ao_bytes_pt + (int64_t)ndot * 8) + nslot * slot_pt;
   // screened: keys (2 x 4 B), indices (2 x 4 B), sorted points (24 B) per
   // point; k-step list (2 nk B), count and mask (4 + 4 nwords B) per tile
   const int64_t per_point_s = screened ? 40 + (4 * nk + 4 + 4 * nwords + tp - 1) / tp : 0;
-  // flag counter, max estimate; from byte 256 the active k-step counters (one
-  // per 128-byte line)
+  // flag counters (one per slot, 16 bytes apart), max estimate (bytes 8-15);
+  // from byte 256 the active k-step counters (one per 128-byte line)
   const size_t head = 256 + 128 * kScreenCounters;
   size_t sort_bytes = 0;
   int64_t Nc = 0;
@@ -1052,6 +1114,7 @@ void Mol::eval(const double* points, int64_t n, int layout, uint32_t q, int prec
   size_t ws64_bytes = 0;
   if (!fixed) {
     Nc = chunk_points(per_point + per_point_s, n);
+    if (piped) Nc = std::min<int64_t>(Nc, kHostChunk);
     for (;;) {
       sort_bytes = screened ? (screen_sort_workspace(Nc) + 255) / 256 * 256 : 0;
       if (d_ws_.reserve(head + (size_t)Nc * (per_point + per_point_s) + sort_bytes + 4096)) break;
@@ -1086,7 +1149,6 @@ void Mol::eval(const double* points, int64_t n, int layout, uint32_t q, int prec
     ws_size = main;
     ws64 = fixed->base + main;
   }
-  int32_t* d_count = reinterpret_cast<int32_t*>(ws);
   unsigned long long* d_maxest = reinterpret_cast<unsigned long long*>(ws + 8);
   unsigned long long* d_active = reinterpret_cast<unsigned long long*>(ws + 256);
   char* cur = ws + head;
@@ -1094,16 +1156,61 @@ void Mol::eval(const double* points, int64_t n, int layout, uint32_t q, int prec
   cur += (size_t)Nc * ao_bytes_pt;
   double* dots = reinterpret_cast<double*>(cur);
   cur += (size_t)Nc * (mixed ? nrt * (ndot + 1) : ndot) * 8;
-  double* pts_stage = reinterpret_cast<double*>(cur);
-  cur += (size_t)Nc * 24;
-  double* out_stage = reinterpret_cast<double*>(cur);
-  cur += (size_t)Nc * stage_width * 8;
-  int32_t* flag_idx = reinterpret_cast<int32_t*>(cur);
-  cur += (size_t)Nc * 4;
-  double* col_stage = reinterpret_cast<double*>(cur);
-  cur += (size_t)Nc * col_stage_pt;
-  double* tgt_stage = reinterpret_cast<double*>(cur);
-  cur += (size_t)Nc * tgt_stage_pt;
+
+  // a pipeline slot: its staging and the chunk it holds
+  struct Slot {
+    double* pts_stage;  // device: the chunk's points, row layout
+    char* in_dev;       // device: uploaded points as given ([0, 24 Nc)), targets (after them)
+    double* out_stage;  // device: outputs bound for host arrays, each quantity w x nc
+    int32_t* flag_idx;  // device: the chunk's flagged points
+    int32_t* count;     // device: their number
+    char* in_host;      // pinned: inputs to upload
+    char* out_host;     // pinned: downloaded outputs
+    int64_t* cnt_host;  // pinned: the count read back
+    hipEvent_t* ev;     // PipeEvent events
+    bool used[NPIPEEV];  // which events this call has recorded
+    // the chunk in the slot
+    int64_t start, nc;
+    const double* pts;
+    FinalizeArgs fa;
+    bool out_pending;  // downloaded (or downloading) outputs not yet copied to the caller
+  };
+  Slot slots[kPipeSlots];
+  // readbacks of the counters: pinned, or (allocation-free) pageable memory
+  // on this stack, read after the stream synchronises
+  int64_t h_local[8 + 16 * kScreenCounters];
+  int64_t* h_small = h_local;
+  if (!fixed) {
+    h_small_.reserve(sizeof(h_local));
+    h_small = static_cast<int64_t*>(h_small_.get());
+  }
+  const size_t hslot = (size_t)Nc * (size_t)(in_pt + stage_width * 8);
+  if (piped) {
+    ensure_pipeline();
+    // (idle unless an earlier call ended in an error with copies in flight)
+    CT_HIP_CHECK(hipStreamSynchronize(h2d_stream_));
+    CT_HIP_CHECK(hipStreamSynchronize(d2h_stream_));
+    h_stage_.reserve(hslot * nslot);
+  }
+  for (int s = 0; s < nslot; ++s) {
+    Slot& sl = slots[s];
+    std::memset(&sl, 0, sizeof(sl));
+    sl.pts_stage = reinterpret_cast<double*>(cur);
+    cur += (size_t)Nc * 24;
+    sl.in_dev = cur;
+    cur += (size_t)Nc * in_pt;
+    sl.out_stage = reinterpret_cast<double*>(cur);
+    cur += (size_t)Nc * stage_width * 8;
+    sl.flag_idx = reinterpret_cast<int32_t*>(cur);
+    cur += (size_t)Nc * 4;
+    sl.count = reinterpret_cast<int32_t*>(ws + 16 * s);
+    sl.cnt_host = h_small + 2 + s;
+    if (piped) {
+      sl.in_host = static_cast<char*>(h_stage_.get()) + hslot * s;
+      sl.out_host = sl.in_host + (size_t)Nc * in_pt;
+      sl.ev = pipe_ev_[s];
+    }
+  }
   // screened layout buffers (256-byte aligned pieces)
   auto take = [&](size_t bytes) {
     char* p = cur;
@@ -1129,51 +1236,76 @@ void Mol::eval(const double* points, int64_t n, int layout, uint32_t q, int prec
   }
   CT_REQUIRE(cur <= ws + ws_size, "internal error: evaluation workspace overflow");
   int64_t valid_tiles = 0;
-  // readbacks of the counters: pinned, or (allocation-free) pageable memory
-  // on this stack, read after the stream synchronises
-  int64_t h_local[8 + 16 * kScreenCounters];
-  int64_t* h_small = h_local;
-  if (!fixed) {
-    h_small_.reserve(sizeof(h_local));
-    h_small = static_cast<int64_t*>(h_small_.get());
-  }
 
   const auto* meta = static_cast<const ShellMeta*>(d_meta_.get());
   const auto* geom = static_cast<const ShellGeom*>(d_geom_.get());
   const auto* prim = static_cast<const PrimData*>(d_prim_.get());
-  int32_t* list = flag_idx;
+  int32_t* list = slots[0].flag_idx;
   if (batch) {
     if (!d_flags_.reserve(sizeof(int32_t) * (size_t)n)) throw Error(CT_E_NOMEM, "cannot allocate the flag list");
     list = static_cast<int32_t*>(d_flags_.get());
   }
-  // the flag counter (bytes 0-3), the largest estimate (8-15) and the
-  // active k-step counters start at zero
+  // the flag counters (bytes 0-3, 16-19, 32-35), the largest estimate (8-15)
+  // and the active k-step counters start at zero
   if (mixed) launch_fill_zero(reinterpret_cast<unsigned long long*>(ws), (int64_t)(head / 8), stream_);
 
   int64_t chunks = 0, reevaluated = 0;
-  for (int64_t start = 0; start < n; start += Nc) {
-    const int64_t nc = std::min(Nc, n - start);
+  // Queues chunk [start, start + nc) in slot sl: upload, kernels, finaliser
+  // and (mixed, per chunk) the read-back of its flag count.
+  auto enqueue_chunk = [&](Slot& sl, int64_t start, int64_t nc) {
     const int64_t ld = round_up(nc, kPointAlign);
-    const double* pts = cube ? nullptr : points + 3 * start;
-    if (cube) {
-      launch_cubic_points(*cube, start, nc, pts_stage, stream_);
-      pts = pts_stage;
-    } else if (col_in) {
-      const double* cols = points + start;
-      int64_t stride = n;
-      if (!pts_dev) {
-        CT_HIP_CHECK(hipMemcpy2DAsync(col_stage, sizeof(double) * nc, points + start, sizeof(double) * n,
-                                      sizeof(double) * nc, 3, hipMemcpyHostToDevice, stream_));
-        cols = col_stage;
-        stride = nc;
+    sl.start = start;
+    sl.nc = nc;
+    if (piped) {
+      // the slot's device buffers are free once its previous chunk is final
+      // (re-evaluated) and downloaded (the host has copied that chunk out
+      // before this call)
+      if (sl.used[EV_READY]) {
+        CT_HIP_CHECK(hipStreamWaitEvent(h2d_stream_, sl.ev[EV_READY], 0));
+        CT_HIP_CHECK(hipStreamWaitEvent(stream_, sl.ev[EV_READY], 0));
       }
-      launch_gather_points(cols, nullptr, nc, pts_stage, stream_, stride);
-      pts = pts_stage;
-    } else if (!pts_dev) {
-      CT_HIP_CHECK(hipMemcpyAsync(pts_stage, pts, sizeof(double) * 3 * nc, hipMemcpyHostToDevice, stream_));
-      pts = pts_stage;
+      if (sl.used[EV_D2H]) CT_HIP_CHECK(hipStreamWaitEvent(stream_, sl.ev[EV_D2H], 0));
+      if (in_pt > 0) {
+        // the pinned region is free once its previous upload is done
+        if (sl.used[EV_H2D]) CT_HIP_CHECK(hipEventSynchronize(sl.ev[EV_H2D]));
+        HostCopy jobs[4];
+        int nj = 0;
+        if (host_in) {
+          if (col_in)
+            for (int c = 0; c < 3; ++c)
+              jobs[nj++] = HostCopy{sl.in_host + (size_t)c * nc * 8, points + (size_t)c * n + start, (size_t)nc * 8};
+          else
+            jobs[nj++] = HostCopy{sl.in_host, points + 3 * start, (size_t)nc * 24};
+        }
+        const size_t tgt_off = host_in ? (size_t)Nc * 24 : 0;
+        if (host_tgt) jobs[nj++] = HostCopy{sl.in_host + tgt_off, par.rho_target + start, (size_t)nc * 8};
+        host_copy(jobs, nj);
+        if (host_in)
+          CT_HIP_CHECK(hipMemcpyAsync(sl.in_dev, sl.in_host, (size_t)nc * 24, hipMemcpyHostToDevice, h2d_stream_));
+        if (host_tgt)
+          CT_HIP_CHECK(hipMemcpyAsync(sl.in_dev + tgt_off, sl.in_host + tgt_off, (size_t)nc * 8,
+                                      hipMemcpyHostToDevice, h2d_stream_));
+        CT_HIP_CHECK(hipEventRecord(sl.ev[EV_H2D], h2d_stream_));
+        sl.used[EV_H2D] = true;
+        CT_HIP_CHECK(hipStreamWaitEvent(stream_, sl.ev[EV_H2D], 0));
+      }
     }
-    FinalizeArgs fa;
+    const double* pts = nullptr;
+    if (cube) {
+      launch_cubic_points(*cube, start, nc, sl.pts_stage, stream_);
+      pts = sl.pts_stage;
+    } else if (col_in) {
+      // column-major points: (3, n), x, y, z of point p at points[p],
+      // points[n + p], points[2n + p]; gathered into row layout (uploaded
+      // columns: stride nc)
+      launch_gather_points(host_in ? reinterpret_cast<const double*>(sl.in_dev) : points + start, nullptr, nc,
+                           sl.pts_stage, stream_, host_in ? nc : n);
+      pts = sl.pts_stage;
+    } else {
+      pts = host_in ? reinterpret_cast<const double*>(sl.in_dev) : points + 3 * start;
+    }
+    sl.pts = pts;
+    FinalizeArgs& fa = sl.fa;
     std::memset(&fa, 0, sizeof(fa));
     fa.dots = dots;
     fa.index = nullptr;
@@ -1184,7 +1316,7 @@ void Mol::eval(const double* points, int64_t n, int layout, uint32_t q, int prec
     fa.mode = mode;
     fa.q = q;
     fa.par = par;
-    double* stage_cursor = out_stage;
+    double* stage_cursor = sl.out_stage;
     for (size_t k = 0; k < NS; ++k) {
       if (!(q & kSlots[k].bit)) continue;
       double* user = out.*(kSlots[k].field);
@@ -1261,51 +1393,117 @@ void Mol::eval(const double* points, int64_t n, int layout, uint32_t q, int prec
       fl.gamma = use_factor ? kMixedGammaFactor : mixed_gamma(mode);
       fl.tol = par.mixed_tol;
       fl.rho_floor = par.rho_floor;
-      fl.count = d_count;
-      fl.index = list;
+      fl.count = batch ? slots[0].count : sl.count;
+      fl.index = batch ? list : sl.flag_idx;
       fl.index_base = batch ? start : 0;
       fl.max_est = d_maxest;
       fl.trunc = screened ? screen_trunc_ : 0.0;
       fl.target = nullptr;
-      if (has_target) {
-        if (tgt_dev) {
-          fl.target = par.rho_target + start;
-        } else {
-          CT_HIP_CHECK(hipMemcpyAsync(tgt_stage, par.rho_target + start, sizeof(double) * nc, hipMemcpyHostToDevice,
-                                      stream_));
-          fl.target = tgt_stage;
-        }
-      }
+      if (has_target)
+        fl.target = host_tgt ? reinterpret_cast<const double*>(sl.in_dev + (host_in ? (size_t)Nc * 24 : 0))
+                             : par.rho_target + start;
+      if (!batch) launch_fill_zero(reinterpret_cast<unsigned long long*>(sl.count), 1, stream_);
       timer_.begin(KernelTimer::FINALIZE, stream_);
       launch_finalize_mixed(fa, fl, stream_);
       timer_.end(stream_);
       if (!batch) {
-        CT_HIP_CHECK(hipMemcpyAsync(h_small, d_count, 4, hipMemcpyDeviceToHost, stream_));
-        CT_HIP_CHECK(hipStreamSynchronize(stream_));
-        const int64_t count = (int64_t) * reinterpret_cast<int32_t*>(h_small);
-        if (count > 0) {
-          timer_.begin(KernelTimer::REEVAL, stream_);
-          reevaluate(mode, pts, 0, count, flag_idx, fa, use_factor, ws64, ws64_bytes);
-          timer_.end(stream_);
-          reevaluated += count;
+        CT_HIP_CHECK(hipMemcpyAsync(sl.cnt_host, sl.count, 4, hipMemcpyDeviceToHost, stream_));
+        if (piped) {
+          CT_HIP_CHECK(hipEventRecord(sl.ev[EV_COUNT], stream_));
+          sl.used[EV_COUNT] = true;
         }
-        launch_fill_zero(reinterpret_cast<unsigned long long*>(d_count), 1, stream_);
       }
     }
+    ++chunks;
+  };
+  // Pipelined, the FP64 re-evaluation of a chunk's flagged points runs on
+  // its own stream, beside the next chunk's kernels, when its workspace (for
+  // the largest chunk) can be held for the whole call: growing it would free
+  // memory that a re-evaluation in flight may still use.
+  const bool re_side = piped && mixed &&
+                       d_ws64_.reserve(reeval_bytes(
+                           mode, chunk_points((int64_t)(reeval_bytes(mode, kPointAlign, use_factor) / kPointAlign), Nc),
+                           use_factor));
+  hipStream_t rs = re_side ? re_stream_ : stream_;
+  // Completes the chunk in slot sl: (mixed, per chunk) waits for its flag
+  // count and queues the FP64 re-evaluation of its flagged points; then
+  // (piped) marks the slot's results final and queues their download.
+  auto finish_chunk = [&](Slot& sl) {
+    if (mixed && !batch) {
+      if (piped)
+        CT_HIP_CHECK(hipEventSynchronize(sl.ev[EV_COUNT]));
+      else
+        CT_HIP_CHECK(hipStreamSynchronize(stream_));
+      const int64_t count = (int64_t) * reinterpret_cast<const int32_t*>(sl.cnt_host);
+      // (after the chunk's finaliser)
+      if (rs != stream_) CT_HIP_CHECK(hipStreamWaitEvent(rs, sl.ev[EV_COUNT], 0));
+      if (count > 0) {
+        timer_.begin(KernelTimer::REEVAL, rs);
+        reevaluate(mode, sl.pts, 0, count, sl.flag_idx, sl.fa, use_factor, ws64, ws64_bytes, rs);
+        timer_.end(rs);
+        reevaluated += count;
+      }
+    }
+    if (!piped) return;
+    CT_HIP_CHECK(hipEventRecord(sl.ev[EV_READY], rs));
+    sl.used[EV_READY] = true;
+    if (stage_width > 0) {
+      CT_HIP_CHECK(hipStreamWaitEvent(d2h_stream_, sl.ev[EV_READY], 0));
+      CT_HIP_CHECK(hipMemcpyAsync(sl.out_host, sl.out_stage, (size_t)sl.nc * stage_width * 8, hipMemcpyDeviceToHost,
+                                  d2h_stream_));
+      CT_HIP_CHECK(hipEventRecord(sl.ev[EV_D2H], d2h_stream_));
+      sl.used[EV_D2H] = true;
+      sl.out_pending = true;
+    }
+  };
+  // Copies the downloaded outputs of the chunk in slot sl into the caller's
+  // host arrays.
+  auto drain_slot = [&](Slot& sl) {
+    if (!sl.out_pending) return;
+    CT_HIP_CHECK(hipEventSynchronize(sl.ev[EV_D2H]));
+    std::vector<HostCopy> jobs;
+    const double* src = reinterpret_cast<const double*>(sl.out_host);
     for (size_t k = 0; k < NS; ++k) {
       if (!(q & kSlots[k].bit) || out_dev[k]) continue;
       const int w = kSlots[k].width;
+      double* user = out.*(kSlots[k].field);
       if (col_out && w > 1)
-        CT_HIP_CHECK(hipMemcpy2DAsync(out.*(kSlots[k].field) + start, sizeof(double) * n, fa.out.*(kSlots[k].field),
-                                      sizeof(double) * nc, sizeof(double) * nc, w, hipMemcpyDeviceToHost, stream_));
+        for (int c = 0; c < w; ++c)
+          jobs.push_back(HostCopy{user + (size_t)c * n + sl.start, src + (size_t)c * sl.nc, (size_t)sl.nc * 8});
       else
-        CT_HIP_CHECK(hipMemcpyAsync(out.*(kSlots[k].field) + start * w, fa.out.*(kSlots[k].field),
-                                    sizeof(double) * w * nc, hipMemcpyDeviceToHost, stream_));
+        jobs.push_back(HostCopy{user + sl.start * w, src, (size_t)sl.nc * w * 8});
+      src += (size_t)w * sl.nc;
     }
-    ++chunks;
+    host_copy(jobs.data(), (int)jobs.size());
+    sl.out_pending = false;
+  };
+
+  if (piped) {
+    // slot i % kPipeSlots; chunk i - 1 is completed once chunk i is queued
+    Slot* prev = nullptr;
+    int64_t i = 0, start = 0;
+    for (int64_t nc : pipeline_chunks(n, Nc)) {
+      Slot& sl = slots[i % kPipeSlots];
+      drain_slot(sl);
+      enqueue_chunk(sl, start, nc);
+      if (prev) finish_chunk(*prev);
+      prev = &sl;
+      start += nc;
+      ++i;
+    }
+    if (prev) finish_chunk(*prev);
+    for (int s = 0; s < kPipeSlots; ++s) drain_slot(slots[(i + s) % kPipeSlots]);
+    // (outputs in device memory: nothing above waited for the last
+    // re-evaluation)
+    CT_HIP_CHECK(hipStreamSynchronize(rs));
+  } else {
+    for (int64_t start = 0; start < n; start += Nc) {
+      enqueue_chunk(slots[0], start, std::min(Nc, n - start));
+      finish_chunk(slots[0]);
+    }
   }
   if (batch) {
-    CT_HIP_CHECK(hipMemcpyAsync(h_small, d_count, 4, hipMemcpyDeviceToHost, stream_));
+    CT_HIP_CHECK(hipMemcpyAsync(h_small, slots[0].count, 4, hipMemcpyDeviceToHost, stream_));
     CT_HIP_CHECK(hipStreamSynchronize(stream_));
     const int64_t count = (int64_t) * reinterpret_cast<int32_t*>(h_small);
     if (count > 0) {

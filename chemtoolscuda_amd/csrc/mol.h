@@ -40,6 +40,15 @@ int64_t device_allocation_held();
 // (CT_NOALLOC_SCRATCH_BYTES; looked up on the first call per device).
 void* noalloc_scratch(int device);
 
+// Host memory copies (dst, src, bytes), split over a few threads when large
+// (host_copy.cpp); returns when all are done.
+struct HostCopy {
+  void* dst;
+  const void* src;
+  size_t bytes;
+};
+void host_copy(const HostCopy* jobs, int njobs);
+
 // A caller-provided device region that an evaluation uses for all of its
 // scratch instead of the handle's growing buffers.
 struct Workspace {
@@ -159,9 +168,11 @@ class Mol {
   int64_t chunk_points(int64_t per_point, int64_t n) const;
   // FP64 re-evaluation of `count` points (pts: (m, 3) row-major, col_stride
   // 0, or column-major with that stride; index: positions in pts), in slices
-  // that fit ws (ws_bytes) or, ws null, the handle's growing FP64 workspace
+  // that fit ws (ws_bytes) or, ws null, the handle's growing FP64 workspace;
+  // on stream rs (null: the handle's stream)
   void reevaluate(int mode, const double* pts, int64_t col_stride, int64_t count, const int32_t* index,
-                  const FinalizeArgs& fin, bool factor, char* ws = nullptr, size_t ws_bytes = 0);
+                  const FinalizeArgs& fin, bool factor, char* ws = nullptr, size_t ws_bytes = 0,
+                  hipStream_t rs = nullptr);
   // workspace bytes of one re-evaluation slice of at most count points
   size_t reeval_bytes(int mode, int64_t count, bool factor) const;
   // whether this mode runs on the factor of P (ct_eval_params.route; the FP64
@@ -210,7 +221,18 @@ class Mol {
   DeviceBuffer d_flags_;   // mixed path: flagged point list of a whole call (device outputs)
   DeviceBuffer d_flags_sorted_;  // the same in index order, and the sort's scratch
   PinnedBuffer h_small_;   // pinned readback of the flag counter / error estimate
-  PinnedBuffer h_stage_;   // host staging for pageable inputs/outputs
+  PinnedBuffer h_stage_;   // pinned staging of host-array inputs/outputs, one region per pipeline slot
+  // Host-array pipeline (eval with points, outputs or targets in host
+  // memory): chunks pass through kPipeSlots slots of pinned and device
+  // staging; uploads, downloads and the FP64 re-evaluation of flagged points
+  // run on their own streams, overlapping the kernels of the neighbouring
+  // chunks.  Created on first use.
+  static constexpr int kPipeSlots = 3;
+  static constexpr int64_t kHostChunk = int64_t(1) << 20;  // largest pipelined chunk (points)
+  enum PipeEvent { EV_H2D = 0, EV_COUNT, EV_READY, EV_D2H, NPIPEEV };
+  hipStream_t h2d_stream_ = nullptr, d2h_stream_ = nullptr, re_stream_ = nullptr;
+  hipEvent_t pipe_ev_[kPipeSlots][NPIPEEV] = {};
+  void ensure_pipeline();
   int64_t ws_limit_ = int64_t(8) << 30;  // set from the device's memory at construction
   KernelTimer timer_;
   // ESP: Hermite pre-contraction of P, rebuilt when P or the centres change

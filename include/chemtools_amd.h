@@ -5,7 +5,11 @@
  * or a negative CT_E* status; the message of the last failure on the calling
  * thread is available from ct_last_error().  Pointer arguments for points and
  * outputs may be host memory or device (HBM) memory: the library detects
- * which and copies through pinned staging buffers only when needed.
+ * which.  ct_eval with host arrays runs its chunks as a pipeline: inputs are
+ * copied (several host threads) into pinned staging and uploaded, outputs
+ * downloaded into pinned staging and copied out, on streams of their own,
+ * overlapping the kernels of the neighbouring chunks.  ct_esp and
+ * ct_backward copy host arrays in and out around their kernels.
  *
  * Each ct_mol owns its device copies of the shell table and of P, its own HIP
  * stream and workspace, so handles are independent and reentrant (the
