@@ -494,8 +494,9 @@ int64_t contract_f64_factor_split_workspace(int mode, int rows_pad, int64_t ld) 
 }
 
 void launch_contract_f64_factor_split(int mode, const double* L, int rows_pad, int nbasis_pad, const double* ao,
-                                      int64_t ld, double* part, double* dots, hipStream_t s) {
-  factor_modes(mode, L, rows_pad, nbasis_pad, ao, ld, part, s, true);
+                                      int64_t ld, double* part, double* dots, hipStream_t s, const uint32_t* kmask,
+                                      int nwords, int tp) {
+  factor_modes(mode, L, rows_pad, nbasis_pad, ao, ld, part, s, true, kmask, nwords, tp);
   const int64_t count = (int64_t)mode_ndot(mode) * ld;
   hipLaunchKernelGGL(reduce_row_blocks_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, part,
                      rows_pad / 64, count, dots);

@@ -144,7 +144,8 @@ void launch_contract_f64_factor(int mode, const double* L, int rows_pad, int nba
 // holds contract_f64_factor_split_workspace() doubles.
 int64_t contract_f64_factor_split_workspace(int mode, int rows_pad, int64_t ld);
 void launch_contract_f64_factor_split(int mode, const double* L, int rows_pad, int nbasis_pad, const double* ao,
-                                      int64_t ld, double* part, double* dots, hipStream_t s);
+                                      int64_t ld, double* part, double* dots, hipStream_t s,
+                                      const uint32_t* kmask = nullptr, int nwords = 0, int tile_points = 64);
 // The same contraction with one 64-row block of P per wave and an ordered
 // sum of the blocks' partial dots (for small point counts); part holds
 // contract_f64_split_workspace() doubles.

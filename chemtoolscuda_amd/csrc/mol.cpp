@@ -954,11 +954,18 @@ int64_t Mol::chunk_points(int64_t per_point, int64_t n) const {
 
 size_t Mol::reeval_bytes(int mode, int64_t count, bool factor) const {
   // AO arrays, dots, gathered points and the split contraction's partials
-  // (an upper bound for every slice of at most count points)
+  // (an upper bound for every slice of at most count points); the screened
+  // factor route: the active k-step counters, lists, counts and masks of the
+  // slice's point tiles
   const int nder = mode_nder(mode), ndot = mode_ndot(mode);
   const int64_t ld = round_up(count, kPointAlign);
   const int blocks = factor ? factor_rows_ / 64 : Mp_ / 64;
-  return (size_t)ld * ((size_t)nder * Mp_ * 8 + (size_t)ndot * 8 + 24 + (size_t)blocks * ndot * 8);
+  size_t bytes = (size_t)ld * ((size_t)nder * Mp_ * 8 + (size_t)ndot * 8 + 24 + (size_t)blocks * ndot * 8);
+  if (factor && screen_eps_ > 0.0) {
+    const int nk = Mp_ / 32, nwords = (nk + 31) / 32;
+    bytes += 128 * kScreenCounters + (size_t)(ld / factor_tile_points(mode)) * (4 * nk + 4 + 4 * nwords);
+  }
+  return bytes;
 }
 
 // FP64 evaluation of `count` points (index: their positions in pts), written
@@ -997,17 +1004,35 @@ void Mol::reevaluate(int mode, const double* pts, int64_t col_stride, int64_t co
     double* gp = dots + (size_t)ndot * ld;
     double* part = gp + 3 * (size_t)ld;
     launch_gather_points(pts, idx, c, gp, rs, col_stride);
-    // the factor route in the order of its factor: the screened shell order
-    // (every shell evaluated here)
+    // the factor route in the order of its factor (the screened shell
+    // order), screened as the FP64 path screens it: per tile of the slice's
+    // points (in index order), the k-steps of its shells within the FP64
+    // cutoff radii
+    const bool scr = factor && screen_eps_ > 0.0;
+    const int tp = scr ? factor_tile_points(mode) : 64;
+    const int nk = Mp_ / 32, nwords = (nk + 31) / 32;
+    uint32_t* kmask = nullptr;
+    if (scr) {
+      char* sp = reinterpret_cast<char*>(part + (size_t)(factor_rows_ / 64) * ndot * ld);
+      auto* counters = reinterpret_cast<unsigned long long*>(sp);  // (unused here)
+      const int64_t ntile = ld / tp;
+      int32_t* klist = reinterpret_cast<int32_t*>(sp + 128 * kScreenCounters);
+      int32_t* kcount = klist + ntile * nk;
+      kmask = reinterpret_cast<uint32_t*>(kcount + ntile);
+      launch_tile_activity(gp, c, tp, ntile, static_cast<const int32_t*>(d_span_off64_.get()),
+                           static_cast<const ScreenSpan*>(d_spans64_.get()), nk, klist, kcount, kmask, nwords,
+                           counters, rs);
+    }
     launch_ao_f64(mode, lmax_, static_cast<const ShellMeta*>(factor ? d_smeta_.get() : d_meta_.get()),
                   static_cast<const ShellGeom*>(factor ? d_sgeom_.get() : d_geom_.get()),
-                  static_cast<const PrimData*>(d_prim_.get()), nshell_, M_, Mp_, gp, c, ld, ao, 700.0, rs);
+                  static_cast<const PrimData*>(d_prim_.get()), nshell_, M_, Mp_, gp, c, ld, ao, 700.0, rs, kmask,
+                  nwords, tp);
     if (factor && split)
       launch_contract_f64_factor_split(mode, static_cast<const double*>(d_L64s_.get()), factor_rows_, Mp_, ao, ld,
-                                       part, dots, rs);
+                                       part, dots, rs, kmask, nwords, tp);
     else if (factor)
       launch_contract_f64_factor(mode, static_cast<const double*>(d_L64s_.get()), factor_rows_, Mp_, ao, ld, dots,
-                                 rs);
+                                 rs, kmask, nwords, tp);
     else if (split)
       launch_contract_f64_split(mode, static_cast<const double*>(d_P_.get()), Mp_, ao, ld, part, dots, rs);
     else
