@@ -175,8 +175,8 @@ class Mol {
                   hipStream_t rs = nullptr);
   // workspace bytes of one re-evaluation slice of at most count points
   size_t reeval_bytes(int mode, int64_t count, bool factor) const;
-  // whether this mode runs on the factor of P (ct_eval_params.route; the FP64
-  // path only when asked)
+  // whether this mode runs on the factor of P (ct_eval_params.route, the
+  // same decision in both precisions)
   bool factor_route(int mode, int route, bool mixed) const;
 
   HostBasis basis_;

@@ -113,7 +113,7 @@ typedef struct {
   double ked_alpha;        /* CT_Q_KED_ALPHA */
   double mixed_tol;        /* relative error target of the mixed path (0 => 1e-6) */
   double rho_floor;        /* points with rho below this are not re-evaluated (0 => 1e-10) */
-  int32_t route;           /* mixed path: CT_ROUTE_AUTO (0), CT_ROUTE_DENSITY, CT_ROUTE_FACTOR */
+  int32_t route;           /* CT_ROUTE_AUTO (0), CT_ROUTE_DENSITY, CT_ROUTE_FACTOR (both precisions) */
   int32_t out_layout;      /* grad / hess outputs: CT_LAYOUT_ROW (0) or CT_LAYOUT_COL */
   /* Mixed path, optional (NULL: off): a target density per point (n
    * entries, host or device).  The error of rho is then bounded by
@@ -124,16 +124,16 @@ typedef struct {
   const double* rho_target;
 } ct_eval_params;
 
-/* Routes of the mixed contraction (rho, grad and lap quantities).  DENSITY
+/* Routes of the contraction (every density-family quantity).  DENSITY
  * contracts the AOs with P itself, as the reference does (rho = phi^T P phi).
  * FACTOR contracts them with a pivoted Cholesky factor P = L L^T computed on
  * the host when P is set (psi = L^T phi, rho = psi^T psi, grad rho =
  * 2 psi^T grad psi, ...): rank(P) columns instead of nbasis, available when P
  * is positive semidefinite of rank at most ~half the basis (SCF densities).
  * AUTO takes FACTOR when it is available and needs fewer matrix-core
- * products (all modes).  Both routes meet the same mixed_tol through
- * an FP64 re-evaluation of flagged points (on the route that flagged them).
- * The FP64 path contracts with P unless FACTOR is asked for. */
+ * products (all modes, both precisions).  In FP64 the two routes agree to
+ * rounding; in the mixed path both meet the same mixed_tol through an FP64
+ * re-evaluation of flagged points (on the route that flagged them). */
 #define CT_ROUTE_AUTO 0
 #define CT_ROUTE_DENSITY 1
 #define CT_ROUTE_FACTOR 2
@@ -158,8 +158,8 @@ typedef struct {
   int64_t n_reevaluated;   /* mixed path: points recomputed in FP64 */
   int64_t n_chunks;
   double max_error_estimate;
-  int64_t factor_rank;     /* mixed path: rank of the factor if the FACTOR route ran, else -1 */
-  double active_fraction;  /* mixed FACTOR route with screening: fraction of the (point tile, k-step)
+  int64_t factor_rank;     /* rank of the factor if the FACTOR route ran, else -1 */
+  double active_fraction;  /* FACTOR route with screening: fraction of the (point tile, k-step)
                               blocks contracted; 1 otherwise */
 } ct_eval_report;
 
@@ -208,7 +208,7 @@ int ct_set_device_allocation_limit(int64_t bytes);
 /* Device memory the library's buffers hold now (all handles; bytes). */
 int ct_device_allocation_held(int64_t* bytes);
 
-/* Spatial screening of the mixed FACTOR route (no counterpart in the
+/* Spatial screening of the FACTOR route (no counterpart in the
  * reference, which evaluates every basis function at every point,
  * src/evaluate_density.cu:25-411).  Points are taken in Morton order and,
  * per tile of 16 (rho+grad+lap+tau), 32 (grad, Hessian) or 64 (rho) of them,
@@ -217,7 +217,8 @@ int ct_device_allocation_held(int64_t* bytes);
  * skipped; 4 eps ||c||_2 sqrt(rho) (c_j = ||L_j||_1, P = L L^T) is added to
  * each point's error estimate, so mixed_tol still holds for rho.  eps = 0 turns it off; default
  * 1e-15.  The other quantities change by at most about eps times their
- * derivative order's AO scale. */
+ * derivative order's AO scale.  The FP64 path screens at eps * 1e-7: what it
+ * skips stays below FP64 rounding of rho wherever rho is above ~1e-12. */
 int ct_mol_set_screening(ct_mol* mol, double eps);
 /* Distance from the centre beyond which every function of one shell (l,
  * exponents and contraction coefficients as in ct_basis_desc) and its first
