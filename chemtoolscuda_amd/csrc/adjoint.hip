@@ -5,8 +5,9 @@
 //
 // Two FP64 matrix-core kernels per point chunk:
 //   * gram_f64: G = Phi diag(w) Phi^T, split over point slices.  One wave owns
-//     a 64 x 64 tile of G and one slice of points (the GEMM's K); each slice
-//     writes its own partial tile, and a reduction adds the slices in a fixed
+//     a 64 x 64 tile of the upper triangle of G and one slice of points (the
+//     GEMM's K); each slice writes its own partial tile and, off the
+//     diagonal, its transpose, and a reduction adds the slices in a fixed
 //     order (deterministic).
 //   * contract_adjoint_f64: the forward contraction's GEMM Y = P phi with an
 //     epilogue that sums w_g d_k phi_i(g) Y_i(g) over the wave's points per
@@ -32,9 +33,16 @@ __global__ __launch_bounds__(64) void gram_f64_kernel(const double* __restrict__
                                                       double* __restrict__ part, int ntile) {
   const int lane = threadIdx.x;
   const int r = lane & 15, q = lane >> 4;
-  const int tile = blockIdx.x % (ntile * ntile);
-  const int slice = blockIdx.x / (ntile * ntile);
-  const int I = (tile / ntile) * 64, J = (tile % ntile) * 64;
+  const int ntri = ntile * (ntile + 1) / 2;
+  int t = blockIdx.x % ntri;
+  const int slice = blockIdx.x / ntri;
+  // (row block a, column block b >= a) of the t-th upper-triangle tile
+  int ra = 0;
+  while (t >= ntile - ra) {
+    t -= ntile - ra;
+    ++ra;
+  }
+  const int I = ra * 64, J = (ra + t) * 64;
   const int64_t pbeg = (int64_t)slice * kGramSlice;
   const int64_t pend = pbeg + kGramSlice < npts ? pbeg + kGramSlice : npts;
   d4 acc[4][4];
@@ -63,7 +71,11 @@ __global__ __launch_bounds__(64) void gram_f64_kernel(const double* __restrict__
 #pragma unroll
     for (int cb = 0; cb < 4; ++cb)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) out[(int64_t)(I + 16 * rb + q + 4 * j) * mp + J + 16 * cb + r] = acc[rb][cb][j];
+      for (int j = 0; j < 4; ++j) {
+        const int64_t i = I + 16 * rb + q + 4 * j, jj = J + 16 * cb + r;
+        out[i * mp + jj] = acc[rb][cb][j];
+        if (I != J) out[jj * mp + i] = acc[rb][cb][j];
+      }
 }
 
 __global__ __launch_bounds__(256) void gram_reduce_kernel(const double* __restrict__ part, int nslice, int64_t count,
@@ -153,8 +165,8 @@ void launch_gram_f64(const double* ao, int mp, int64_t ld, const double* w, int6
                      hipStream_t s) {
   const int ntile = mp / 64;
   const int nslice = gram_slices(npts);
-  hipLaunchKernelGGL(gram_f64_kernel, dim3((unsigned)(ntile * ntile * nslice)), dim3(64), 0, s, ao, mp, ld, w, npts,
-                     part, ntile);
+  hipLaunchKernelGGL(gram_f64_kernel, dim3((unsigned)(ntile * (ntile + 1) / 2 * nslice)), dim3(64), 0, s, ao, mp, ld,
+                     w, npts, part, ntile);
   CT_HIP_CHECK(hipGetLastError());
 }
 

@@ -245,16 +245,36 @@ void launch_contract_adjoint_mixed(const uint16_t* Pf, int mp, const float* ao, 
   CT_HIP_CHECK(hipGetLastError());
 }
 
-void gram_mixed_plan(int64_t npts, int mp, int* nslice, int64_t* slice_len) {
+void gram_mixed_plan(int64_t npts, int mp, int* nslice, int64_t* slice_len, int max_slices) {
   const int nb = mp / TR;
   const int ntri = nb * (nb + 1) / 2;
-  int64_t ns = (2048 + ntri - 1) / ntri;
-  const int64_t max_by_points = (npts + 1023) / 1024;  // slices of at least 1024 points
-  if (ns > max_by_points) ns = max_by_points;
-  if (ns > 64) ns = 64;
-  if (ns < 1) ns = 1;
-  int64_t len = (npts + ns - 1) / ns;
-  len = (len + KB - 1) / KB * KB;
+  // The (tile, slice) workgroups run two per CU in rounds: the call takes
+  // (rounds) x (points per slice).  Of the slice counts up to 64 (slices of
+  // at least 1024 points) the one with the least of that product, the
+  // smallest on a tie (fewer partial tiles to add): 10 tiles (M = 416) on 256
+  // CUs take 51 slices, one round, where 64 slices ran a second round for
+  // 128 of 640 workgroups.
+  int dev = 0, cu = 0;
+  CT_HIP_CHECK(hipGetDevice(&dev));
+  CT_HIP_CHECK(hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev));
+  const int64_t slots = 2 * (int64_t)(cu > 0 ? cu : 1);
+  int64_t cap = (npts + 1023) / 1024;
+  if (cap > 64) cap = 64;
+  if (cap > max_slices) cap = max_slices;
+  if (cap < 1) cap = 1;
+  int64_t best = 1;
+  double best_cost = 0.0;
+  for (int64_t ns = 1; ns <= cap; ++ns) {
+    const int64_t len = ((npts + ns - 1) / ns + KB - 1) / KB * KB;
+    const int64_t used = (npts + len - 1) / len;
+    const int64_t rounds = (ntri * used + slots - 1) / slots;
+    const double cost = (double)rounds * (double)len;
+    if (ns == 1 || cost < best_cost) {
+      best = ns;
+      best_cost = cost;
+    }
+  }
+  int64_t len = ((npts + best - 1) / best + KB - 1) / KB * KB;
   *slice_len = len;
   *nslice = (int)((npts + len - 1) / len);
 }

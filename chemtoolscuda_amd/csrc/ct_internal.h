@@ -285,7 +285,9 @@ void launch_gram_reduce(const double* part, int nslice, int64_t count, double* G
 void launch_contract_adjoint_f64(const double* P, int mp, const double* ao, int64_t ld, const double* w,
                                  int64_t npts, double* T, hipStream_t s);
 // Mixed adjoint (FP32 row-quad AOs, split-bf16 MFMA, FP64 partials).
-void gram_mixed_plan(int64_t npts, int mp, int* nslice, int64_t* slice_len);
+// Point slices of the mixed Gram kernel for npts points (at most max_slices):
+// the number of slices and the points per slice (a multiple of 32).
+void gram_mixed_plan(int64_t npts, int mp, int* nslice, int64_t* slice_len, int max_slices = 64);
 void launch_gram_mixed(const float* ao, int mp, int64_t ld, const double* w, int64_t npts, const int32_t* tiles,
                        int nslice, int64_t slice_len, double* part, hipStream_t s);
 void launch_contract_adjoint_mixed(const uint16_t* Pf, int mp, const float* ao, int64_t ld, const double* w,

@@ -752,7 +752,7 @@ void Mol::backward(const double* points, int64_t n, int layout, const double* w,
   double* G = static_cast<double*>(d_adj_.get());
   double* T = G + gsize;
   double* part = T + (size_t)Mp_ * 3;
-  CT_HIP_CHECK(hipMemsetAsync(G, 0, sizeof(double) * (gsize + (size_t)Mp_ * 3), stream_));
+  launch_fill_zero(reinterpret_cast<unsigned long long*>(G), (int64_t)(gsize + (size_t)Mp_ * 3), stream_);
   const int32_t* tiles = nullptr;
   if (mixed) {
     // upper-triangle 128 x 128 tiles of the Gram matrix, (I, J) with I <= J
@@ -811,7 +811,7 @@ void Mol::backward(const double* points, int64_t n, int layout, const double* w,
       launch_ao_f32(mode, lmax_, meta, geom, prim, nshell_, M_, Mp_, pts, nc, ld, ao, 50.0, nullptr, 0, 64, stream_);
       int nslice = 0;
       int64_t len = 0;
-      gram_mixed_plan(nc, Mp_, &nslice, &len);
+      gram_mixed_plan(nc, Mp_, &nslice, &len, nslice_max);
       launch_gram_mixed(ao, Mp_, ld, wc, nc, tiles, nslice, len, part, stream_);
       launch_gram_reduce(part, nslice, (int64_t)gsize, G, stream_);
       launch_contract_adjoint_mixed(static_cast<const uint16_t*>(d_Pb_.get()), Mp_, ao, ld, wc, nc, T, stream_);
