@@ -65,7 +65,7 @@ def parse():
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64-vs-mixed comparison run")
     ap.add_argument("--workspace-mb", type=int, default=0, help="per-chunk workspace budget (0: library default)")
     ap.add_argument("--screening", type=float, default=None, metavar="EPS",
-                    help="spatial screening threshold of the mixed factor route (0 turns it off; "
+                    help="spatial screening threshold of the factor route (0 turns it off; "
                          "default: the library's, 1e-15)")
     ap.add_argument("--no-extra", action="store_true", help="skip the fwd+bwd and ESP secondary measurements")
     ap.add_argument("--part", default=None, choices=["esp", "fwd_bwd", "autograd", "large300", "host"],

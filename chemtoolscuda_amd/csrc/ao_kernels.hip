@@ -18,7 +18,7 @@
 //     when every lane has a r^2 > screen (exp underflow region in parity
 //     mode; below FP32 resolution of any density above the re-evaluation
 //     floor in the mixed path);
-//   * in the screened layout of the mixed factor route (points in Morton
+//   * in the screened layout of the factor route (points in Morton
 //     order, per-tile activity masks) a wave skips, stores included, every
 //     shell that lies only in k-steps the contraction skips for its points;
 //   * values are computed in FP64 for both paths; the mixed path rounds them

@@ -193,7 +193,7 @@ void launch_factor_residual(const double* A, int mp, const double* L, int rows, 
                             unsigned long long* out, hipStream_t s);
 
 // ---------------------------------------------------------------------------
-// Spatial screening of the mixed factor route (screen_host.cpp, screen.hip).
+// Spatial screening of the factor route (screen_host.cpp, screen.hip).
 // sqrt((2a/pi)^{3/2} (4a)^L): radial part of a primitive's L2 normalisation.
 double prim_radial_norm(double a, int L);
 // Distance from the centre beyond which the value, every first and second

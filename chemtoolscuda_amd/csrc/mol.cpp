@@ -604,7 +604,8 @@ void Mol::build_screen() {
   CT_HIP_CHECK(hipMemcpy(d_Lb_.get(), Lf.data(), sizeof(uint16_t) * Lf.size(), hipMemcpyHostToDevice));
   // FP64 copy in the same function order, k-major (L64s[i * rows + j] =
   // L_ij for screened function i): the FP64 factor route and the
-  // re-evaluation of the mixed factor route's flagged points
+  // FP64 factor route in the screened order (also the re-evaluation of the
+  // mixed factor route's flagged points)
   std::vector<double> L64s((size_t)Mp_ * factor_rows_, 0.0);
   for (int j = 0; j < rank; ++j)
     for (int i = 0; i < M_; ++i) L64s[(size_t)i * factor_rows_ + j] = L_[(size_t)j * M_ + layout_.func_src[i]];
@@ -1067,7 +1068,7 @@ void Mol::eval(const double* points, int64_t n, int layout, uint32_t q, int prec
   const int nder = mode_nder(mode), ndot = mode_ndot(mode);
   const bool use_factor = factor_route(mode, par.route, mixed);
   const int nrt = (use_factor ? factor_rows_ : Mp_) / kRowBlock;  // row blocks of the mixed partials
-  // screened layout (mixed factor route): points in Morton order, per-tile
+  // screened layout (factor route): points in Morton order, per-tile
   // lists of the active k-steps
   const int nk = Mp_ / 32, nwords = (nk + 31) / 32;
   const bool screened = use_factor && screen_eps_ > 0.0;

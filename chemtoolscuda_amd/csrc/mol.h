@@ -133,9 +133,10 @@ class Mol {
   void set_coordinates(const double* xyz);  // host or device
   void set_density_matrix(const double* P);  // host or device
   void set_workspace_limit(int64_t bytes) { ws_limit_ = bytes; }
-  // Spatial screening of the mixed factor route: k-steps of basis functions
-  // whose values and derivatives are at most eps at every point of a tile
-  // are skipped (their bound enters the error estimate); 0 turns it off.
+  // Spatial screening of the factor route: k-steps of basis functions whose
+  // values and derivatives are at most eps (FP64 path: eps x
+  // kFp64ScreenFactor) at every point of a tile are skipped (mixed path: their
+  // bound enters the error estimate); 0 turns it off.
   void set_screening(double eps);
   double screening() const { return screen_eps_; }
   hipStream_t stream() const { return stream_; }
@@ -160,7 +161,7 @@ class Mol {
  private:
   void upload_geometry();
   void upload_P();
-  // screened layout of the mixed factor route (shell order, bf16 factor in
+  // screened layout of the factor route (shell order, bf16 and FP64 factor in
   // that order, k-step spans, truncation bounds); after P, the geometry or
   // the threshold change
   void build_screen();
@@ -196,7 +197,7 @@ class Mol {
   int factor_rank_ = -1;   // rank of the factor, -1 when P has none (not PSD or rank too high)
   int factor_rows_ = 0;    // rows of L^T padded to a multiple of 128
   std::vector<double> L_;  // the factor, rank x M (row j = column j of L), natural order
-  // spatial screening of the mixed factor route
+  // spatial screening of the factor route
   double screen_eps_ = 1e-15;
   bool screen_built_ = false;
   bool layout_stale_ = true;           // the shell order must be rebuilt (threshold changed, centres moved far)

@@ -424,8 +424,8 @@ PYBIND11_MODULE(_chemtools_amd, m) {
       .def("set_density_matrix", &Molecule::set_density_matrix)
       .def("set_workspace_limit", &Molecule::set_workspace_limit)
       .def("set_screening", &Molecule::set_screening, py::arg("eps"),
-           "Spatial screening of the mixed factor route: basis-function blocks whose values and derivatives "
-           "are at most eps on a point tile are skipped (0 turns it off; default 1e-15).")
+           "Spatial screening of the factor route: basis-function blocks whose values and derivatives "
+           "are at most eps on a point tile are skipped (0 turns it off; default 1e-15; FP64 at 1e-7 of it).")
       .def("last_report", &Molecule::last_report)
       .def_property_readonly("coordinates", &Molecule::coordinates, "Cartesian coordinates of the atomic centres.")
       .def_property_readonly("numbers", &Molecule::numbers, "Atomic numbers of the atomic centres.")

@@ -1,9 +1,10 @@
-// Spatial screening of the mixed factor route, device side.
+// Spatial screening of the factor route, device side.
 //
 //   1. points are put in Morton order (24-bit keys on a lattice around the
 //      molecule; stable radix sort of (key, index) pairs), so that a point
 //      tile of the contraction is a compact box in space;
-//   2. per point tile (64 points per wave) the tile's bounding box marks the
+//   2. per point tile (16, 32 or 64 points by mode; a wave takes 64 points)
+//      the tile's bounding box marks the
 //      32-function k-steps holding a shell within reach of the box (distance
 //      below the shell's cutoff radius); the others are skipped by the AO
 //      kernel (no evaluation, no stores) and by the contraction (no loads, no

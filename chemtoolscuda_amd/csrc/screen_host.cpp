@@ -1,4 +1,4 @@
-// Spatial screening of the mixed factor route, host side: rigorous cutoff
+// Spatial screening of the factor route, host side: rigorous cutoff
 // radii per shell, the shell order of the screened AO layout and the table of
 // shells per 32-function k-step that the tile-activity kernel tests.
 //
