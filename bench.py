@@ -438,7 +438,17 @@ def main():
             roofline["executed_screened"] = {"flops_per_point": f_scr, "achieved": a_scr / 1e12,
                                              "frac": a_scr / PEAK[peak_key]}
 
-    extra = {}
+    # what this device sustains for the same instruction mixes (clocks and
+    # power as under load): the published peaks assume the top clock
+    measured = capi.measure_peaks()
+    m_key = "bf16_mfma_tflops" if precision != "fp64" else "f64_mfma_tflops"
+    roofline["measured_peak"] = measured[m_key]
+    roofline["frac_of_measured"] = roofline["achieved"] / measured[m_key]
+    for sub in ("executed", "executed_screened"):
+        if sub in roofline:
+            roofline[sub]["frac_of_measured"] = roofline[sub]["achieved"] / measured[m_key]
+
+    extra = {"measured_peaks": measured}
     if precision == "mixed" and not args.no_compare:
         # FP64 on the same route as the mixed run (AUTO: the screened factor
         # route when P has a low-rank factor): the like-for-like ratio
@@ -485,7 +495,10 @@ def main():
                                      "avg_launch_ms": bw_ms["mixed"], "flops_per_point": 4.0 * m_d * m_d,
                                      "fp64": {"achieved": f_bw / (bw_ms["fp64"] * 1e-3) / 1e12,
                                               "peak": PEAK["fp64"] / 1e12,
-                                              "frac": f_bw / (bw_ms["fp64"] * 1e-3) / PEAK["fp64"]}}
+                                              "frac": f_bw / (bw_ms["fp64"] * 1e-3) / PEAK["fp64"],
+                                              "frac_of_measured": f_bw / (bw_ms["fp64"] * 1e-3) / 1e12 /
+                                              measured["f64_mfma_tflops"]},
+                                     "frac_of_measured": a_bw / 1e12 / measured["bf16_mfma_tflops"]}
         ag = {}
         for p in ("fp64", "mixed"):
             rate, fr = fwd_bwd_autograd(torch, synthetic, p)
@@ -497,6 +510,8 @@ def main():
                                      "autograd: L = sum w (rho - rho_t)^2, mixed forward held to 1e-6 of "
                                      "|rho - rho_t| (rho_target)")
         extra["esp"] = esp_rate(torch, capi, synthetic)
+        extra["esp"]["roofline"]["frac_of_measured"] = (extra["esp"]["roofline"]["achieved"] /
+                                                        measured["f64_valu_tflops"])
         extra["large300"] = large_rate(torch, capi, synthetic)
 
     kernel_ms = {k: stats[f"{k}_ms"] / args.steps for k in ("screen", "ao", "contract", "finalize", "reeval")}

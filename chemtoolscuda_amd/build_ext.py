@@ -23,7 +23,7 @@ LIB = os.path.join(HERE, "libchemtools_amd.so")
 ARCH = os.environ.get("CT_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["ao_kernels.hip", "contract_f64.hip", "contract_mixed.hip", "finalize.hip", "esp.hip", "adjoint.hip",
-               "adjoint_mixed.hip", "screen.hip", "noalloc.hip"]
+               "adjoint_mixed.hip", "screen.hip", "noalloc.hip", "peak.hip"]
 HOST_SOURCES = ["mol.cpp", "capi.cpp", "fchk.cpp", "wavefunction.cpp", "esp_host.cpp", "screen_host.cpp",
                 "host_copy.cpp"]
 

@@ -86,6 +86,7 @@ def lib():
         L.ct_mol_set_screening.argtypes = [vp, ctypes.c_double]
         L.ct_set_device_allocation_limit.argtypes = [i64]
         L.ct_device_allocation_held.argtypes = [ctypes.POINTER(i64)]
+        L.ct_measure_peaks.argtypes = [ctypes.POINTER(ctypes.c_double)] * 4
         L.ct_shell_screening_radius.argtypes = [i32, i32, vp, vp, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
         L.ct_mol_synchronize.argtypes = [vp]
         L.ct_mol_enable_profiling.argtypes = [vp, ctypes.c_int]
@@ -243,6 +244,15 @@ def device_allocation_held() -> int:
     v = ctypes.c_int64()
     check(lib().ct_device_allocation_held(ctypes.byref(v)))
     return v.value
+
+
+def measure_peaks() -> dict:
+    """ct_measure_peaks: bf16 and FP64 matrix-core TFLOP/s, FP64 vector FMA
+    TFLOP/s and HBM copy GB/s (read + write) sustained by the current device."""
+    v = [ctypes.c_double() for _ in range(4)]
+    check(lib().ct_measure_peaks(*[ctypes.byref(x) for x in v]))
+    return {"bf16_mfma_tflops": v[0].value, "f64_mfma_tflops": v[1].value, "f64_valu_tflops": v[2].value,
+            "hbm_copy_gbs": v[3].value}
 
 
 def boys(T, m: int, lmax_pairs: int):

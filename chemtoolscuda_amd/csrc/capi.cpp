@@ -238,6 +238,12 @@ int ct_device_allocation_held(int64_t* bytes) {
     *bytes = ct::device_allocation_held();
   });
 }
+int ct_measure_peaks(double* bf16_mfma_tflops, double* f64_mfma_tflops, double* f64_valu_tflops, double* hbm_gbs) {
+  return guard([&] {
+    CT_REQUIRE(bf16_mfma_tflops && f64_mfma_tflops && f64_valu_tflops && hbm_gbs, "null output");
+    ct::measure_peaks(bf16_mfma_tflops, f64_mfma_tflops, f64_valu_tflops, hbm_gbs);
+  });
+}
 int ct_mol_set_screening(ct_mol* mol, double eps) {
   return guard([&] {
     CT_REQUIRE(eps >= 0.0 && std::isfinite(eps), "screening threshold must be finite and non-negative");

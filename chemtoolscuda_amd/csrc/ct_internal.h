@@ -285,6 +285,9 @@ void launch_gram_reduce(const double* part, int nslice, int64_t count, double* G
 void launch_contract_adjoint_f64(const double* P, int mp, const double* ao, int64_t ld, const double* w,
                                  int64_t npts, double* T, hipStream_t s);
 // Mixed adjoint (FP32 row-quad AOs, split-bf16 MFMA, FP64 partials).
+// Sustained rates of the device (peak.hip): bf16 and FP64 matrix cores,
+// FP64 vector FMA (TFLOP/s), HBM copy (GB/s, read + write).
+void measure_peaks(double* bf16_tflops, double* f64_mfma_tflops, double* f64_valu_tflops, double* hbm_gbs);
 // Point slices of the mixed Gram kernel for npts points (at most max_slices):
 // the number of slices and the points per slice (a multiple of 32).
 void gram_mixed_plan(int64_t npts, int mp, int* nslice, int64_t* slice_len, int max_slices = 64);

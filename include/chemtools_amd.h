@@ -208,6 +208,14 @@ int ct_set_device_allocation_limit(int64_t bytes);
 /* Device memory the library's buffers hold now (all handles; bytes). */
 int ct_device_allocation_held(int64_t* bytes);
 
+/* Rates this device sustains, measured now on the current device with
+ * short probe kernels at the clocks and power of a real run (no reference
+ * counterpart; a measuring aid for roofline fractions): bf16 matrix-core
+ * TFLOP/s (v_mfma_f32_16x16x32_bf16), FP64 matrix-core TFLOP/s
+ * (v_mfma_f64_16x16x4f64), FP64 vector FMA TFLOP/s, HBM copy GB/s (read +
+ * write).  Allocates 4 GiB for the copy for the duration of the call. */
+int ct_measure_peaks(double* bf16_mfma_tflops, double* f64_mfma_tflops, double* f64_valu_tflops, double* hbm_gbs);
+
 /* Spatial screening of the FACTOR route (no counterpart in the
  * reference, which evaluates every basis function at every point,
  * src/evaluate_density.cu:25-411).  Points are taken in Morton order and,
