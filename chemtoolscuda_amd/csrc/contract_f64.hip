@@ -11,17 +11,19 @@
 // AO rows, so Y never leaves the registers and every quantity of a mode comes
 // out of a single pass.
 //
-// Work decomposition: one wavefront owns a set of points and all rows of P.
+// Work decomposition: a pair of wavefronts owns a set of points and all rows
+// of P, each wave taking half of the 64-row blocks of P.
 //   * 1 GEMM column type (MODE_RHO, MODE_GRAD): 4 column blocks x 16 points
-//     = 64 points per wave.
+//     = 64 points per wave pair.
 //   * 4 column types (MODE_LAP, MODE_HESS): the 4 column blocks are phi,
 //     d_x phi, d_y phi, d_z phi of the same 16 points.
-// For each 64-row block of P (4 MFMA row blocks) the wave streams all of K,
-// accumulating a 64 x 64 FP64 tile (16 MFMA accumulators), then runs the
-// epilogue.  Operands are read straight from global memory with coalesced
-// 128-byte row segments: A[i][k] = P[k][i] (P is symmetric), B[k][j] =
-// AO[k][p0 + j]; P (<= a few MB) and the chunk's AO rows stay L2-resident.
-// Waves are independent (no LDS, no barriers).
+// For each of its 64-row blocks of P (4 MFMA row blocks) a wave streams all
+// of K, accumulating a 64 x 64 FP64 tile (16 MFMA accumulators), then runs
+// the epilogue; the pair adds its partial dots through LDS at the end.
+// Operands are read straight from global memory with coalesced 128-byte row
+// segments: A[i][k] = P[k][i] (P is symmetric), B[k][j] = AO[k][p0 + j]; P
+// (<= a few MB) stays L2-resident, and the two waves of a pair stream the
+// same AO columns at about the same time.
 //
 // MFMA 16x16x4 f64 lane layout on gfx950 (lane l, r = l % 16, q = l / 16),
 // measured with an in-bounds probe:
@@ -39,13 +41,46 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 
 constexpr int kIBlock = 64;  // rows of P per wave pass (4 MFMA row blocks); the basis padding is a multiple
 
+// Epilogue slots and products.  Slot l of a lane's row: derivative
+// ep_slot_d(l) of the point in column block ep_slot_cb(l) (one column type:
+// four blocks of 16 points; four column types: one block).  Product (cb, l):
+// the Y column block cb times slot l, accumulated into dot ep_dot(cb, l), or
+// -1 if the mode forms no such product.
+//   rho:  cb x {phi}                        -> dot cb
+//   grad: cb x {phi, d_x, d_y, d_z phi}     -> dot 4 cb + d
+//   lap:  Y(phi) x {phi, d_k phi, lap phi}  -> dots 0..4;  Y(d_k phi) x d_k phi -> dot 4 + k
+//   hess: Y(phi) x {phi, d_k, d_kl phi}     -> dots 0..9;  Y(d_k phi) x d_l phi (l <= k) -> 10..15
+// Each dot receives one product per (row block, row), in row order.
+__host__ __device__ constexpr int ep_slots(int mode) {
+  return mode == MODE_RHO ? 4 : mode == MODE_GRAD ? 16 : mode == MODE_LAP ? 5 : 10;
+}
+__host__ __device__ constexpr int ep_slot_cb(int mode, int l) {
+  return mode == MODE_RHO ? l : mode == MODE_GRAD ? l / 4 : 0;
+}
+__host__ __device__ constexpr int ep_slot_d(int mode, int l) {
+  return mode == MODE_RHO ? 0 : mode == MODE_GRAD ? l % 4 : l;
+}
+__host__ __device__ constexpr int ep_dot(int mode, int cb, int l) {
+  if (mode == MODE_RHO) return l == cb ? cb : -1;
+  if (mode == MODE_GRAD) return l / 4 == cb ? l : -1;
+  if (cb == 0) return l;
+  if (mode == MODE_LAP) return l == cb ? 4 + cb : -1;
+  // hess: t_kl for l <= k, in the order xx, xy, yy, xz, yz, zz
+  const int base = cb == 1 ? 10 : cb == 2 ? 11 : 13;
+  return (l >= 1 && l <= cb) ? base + l - 1 : -1;
+}
+
 // ISPLIT: one 64-row block of P per wave (blockIdx.y), each writing its own
 // partial dots [block][dot][point] that reduce_row_blocks adds in order --
 // enough waves for small point counts (the mixed path's re-evaluation).
+// Unsplit: waves 2h and 2h + 1 of a workgroup share a point set; wave 2h
+// takes the row blocks b with b % 4 in {0, 3}, wave 2h + 1 those with
+// b % 4 in {1, 2} (equal work under the triangular k ranges below, whose
+// length falls linearly with b).
 // Two waves per SIMD: left to itself the compiler keeps the 64 x 64 FP64
 // accumulator tile in 128 AGPRs beside ~190 VGPRs (one wave per SIMD); with
-// the bound it fits both in 256 registers without spills (organic60 FP64
-// contraction 183 -> 129 ms per step).
+// the bound it fits both in 256 registers (organic60 FP64 contraction
+// 183 -> 129 ms per step; the Hessian mode spills two registers).
 template <int MODE, bool ISPLIT>
 __global__ __launch_bounds__(256, 2) void contract_f64_kernel(const double* __restrict__ P, int mp,
                                                            const double* __restrict__ ao, int64_t ld,
@@ -57,7 +92,8 @@ __global__ __launch_bounds__(256, 2) void contract_f64_kernel(const double* __re
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int r = lane & 15, q = lane >> 4;
-  const int64_t p0 = ((int64_t)blockIdx.x * 4 + wave) * PPW;
+  const int half = ISPLIT ? 0 : (wave & 1);
+  const int64_t p0 = ISPLIT ? ((int64_t)blockIdx.x * 4 + wave) * PPW : ((int64_t)blockIdx.x * 2 + (wave >> 1)) * PPW;
   const int64_t sd = (int64_t)mp * ld;
 
   const double* bptr[4];
@@ -75,6 +111,10 @@ __global__ __launch_bounds__(256, 2) void contract_f64_kernel(const double* __re
   const int i_end = ISPLIT ? i_begin + kIBlock : mp;
   if (ISPLIT) dots += (int64_t)blockIdx.y * ND * ld;
   for (int i0 = i_begin; i0 < i_end; i0 += kIBlock) {
+    if (!ISPLIT) {
+      const int m4 = (i0 / kIBlock) & 3;
+      if ((m4 == 0 || m4 == 3) != (half == 0)) continue;
+    }
     d4 acc[4][4];
 #pragma unroll
     for (int rb = 0; rb < 4; ++rb)
@@ -88,23 +128,28 @@ __global__ __launch_bounds__(256, 2) void contract_f64_kernel(const double* __re
     // column (rho = phi^T P phi).  Lap mode: the derivative columns (cb 1..3),
     // which enter only tau_kk = d_k phi^T P d_k phi.
     constexpr bool TRI_ALL = MODE == MODE_RHO;
-    const int kstart = TRI_ALL ? i0 : 0;
+    constexpr bool TRI_DER = MODE == MODE_LAP;
     const double* aptr = P + (int64_t)q * mp + i0 + r;
+    // k order: right of the diagonal block, left of it, the diagonal block
+    // last -- the epilogue then reads AO rows i0..i0+63 while they are still
+    // in L2 from the last k-steps
+    const int ka = i0 + kIBlock;
+    const int kfirst = ka < mp ? ka : (TRI_ALL || i0 == 0 ? i0 : 0);
     double a[4], b[4];
 #pragma unroll
-    for (int rb = 0; rb < 4; ++rb) a[rb] = aptr[(int64_t)kstart * mp + 16 * rb];
+    for (int rb = 0; rb < 4; ++rb) a[rb] = aptr[(int64_t)kfirst * mp + 16 * rb];
 #pragma unroll
-    for (int cb = 0; cb < 4; ++cb) b[cb] = bptr[cb][(int64_t)kstart * ld];
+    for (int cb = 0; cb < 4; ++cb) b[cb] = bptr[cb][(int64_t)kfirst * ld];
 
-    auto kstep = [&](int k0, bool all_cols, bool dbl) {
-      // prefetch the next k-step (clamped on the last step; the value is unused)
-      const int kn = (k0 + 4 < mp) ? k0 + 4 : k0;
+    // one k-step on the operands in a/b, prefetching k-step kn (all rows of P;
+    // AO columns cb = 0 only unless load_all)
+    auto kstep = [&](int kn, bool all_cols, bool dbl, bool load_all) {
       double an[4], bn[4];
 #pragma unroll
       for (int rb = 0; rb < 4; ++rb) an[rb] = aptr[(int64_t)kn * mp + 16 * rb];
 #pragma unroll
       for (int cb = 0; cb < 4; ++cb)
-        if (cb == 0 || all_cols || kn >= i0) bn[cb] = bptr[cb][(int64_t)kn * ld];
+        if (cb == 0 || load_all) bn[cb] = bptr[cb][(int64_t)kn * ld];
 #pragma unroll
       for (int rb = 0; rb < 4; ++rb)
 #pragma unroll
@@ -117,52 +162,53 @@ __global__ __launch_bounds__(256, 2) void contract_f64_kernel(const double* __re
 #pragma unroll
       for (int cb = 0; cb < 4; ++cb) b[cb] = bn[cb];
     };
-    if constexpr (MODE == MODE_LAP || TRI_ALL) {
-      for (int k0 = kstart; k0 < i0; k0 += 4) kstep(k0, false, false);  // lap: the phi column left of i0
-      for (int k0 = i0; k0 < i0 + kIBlock; k0 += 4) kstep(k0, true, false);
-      for (int k0 = i0 + kIBlock; k0 < mp; k0 += 4) kstep(k0, true, true);
-    } else {
-      for (int k0 = 0; k0 < mp; k0 += 4) kstep(k0, true, false);
-    }
+    // k-steps kb, kb + 4, .. < ke; the last one prefetches knext with every
+    // column (peeled, so the loop's loads do not depend on the k index)
+    auto phase = [&](int kb, int ke, int knext, bool all_cols, bool dbl, bool load_all) {
+      int k0 = kb;
+      for (; k0 + 4 < ke; k0 += 4) kstep(k0 + 4, all_cols, dbl, load_all);
+      if (k0 < ke) kstep(knext, all_cols, dbl, true);
+    };
+    // phase B: left of the diagonal block (not in rho mode); lap mode needs
+    // only the phi column there
+    const int kb_end = TRI_ALL ? 0 : i0;
+    phase(ka, mp, kb_end > 0 ? 0 : i0, true, TRI_ALL || TRI_DER, true);
+    phase(0, kb_end, i0, !TRI_DER, false, !TRI_DER);
+    phase(i0, i0 + kIBlock, i0, true, false, true);
 
-    // epilogue: dot the Y tile with the AO rows i0..i0+63 of this wave's points
+    // epilogue: dot the Y tile with the AO rows i0..i0+63 of this wave's
+    // points.  Per lane, slot l of row i is the AO value of derivative
+    // ep_slot_d(l) at point column block ep_slot_cb(l); the values of JC rows
+    // x LC slots are loaded as one batch before their products (lap mode: 20
+    // loads in flight per 16-row block) rather than one at a time beside
+    // each product.
+    constexpr int NL = ep_slots(MODE);
+    constexpr int JC = (MODE == MODE_GRAD || MODE == MODE_HESS) ? 1 : 4;
+    constexpr int LC = MODE == MODE_GRAD ? 8 : MODE == MODE_HESS ? 5 : NL;
 #pragma unroll
     for (int rb = 0; rb < 4; ++rb) {
 #pragma unroll
-      for (int cb = 0; cb < 4; ++cb) {
-        const int64_t p = p0 + (NT == 1 ? 16 * cb : 0) + r;
+      for (int j0 = 0; j0 < 4; j0 += JC) {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const int64_t i = i0 + 16 * rb + q + 4 * j;
-          const double y = acc[rb][cb][j];
-          const double* ai = ao + i * ld + p;
-          if constexpr (MODE == MODE_RHO) {
-            dacc[cb] += ai[0] * y;
-          } else if constexpr (MODE == MODE_GRAD) {
+        for (int l0 = 0; l0 < NL; l0 += LC) {
+          double v[JC][LC];
 #pragma unroll
-            for (int d = 0; d < 4; ++d) dacc[cb * 4 + d] += ai[d * sd] * y;
-          } else if constexpr (MODE == MODE_LAP) {
-            if (cb == 0) {
+          for (int jj = 0; jj < JC; ++jj) {
+            const int64_t i = i0 + 16 * rb + q + 4 * (j0 + jj);
 #pragma unroll
-              for (int d = 0; d < 5; ++d) dacc[d] += ai[d * sd] * y;
-            } else {
-              dacc[4 + cb] += ai[cb * sd] * y;
-            }
-          } else {  // MODE_HESS
-            if (cb == 0) {
-#pragma unroll
-              for (int d = 0; d < 10; ++d) dacc[d] += ai[d * sd] * y;
-            } else if (cb == 1) {
-              dacc[10] += ai[1 * sd] * y;  // t_xx
-            } else if (cb == 2) {
-              dacc[11] += ai[1 * sd] * y;  // t_xy
-              dacc[12] += ai[2 * sd] * y;  // t_yy
-            } else {
-              dacc[13] += ai[1 * sd] * y;  // t_xz
-              dacc[14] += ai[2 * sd] * y;  // t_yz
-              dacc[15] += ai[3 * sd] * y;  // t_zz
-            }
+            for (int ll = 0; ll < LC; ++ll)
+              v[jj][ll] = ao[ep_slot_d(MODE, l0 + ll) * sd + i * ld + p0 + 16 * ep_slot_cb(MODE, l0 + ll) + r];
           }
+#pragma unroll
+          for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+            for (int jj = 0; jj < JC; ++jj)
+#pragma unroll
+              for (int ll = 0; ll < LC; ++ll) {
+                constexpr int kNone = -1;
+                const int k = ep_dot(MODE, cb, l0 + ll);
+                if (k != kNone) dacc[k] += v[jj][ll] * acc[rb][cb][j0 + jj];
+              }
         }
       }
     }
@@ -175,6 +221,17 @@ __global__ __launch_bounds__(256, 2) void contract_f64_kernel(const double* __re
     v += __shfl_xor(v, 16);
     v += __shfl_xor(v, 32);
     dacc[k] = v;
+  }
+  if (!ISPLIT) {
+    __shared__ double pair_part[2][NACC][16];
+    if (half == 1 && q == 0) {
+#pragma unroll
+      for (int k = 0; k < NACC; ++k) pair_part[wave >> 1][k][r] = dacc[k];
+    }
+    __syncthreads();
+    if (half == 1) return;
+#pragma unroll
+    for (int k = 0; k < NACC; ++k) dacc[k] += pair_part[wave >> 1][k][r];
   }
   if (q == 0) {
     if constexpr (NT == 1) {
@@ -202,7 +259,7 @@ template <bool ISPLIT>
 void launch_modes(int mode, const double* P, int nbasis_pad, const double* ao, int64_t ld, double* dots,
                   hipStream_t s) {
   const int ppw = (mode == MODE_RHO || mode == MODE_GRAD) ? 64 : 16;
-  const dim3 block(256), grid((unsigned)(ld / (4 * ppw)), ISPLIT ? (unsigned)(nbasis_pad / kIBlock) : 1u);
+  const dim3 block(256), grid((unsigned)(ld / ((ISPLIT ? 4 : 2) * ppw)), ISPLIT ? (unsigned)(nbasis_pad / kIBlock) : 1u);
   switch (mode) {
     case MODE_RHO:
       hipLaunchKernelGGL((contract_f64_kernel<MODE_RHO, ISPLIT>), grid, block, 0, s, P, nbasis_pad, ao, ld, dots);
