@@ -28,9 +28,16 @@ constexpr int kGramSlice = 2048;  // points per partial (K of one wave)
 
 // MFMA 16x16x4 f64 layout (lane l, r = l % 16, q = l / 16): A[r][q], B[q][r],
 // D[q + 4j][r].
-__global__ __launch_bounds__(64) void gram_f64_kernel(const double* __restrict__ ao, int mp, int64_t ld,
+// One k-step covers 8 points: lane q reads points p0 + 2q and p0 + 2q + 1 of
+// each of its rows as one 16-byte load and feeds them to two MFMAs (the
+// point order inside a slice is fixed, so the sums are deterministic).  The
+// next k-step's loads are issued before the current step's MFMAs.  Points
+// at or beyond the slice end contribute zero (the AO buffer holds ld >= the
+// rounded-up slice end columns; w is read only below npts).
+__global__ __launch_bounds__(64, 2) void gram_f64_kernel(const double* __restrict__ ao, int mp, int64_t ld,
                                                       const double* __restrict__ w, int64_t npts,
                                                       double* __restrict__ part, int ntile) {
+  typedef double d2 __attribute__((ext_vector_type(2)));
   const int lane = threadIdx.x;
   const int r = lane & 15, q = lane >> 4;
   const int ntri = ntile * (ntile + 1) / 2;
@@ -50,20 +57,46 @@ __global__ __launch_bounds__(64) void gram_f64_kernel(const double* __restrict__
   for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int b = 0; b < 4; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
-  for (int64_t p0 = pbeg; p0 < pend; p0 += 4) {
-    const int64_t p = p0 + q;
-    const bool ok = p < pend;
-    const double wp = ok ? w[p] : 0.0;
-    double a[4], b[4];
+  const double* arow = ao + (int64_t)(I + r) * ld + 2 * q;
+  const double* brow = ao + (int64_t)(J + r) * ld + 2 * q;
+  const int64_t r16 = 16 * ld;
+  d2 a[4], b[4];
+  auto load = [&](int64_t p, d2* av, d2* bv) {
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      a[k] = ok ? ao[(int64_t)(I + 16 * k + r) * ld + p] : 0.0;
-      b[k] = ok ? wp * ao[(int64_t)(J + 16 * k + r) * ld + p] : 0.0;
+      av[k] = *reinterpret_cast<const d2*>(arow + k * r16 + p);
+      bv[k] = *reinterpret_cast<const d2*>(brow + k * r16 + p);
+    }
+  };
+  load(pbeg, a, b);
+  for (int64_t p0 = pbeg; p0 < pend; p0 += 8) {
+    const int64_t pn = p0 + 8 < pend ? p0 + 8 : p0;
+    d2 an[4], bn[4];
+    load(pn, an, bn);
+    const int64_t pa = p0 + 2 * q;
+    const bool ok0 = pa < pend, ok1 = pa + 1 < pend;
+    const double w0 = ok0 ? w[pa] : 0.0, w1 = ok1 ? w[pa + 1] : 0.0;
+    double a0[4], a1[4], b0[4], b1[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      a0[k] = ok0 ? a[k][0] : 0.0;
+      a1[k] = ok1 ? a[k][1] : 0.0;
+      b0[k] = ok0 ? w0 * b[k][0] : 0.0;
+      b1[k] = ok1 ? w1 * b[k][1] : 0.0;
     }
 #pragma unroll
     for (int rb = 0; rb < 4; ++rb)
 #pragma unroll
-      for (int cb = 0; cb < 4; ++cb) acc[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb], b[cb], acc[rb][cb], 0, 0, 0);
+      for (int cb = 0; cb < 4; ++cb) acc[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[rb], b0[cb], acc[rb][cb], 0, 0, 0);
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+      for (int cb = 0; cb < 4; ++cb) acc[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[rb], b1[cb], acc[rb][cb], 0, 0, 0);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      a[k] = an[k];
+      b[k] = bn[k];
+    }
   }
   double* out = part + (int64_t)slice * mp * mp;
 #pragma unroll
@@ -87,16 +120,21 @@ __global__ __launch_bounds__(256) void gram_reduce_kernel(const double* __restri
   G[k] = s;
 }
 
-// Y = P phi (64 points per wave, like the forward MODE_GRAD kernel) with the
-// row-wise weighted epilogue.
-__global__ __launch_bounds__(256) void contract_adjoint_f64_kernel(const double* __restrict__ P, int mp,
-                                                                   const double* __restrict__ ao, int64_t ld,
-                                                                   const double* __restrict__ w, int64_t npts,
-                                                                   double* __restrict__ T) {
+// Y = P phi (64 points per wave pair, like the forward MODE_GRAD kernel) with
+// the row-wise weighted epilogue.  Waves 2h and 2h + 1 of a workgroup share
+// 64 points and take alternate 64-row blocks of P; their rows are disjoint,
+// so each adds its own row totals into T.  The k-loop prefetches one k-step
+// ahead; the epilogue loads the 12 AO values of two rows per lane as one
+// batch before their products.
+__global__ __launch_bounds__(256, 2) void contract_adjoint_f64_kernel(const double* __restrict__ P, int mp,
+                                                                      const double* __restrict__ ao, int64_t ld,
+                                                                      const double* __restrict__ w, int64_t npts,
+                                                                      double* __restrict__ T) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int r = lane & 15, q = lane >> 4;
-  const int64_t p0 = ((int64_t)blockIdx.x * 4 + wave) * 64;
+  const int half = wave & 1;
+  const int64_t p0 = ((int64_t)blockIdx.x * 2 + (wave >> 1)) * 64;
   const int64_t sd = (int64_t)mp * ld;
   const double* bptr[4];
   double wl[4];
@@ -106,51 +144,73 @@ __global__ __launch_bounds__(256) void contract_adjoint_f64_kernel(const double*
     const int64_t p = p0 + 16 * cb + r;
     wl[cb] = p < npts ? w[p] : 0.0;
   }
-  for (int i0 = 0; i0 < mp; i0 += 64) {
+  for (int i0 = 64 * half; i0 < mp; i0 += 128) {
     d4 acc[4][4];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
       for (int b = 0; b < 4; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
     const double* aptr = P + (int64_t)q * mp + i0 + r;
+    double a[4], b[4];
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb) a[rb] = aptr[16 * rb];
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) b[cb] = bptr[cb][0];
     for (int k0 = 0; k0 < mp; k0 += 4) {
-      double a[4], b[4];
+      // prefetch the next k-step (clamped on the last one; the value is unused)
+      const int kn = k0 + 4 < mp ? k0 + 4 : k0;
+      double an[4], bn[4];
 #pragma unroll
-      for (int rb = 0; rb < 4; ++rb) a[rb] = aptr[(int64_t)k0 * mp + 16 * rb];
+      for (int rb = 0; rb < 4; ++rb) an[rb] = aptr[(int64_t)kn * mp + 16 * rb];
 #pragma unroll
-      for (int cb = 0; cb < 4; ++cb) b[cb] = bptr[cb][(int64_t)k0 * ld];
+      for (int cb = 0; cb < 4; ++cb) bn[cb] = bptr[cb][(int64_t)kn * ld];
 #pragma unroll
       for (int rb = 0; rb < 4; ++rb)
 #pragma unroll
         for (int cb = 0; cb < 4; ++cb)
           acc[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb], b[cb], acc[rb][cb], 0, 0, 0);
+#pragma unroll
+      for (int rb = 0; rb < 4; ++rb) a[rb] = an[rb];
+#pragma unroll
+      for (int cb = 0; cb < 4; ++cb) b[cb] = bn[cb];
     }
-    // row sums over this wave's 64 points: lane (q, r) holds rows i0+16rb+q+4j
+    // row sums over the pair's 64 points: lane (q, r) holds rows i0+16rb+q+4j
 #pragma unroll
     for (int rb = 0; rb < 4; ++rb) {
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int64_t i = i0 + 16 * rb + q + 4 * j;
-        double s[3] = {0.0, 0.0, 0.0};
+      for (int j0 = 0; j0 < 4; j0 += 2) {
+        double v[2][4][3];
 #pragma unroll
-        for (int cb = 0; cb < 4; ++cb) {
-          const int64_t p = p0 + 16 * cb + r;
-          const double y = acc[rb][cb][j] * wl[cb];
+        for (int jj = 0; jj < 2; ++jj) {
+          const int64_t i = i0 + 16 * rb + q + 4 * (j0 + jj);
 #pragma unroll
-          for (int k = 0; k < 3; ++k) s[k] += ao[(1 + k) * sd + i * ld + p] * y;
+          for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) v[jj][cb][k] = ao[(1 + k) * sd + i * ld + p0 + 16 * cb + r];
         }
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          double v = s[k];
-          v += __shfl_xor(v, 1);
-          v += __shfl_xor(v, 2);
-          v += __shfl_xor(v, 4);
-          v += __shfl_xor(v, 8);
-          s[k] = v;
-        }
-        if (r == 0) {
+        for (int jj = 0; jj < 2; ++jj) {
+          const int64_t i = i0 + 16 * rb + q + 4 * (j0 + jj);
+          double s[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-          for (int k = 0; k < 3; ++k) atomicAdd(&T[i * 3 + k], s[k]);
+          for (int cb = 0; cb < 4; ++cb) {
+            const double y = acc[rb][cb][j0 + jj] * wl[cb];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) s[k] += v[jj][cb][k] * y;
+          }
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            double t = s[k];
+            t += __shfl_xor(t, 1);
+            t += __shfl_xor(t, 2);
+            t += __shfl_xor(t, 4);
+            t += __shfl_xor(t, 8);
+            s[k] = t;
+          }
+          if (r == 0) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) atomicAdd(&T[i * 3 + k], s[k]);
+          }
         }
       }
     }
@@ -178,7 +238,8 @@ void launch_gram_reduce(const double* part, int nslice, int64_t count, double* G
 
 void launch_contract_adjoint_f64(const double* P, int mp, const double* ao, int64_t ld, const double* w,
                                  int64_t npts, double* T, hipStream_t s) {
-  hipLaunchKernelGGL(contract_adjoint_f64_kernel, dim3((unsigned)(ld / 256)), dim3(256), 0, s, P, mp, ao, ld, w, npts,
+  // 128 points per workgroup (two wave pairs); ld is a multiple of 256
+  hipLaunchKernelGGL(contract_adjoint_f64_kernel, dim3((unsigned)(ld / 128)), dim3(256), 0, s, P, mp, ao, ld, w, npts,
                      T);
   CT_HIP_CHECK(hipGetLastError());
 }
