@@ -251,15 +251,20 @@ __device__ __forceinline__ void y_tile(const u4* __restrict__ Pf, int rows, int 
 
   // A (rows x mp: P, or the transposed factor of P): fragment (piece pl,
   // 16-row block b16, k-step kb) is the 64 x 16 bytes at
-  // Pf[((pl * rows/16 + b16) * nk + kb) * 64 + lane].
-  const u4* abase = Pf + (int64_t)((i0 + RW * w) / 16) * nk * 64 + lane;
+  // Pf[((pl * rows/16 + b16) * nk + kb) * 64 + lane]: a wave-uniform
+  // fragment address (scalar arithmetic per k-step) plus the lane's 16-byte
+  // slot.
+  const u4* awave = Pf + (int64_t)((i0 + RW * w) / 16) * nk * 64;
   const int64_t aplane = (int64_t)(rows / 16) * nk * 64;
   u4 areg[NPL][NRB];
   auto load_a = [&](int kb) {
 #pragma unroll
     for (int pl = 0; pl < NPL; ++pl)
 #pragma unroll
-      for (int rb = 0; rb < NRB; ++rb) areg[pl][rb] = abase[pl * aplane + ((int64_t)rb * nk + kb) * 64];
+      for (int rb = 0; rb < NRB; ++rb) {
+        const u4* frag = awave + (pl * aplane + ((int64_t)rb * nk + kb) * 64);
+        areg[pl][rb] = frag[lane];
+      }
   };
 
   // B: (column, 8-row octet) units, unit u -> column u % TC, octet u / TC
@@ -291,13 +296,20 @@ __device__ __forceinline__ void y_tile(const u4* __restrict__ Pf, int rows, int 
   // the next step's B only where it is split into LDS, a whole k-step after
   // the load.  Masked loads make the count dynamic, and the wait for A at
   // the top of each k-step then also drains the B prefetch.
+  // The lane's row quads of k-step 0 (per unit and half); k-step kb lies a
+  // wave-uniform 32 ld floats per step further.
+  const float* bq[NU][2];
+#pragma unroll
+  for (int u = 0; u < NU; ++u)
+#pragma unroll
+    for (int h = 0; h < 2; ++h) bq[u][h] = ubase[u] + (int64_t)(2 * uoct[u] + h) * ld * 4;
   f4 rbuf[NU][2];
   auto load_b = [&](int kb) {
+    const int64_t koff = (int64_t)kb * (32 * ld);
 #pragma unroll
     for (int u = 0; u < NU; ++u)
 #pragma unroll
-      for (int h = 0; h < 2; ++h)
-        rbuf[u][h] = *reinterpret_cast<const f4*>(ubase[u] + (int64_t)(8 * kb + 2 * uoct[u] + h) * ld * 4);
+      for (int h = 0; h < 2; ++h) rbuf[u][h] = *reinterpret_cast<const f4*>(bq[u][h] + koff);
   };
   auto store_b = [&](int s) { tile_store_b<TC, NU>(Bs, s, rbuf, ucol, uoct, uok); };
 
