@@ -13,9 +13,12 @@
 //     writes its FP64 partial tile (and its transpose); gram_reduce adds the
 //     slices in a fixed order.
 //   * contract_adjoint_mixed: Y = P phi with the shared tile engine
-//     (mixed_tile.h); the epilogue sums w_g d_k phi_i(g) Y_i(g) over the
-//     workgroup's points per row i in FP64, over several point tiles per
-//     workgroup, then adds the row totals into T[i][k] with FP64 atomics.
+//     (mixed_tile.h), its products accumulated in FP32 pairs (hi*hi apart
+//     from the five smaller ones: no FP64 work per k-step, which would
+//     outweigh the matrix-core work of this 64-column tile); the epilogue
+//     sums w_g d_k phi_i(g) Y_i(g) over the workgroup's points per row i in
+//     FP64, over several point tiles per workgroup, then adds the row totals
+//     into T[i][k] with FP64 atomics.
 
 #include "mixed_tile.h"
 
@@ -61,8 +64,8 @@ __global__ __launch_bounds__(256, 2) void contract_adjoint_mixed_kernel(const u4
 
   for (int64_t pt = t0; pt < t1; ++pt) {
     const int64_t pbase = pt * kAdjTC;
-    YTile<kAdjTC, 0> yt;
-    y_tile<kAdjTC, 0, kAdjTC / 16, false>(Pf, mp, mp, ld, i0, tid, ao + (pbase + (tid & (kAdjTC - 1))) * 4, Bs, yt, 0);
+    YTile<kAdjTC, 0, 0> yt;
+    y_tile<kAdjTC, 0, 0, false>(Pf, mp, mp, ld, i0, tid, ao + (pbase + (tid & (kAdjTC - 1))) * 4, Bs, yt, 0);
 #pragma unroll
     for (int cb = 0; cb < kAdjTC / 16; ++cb) {
       const int64_t p = pbase + 16 * cb + r;
