@@ -81,12 +81,20 @@ __global__ __launch_bounds__(256) void ao_eval_kernel(const ShellMeta* __restric
     lm0 = lmask[0];
     if (nwords == 2) lm1 = lmask[1];
   }
+  // Screened layout: tile-packed blocks (ao_packed_index), so that the quads
+  // one wave stores for a (tile, k-step) land in one 10 KiB region (lap mode,
+  // 16-point tiles) instead of 256-byte pieces a whole array apart.
+  const int64_t tile = kmask ? p / tile_points : 0;
+  const int pp = kmask ? (int)(p - tile * tile_points) : 0;
+  const int nk = nbasis_pad >> 5;
   auto store_quad = [&](int d, int quad) {
     if constexpr (quads) {
       const int k = quad >> 3;  // 32-function k-step of the quad
-      if ((((k >> 5) ? lm1 : lm0) >> (k & 31)) & 1u)
-        *reinterpret_cast<f4*>(ao + ((int64_t)d * mq + quad) * ld * 4 + p * 4) =
-            f4{qb[d][0], qb[d][1], qb[d][2], qb[d][3]};
+      if ((((k >> 5) ? lm1 : lm0) >> (k & 31)) & 1u) {
+        const int64_t off = kmask ? ao_packed_index(tile, k, d, quad & 7, pp, nk, nder, tile_points)
+                                  : (((int64_t)d * mq + quad) * ld + p) * 4;
+        *reinterpret_cast<f4*>(ao + off) = f4{qb[d][0], qb[d][1], qb[d][2], qb[d][3]};
+      }
     }
   };
   auto dispatch = [&](int code, T x, T y, T z, T s0, T s1, T s2, auto&& st) {

@@ -250,15 +250,23 @@ __global__ __launch_bounds__(256, 2) void contract_mixed_kernel(const u4* __rest
   const int64_t mq = mp / 4;  // row quads per AO array
 
   YTile<TC, NBC, C::NF> yt;
+  // screened factor route (per-tile lists): the AOs are tile-packed
+  // (ao_packed_index), this tile's k-step k at a fixed stride from k-step 0
+  constexpr int NDER = mode_nder(C::MODE);
+  const bool packed = C::FAC && kcount != nullptr;
+  const int64_t tile_floats = (int64_t)(mp / KB) * NDER * 8 * C::PTS * 4;
   // this thread's B column (TC a multiple of 64), or any column's (bcol)
-  auto bcol = [&](int c) { return ao + ((int64_t)C::type(c) * mq * ld + pbase + C::point(c)) * 4; };
+  auto bcol = [&](int c) {
+    return packed ? ao + ptile * tile_floats + ((int64_t)C::type(c) * 8 * C::PTS + C::point(c)) * 4
+                  : ao + ((int64_t)C::type(c) * mq * ld + pbase + C::point(c)) * 4;
+  };
   const float* bsrc = TC % 64 == 0 ? bcol(tid & (TC - 1)) : nullptr;
   // factor route: the point tile's list of k-steps (screened: its own list
   // and length; else one shared list of all k-steps, kstride 0)
   const int32_t* kl = C::FAC ? klist + ptile * kstride : nullptr;
   const int kc = C::FAC ? (kcount ? kcount[ptile] : kcount_all) : 0;
   y_tile<TC, NBC, C::NF, C::TRI, C::FAC>(Pf, rows, mp, ld, i0, tid, bsrc, Bs, yt, C::TRI ? i0 / KB : 0, kl, kc,
-                                          bcol);
+                                          bcol, packed ? 4 * C::PTS : 0, packed ? (int64_t)NDER * 32 * C::PTS : 0);
   if constexpr (C::FAC) {
     factor_epilogue<V, DT>(yt, &Bs[0][0][0][0], part, ld, rt, pbase, tid);
     return;

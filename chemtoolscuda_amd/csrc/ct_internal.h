@@ -100,6 +100,16 @@ constexpr int mode_ndot(int m) { return m == MODE_RHO ? 1 : m == MODE_GRAD ? 4 :
 __host__ __device__ inline int64_t ao_quad_index(int d, int i, int64_t p, int mp, int64_t ld) {
   return (((int64_t)d * (mp >> 2) + (i >> 2)) * ld + p) * 4 + (i & 3);
 }
+// Mixed path, screened factor route ("tile-packed" row quads): the chunk's
+// sorted points in tiles of tp, and per (tile, 32-function k-step) one
+// contiguous block of nder arrays x 8 quads x tp points x 4 floats, so the
+// block a contraction tile reads per k-step (and the AO kernel writes for it)
+// is one region (lap mode: 5 x 8 x 16 x 16 B = 10 KiB).  Offset in floats of
+// quad qq (0..7) of k-step k, array d, point pp of tile `tile`; nk = mp / 32.
+__host__ __device__ inline int64_t ao_packed_index(int64_t tile, int k, int d, int qq, int pp, int nk, int nder,
+                                                   int tp) {
+  return ((((tile * nk + k) * nder + d) * 8 + qq) * (int64_t)tp + pp) * 4;
+}
 
 // ---------------------------------------------------------------------------
 // Kernel launchers (defined in the .hip files).

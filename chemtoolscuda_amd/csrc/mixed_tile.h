@@ -219,8 +219,10 @@ __device__ __forceinline__ void tile_store_b(uint16_t (&Bs)[2][NPL][TC][LROW], i
 }
 
 // Computes the tile.  bsrc: this thread's B column at row quad 0, i.e. the
-// column tid % TC; its row quad k lies at bsrc + k * ld * 4.  Bs: the
-// workgroup's LDS.  Ends with a barrier (Bs free for reuse).
+// column tid % TC; its row quad k lies at bsrc + k * ld * 4 (row-quad
+// layout), or, with qstride and kstride (tile-packed layout), quad q of
+// k-step kb at bsrc + kb * kstride + q * qstride.  Bs: the workgroup's LDS.
+// Ends with a barrier (Bs free for reuse).
 //
 // K runs over the k-steps k_begin .. mp/32 - 1.  TRI (symmetric P, columns
 // used only in quadratic forms x^T P x): k_begin is the row block's diagonal
@@ -239,7 +241,8 @@ template <int TC, int NBC, int NF, bool TRI, bool LIST = false, class BCol = std
 __device__ __forceinline__ void y_tile(const u4* __restrict__ Pf, int rows, int mp, int64_t ld, int i0, int tid,
                                        const float* __restrict__ bsrc, uint16_t (&Bs)[2][NPL][TC][LROW],
                                        YTile<TC, NBC, NF>& t, int k_begin, const int32_t* klist = nullptr,
-                                       int kcount = 0, BCol bcol = nullptr) {
+                                       int kcount = 0, BCol bcol = nullptr, int64_t qstride = 0,
+                                       int64_t kstride = 0) {
   static_assert(!TRI || NF == TC / 16, "the triangle needs the FP64 running sums");
   static_assert(!(TRI && LIST), "the triangle walks all k-steps from the diagonal");
   static_assert(TC % 16 == 0, "column blocks of 16");
@@ -298,14 +301,17 @@ __device__ __forceinline__ void y_tile(const u4* __restrict__ Pf, int rows, int 
   // the top of each k-step then also drains the B prefetch.
   // The lane's row quads of k-step 0 (per unit and half); k-step kb lies a
   // wave-uniform 32 ld floats per step further.
+  // (qstride, kstride: floats between consecutive row quads and k-steps of
+  // a column; 0: the row-quad layout's 4 ld and 32 ld)
+  const int64_t qs = qstride ? qstride : ld * 4, ks = kstride ? kstride : 32 * ld;
   const float* bq[NU][2];
 #pragma unroll
   for (int u = 0; u < NU; ++u)
 #pragma unroll
-    for (int h = 0; h < 2; ++h) bq[u][h] = ubase[u] + (int64_t)(2 * uoct[u] + h) * ld * 4;
+    for (int h = 0; h < 2; ++h) bq[u][h] = ubase[u] + (int64_t)(2 * uoct[u] + h) * qs;
   f4 rbuf[NU][2];
   auto load_b = [&](int kb) {
-    const int64_t koff = (int64_t)kb * (32 * ld);
+    const int64_t koff = (int64_t)kb * ks;
 #pragma unroll
     for (int u = 0; u < NU; ++u)
 #pragma unroll
