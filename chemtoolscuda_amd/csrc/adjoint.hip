@@ -5,10 +5,10 @@
 //
 // Two FP64 matrix-core kernels per point chunk:
 //   * gram_f64: G = Phi diag(w) Phi^T, split over point slices.  One wave owns
-//     a 64 x 64 tile of the upper triangle of G and one slice of points (the
-//     GEMM's K); each slice writes its own partial tile and, off the
-//     diagonal, its transpose, and a reduction adds the slices in a fixed
-//     order (deterministic).
+//     a 64 x 64 tile of the upper triangle of G (over the row blocks holding
+//     real functions) and one slice of points (the GEMM's K); each slice
+//     writes its own partial tile and, off the diagonal, its transpose, and a
+//     reduction adds the slices in a fixed order (deterministic).
 //   * contract_adjoint_f64: the forward contraction's GEMM Y = P phi with an
 //     epilogue that sums w_g d_k phi_i(g) Y_i(g) over the wave's points per
 //     row i and adds the row totals into T[i][k] with FP64 atomics (the order
@@ -126,7 +126,7 @@ __global__ __launch_bounds__(256) void gram_reduce_kernel(const double* __restri
 // so each adds its own row totals into T.  The k-loop prefetches one k-step
 // ahead; the epilogue loads the 12 AO values of two rows per lane as one
 // batch before their products.
-__global__ __launch_bounds__(256, 2) void contract_adjoint_f64_kernel(const double* __restrict__ P, int mp,
+__global__ __launch_bounds__(256, 2) void contract_adjoint_f64_kernel(const double* __restrict__ P, int mp, int m,
                                                                       const double* __restrict__ ao, int64_t ld,
                                                                       const double* __restrict__ w, int64_t npts,
                                                                       double* __restrict__ T) {
@@ -144,7 +144,10 @@ __global__ __launch_bounds__(256, 2) void contract_adjoint_f64_kernel(const doub
     const int64_t p = p0 + 16 * cb + r;
     wl[cb] = p < npts ? w[p] : 0.0;
   }
-  for (int i0 = 64 * half; i0 < mp; i0 += 128) {
+  // P's rows and columns beyond M are zero: row blocks up to the last real
+  // row, K up to the last real column (a multiple of 4)
+  const int mr = (m + 63) / 64 * 64, mk = (m + 3) / 4 * 4;
+  for (int i0 = 64 * half; i0 < mr; i0 += 128) {
     d4 acc[4][4];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
@@ -156,9 +159,9 @@ __global__ __launch_bounds__(256, 2) void contract_adjoint_f64_kernel(const doub
     for (int rb = 0; rb < 4; ++rb) a[rb] = aptr[16 * rb];
 #pragma unroll
     for (int cb = 0; cb < 4; ++cb) b[cb] = bptr[cb][0];
-    for (int k0 = 0; k0 < mp; k0 += 4) {
+    for (int k0 = 0; k0 < mk; k0 += 4) {
       // prefetch the next k-step (clamped on the last one; the value is unused)
-      const int kn = k0 + 4 < mp ? k0 + 4 : k0;
+      const int kn = k0 + 4 < mk ? k0 + 4 : k0;
       double an[4], bn[4];
 #pragma unroll
       for (int rb = 0; rb < 4; ++rb) an[rb] = aptr[(int64_t)kn * mp + 16 * rb];
@@ -221,9 +224,10 @@ __global__ __launch_bounds__(256, 2) void contract_adjoint_f64_kernel(const doub
 
 int gram_slices(int64_t npts) { return (int)((npts + kGramSlice - 1) / kGramSlice); }
 
-void launch_gram_f64(const double* ao, int mp, int64_t ld, const double* w, int64_t npts, double* part,
+void launch_gram_f64(const double* ao, int mp, int m, int64_t ld, const double* w, int64_t npts, double* part,
                      hipStream_t s) {
-  const int ntile = mp / 64;
+  CT_REQUIRE(m > 0 && m <= mp, "basis size");
+  const int ntile = (m + 63) / 64;  // tiles over the row blocks with a real function
   const int nslice = gram_slices(npts);
   hipLaunchKernelGGL(gram_f64_kernel, dim3((unsigned)(ntile * (ntile + 1) / 2 * nslice)), dim3(64), 0, s, ao, mp, ld,
                      w, npts, part, ntile);
@@ -236,11 +240,12 @@ void launch_gram_reduce(const double* part, int nslice, int64_t count, double* G
   CT_HIP_CHECK(hipGetLastError());
 }
 
-void launch_contract_adjoint_f64(const double* P, int mp, const double* ao, int64_t ld, const double* w,
+void launch_contract_adjoint_f64(const double* P, int mp, int m, const double* ao, int64_t ld, const double* w,
                                  int64_t npts, double* T, hipStream_t s) {
+  CT_REQUIRE(m > 0 && m <= mp, "basis size");
   // 128 points per workgroup (two wave pairs); ld is a multiple of 256
-  hipLaunchKernelGGL(contract_adjoint_f64_kernel, dim3((unsigned)(ld / 128)), dim3(256), 0, s, P, mp, ao, ld, w, npts,
-                     T);
+  hipLaunchKernelGGL(contract_adjoint_f64_kernel, dim3((unsigned)(ld / 128)), dim3(256), 0, s, P, mp, m, ao, ld, w,
+                     npts, T);
   CT_HIP_CHECK(hipGetLastError());
 }
 

@@ -31,7 +31,7 @@ using namespace mixed;
 // ------------------------------------------------------------------ dL/dR
 constexpr int kAdjTC = 64;  // points per tile
 
-__global__ __launch_bounds__(256, 2) void contract_adjoint_mixed_kernel(const u4* __restrict__ Pf, int mp,
+__global__ __launch_bounds__(256, 2) void contract_adjoint_mixed_kernel(const u4* __restrict__ Pf, int mp, int m,
                                                                         const float* __restrict__ ao, int64_t ld,
                                                                         const double* __restrict__ w, int64_t npts,
                                                                         int nrt, int64_t nptile, int tiles_per_wg,
@@ -53,6 +53,10 @@ __global__ __launch_bounds__(256, 2) void contract_adjoint_mixed_kernel(const u4
   const int i0 = rt * TR;
   const int64_t mq = mp / 4;
   const int64_t quad0 = (i0 + RW * wv) / 4 + q;
+  // P's rows and columns beyond M are zero: K stops at the last k-step with a
+  // real function, and a wave whose rows are all padding adds nothing to T
+  const bool live = i0 + RW * wv < m;
+  const int kend = (m + KB - 1) / KB;
 
   double rs[NRB][4][3];  // row sums: rows i0 + 32 wv + 16 rb + 4 q + j, axis k
 #pragma unroll
@@ -65,7 +69,9 @@ __global__ __launch_bounds__(256, 2) void contract_adjoint_mixed_kernel(const u4
   for (int64_t pt = t0; pt < t1; ++pt) {
     const int64_t pbase = pt * kAdjTC;
     YTile<kAdjTC, 0, 0> yt;
-    y_tile<kAdjTC, 0, 0, false>(Pf, mp, mp, ld, i0, tid, ao + (pbase + (tid & (kAdjTC - 1))) * 4, Bs, yt, 0);
+    y_tile<kAdjTC, 0, 0, false>(Pf, mp, mp, ld, i0, tid, ao + (pbase + (tid & (kAdjTC - 1))) * 4, Bs, yt, 0,
+                                 nullptr, 0, nullptr, 0, 0, kend);
+    if (!live) continue;
 #pragma unroll
     for (int cb = 0; cb < kAdjTC / 16; ++cb) {
       const int64_t p = pbase + 16 * cb + r;
@@ -99,7 +105,7 @@ __global__ __launch_bounds__(256, 2) void contract_adjoint_mixed_kernel(const u4
         v += __shfl_xor(v, 8);
         rs[rb][j][k] = v;
       }
-  if (r == 0) {
+  if (r == 0 && live) {
 #pragma unroll
     for (int rb = 0; rb < NRB; ++rb)
 #pragma unroll
@@ -235,16 +241,16 @@ int adjoint_mixed_tiles_per_wg(int64_t npts, int mp) {
   return (int)(t < 1 ? 1 : t);
 }
 
-void launch_contract_adjoint_mixed(const uint16_t* Pf, int mp, const float* ao, int64_t ld, const double* w,
+void launch_contract_adjoint_mixed(const uint16_t* Pf, int mp, int m, const float* ao, int64_t ld, const double* w,
                                    int64_t npts, double* T, hipStream_t s) {
-  CT_REQUIRE(mp % TR == 0, "padded basis size must be a multiple of 128");
+  CT_REQUIRE(mp % TR == 0 && m > 0 && m <= mp, "padded basis size must be a multiple of 128");
   const int nrt = mp / TR;
   const int64_t nptile = (npts + kAdjTC - 1) / kAdjTC;  // ld >= nptile * 64: ld is a multiple of 256
   const int tpw = adjoint_mixed_tiles_per_wg(npts, mp);
   const int64_t ngrp = (nptile + tpw - 1) / tpw;
   const dim3 grid((unsigned)(((ngrp + 7) / 8) * 8 * nrt)), block(256);
-  hipLaunchKernelGGL(contract_adjoint_mixed_kernel, grid, block, 0, s, reinterpret_cast<const u4*>(Pf), mp, ao, ld, w,
-                     npts, nrt, nptile, tpw, T);
+  hipLaunchKernelGGL(contract_adjoint_mixed_kernel, grid, block, 0, s, reinterpret_cast<const u4*>(Pf), mp, m, ao, ld,
+                     w, npts, nrt, nptile, tpw, T);
   CT_HIP_CHECK(hipGetLastError());
 }
 

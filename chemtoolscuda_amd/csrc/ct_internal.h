@@ -289,10 +289,12 @@ void launch_fill_zero(unsigned long long* p, int64_t n, hipStream_t s);
 
 // Adjoint.
 int gram_slices(int64_t npts);
-void launch_gram_f64(const double* ao, int mp, int64_t ld, const double* w, int64_t npts, double* part,
+// tiles over the row blocks holding the m real functions only (partials of
+// G's other entries are left unwritten)
+void launch_gram_f64(const double* ao, int mp, int m, int64_t ld, const double* w, int64_t npts, double* part,
                      hipStream_t s);
 void launch_gram_reduce(const double* part, int nslice, int64_t count, double* G, hipStream_t s);
-void launch_contract_adjoint_f64(const double* P, int mp, const double* ao, int64_t ld, const double* w,
+void launch_contract_adjoint_f64(const double* P, int mp, int m, const double* ao, int64_t ld, const double* w,
                                  int64_t npts, double* T, hipStream_t s);
 // Mixed adjoint (FP32 row-quad AOs, split-bf16 MFMA, FP64 partials).
 // Sustained rates of the device (peak.hip): bf16 and FP64 matrix cores,
@@ -303,7 +305,7 @@ void measure_peaks(double* bf16_tflops, double* f64_mfma_tflops, double* f64_val
 void gram_mixed_plan(int64_t npts, int mp, int* nslice, int64_t* slice_len, int max_slices = 64);
 void launch_gram_mixed(const float* ao, int mp, int64_t ld, const double* w, int64_t npts, const int32_t* tiles,
                        int nslice, int64_t slice_len, double* part, hipStream_t s);
-void launch_contract_adjoint_mixed(const uint16_t* Pf, int mp, const float* ao, int64_t ld, const double* w,
+void launch_contract_adjoint_mixed(const uint16_t* Pf, int mp, int m, const float* ao, int64_t ld, const double* w,
                                    int64_t npts, double* T, hipStream_t s);
 
 }  // namespace ct

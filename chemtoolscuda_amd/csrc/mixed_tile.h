@@ -224,7 +224,8 @@ __device__ __forceinline__ void tile_store_b(uint16_t (&Bs)[2][NPL][TC][LROW], i
 // k-step kb at bsrc + kb * kstride + q * qstride.  Bs: the workgroup's LDS.
 // Ends with a barrier (Bs free for reuse).
 //
-// K runs over the k-steps k_begin .. mp/32 - 1.  TRI (symmetric P, columns
+// K runs over the k-steps k_begin .. mp/32 - 1 (or .. k_end - 1: the rest of
+// A's columns are zero).  TRI (symmetric P, columns
 // used only in quadratic forms x^T P x): k_begin is the row block's diagonal
 // k-step and wave w, whose 32 rows are k-step kd = k_begin + w, uses the
 // k-steps from kd on, adding the products of k-steps right of kd twice.  By
@@ -242,7 +243,7 @@ __device__ __forceinline__ void y_tile(const u4* __restrict__ Pf, int rows, int 
                                        const float* __restrict__ bsrc, uint16_t (&Bs)[2][NPL][TC][LROW],
                                        YTile<TC, NBC, NF>& t, int k_begin, const int32_t* klist = nullptr,
                                        int kcount = 0, BCol bcol = nullptr, int64_t qstride = 0,
-                                       int64_t kstride = 0) {
+                                       int64_t kstride = 0, int k_end = -1) {
   static_assert(!TRI || NF == TC / 16, "the triangle needs the FP64 running sums");
   static_assert(!(TRI && LIST), "the triangle walks all k-steps from the diagonal");
   static_assert(TC % 16 == 0, "column blocks of 16");
@@ -331,7 +332,7 @@ __device__ __forceinline__ void y_tile(const u4* __restrict__ Pf, int rows, int 
   // increasing; list and length are the same for the whole workgroup, so the
   // entries are scalar loads, fetched three steps ahead); else
   // k_begin .. nk - 1.
-  const int kn = LIST ? kcount : nk - k_begin;
+  const int kn = LIST ? kcount : (k_end >= 0 ? k_end : nk) - k_begin;
   auto kat = [&](int i) {
     if constexpr (LIST)
       return klist[i];

@@ -804,9 +804,9 @@ void Mol::backward(const double* points, int64_t n, int layout, const double* w,
     if (!mixed) {
       double* ao = reinterpret_cast<double*>(ws);
       launch_ao_f64(mode, lmax_, meta, geom, prim, nshell_, M_, Mp_, pts, nc, ld, ao, 700.0, stream_);
-      launch_gram_f64(ao, Mp_, ld, wc, nc, part, stream_);
+      launch_gram_f64(ao, Mp_, M_, ld, wc, nc, part, stream_);
       launch_gram_reduce(part, gram_slices(nc), (int64_t)gsize, G, stream_);
-      launch_contract_adjoint_f64(static_cast<const double*>(d_P_.get()), Mp_, ao, ld, wc, nc, T, stream_);
+      launch_contract_adjoint_f64(static_cast<const double*>(d_P_.get()), Mp_, M_, ao, ld, wc, nc, T, stream_);
     } else {
       float* ao = reinterpret_cast<float*>(ws);
       launch_ao_f32(mode, lmax_, meta, geom, prim, nshell_, M_, Mp_, pts, nc, ld, ao, 50.0, nullptr, 0, 64, stream_);
@@ -815,7 +815,7 @@ void Mol::backward(const double* points, int64_t n, int layout, const double* w,
       gram_mixed_plan(nc, Mp_, &nslice, &len, nslice_max);
       launch_gram_mixed(ao, Mp_, ld, wc, nc, tiles, nslice, len, part, stream_);
       launch_gram_reduce(part, nslice, (int64_t)gsize, G, stream_);
-      launch_contract_adjoint_mixed(static_cast<const uint16_t*>(d_Pb_.get()), Mp_, ao, ld, wc, nc, T, stream_);
+      launch_contract_adjoint_mixed(static_cast<const uint16_t*>(d_Pb_.get()), Mp_, M_, ao, ld, wc, nc, T, stream_);
     }
     timer_.end(stream_);
   }
