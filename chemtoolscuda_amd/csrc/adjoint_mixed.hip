@@ -133,43 +133,47 @@ __global__ __launch_bounds__(256, 2) void gram_mixed_kernel(const float* __restr
   const int I = tiles[tile].x * TR, J = tiles[tile].y * TR;
   const int64_t pbeg = (int64_t)slice * slice_len;
   const int64_t pend = pbeg + slice_len < npts ? pbeg + slice_len : npts;
-  const int64_t mq = mp / 4;
 
-  // staging unit of this thread: operand op (0: phi rows of block I, 1: w phi
-  // rows of block J), row quad rq, point octet o: 8 points x 4 rows.
-  const int op = tid >> 7;
-  const int u = tid & 127;
-  const int rq = u >> 2, o = u & 3;
-  const int row0 = (op == 0 ? I : J) + 4 * rq;
-  const float* src = ao + ((int64_t)(row0 / 4) * ld) * 4;
-  f4 st[8];
-  double ws[8];
+  // staging units of this thread: row quad rq, point quad pq (4 rows x 4
+  // points) of both operands, phi rows of block I (op 0) and w phi rows of
+  // block J (op 1): every thread scales the same number of values by w
+  const int rq = tid >> 3, pq = tid & 7;
+  const float* src[2] = {ao + ((int64_t)(I / 4 + rq) * ld) * 4, ao + ((int64_t)(J / 4 + rq) * ld) * 4};
+  f4 st[2][4];
+  double ws[4];
   auto load = [&](int64_t p0) {
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const int64_t p = p0 + 8 * o + e;
+    for (int e = 0; e < 4; ++e) {
+      const int64_t p = p0 + 4 * pq + e;
       const bool ok = p < pend;
-      st[e] = ok ? *reinterpret_cast<const f4*>(src + p * 4) : f4{0.f, 0.f, 0.f, 0.f};
-      ws[e] = (ok && op == 1) ? w[p] : 1.0;
+#pragma unroll
+      for (int op = 0; op < 2; ++op) st[op][e] = ok ? *reinterpret_cast<const f4*>(src[op] + p * 4) : f4{0.f, 0.f, 0.f, 0.f};
+      ws[e] = ok ? w[p] : 0.0;
     }
   };
   auto store = [&]() {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      u4 vh, vm, vl;
+    for (int op = 0; op < 2; ++op)
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const float x0 = (float)(ws[2 * e] * (double)st[2 * e][j]);
-        const float x1 = (float)(ws[2 * e + 1] * (double)st[2 * e + 1][j]);
-        const Split2 sp = split2(x0, x1);
-        vh[e] = sp.h;
-        vm[e] = sp.m;
-        vl[e] = sp.l;
+      for (int j = 0; j < 4; ++j) {
+        uint32_t vh[2], vm[2], vl[2];
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+          float x0 = st[op][2 * e][j], x1 = st[op][2 * e + 1][j];
+          if (op == 1) {
+            x0 = (float)(ws[2 * e] * (double)x0);
+            x1 = (float)(ws[2 * e + 1] * (double)x1);
+          }
+          const Split2 sp = split2(x0, x1);
+          vh[e] = sp.h;
+          vm[e] = sp.m;
+          vl[e] = sp.l;
+        }
+        typedef uint32_t u2 __attribute__((ext_vector_type(2)));
+        *reinterpret_cast<u2*>(&S[op][0][4 * rq + j][4 * pq]) = u2{vh[0], vh[1]};
+        *reinterpret_cast<u2*>(&S[op][1][4 * rq + j][4 * pq]) = u2{vm[0], vm[1]};
+        *reinterpret_cast<u2*>(&S[op][2][4 * rq + j][4 * pq]) = u2{vl[0], vl[1]};
       }
-      *reinterpret_cast<u4*>(&S[op][0][4 * rq + j][8 * o]) = vh;
-      *reinterpret_cast<u4*>(&S[op][1][4 * rq + j][8 * o]) = vm;
-      *reinterpret_cast<u4*>(&S[op][2][4 * rq + j][8 * o]) = vl;
-    }
   };
 
   f4 hh[NRB][8], sm[NRB][8];
