@@ -119,7 +119,7 @@ __global__ __launch_bounds__(256, 2) void contract_adjoint_mixed_kernel(const u4
 
 // ------------------------------------------------------------------ dL/dP
 // LDS: one stage of both operands, [operand][piece][row][k], 80-byte rows.
-__global__ __launch_bounds__(256, 2) void gram_mixed_kernel(const float* __restrict__ ao, int mp, int64_t ld,
+__global__ __launch_bounds__(256, 2) void gram_mixed_kernel(const float* __restrict__ ao, int mp, int m, int64_t ld,
                                                             const double* __restrict__ w, int64_t npts,
                                                             const int2* __restrict__ tiles, int ntri,
                                                             int64_t slice_len, double* __restrict__ part) {
@@ -139,6 +139,9 @@ __global__ __launch_bounds__(256, 2) void gram_mixed_kernel(const float* __restr
   // block J (op 1): every thread scales the same number of values by w
   const int rq = tid >> 3, pq = tid & 7;
   const float* src[2] = {ao + ((int64_t)(I / 4 + rq) * ld) * 4, ao + ((int64_t)(J / 4 + rq) * ld) * 4};
+  // rows at or past the last real k-step are not in the AO buffer: zeros
+  const int kend_rows = (m + KB - 1) / KB * KB;
+  const bool real[2] = {I + 4 * rq < kend_rows, J + 4 * rq < kend_rows};
   f4 st[2][4];
   double ws[4];
   auto load = [&](int64_t p0) {
@@ -147,7 +150,8 @@ __global__ __launch_bounds__(256, 2) void gram_mixed_kernel(const float* __restr
       const int64_t p = p0 + 4 * pq + e;
       const bool ok = p < pend;
 #pragma unroll
-      for (int op = 0; op < 2; ++op) st[op][e] = ok ? *reinterpret_cast<const f4*>(src[op] + p * 4) : f4{0.f, 0.f, 0.f, 0.f};
+      for (int op = 0; op < 2; ++op)
+        st[op][e] = ok && real[op] ? *reinterpret_cast<const f4*>(src[op] + p * 4) : f4{0.f, 0.f, 0.f, 0.f};
       ws[e] = ok ? w[p] : 0.0;
     }
   };
@@ -292,11 +296,11 @@ void gram_mixed_plan(int64_t npts, int mp, int* nslice, int64_t* slice_len, int 
   *nslice = (int)((npts + len - 1) / len);
 }
 
-void launch_gram_mixed(const float* ao, int mp, int64_t ld, const double* w, int64_t npts, const int32_t* tiles,
+void launch_gram_mixed(const float* ao, int mp, int m, int64_t ld, const double* w, int64_t npts, const int32_t* tiles,
                        int nslice, int64_t slice_len, double* part, hipStream_t s) {
   const int nb = mp / TR;
   const int ntri = nb * (nb + 1) / 2;
-  hipLaunchKernelGGL(gram_mixed_kernel, dim3((unsigned)(ntri * nslice)), dim3(256), 0, s, ao, mp, ld, w, npts,
+  hipLaunchKernelGGL(gram_mixed_kernel, dim3((unsigned)(ntri * nslice)), dim3(256), 0, s, ao, mp, m, ld, w, npts,
                      reinterpret_cast<const int2*>(tiles), ntri, slice_len, part);
   CT_HIP_CHECK(hipGetLastError());
 }

@@ -54,7 +54,7 @@ __global__ __launch_bounds__(256) void ao_eval_kernel(const ShellMeta* __restric
                                                       int nbasis_pad, const double* __restrict__ pts, int64_t npts,
                                                       int64_t ld, TS* __restrict__ ao, double screen,
                                                       const uint32_t* __restrict__ kmask, int nwords,
-                                                      int tile_points) {
+                                                      int tile_points, int rows_out) {
   using T = double;
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // p < ld by construction
   const int64_t pc = p < npts ? p : (npts - 1);
@@ -192,10 +192,12 @@ __global__ __launch_bounds__(256) void ao_eval_kernel(const ShellMeta* __restric
     }
   }
   // zero rows of the padded basis so the contraction never reads garbage
+  // (up to rows_out: a caller whose kernels stop short of nbasis_pad leaves
+  // the rest unwritten)
   if constexpr (!quads) {
     if (blockIdx.y + 1 == gridDim.y)
       for (int d = 0; d < nder; ++d)
-        for (int i = nbasis; i < nbasis_pad; ++i) ao[((int64_t)d * nbasis_pad + i) * ld + p] = TS(0);
+        for (int i = nbasis; i < rows_out; ++i) ao[((int64_t)d * nbasis_pad + i) * ld + p] = TS(0);
   } else {
     const int a = nbasis & 3;
     if (a != 0) {
@@ -211,7 +213,7 @@ __global__ __launch_bounds__(256) void ao_eval_kernel(const ShellMeta* __restric
     for (int d = 0; d < nder; ++d) {
 #pragma unroll
       for (int e = 0; e < 4; ++e) qb[d][e] = TS(0);
-      for (int quad = (nbasis + 3) >> 2; quad < mq; ++quad) store_quad(d, quad);
+      for (int quad = (nbasis + 3) >> 2; quad < (rows_out >> 2); ++quad) store_quad(d, quad);
     }
   }
 }
@@ -219,7 +221,7 @@ __global__ __launch_bounds__(256) void ao_eval_kernel(const ShellMeta* __restric
 template <typename TS, int MODE>
 void launch_lmax(int lmax, const ShellMeta* meta, const ShellGeom* geom, const PrimData* prim, int nshell, int nbasis,
                  int nbasis_pad, const double* pts, int64_t npts, int64_t ld, TS* ao, double screen,
-                 const uint32_t* kmask, int nwords, int tile_points, hipStream_t s) {
+                 const uint32_t* kmask, int nwords, int tile_points, int rows_out, hipStream_t s) {
   // FP64 rows: shell ranges over grid.y until ~2048 waves (a few per SIMD)
   // are in flight; the FP32 row quads carry partial quads from shell to shell
   const int64_t waves = ld / 64;
@@ -229,36 +231,36 @@ void launch_lmax(int lmax, const ShellMeta* meta, const ShellGeom* geom, const P
   const dim3 block(256), grid((unsigned)(ld / 256), (unsigned)nsplit);
   if (lmax <= 2)
     hipLaunchKernelGGL((ao_eval_kernel<TS, MODE, 2>), grid, block, 0, s, meta, geom, prim, nshell, nbasis, nbasis_pad,
-                       pts, npts, ld, ao, screen, kmask, nwords, tile_points);
+                       pts, npts, ld, ao, screen, kmask, nwords, tile_points, rows_out);
   else if (lmax == 3)
     hipLaunchKernelGGL((ao_eval_kernel<TS, MODE, 3>), grid, block, 0, s, meta, geom, prim, nshell, nbasis, nbasis_pad,
-                       pts, npts, ld, ao, screen, kmask, nwords, tile_points);
+                       pts, npts, ld, ao, screen, kmask, nwords, tile_points, rows_out);
   else
     hipLaunchKernelGGL((ao_eval_kernel<TS, MODE, 4>), grid, block, 0, s, meta, geom, prim, nshell, nbasis, nbasis_pad,
-                       pts, npts, ld, ao, screen, kmask, nwords, tile_points);
+                       pts, npts, ld, ao, screen, kmask, nwords, tile_points, rows_out);
   CT_HIP_CHECK(hipGetLastError());
 }
 
 template <typename TS>
 void launch_mode(int mode, int lmax, const ShellMeta* meta, const ShellGeom* geom, const PrimData* prim, int nshell,
                  int nbasis, int nbasis_pad, const double* pts, int64_t npts, int64_t ld, TS* ao, double screen,
-                 const uint32_t* kmask, int nwords, int tile_points, hipStream_t s) {
+                 const uint32_t* kmask, int nwords, int tile_points, int rows_out, hipStream_t s) {
   switch (mode) {
     case MODE_RHO:
       launch_lmax<TS, MODE_RHO>(lmax, meta, geom, prim, nshell, nbasis, nbasis_pad, pts, npts, ld, ao, screen, kmask,
-                          nwords, tile_points, s);
+                          nwords, tile_points, rows_out, s);
       break;
     case MODE_GRAD:
       launch_lmax<TS, MODE_GRAD>(lmax, meta, geom, prim, nshell, nbasis, nbasis_pad, pts, npts, ld, ao, screen, kmask,
-                          nwords, tile_points, s);
+                          nwords, tile_points, rows_out, s);
       break;
     case MODE_LAP:
       launch_lmax<TS, MODE_LAP>(lmax, meta, geom, prim, nshell, nbasis, nbasis_pad, pts, npts, ld, ao, screen, kmask,
-                          nwords, tile_points, s);
+                          nwords, tile_points, rows_out, s);
       break;
     default:
       launch_lmax<TS, MODE_HESS>(lmax, meta, geom, prim, nshell, nbasis, nbasis_pad, pts, npts, ld, ao, screen, kmask,
-                          nwords, tile_points, s);
+                          nwords, tile_points, rows_out, s);
       break;
   }
 }
@@ -267,20 +269,24 @@ void launch_mode(int mode, int lmax, const ShellMeta* meta, const ShellGeom* geo
 
 void launch_ao_f64(int mode, int lmax, const ShellMeta* meta, const ShellGeom* geom, const PrimData* prim, int nshell,
                    int nbasis, int nbasis_pad, const double* pts, int64_t npts, int64_t ld, double* ao,
-                   double screen, hipStream_t s, const uint32_t* kmask, int nwords, int tile_points) {
+                   double screen, hipStream_t s, const uint32_t* kmask, int nwords, int tile_points, int rows_out) {
   CT_REQUIRE(!kmask || (tile_points > 0 && 64 % tile_points == 0 && nwords == (nbasis_pad / 32 + 31) / 32),
              "activity masks do not match the AO kernel's waves");
+  if (rows_out <= 0) rows_out = nbasis_pad;
+  CT_REQUIRE(rows_out >= nbasis && rows_out <= nbasis_pad && rows_out % 4 == 0, "rows written by the AO kernel");
   launch_mode<double>(mode, lmax, meta, geom, prim, nshell, nbasis, nbasis_pad, pts, npts, ld, ao, screen, kmask,
-                      nwords, tile_points, s);
+                      nwords, tile_points, rows_out, s);
 }
 
 void launch_ao_f32(int mode, int lmax, const ShellMeta* meta, const ShellGeom* geom, const PrimData* prim, int nshell,
                    int nbasis, int nbasis_pad, const double* pts, int64_t npts, int64_t ld, float* ao, double screen,
-                   const uint32_t* kmask, int nwords, int tile_points, hipStream_t s) {
+                   const uint32_t* kmask, int nwords, int tile_points, hipStream_t s, int rows_out) {
   CT_REQUIRE(!kmask || (tile_points > 0 && 64 % tile_points == 0 && nwords == (nbasis_pad / 32 + 31) / 32),
              "activity masks do not match the AO kernel's waves");
+  if (rows_out <= 0) rows_out = nbasis_pad;
+  CT_REQUIRE(rows_out >= nbasis && rows_out <= nbasis_pad && rows_out % 4 == 0, "rows written by the AO kernel");
   launch_mode<float>(mode, lmax, meta, geom, prim, nshell, nbasis, nbasis_pad, pts, npts, ld, ao, screen, kmask,
-                     nwords, tile_points, s);
+                     nwords, tile_points, rows_out, s);
 }
 
 }  // namespace ct

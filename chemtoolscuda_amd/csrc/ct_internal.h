@@ -137,7 +137,7 @@ struct FinalizeArgs {
 void launch_ao_f64(int mode, int lmax, const ShellMeta* meta, const ShellGeom* geom, const PrimData* prim, int nshell,
                    int nbasis, int nbasis_pad, const double* pts, int64_t npts, int64_t ld, double* ao,
                    double screen, hipStream_t s, const uint32_t* kmask = nullptr, int nwords = 0,
-                   int tile_points = 64);
+                   int tile_points = 64, int rows_out = 0);
 void launch_contract_f64(int mode, const double* P, int nbasis_pad, const double* ao, int64_t ld, double* dots,
                          hipStream_t s);
 void launch_finalize(const FinalizeArgs& a, hipStream_t s);
@@ -169,7 +169,7 @@ void launch_contract_f64_split(int mode, const double* P, int nbasis_pad, const 
 // included, when no k-step the shell touches is active for its points.
 void launch_ao_f32(int mode, int lmax, const ShellMeta* meta, const ShellGeom* geom, const PrimData* prim, int nshell,
                    int nbasis, int nbasis_pad, const double* pts, int64_t npts, int64_t ld, float* ao, double screen,
-                   const uint32_t* kmask, int nwords, int tile_points, hipStream_t s);
+                   const uint32_t* kmask, int nwords, int tile_points, hipStream_t s, int rows_out = 0);
 // Reorders three row-major bf16 planes of a rows x cols matrix
 // ([3][rows][cols]; rows a multiple of 16, cols of 32) into the MFMA fragment
 // order of the mixed tile engine's A operand (host side, same size).
@@ -303,7 +303,7 @@ void measure_peaks(double* bf16_tflops, double* f64_mfma_tflops, double* f64_val
 // Point slices of the mixed Gram kernel for npts points (at most max_slices):
 // the number of slices and the points per slice (a multiple of 32).
 void gram_mixed_plan(int64_t npts, int mp, int* nslice, int64_t* slice_len, int max_slices = 64);
-void launch_gram_mixed(const float* ao, int mp, int64_t ld, const double* w, int64_t npts, const int32_t* tiles,
+void launch_gram_mixed(const float* ao, int mp, int m, int64_t ld, const double* w, int64_t npts, const int32_t* tiles,
                        int nslice, int64_t slice_len, double* part, hipStream_t s);
 void launch_contract_adjoint_mixed(const uint16_t* Pf, int mp, int m, const float* ao, int64_t ld, const double* w,
                                    int64_t npts, double* T, hipStream_t s);

@@ -803,17 +803,22 @@ void Mol::backward(const double* points, int64_t n, int layout, const double* w,
     timer_.begin(KernelTimer::BACKWARD, stream_);
     if (!mixed) {
       double* ao = reinterpret_cast<double*>(ws);
-      launch_ao_f64(mode, lmax_, meta, geom, prim, nshell_, M_, Mp_, pts, nc, ld, ao, 700.0, stream_);
+      // rows up to the 64-row blocks of the FP64 Gram and adjoint (zeros past M)
+      launch_ao_f64(mode, lmax_, meta, geom, prim, nshell_, M_, Mp_, pts, nc, ld, ao, 700.0, stream_, nullptr, 0, 64,
+                    std::min(Mp_, (M_ + 63) / 64 * 64));
       launch_gram_f64(ao, Mp_, M_, ld, wc, nc, part, stream_);
       launch_gram_reduce(part, gram_slices(nc), (int64_t)gsize, G, stream_);
       launch_contract_adjoint_f64(static_cast<const double*>(d_P_.get()), Mp_, M_, ao, ld, wc, nc, T, stream_);
     } else {
       float* ao = reinterpret_cast<float*>(ws);
-      launch_ao_f32(mode, lmax_, meta, geom, prim, nshell_, M_, Mp_, pts, nc, ld, ao, 50.0, nullptr, 0, 64, stream_);
+      // rows up to the last k-step with a real function (zeros past M); the
+      // Gram stages zeros for the rest of its last row block itself
+      launch_ao_f32(mode, lmax_, meta, geom, prim, nshell_, M_, Mp_, pts, nc, ld, ao, 50.0, nullptr, 0, 64, stream_,
+                    std::min(Mp_, (M_ + 31) / 32 * 32));
       int nslice = 0;
       int64_t len = 0;
       gram_mixed_plan(nc, Mp_, &nslice, &len, nslice_max);
-      launch_gram_mixed(ao, Mp_, ld, wc, nc, tiles, nslice, len, part, stream_);
+      launch_gram_mixed(ao, Mp_, M_, ld, wc, nc, tiles, nslice, len, part, stream_);
       launch_gram_reduce(part, nslice, (int64_t)gsize, G, stream_);
       launch_contract_adjoint_mixed(static_cast<const uint16_t*>(d_Pb_.get()), Mp_, M_, ao, ld, wc, nc, T, stream_);
     }
