@@ -18,9 +18,10 @@
 //     straight into registers with one coalesced 1 KiB load per (piece,
 //     16-row block); the next step's fragments are loaded after the last use
 //     of the current ones (the other workgroup on the CU covers the wait);
-//   B: FP32 row quads (ao_quad_index layout) read one k-step ahead, split on
-//     the way into double-buffered LDS, column-major with 80-byte rows
-//     (conflict-free 16-byte fragment reads).
+//   B: FP32 row quads (ao_quad_index layout, or ao_packed_index for the
+//     screened factor route: quad and k-step strides given) read one k-step
+//     ahead, split on the way into double-buffered LDS, column-major with
+//     80-byte rows (conflict-free 16-byte fragment reads).
 //
 // MFMA 16x16x32 bf16 lane layout (lane l, r = l % 16, q = l / 16), measured:
 //   A: A[r][8q + e], B: B[8q + e][r] (e = 0..7), D: D[4q + j][r] (j = 0..3).
