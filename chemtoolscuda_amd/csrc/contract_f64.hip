@@ -11,19 +11,20 @@
 // AO rows, so Y never leaves the registers and every quantity of a mode comes
 // out of a single pass.
 //
-// Work decomposition: a pair of wavefronts owns a set of points and all rows
-// of P, each wave taking half of the 64-row blocks of P.
-//   * 1 GEMM column type (MODE_RHO, MODE_GRAD): 4 column blocks x 16 points
-//     = 64 points per wave pair.
-//   * 4 column types (MODE_LAP, MODE_HESS): the 4 column blocks are phi,
-//     d_x phi, d_y phi, d_z phi of the same 16 points.
-// For each of its 64-row blocks of P (4 MFMA row blocks) a wave streams all
-// of K, accumulating a 64 x 64 FP64 tile (16 MFMA accumulators), then runs
-// the epilogue; the pair adds its partial dots through LDS at the end.
-// Operands are read straight from global memory with coalesced 128-byte row
-// segments: A[i][k] = P[k][i] (P is symmetric), B[k][j] = AO[k][p0 + j]; P
-// (<= a few MB) stays L2-resident, and the two waves of a pair stream the
-// same AO columns at about the same time.
+// GEMM columns: rho and gradient modes take one column type, phi, for 64
+// points (four column blocks of 16); lap and Hessian modes take four, phi,
+// d_x phi, d_y phi, d_z phi of 16 points.  Two work decompositions:
+//   * contract_f64_pointset_kernel (gradient, lap and Hessian modes): the four
+//     waves of a workgroup share a point set and split the rows of P, the
+//     point set's AO rows staged through LDS once per round of 256 rows.
+//   * contract_f64_rho_kernel (rho mode): a pair of wavefronts owns 64 points
+//     and all rows of P, each wave taking half of the 64-row blocks of P and
+//     streaming all of K per block into a 64 x 64 FP64 tile (16 MFMA
+//     accumulators), its operands read straight from global memory in
+//     coalesced 128-byte row segments (A[i][k] = P[k][i], P being symmetric;
+//     B[k][j] = AO[k][p0 + j]); the pair adds its partial sums through LDS.
+//     Its split form (one wave per 64 points and 64-row block) serves small
+//     point counts.
 //
 // MFMA 16x16x4 f64 lane layout on gfx950 (lane l, r = l % 16, q = l / 16),
 // measured with an in-bounds probe:
@@ -38,30 +39,23 @@ namespace ct {
 namespace {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
+typedef double d2 __attribute__((ext_vector_type(2)));
 
-constexpr int kIBlock = 64;  // rows of P per wave pass (4 MFMA row blocks); the basis padding is a multiple
+constexpr int kIBlock = 64;  // rows of P per wave pass of the rho kernel; the basis padding is a multiple
 
-// Epilogue slots and products.  Slot l of a lane's row: derivative
-// ep_slot_d(l) of the point in column block ep_slot_cb(l) (one column type:
-// four blocks of 16 points; four column types: one block).  Product (cb, l):
-// the Y column block cb times slot l, accumulated into dot ep_dot(cb, l), or
-// -1 if the mode forms no such product.
-//   rho:  cb x {phi}                        -> dot cb
+// Epilogue slots and products of the point-set kernel.  Slot l of a lane's
+// row: derivative ep_slot_d(l) of the point in column block ep_slot_cb(l)
+// (gradient mode: four blocks of 16 points; lap and Hessian: one block).
+// Product (cb, l): the Y column block cb times slot l, accumulated into dot
+// ep_dot(cb, l), or -1 if the mode forms no such product.
 //   grad: cb x {phi, d_x, d_y, d_z phi}     -> dot 4 cb + d
 //   lap:  Y(phi) x {phi, d_k phi, lap phi}  -> dots 0..4;  Y(d_k phi) x d_k phi -> dot 4 + k
 //   hess: Y(phi) x {phi, d_k, d_kl phi}     -> dots 0..9;  Y(d_k phi) x d_l phi (l <= k) -> 10..15
 // Each dot receives one product per (row block, row), in row order.
-__host__ __device__ constexpr int ep_slots(int mode) {
-  return mode == MODE_RHO ? 4 : mode == MODE_GRAD ? 16 : mode == MODE_LAP ? 5 : 10;
-}
-__host__ __device__ constexpr int ep_slot_cb(int mode, int l) {
-  return mode == MODE_RHO ? l : mode == MODE_GRAD ? l / 4 : 0;
-}
-__host__ __device__ constexpr int ep_slot_d(int mode, int l) {
-  return mode == MODE_RHO ? 0 : mode == MODE_GRAD ? l % 4 : l;
-}
+__host__ __device__ constexpr int ep_slots(int mode) { return mode == MODE_GRAD ? 16 : mode == MODE_LAP ? 5 : 10; }
+__host__ __device__ constexpr int ep_slot_cb(int mode, int l) { return mode == MODE_GRAD ? l / 4 : 0; }
+__host__ __device__ constexpr int ep_slot_d(int mode, int l) { return mode == MODE_GRAD ? l % 4 : l; }
 __host__ __device__ constexpr int ep_dot(int mode, int cb, int l) {
-  if (mode == MODE_RHO) return l == cb ? cb : -1;
   if (mode == MODE_GRAD) return l / 4 == cb ? l : -1;
   if (cb == 0) return l;
   if (mode == MODE_LAP) return l == cb ? 4 + cb : -1;
@@ -70,46 +64,32 @@ __host__ __device__ constexpr int ep_dot(int mode, int cb, int l) {
   return (l >= 1 && l <= cb) ? base + l - 1 : -1;
 }
 
-// ISPLIT: one 64-row block of P per wave (blockIdx.y), each writing its own
-// partial dots [block][dot][point] that reduce_row_blocks adds in order --
-// enough waves for small point counts (the mixed path's re-evaluation).
-// Unsplit: waves 2h and 2h + 1 of a workgroup share a point set; wave 2h
-// takes the row blocks b with b % 4 in {0, 3}, wave 2h + 1 those with
-// b % 4 in {1, 2} (equal work under the triangular k ranges below, whose
-// length falls linearly with b).
+// Rho mode, wave pairs: 64 points (four column blocks of 16) per pair of
+// waves.  ISPLIT: one 64-row block of P per wave (blockIdx.y), each writing
+// its own partial rho [block][point] that reduce_row_blocks adds in order --
+// enough waves for small point counts.  Unsplit: waves 2h and 2h + 1 of a
+// workgroup share a point set; wave 2h takes the row blocks b with b % 4 in
+// {0, 3}, wave 2h + 1 those with b % 4 in {1, 2} (equal work under the
+// triangular k ranges below, whose length falls linearly with b).  For
+// nbasis_pad = 128 both forms add the same terms in the same order.
 // Two waves per SIMD: left to itself the compiler keeps the 64 x 64 FP64
-// accumulator tile in 128 AGPRs beside ~190 VGPRs (one wave per SIMD); with
-// the bound it fits both in 256 registers (organic60 FP64 contraction
-// 183 -> 129 ms per step; the Hessian mode spills two registers).
-template <int MODE, bool ISPLIT>
-__global__ __launch_bounds__(256, 2) void contract_f64_kernel(const double* __restrict__ P, int mp,
-                                                           const double* __restrict__ ao, int64_t ld,
-                                                           double* __restrict__ dots) {
-  constexpr int NT = (MODE == MODE_RHO || MODE == MODE_GRAD) ? 1 : 4;
-  constexpr int PPW = NT == 1 ? 64 : 16;
-  constexpr int ND = mode_ndot(MODE);
-  constexpr int NACC = NT == 1 ? 4 * ND : ND;
+// accumulator tile in 128 AGPRs beside the other registers (one wave per
+// SIMD); the bound fits both in 256.
+template <bool ISPLIT>
+__global__ __launch_bounds__(256, 2) void contract_f64_rho_kernel(const double* __restrict__ P, int mp,
+                                                               const double* __restrict__ ao, int64_t ld,
+                                                               double* __restrict__ dots) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int r = lane & 15, q = lane >> 4;
   const int half = ISPLIT ? 0 : (wave & 1);
-  const int64_t p0 = ISPLIT ? ((int64_t)blockIdx.x * 4 + wave) * PPW : ((int64_t)blockIdx.x * 2 + (wave >> 1)) * PPW;
-  const int64_t sd = (int64_t)mp * ld;
-
-  const double* bptr[4];
-#pragma unroll
-  for (int cb = 0; cb < 4; ++cb) {
-    const int t = NT == 1 ? 0 : cb;
-    const int poff = NT == 1 ? 16 * cb : 0;
-    bptr[cb] = ao + t * sd + (int64_t)q * ld + p0 + poff + r;
-  }
-  double dacc[NACC];
-#pragma unroll
-  for (int k = 0; k < NACC; ++k) dacc[k] = 0.0;
+  const int64_t p0 = ISPLIT ? ((int64_t)blockIdx.x * 4 + wave) * 64 : ((int64_t)blockIdx.x * 2 + (wave >> 1)) * 64;
+  const double* bptr = ao + (int64_t)q * ld + p0 + r;  // column block cb at + 16 cb
+  double dacc[4] = {0.0, 0.0, 0.0, 0.0};
 
   const int i_begin = ISPLIT ? (int)blockIdx.y * kIBlock : 0;
   const int i_end = ISPLIT ? i_begin + kIBlock : mp;
-  if (ISPLIT) dots += (int64_t)blockIdx.y * ND * ld;
+  if (ISPLIT) dots += (int64_t)blockIdx.y * ld;
   for (int i0 = i_begin; i0 < i_end; i0 += kIBlock) {
     if (!ISPLIT) {
       const int m4 = (i0 / kIBlock) & 3;
@@ -121,72 +101,251 @@ __global__ __launch_bounds__(256, 2) void contract_f64_kernel(const double* __re
 #pragma unroll
       for (int cb = 0; cb < 4; ++cb) acc[rb][cb] = d4{0.0, 0.0, 0.0, 0.0};
 
-    // Quadratic forms x^T P x need only the diagonal-and-right blocks: by the
+    // rho = phi^T P phi needs only the diagonal-and-right blocks: by the
     // symmetry of P, sum_IJ x_I^T P_IJ x_J = sum_I x_I^T (P_II x_I +
-    // 2 sum_{J>I} P_IJ x_J), so those columns use k >= i0 only, with their
-    // values doubled (exactly) right of the diagonal block.  Rho mode: every
-    // column (rho = phi^T P phi).  Lap mode: the derivative columns (cb 1..3),
-    // which enter only tau_kk = d_k phi^T P d_k phi.
-    constexpr bool TRI_ALL = MODE == MODE_RHO;
-    constexpr bool TRI_DER = MODE == MODE_LAP;
+    // 2 sum_{J>I} P_IJ x_J), so k >= i0 only, with the B values doubled
+    // (exactly) right of the diagonal block.  k order: right of the diagonal
+    // block, then the diagonal block -- the epilogue then reads AO rows
+    // i0..i0+63 while they are still in L2 from the last k-steps.
     const double* aptr = P + (int64_t)q * mp + i0 + r;
-    // k order: right of the diagonal block, left of it, the diagonal block
-    // last -- the epilogue then reads AO rows i0..i0+63 while they are still
-    // in L2 from the last k-steps
     const int ka = i0 + kIBlock;
-    const int kfirst = ka < mp ? ka : (TRI_ALL || i0 == 0 ? i0 : 0);
+    const int kfirst = ka < mp ? ka : i0;
     double a[4], b[4];
 #pragma unroll
     for (int rb = 0; rb < 4; ++rb) a[rb] = aptr[(int64_t)kfirst * mp + 16 * rb];
 #pragma unroll
-    for (int cb = 0; cb < 4; ++cb) b[cb] = bptr[cb][(int64_t)kfirst * ld];
+    for (int cb = 0; cb < 4; ++cb) b[cb] = bptr[(int64_t)kfirst * ld + 16 * cb];
 
-    // one k-step on the operands in a/b, prefetching k-step kn (all rows of P;
-    // AO columns cb = 0 only unless load_all)
-    auto kstep = [&](int kn, bool all_cols, bool dbl, bool load_all) {
+    // one k-step on the operands in a/b, prefetching k-step kn
+    auto kstep = [&](int kn, bool dbl) {
       double an[4], bn[4];
 #pragma unroll
       for (int rb = 0; rb < 4; ++rb) an[rb] = aptr[(int64_t)kn * mp + 16 * rb];
 #pragma unroll
-      for (int cb = 0; cb < 4; ++cb)
-        if (cb == 0 || load_all) bn[cb] = bptr[cb][(int64_t)kn * ld];
+      for (int cb = 0; cb < 4; ++cb) bn[cb] = bptr[(int64_t)kn * ld + 16 * cb];
 #pragma unroll
       for (int rb = 0; rb < 4; ++rb)
 #pragma unroll
         for (int cb = 0; cb < 4; ++cb)
-          if (cb == 0 || all_cols)
-            acc[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(
-                a[rb], ((TRI_ALL || cb > 0) && dbl) ? 2.0 * b[cb] : b[cb], acc[rb][cb], 0, 0, 0);
+          acc[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb], dbl ? 2.0 * b[cb] : b[cb], acc[rb][cb], 0, 0, 0);
 #pragma unroll
       for (int rb = 0; rb < 4; ++rb) a[rb] = an[rb];
 #pragma unroll
       for (int cb = 0; cb < 4; ++cb) b[cb] = bn[cb];
     };
-    // k-steps kb, kb + 4, .. < ke; the last one prefetches knext with every
-    // column (peeled, so the loop's loads do not depend on the k index)
-    auto phase = [&](int kb, int ke, int knext, bool all_cols, bool dbl, bool load_all) {
+    // k-steps kb, kb + 4, .. < ke; the last one prefetches knext (peeled, so
+    // the loop's loads do not depend on the k index)
+    auto phase = [&](int kb, int ke, int knext, bool dbl) {
       int k0 = kb;
-      for (; k0 + 4 < ke; k0 += 4) kstep(k0 + 4, all_cols, dbl, load_all);
-      if (k0 < ke) kstep(knext, all_cols, dbl, true);
+      for (; k0 + 4 < ke; k0 += 4) kstep(k0 + 4, dbl);
+      if (k0 < ke) kstep(knext, dbl);
     };
-    // phase B: left of the diagonal block (not in rho mode); lap mode needs
-    // only the phi column there
-    const int kb_end = TRI_ALL ? 0 : i0;
-    phase(ka, mp, kb_end > 0 ? 0 : i0, true, TRI_ALL || TRI_DER, true);
-    phase(0, kb_end, i0, !TRI_DER, false, !TRI_DER);
-    phase(i0, i0 + kIBlock, i0, true, false, true);
+    phase(ka, mp, i0, true);
+    phase(i0, i0 + kIBlock, i0, false);
 
-    // epilogue: dot the Y tile with the AO rows i0..i0+63 of this wave's
-    // points.  Per lane, slot l of row i is the AO value of derivative
-    // ep_slot_d(l) at point column block ep_slot_cb(l); the values of JC rows
-    // x LC slots are loaded as one batch before their products (lap mode: 20
-    // loads in flight per 16-row block) rather than one at a time beside
-    // each product.
-    constexpr int NL = ep_slots(MODE);
-    constexpr int JC = (MODE == MODE_GRAD || MODE == MODE_HESS) ? 1 : 4;
-    constexpr int LC = MODE == MODE_GRAD ? 8 : MODE == MODE_HESS ? 5 : NL;
+    // epilogue: rho += phi_i (P phi)_i over the rows i0..i0+63 of this wave's
+    // points, the 16 AO values of a 16-row block loaded as one batch
 #pragma unroll
     for (int rb = 0; rb < 4; ++rb) {
+      double v[4][4];
+#pragma unroll
+      for (int jj = 0; jj < 4; ++jj)
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb) v[jj][cb] = ao[(i0 + 16 * rb + q + 4 * jj) * ld + p0 + 16 * cb + r];
+#pragma unroll
+      for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) dacc[cb] += v[jj][cb] * acc[rb][cb][jj];
+    }
+  }
+
+  // reduce the four lane groups that hold the same point
+#pragma unroll
+  for (int cb = 0; cb < 4; ++cb) {
+    double v = dacc[cb];
+    v += __shfl_xor(v, 16);
+    v += __shfl_xor(v, 32);
+    dacc[cb] = v;
+  }
+  if (!ISPLIT) {
+    __shared__ double pair_part[2][4][16];
+    if (half == 1 && q == 0) {
+#pragma unroll
+      for (int cb = 0; cb < 4; ++cb) pair_part[wave >> 1][cb][r] = dacc[cb];
+    }
+    __syncthreads();
+    if (half == 1) return;
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) dacc[cb] += pair_part[wave >> 1][cb][r];
+  }
+  if (q == 0) {
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) dots[p0 + 16 * cb + r] = dacc[cb];
+  }
+}
+
+// Point-set workgroups (gradient, lap and Hessian modes; rho mode stays with
+// the wave pairs above, whose independent waves run its short triangular k
+// ranges faster): the four waves of a workgroup share one point set (the 64
+// GEMM columns above) and split the rows of P between them, so the point
+// set's AO columns are read from HBM once per round of rows instead of once
+// per 64-row block (organic60 lap mode: 69 KB per point against 187 KB).
+// A round covers up to 256 rows of P as 16 row blocks of 16; wave w holds
+// blocks base + w, base + 7 - w, base + 8 + w, base + 15 - w (equal totals of
+// the triangular k ranges), accumulating each block's 16 x 64 FP64 tile in
+// the MFMA registers.  The k loop streams the point set's AO rows through LDS
+// once per round, 16 rows (four MFMA k-steps) per chunk, double-buffered with
+// one barrier per chunk; B is read from LDS one k-step ahead, A (P, a few MB)
+// from global memory through a ring one chunk ahead.  The AO stream is read
+// non-temporally: it is used once, and in L2 it would push out P.
+// Lap mode's derivative columns enter only tau_kk = d_k phi^T P d_k phi, a
+// symmetric form: the row block at rows i0..i0+15 needs k >= i0 only, doubled
+// from i0 + 16 on (see contract_f64_rho_kernel), so the chunks below a round's
+// first row carry only the phi columns.  AO bytes per point set and round:
+// the phi columns over all k, the derivative columns over k >= the round's
+// first row (all k in the gradient and Hessian modes), and the epilogue's AO
+// rows of the round.
+constexpr int kKC = 16;            // AO rows per LDS chunk
+constexpr int kLdsPitch = 64 + 2;  // doubles per LDS row (the four k rows of a B read start 4 banks apart)
+
+template <int MODE>
+__global__ __launch_bounds__(256, 2) void contract_f64_pointset_kernel(const double* __restrict__ P, int mp,
+                                                                    const double* __restrict__ ao, int64_t ld,
+                                                                    double* __restrict__ dots) {
+  static_assert(MODE != MODE_RHO, "rho mode: contract_f64_rho_kernel");
+  constexpr int NT = MODE == MODE_GRAD ? 1 : 4;
+  constexpr int PPB = NT == 1 ? 64 : 16;  // points per workgroup
+  constexpr int ND = mode_ndot(MODE);
+  constexpr int NACC = NT == 1 ? 4 * ND : ND;
+  constexpr bool TRI = MODE == MODE_LAP;  // derivative columns triangular
+  __shared__ double sB[2][kKC][kLdsPitch];
+  static_assert(4 * NACC * 16 <= 2 * kKC * kLdsPitch, "the wave partials reuse the B buffers");
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: the k-loop's conditions are scalar
+  const int r = lane & 15, q = lane >> 4;
+  const int64_t p0 = (int64_t)blockIdx.x * PPB;
+  const int64_t sd = (int64_t)mp * ld;
+  const int S = mp >> 4;  // 16-row blocks of P (mp is a multiple of 128)
+  const int nch = mp / kKC;
+
+  // staging: thread tid moves two 16-byte pieces of a chunk, rows tid >> 5 and
+  // 8 + (tid >> 5), LDS columns c, c + 1 with c = 2 (tid & 31); column c is
+  // point c of the set (one column type) or point c % 16 of type c / 16.  In
+  // chunks without derivative columns the derivative pieces re-read the phi
+  // values of the same point (lines the workgroup fetches anyway): every
+  // load stays unconditional.
+  const int srow = tid >> 5, scol = (tid & 31) * 2;
+  const bool sphi = NT == 1 || scol < 16;
+  const double* gsrc = NT == 1 ? ao + (int64_t)srow * ld + p0 + scol
+                               : ao + (int64_t)(scol >> 4) * sd + (int64_t)srow * ld + p0 + (scol & 15);
+  const double* gphi = NT == 1 ? gsrc : ao + (int64_t)srow * ld + p0 + (scol & 15);
+  const int64_t shalf = (int64_t)8 * ld;  // the second piece: 8 rows on
+  auto stage_load = [&](int k, int kd_lo, d2& v0, d2& v1) {
+    const double* src = (sphi || k >= kd_lo) ? gsrc : gphi;
+    v0 = __builtin_nontemporal_load(reinterpret_cast<const d2*>(src + (int64_t)k * ld));
+    v1 = __builtin_nontemporal_load(reinterpret_cast<const d2*>(src + (int64_t)k * ld + shalf));
+  };
+  auto stage_store = [&](int buf, const d2& v0, const d2& v1) {
+    sB[buf][srow][scol] = v0.x;
+    sB[buf][srow][scol + 1] = v0.y;
+    sB[buf][srow + 8][scol] = v1.x;
+    sB[buf][srow + 8][scol + 1] = v1.y;
+  };
+
+  double dacc[NACC];
+#pragma unroll
+  for (int k = 0; k < NACC; ++k) dacc[k] = 0.0;
+
+  for (int base = 0; base < S; base += 16) {
+    const int nrb = (S - base) >= 16 ? 4 : (S - base) >> 2;  // row blocks per wave this round (4 or 2)
+    auto blk = [&](int j) { return base + 8 * (j >> 1) + ((j & 1) ? 7 - wave : wave); };
+    const int kd_lo = TRI ? base * 16 : 0;  // first k the derivative columns need
+    d4 acc[4][4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int cb = 0; cb < 4; ++cb) acc[j][cb] = d4{0.0, 0.0, 0.0, 0.0};
+
+    {
+      d2 v0, v1;
+      stage_load(0, kd_lo, v0, v1);
+      stage_store(0, v0, v1);
+    }
+    __syncthreads();
+
+    // A of the chunk's four k-steps, each refilled with the next chunk's as it
+    // is consumed.  Loads are unconditional (a block j >= nrb reads block
+    // nrb - 1, the last chunk itself again): a load under a branch makes the
+    // compiler wait for it where the branches join.
+    const double* aptr[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) aptr[j] = P + (int64_t)q * mp + blk(j < nrb ? j : nrb - 1) * 16 + r;
+    double a[4][4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) a[s][j] = aptr[j][(int64_t)(4 * s) * mp];
+    double b[4];
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) b[cb] = sB[0][q][16 * cb + r];
+
+    for (int ch = 0; ch < nch; ++ch) {
+      const int buf = ch & 1;
+      const int kc = ch * kKC;
+      const int kn = ch + 1 < nch ? kc + kKC : kc;
+      d2 v0, v1;
+      stage_load(kn, kd_lo, v0, v1);  // stored once this chunk's MFMAs are issued
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        // B of the next k-step (after the last k-step of the chunk that is in
+        // the other buffer, read after the barrier; this read is then dead)
+        double bn[4];
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb) bn[cb] = sB[buf][(4 * (s + 1)) % kKC + q][16 * cb + r];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          if (j >= nrb) continue;
+#pragma unroll
+          for (int cb = 0; cb < 4; ++cb)
+            if (!(TRI && cb > 0))
+              acc[j][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s][j], b[cb], acc[j][cb], 0, 0, 0);
+          // triangular columns: the chunk reaches block j when kc >= i0 and
+          // lies right of its diagonal block when kc >= i0 + 16 (chunks and
+          // blocks are both 16 rows).  The doubling is a factor on A, so the
+          // block's MFMAs are one sequence under one branch (MFMAs on both
+          // sides of an if/else made the compiler copy the accumulators after
+          // every k-step).
+          if (!TRI || kc < blk(j) * 16) continue;
+          const double af = kc >= blk(j) * 16 + 16 ? 2.0 * a[s][j] : a[s][j];  // exact
+#pragma unroll
+          for (int cb = 1; cb < 4; ++cb)
+            acc[j][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(af, b[cb], acc[j][cb], 0, 0, 0);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) a[s][j] = aptr[j][(int64_t)(kn + 4 * s) * mp];
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb) b[cb] = bn[cb];
+      }
+      stage_store(buf ^ 1, v0, v1);
+      __syncthreads();
+#pragma unroll
+      for (int cb = 0; cb < 4; ++cb) b[cb] = sB[buf ^ 1][q][16 * cb + r];
+    }
+
+    // epilogue: dot each row block's Y tile with the AO rows of its 16 rows
+    // (lane row q + 4 j of the block).  Per lane, slot l of a row is the AO
+    // value of derivative ep_slot_d(l) at column block ep_slot_cb(l); the
+    // values of JC rows x LC slots are loaded as one batch before their
+    // products.
+    constexpr int NL = ep_slots(MODE);
+    constexpr int JC = MODE == MODE_LAP ? 2 : 1;
+    constexpr int LC = MODE == MODE_GRAD ? 8 : 5;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (j >= nrb) continue;
+      const int64_t i0 = blk(j) * 16;
 #pragma unroll
       for (int j0 = 0; j0 < 4; j0 += JC) {
 #pragma unroll
@@ -194,10 +353,11 @@ __global__ __launch_bounds__(256, 2) void contract_f64_kernel(const double* __re
           double v[JC][LC];
 #pragma unroll
           for (int jj = 0; jj < JC; ++jj) {
-            const int64_t i = i0 + 16 * rb + q + 4 * (j0 + jj);
+            const int64_t i = i0 + q + 4 * (j0 + jj);
 #pragma unroll
             for (int ll = 0; ll < LC; ++ll)
-              v[jj][ll] = ao[ep_slot_d(MODE, l0 + ll) * sd + i * ld + p0 + 16 * ep_slot_cb(MODE, l0 + ll) + r];
+              v[jj][ll] = __builtin_nontemporal_load(ao + ep_slot_d(MODE, l0 + ll) * sd + i * ld + p0 +
+                                                     16 * ep_slot_cb(MODE, l0 + ll) + r);
           }
 #pragma unroll
           for (int cb = 0; cb < 4; ++cb)
@@ -207,14 +367,15 @@ __global__ __launch_bounds__(256, 2) void contract_f64_kernel(const double* __re
               for (int ll = 0; ll < LC; ++ll) {
                 constexpr int kNone = -1;
                 const int k = ep_dot(MODE, cb, l0 + ll);
-                if (k != kNone) dacc[k] += v[jj][ll] * acc[rb][cb][j0 + jj];
+                if (k != kNone) dacc[k] += v[jj][ll] * acc[j][cb][j0 + jj];
               }
         }
       }
     }
   }
 
-  // reduce the four lane groups that hold the same point
+  // the four lane groups of a point, then the four waves in order; the wave
+  // partials go where the B chunks were, once every wave is done with them
 #pragma unroll
   for (int k = 0; k < NACC; ++k) {
     double v = dacc[k];
@@ -222,26 +383,20 @@ __global__ __launch_bounds__(256, 2) void contract_f64_kernel(const double* __re
     v += __shfl_xor(v, 32);
     dacc[k] = v;
   }
-  if (!ISPLIT) {
-    __shared__ double pair_part[2][NACC][16];
-    if (half == 1 && q == 0) {
-#pragma unroll
-      for (int k = 0; k < NACC; ++k) pair_part[wave >> 1][k][r] = dacc[k];
-    }
-    __syncthreads();
-    if (half == 1) return;
-#pragma unroll
-    for (int k = 0; k < NACC; ++k) dacc[k] += pair_part[wave >> 1][k][r];
-  }
+  double(*red)[NACC][16] = reinterpret_cast<double(*)[NACC][16]>(&sB[0][0][0]);
+  __syncthreads();
   if (q == 0) {
+#pragma unroll
+    for (int k = 0; k < NACC; ++k) red[wave][k][r] = dacc[k];
+  }
+  __syncthreads();
+  for (int e = tid; e < NACC * 16; e += 256) {
+    const int k = e >> 4, c = e & 15;
+    const double v = ((red[0][k][c] + red[1][k][c]) + red[2][k][c]) + red[3][k][c];
     if constexpr (NT == 1) {
-#pragma unroll
-      for (int cb = 0; cb < 4; ++cb)
-#pragma unroll
-        for (int d = 0; d < ND; ++d) dots[(int64_t)d * ld + p0 + 16 * cb + r] = dacc[cb * ND + d];
+      dots[(int64_t)(k % ND) * ld + p0 + 16 * (k / ND) + c] = v;
     } else {
-#pragma unroll
-      for (int d = 0; d < ND; ++d) dots[(int64_t)d * ld + p0 + r] = dacc[d];
+      dots[(int64_t)k * ld + p0 + c] = v;
     }
   }
 }
@@ -255,27 +410,6 @@ __global__ __launch_bounds__(256) void reduce_row_blocks_kernel(const double* __
   out[k] = s;
 }
 
-template <bool ISPLIT>
-void launch_modes(int mode, const double* P, int nbasis_pad, const double* ao, int64_t ld, double* dots,
-                  hipStream_t s) {
-  const int ppw = (mode == MODE_RHO || mode == MODE_GRAD) ? 64 : 16;
-  const dim3 block(256), grid((unsigned)(ld / ((ISPLIT ? 4 : 2) * ppw)), ISPLIT ? (unsigned)(nbasis_pad / kIBlock) : 1u);
-  switch (mode) {
-    case MODE_RHO:
-      hipLaunchKernelGGL((contract_f64_kernel<MODE_RHO, ISPLIT>), grid, block, 0, s, P, nbasis_pad, ao, ld, dots);
-      break;
-    case MODE_GRAD:
-      hipLaunchKernelGGL((contract_f64_kernel<MODE_GRAD, ISPLIT>), grid, block, 0, s, P, nbasis_pad, ao, ld, dots);
-      break;
-    case MODE_LAP:
-      hipLaunchKernelGGL((contract_f64_kernel<MODE_LAP, ISPLIT>), grid, block, 0, s, P, nbasis_pad, ao, ld, dots);
-      break;
-    default:
-      hipLaunchKernelGGL((contract_f64_kernel<MODE_HESS, ISPLIT>), grid, block, 0, s, P, nbasis_pad, ao, ld, dots);
-      break;
-  }
-  CT_HIP_CHECK(hipGetLastError());
-}
 
 // Factor route in FP64 (P = L L^T, L stored k-major: L[k * rows + j]): one
 // wave owns 64 GEMM columns -- NT column types x 64/NT points -- and all
@@ -562,7 +696,25 @@ void launch_contract_f64_factor_split(int mode, const double* L, int rows_pad, i
 
 void launch_contract_f64(int mode, const double* P, int nbasis_pad, const double* ao, int64_t ld, double* dots,
                          hipStream_t s) {
-  launch_modes<false>(mode, P, nbasis_pad, ao, ld, dots, s);
+  const int ppb = (mode == MODE_RHO || mode == MODE_GRAD) ? 64 : 16;  // points per workgroup
+  if (nbasis_pad % kRowBlock != 0 || ld % (2 * ppb) != 0) throw Error(CT_E_INVALID, "contract_f64: padding");
+  const dim3 block(256), grid((unsigned)(ld / ppb));
+  switch (mode) {
+    case MODE_RHO:  // wave pairs: two point sets of 64 per workgroup
+      hipLaunchKernelGGL((contract_f64_rho_kernel<false>), dim3((unsigned)(ld / 128)), block, 0, s, P, nbasis_pad,
+                         ao, ld, dots);
+      break;
+    case MODE_GRAD:
+      hipLaunchKernelGGL((contract_f64_pointset_kernel<MODE_GRAD>), grid, block, 0, s, P, nbasis_pad, ao, ld, dots);
+      break;
+    case MODE_LAP:
+      hipLaunchKernelGGL((contract_f64_pointset_kernel<MODE_LAP>), grid, block, 0, s, P, nbasis_pad, ao, ld, dots);
+      break;
+    default:
+      hipLaunchKernelGGL((contract_f64_pointset_kernel<MODE_HESS>), grid, block, 0, s, P, nbasis_pad, ao, ld, dots);
+      break;
+  }
+  CT_HIP_CHECK(hipGetLastError());
 }
 
 int64_t contract_f64_split_workspace(int mode, int nbasis_pad, int64_t ld) {
@@ -571,7 +723,11 @@ int64_t contract_f64_split_workspace(int mode, int nbasis_pad, int64_t ld) {
 
 void launch_contract_f64_split(int mode, const double* P, int nbasis_pad, const double* ao, int64_t ld,
                                double* part, double* dots, hipStream_t s) {
-  launch_modes<true>(mode, P, nbasis_pad, ao, ld, part, s);
+  if (mode != MODE_RHO || nbasis_pad % kIBlock != 0 || ld % 256 != 0)
+    throw Error(CT_E_INVALID, "contract_f64_split: rho mode, padded sizes");
+  hipLaunchKernelGGL((contract_f64_rho_kernel<true>), dim3((unsigned)(ld / 256), (unsigned)(nbasis_pad / kIBlock)),
+                     dim3(256), 0, s, P, nbasis_pad, ao, ld, part);
+  CT_HIP_CHECK(hipGetLastError());
   const int64_t count = (int64_t)mode_ndot(mode) * ld;
   hipLaunchKernelGGL(reduce_row_blocks_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, part,
                      nbasis_pad / kIBlock, count, dots);

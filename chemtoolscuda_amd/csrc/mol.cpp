@@ -1003,7 +1003,10 @@ void Mol::reevaluate(int mode, const double* pts, int64_t col_stride, int64_t co
     const int32_t* idx = index + s0;
     const int64_t ld = round_up(c, kPointAlign);
     // few points: one wave per (16 or 64 points, 64-row block of P, or
-    // 64-column pass of the factor) so the GPU fills
+    // 64-column pass of the factor) so the GPU fills.  Not for the density
+    // route's gradient, lap and Hessian modes: their point-set kernel has
+    // four waves per point set already, and keeping it for every slice gives
+    // a point the same bits whatever slice it falls in.
     const bool split = ld < 65536;
     double* ao = reinterpret_cast<double*>(ws);
     double* dots = ao + (size_t)nder * Mp_ * ld;
@@ -1039,7 +1042,7 @@ void Mol::reevaluate(int mode, const double* pts, int64_t col_stride, int64_t co
     else if (factor)
       launch_contract_f64_factor(mode, static_cast<const double*>(d_L64s_.get()), factor_rows_, Mp_, ao, ld, dots,
                                  rs, kmask, nwords, tp);
-    else if (split)
+    else if (split && mode == MODE_RHO)
       launch_contract_f64_split(mode, static_cast<const double*>(d_P_.get()), Mp_, ao, ld, part, dots, rs);
     else
       launch_contract_f64(mode, static_cast<const double*>(d_P_.get()), Mp_, ao, ld, dots, rs);
