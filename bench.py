@@ -39,12 +39,15 @@ PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16": 2.5e15}
 # traffic of a call is this figure times its points (the AO reads dominate
 # and scale with the points).  Default (screened factor route, one call over
 # all 2 666 298 points): 7.68e6 + 0.1875e6 KiB calibrated = 15.92 GB per call.
-# The other routes were measured per call over 8 GiB chunks (mean 533 260
-# points mixed, 333 287 FP64) and are given per point the same way.
+# The mixed routes other than the default were measured per call over 8 GiB
+# chunks (mean 533 260 points) and are given per point the same way; the FP64
+# density route over its two calls of a step (2.0 M and 0.67 M points):
+# FETCH 6.798e7 + 2.298e7 KiB, WRITE 1.254e5 + 0.413e5 KiB.
 MEASURED_TRAFFIC_PER_POINT = {("organic60", "mixed", "factor"): (2 * 7.68e6 + 0.1875e6) * 1024 / 2666298,
                               ("organic60", "mixed", "factor-unscreened"): (8.44e9 + 0.038e9) / 533260,
                               ("organic60", "mixed", "density"): (24.1e9 + 0.19e9) / 533260,
-                              ("organic60", "fp64", "density"): (53.7e9 + 0.021e9) / 333287}
+                              ("organic60", "fp64", "density"): ((6.798e7 + 2.298e7) * 2048 +
+                                                                  (1.254e5 + 0.4128e5) * 1024) / 2666298}
 
 
 def _traffic(workload, precision, key, points):
