@@ -220,6 +220,20 @@ __global__ __launch_bounds__(256, 2) void contract_adjoint_f64_kernel(const doub
   }
 }
 
+// dL/dR_A = -2 sum of T[i] over the functions i on atom A (T[i][k] = sum_g
+// w_g d_k phi_i (P phi)_i), in function order: thread 3 A + k.
+__global__ __launch_bounds__(64) void atom_gradient_kernel(const double* __restrict__ T,
+                                                           const int32_t* __restrict__ func_atom, int m, int natom,
+                                                           double* __restrict__ dR) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 3 * natom) return;
+  const int a = t / 3, k = t % 3;
+  double s = 0.0;
+  for (int i = 0; i < m; ++i)
+    if (func_atom[i] == a) s -= 2.0 * T[3 * i + k];
+  dR[t] = s;
+}
+
 }  // namespace
 
 int gram_slices(int64_t npts) { return (int)((npts + kGramSlice - 1) / kGramSlice); }
@@ -237,6 +251,13 @@ void launch_gram_f64(const double* ao, int mp, int m, int64_t ld, const double* 
 void launch_gram_reduce(const double* part, int nslice, int64_t count, double* G, hipStream_t s) {
   hipLaunchKernelGGL(gram_reduce_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, part, nslice, count,
                      G);
+  CT_HIP_CHECK(hipGetLastError());
+}
+
+void launch_atom_gradient(const double* T, const int32_t* func_atom, int m, int natom, double* dR, hipStream_t s) {
+  if (natom <= 0) return;
+  hipLaunchKernelGGL(atom_gradient_kernel, dim3((unsigned)((3 * natom + 63) / 64)), dim3(64), 0, s, T, func_atom, m,
+                     natom, dR);
   CT_HIP_CHECK(hipGetLastError());
 }
 

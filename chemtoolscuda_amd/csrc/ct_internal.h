@@ -296,6 +296,9 @@ void launch_gram_f64(const double* ao, int mp, int m, int64_t ld, const double* 
 void launch_gram_reduce(const double* part, int nslice, int64_t count, double* G, hipStream_t s);
 void launch_contract_adjoint_f64(const double* P, int mp, int m, const double* ao, int64_t ld, const double* w,
                                  int64_t npts, double* T, hipStream_t s);
+// dR (3 natom doubles, device) = -2 sum of T[i] (3 m doubles) over the
+// functions i with func_atom[i] = A
+void launch_atom_gradient(const double* T, const int32_t* func_atom, int m, int natom, double* dR, hipStream_t s);
 // Mixed adjoint (FP32 row-quad AOs, split-bf16 MFMA, FP64 partials).
 // Sustained rates of the device (peak.hip): bf16 and FP64 matrix cores,
 // FP64 vector FMA (TFLOP/s), HBM copy (GB/s, read + write).

@@ -757,16 +757,31 @@ void Mol::backward(const double* points, int64_t n, int layout, const double* w,
   const int32_t* tiles = nullptr;
   if (mixed) {
     // upper-triangle 128 x 128 tiles of the Gram matrix, (I, J) with I <= J
-    const int nb = Mp_ / kRowBlock;
-    std::vector<int32_t> tl;
-    for (int I = 0; I < nb; ++I)
-      for (int J = I; J < nb; ++J) {
-        tl.push_back(I);
-        tl.push_back(J);
-      }
-    if (!d_tiles_.reserve(sizeof(int32_t) * tl.size())) throw Error(CT_E_NOMEM, "cannot allocate the tile list");
-    CT_HIP_CHECK(hipMemcpy(d_tiles_.get(), tl.data(), sizeof(int32_t) * tl.size(), hipMemcpyHostToDevice));
+    // (fixed by the basis: uploaded on the first mixed call)
+    if (!tiles_ready_) {
+      const int nb = Mp_ / kRowBlock;
+      std::vector<int32_t> tl;
+      for (int I = 0; I < nb; ++I)
+        for (int J = I; J < nb; ++J) {
+          tl.push_back(I);
+          tl.push_back(J);
+        }
+      if (!d_tiles_.reserve(sizeof(int32_t) * tl.size())) throw Error(CT_E_NOMEM, "cannot allocate the tile list");
+      CT_HIP_CHECK(hipMemcpy(d_tiles_.get(), tl.data(), sizeof(int32_t) * tl.size(), hipMemcpyHostToDevice));
+      tiles_ready_ = true;
+    }
     tiles = static_cast<const int32_t*>(d_tiles_.get());
+  }
+  const bool dP_dev = dP != nullptr && is_device_pointer(dP);
+  const bool dR_dev = dR != nullptr && is_device_pointer(dR);
+  if (dR_dev && !func_atom_ready_) {
+    std::vector<int32_t> fa((size_t)M_, 0);
+    for (int s = 0; s < nshell_; ++s)
+      for (int f = 0; f < shell_nfunc(basis_.shell_l[s]); ++f) fa[(size_t)meta_[s].first + f] = basis_.shell_atom[s];
+    if (!d_func_atom_.reserve(sizeof(int32_t) * fa.size()))
+      throw Error(CT_E_NOMEM, "cannot allocate the function-to-atom map");
+    CT_HIP_CHECK(hipMemcpy(d_func_atom_.get(), fa.data(), sizeof(int32_t) * fa.size(), hipMemcpyHostToDevice));
+    func_atom_ready_ = true;
   }
   char* ws = static_cast<char*>(d_ws_.get());
   double* pts_stage = reinterpret_cast<double*>(ws + (size_t)nder * Mp_ * Nc * (mixed ? 4 : 8));
@@ -824,33 +839,36 @@ void Mol::backward(const double* points, int64_t n, int layout, const double* w,
     }
     timer_.end(stream_);
   }
-  std::vector<double> Gh(gsize), Th((size_t)Mp_ * 3);
-  CT_HIP_CHECK(hipMemcpyAsync(Gh.data(), G, sizeof(double) * gsize, hipMemcpyDeviceToHost, stream_));
-  CT_HIP_CHECK(hipMemcpyAsync(Th.data(), T, sizeof(double) * Th.size(), hipMemcpyDeviceToHost, stream_));
+  // device outputs are finished on the device (dP: the real M x M block of
+  // G; dR: the per-atom sums of T, in the host loop's order); host outputs
+  // from host copies of G and T
+  if (dP_dev)
+    CT_HIP_CHECK(hipMemcpy2DAsync(dP, sizeof(double) * M_, G, sizeof(double) * Mp_, sizeof(double) * M_, M_,
+                                  hipMemcpyDeviceToDevice, stream_));
+  if (dR_dev)
+    launch_atom_gradient(T, static_cast<const int32_t*>(d_func_atom_.get()), M_, natom_, dR, stream_);
+  std::vector<double> Gh, Th;
+  if (dP && !dP_dev) {
+    Gh.resize(gsize);
+    CT_HIP_CHECK(hipMemcpyAsync(Gh.data(), G, sizeof(double) * gsize, hipMemcpyDeviceToHost, stream_));
+  }
+  if (dR && !dR_dev) {
+    Th.resize((size_t)Mp_ * 3);
+    CT_HIP_CHECK(hipMemcpyAsync(Th.data(), T, sizeof(double) * Th.size(), hipMemcpyDeviceToHost, stream_));
+  }
   CT_HIP_CHECK(hipStreamSynchronize(stream_));
   timer_.collect();
-  if (dP) {
-    std::vector<double> out((size_t)M_ * M_);
-    for (int i = 0; i < M_; ++i)
-      for (int j = 0; j < M_; ++j) out[(size_t)i * M_ + j] = Gh[(size_t)i * Mp_ + j];
-    if (is_device_pointer(dP))
-      CT_HIP_CHECK(hipMemcpy(dP, out.data(), sizeof(double) * out.size(), hipMemcpyHostToDevice));
-    else
-      std::memcpy(dP, out.data(), sizeof(double) * out.size());
-  }
-  if (dR) {
-    std::vector<double> out(3 * (size_t)natom_, 0.0);
+  if (dP && !dP_dev)
+    for (int i = 0; i < M_; ++i) std::memcpy(dP + (size_t)i * M_, Gh.data() + (size_t)i * Mp_, sizeof(double) * M_);
+  if (dR && !dR_dev) {
+    std::fill(dR, dR + 3 * (size_t)natom_, 0.0);
     for (int s = 0; s < nshell_; ++s) {
       const int a = basis_.shell_atom[s];
       for (int f = 0; f < shell_nfunc(basis_.shell_l[s]); ++f) {
         const int i = meta_[s].first + f;
-        for (int k = 0; k < 3; ++k) out[3 * a + k] -= 2.0 * Th[(size_t)i * 3 + k];
+        for (int k = 0; k < 3; ++k) dR[3 * a + k] -= 2.0 * Th[(size_t)i * 3 + k];
       }
     }
-    if (is_device_pointer(dR))
-      CT_HIP_CHECK(hipMemcpy(dR, out.data(), sizeof(double) * out.size(), hipMemcpyHostToDevice));
-    else
-      std::memcpy(dR, out.data(), sizeof(double) * out.size());
   }
 }
 

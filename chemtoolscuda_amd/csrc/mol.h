@@ -243,6 +243,9 @@ class Mol {
   int64_t esp_pairs_by_L_[9] = {0};
   int64_t esp_npairs_ = 0;
   DeviceBuffer d_tiles_;    // mixed adjoint: upper-triangle Gram tile list
+  bool tiles_ready_ = false;
+  DeviceBuffer d_func_atom_;  // adjoint: atom of each basis function (original order)
+  bool func_atom_ready_ = false;
   DeviceBuffer d_esp_rec_, d_esp_rec32_, d_esp_seg_, d_esp_atoms_, d_boys_, d_esp_io_, d_esp_part_;
   DeviceBuffer d_adj_;     // adjoint: G, T, slice partials
   void prepare_esp();
