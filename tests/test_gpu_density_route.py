@@ -1,0 +1,85 @@
+"""FP64 density route (route="density": rho = phi^T P phi with P itself, the
+reference's formulation, src/evaluate_density.cu:780-823) against the FP64
+oracle at basis sizes whose rows of P take several rounds of the point-set
+kernel: organic60 (624 functions, 640 padded rows: rounds of 16, 16 and 8 row
+blocks of 16) and diff40 (416 functions, 512 rows: two full rounds), in every
+mode (rho: wave pairs; gradient, lap and Hessian: point-set workgroups).
+Also that a point's bits do not depend on the other points of the call
+(slices of the FP64 re-evaluation rely on it), and a point count far below
+one workgroup's 16 or 64 points."""
+import numpy as np
+import pytest
+
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+MODES = (["rho"], ["rho", "grad"], ["rho", "grad", "lap", "tau_pd"], ["rho", "grad", "hess"])
+
+
+@pytest.fixture(scope="module")
+def ct():
+    import chemtoolscuda_amd
+
+    return chemtoolscuda_amd
+
+
+def _normwise(got, ref, floor):
+    a = np.asarray(got).reshape(ref.shape[0], -1)
+    b = np.asarray(ref).reshape(ref.shape[0], -1)
+    den = np.maximum(np.linalg.norm(b, axis=1), floor)
+    return float(np.max(np.linalg.norm(a - b, axis=1) / den))
+
+
+def _sample(w, n, seed):
+    rng = np.random.default_rng(seed)
+    return np.ascontiguousarray(w.points[rng.choice(w.points.shape[0], size=n, replace=False)])
+
+
+@pytest.mark.parametrize("workload,npts", [("organic60", 3000), ("diff40", 3000)])
+def test_density_route_matches_oracle(ct, workload, npts):
+    from chemtoolscuda_amd import synthetic
+
+    w = synthetic.WORKLOADS[workload]()
+    pts = _sample(w, npts, 31)
+    mol = ct.molecule_from_basis(w.basis, w.P, w.charges)
+    ref = oracle.density(w.basis.flat(), w.P, pts, rho=True, grad=True, hess=True, lap=True, tau=True)
+    ref["tau_pd"] = ref["tau"]
+    keep = ref["rho"] >= 1e-10
+    for names in MODES:
+        got = mol.compute(pts, names, precision="fp64", route="density")
+        assert mol.last_report()["factor_rank"] == -1, names  # P itself, not its factor
+        rel = np.abs(got["rho"] - ref["rho"]) / np.abs(ref["rho"])
+        assert rel[keep].max() < 1e-12, (names, rel[keep].max())
+        for k in names[1:]:
+            r = ref[k]
+            err = _normwise(got[k], r, 1e-10 * np.abs(r).max())
+            assert err < 1e-12, (names, k, err)
+
+
+def test_density_route_bits_do_not_depend_on_the_call(ct):
+    from chemtoolscuda_amd import synthetic
+
+    w = synthetic.organic60()
+    pts = _sample(w, 4000, 32)
+    mol = ct.molecule_from_basis(w.basis, w.P, w.charges)
+    for names in MODES[1:]:
+        whole = mol.compute(pts, names, precision="fp64", route="density")
+        part = mol.compute(pts[1234:1234 + 777], names, precision="fp64", route="density")
+        for k in names:
+            a = np.ascontiguousarray(whole[k][1234:1234 + 777])
+            assert np.array_equal(a.view(np.uint64), np.ascontiguousarray(part[k]).view(np.uint64)), (names, k)
+
+
+def test_density_route_few_points(ct):
+    from chemtoolscuda_amd import synthetic
+
+    w = synthetic.diff40()
+    pts = _sample(w, 37, 33)
+    mol = ct.molecule_from_basis(w.basis, w.P, w.charges)
+    ref = oracle.density(w.basis.flat(), w.P, pts, rho=True, grad=True, hess=True, lap=True, tau=True)
+    for names in MODES:
+        got = mol.compute(pts, names, precision="fp64", route="density")
+        assert np.max(np.abs(got["rho"] - ref["rho"]) / np.abs(ref["rho"])) < 1e-12, names
+        if "hess" in names:
+            assert _normwise(got["hess"], ref["hess"], 1e-10 * np.abs(ref["hess"]).max()) < 1e-12
