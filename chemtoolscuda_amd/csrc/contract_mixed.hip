@@ -44,10 +44,11 @@ using namespace mixed;
 //   V_F5:          16 points x {phi, d_x phi, d_y phi, d_z phi, lap phi}: the
 //                  whole lap mode in one launch (rho, g_k, t_kk = d_k psi .
 //                  d_k psi, psi . lap psi and the bound);
-//   V_FH<part>:    32 points x {phi, d_kl phi for three kl}: the Hessian
-//                  mode's h_kl = psi . d_kl psi (part 0: xx, xy, xz; part 1:
-//                  yy, yz, zz); with V_F4<hess>, whose t_kl = d_k psi . d_l psi
-//                  cover all six kl.
+//   V_FH<part>:    32 points x {d_kl phi}: the Hessian mode's h_kl =
+//                  psi . d_kl psi (part 0: xx, xy, xz, yy; part 1: yz, zz)
+//                  with psi as V_F4<hess> stored it (the same FP64 running
+//                  sums these launches would form again); with V_F4<hess>,
+//                  whose t_kl = d_k psi . d_l psi cover all six kl.
 // The error bound of the factor route is 2 sum_j |psi_j| (|L|^T |phi|)_j,
 // the first-order propagation of the per-column bound of psi into
 // rho = sum_j psi_j^2.
@@ -64,8 +65,8 @@ struct Cols {
                               : V == V_HESS || V == V_FH ? MODE_HESS
                               : V == V_F4             ? DT
                                                       : MODE_LAP;
-  static constexpr int NT = V == V_F5 ? 5 : V == V_HESS || V == V_F4 || V == V_FH ? 4 : 1;  // column types per tile
-  static constexpr int TC = V == V_F5 ? 80 : V == V_HESS || V == V_LAPDER || V == V_F4 || V == V_FH ? 128 : 64;
+  static constexpr int NT = V == V_F5 ? 5 : V == V_FH ? (DT == 0 ? 4 : 2) : V == V_HESS || V == V_F4 ? 4 : 1;  // column types per tile
+  static constexpr int TC = V == V_F5 ? 80 : V == V_FH ? 32 * NT : V == V_HESS || V == V_LAPDER || V == V_F4 ? 128 : 64;
   static constexpr int NCB = TC / 16;                     // 16-column MFMA blocks
   static constexpr int PTS = TC / NT;                     // points per workgroup
   static constexpr int NBC = V == V_LAPDER || V == V_FH ? 0
@@ -81,7 +82,7 @@ struct Cols {
   // AO array of column c: 0 = phi, 1..3 = d_x, d_y, d_z phi, then 4 = lap phi
   // (lap mode) or 4..9 = d_xx, d_xy, d_xz, d_yy, d_yz, d_zz phi (Hessian mode)
   __host__ __device__ static constexpr int type(int c) {
-    return V == V_FH   ? (c < PTS ? 0 : 4 + 3 * DT + c / PTS - 1)
+    return V == V_FH   ? 4 + 4 * DT + c / PTS
            : NT >= 4   ? c / PTS
            : V == V_LAPDER ? DT
                            : 0;
@@ -123,12 +124,12 @@ constexpr int kMaxDots = 11;  // 10 Hessian-mode dots of the phi column + the bo
 template <int V, int DT>
 __host__ __device__ constexpr int factor_ndf() {
   using C = Cols<V, DT>;
-  return V == V_F1 ? 2 : V == V_FH ? 3 : V == V_F5 ? 9 : C::MODE == MODE_HESS ? 11 : 5;
+  return V == V_F1 ? 2 : V == V_FH ? C::NT : V == V_F5 ? 9 : C::MODE == MODE_HESS ? 11 : 5;
 }
 template <int V, int DT>
 __host__ __device__ constexpr int factor_slot(int k) {
   using C = Cols<V, DT>;
-  return V == V_FH                        ? 4 + 3 * DT + k
+  return V == V_FH                        ? 4 + 4 * DT + k
          : V == V_F5 && k == 7            ? 4
          : k == factor_ndf<V, DT>() - 1   ? mode_ndot(C::MODE)
          : k < 4                          ? k
@@ -139,15 +140,16 @@ __host__ __device__ constexpr int factor_slot(int k) {
 // Epilogue of the factor route: per point, lane-local FP64 sums over the
 // tile's rows (columns of L) of products of the point's psi columns, then over
 // the four lane groups and the four waves in a fixed order; one partial per
-// (row block, dot, point).
+// (row block, dot, point).  psi (Hessian mode): psi of the tile's rows and
+// points at psi[row * ld + point], written by V_F4 and read by V_FH.
 template <int V, int DT, class Tile>
 __device__ __forceinline__ void factor_epilogue(const Tile& yt, uint16_t* lds, double* __restrict__ part,
-                                                int64_t ld, int rt, int64_t pbase, int tid) {
+                                                int64_t ld, int rt, int64_t pbase, int tid, double* __restrict__ psi) {
   using C = Cols<V, DT>;
   constexpr int ND = mode_ndot(C::MODE), ND1 = ND + 1;
   constexpr int NPB = C::PTS / 16;  // 16-point column blocks per column type
   constexpr bool LAP = C::MODE == MODE_LAP, HES = C::MODE == MODE_HESS;
-  // dots per point: V_F1 [rho, bound]; V_FH [psi.d_kl psi x 3]; V_F4 [rho,
+  // dots per point: V_F1 [rho, bound]; V_FH [psi.d_kl psi x NT]; V_F4 [rho,
   // g_x, g_y, g_z, (Hessian: t_xx, t_xy, t_yy, t_xz, t_yz, t_zz,) bound];
   // V_F5 [rho, g_x, g_y, g_z, t_xx, t_yy, t_zz, psi.lap psi, bound]
   static_assert(V != V_F4 || !LAP, "the lap mode runs as V_F5");
@@ -159,15 +161,27 @@ __device__ __forceinline__ void factor_epilogue(const Tile& yt, uint16_t* lds, d
     double d[NDF];
 #pragma unroll
     for (int k = 0; k < NDF; ++k) d[k] = 0.0;
+    // lane row 4 q + j of the wave's 16-row block rb, point 16 pb + r
+    auto psi_at = [&](int rb, int j) {
+      return psi + (int64_t)(rt * TR + RW * w + 16 * rb + 4 * q + j) * ld + pbase + 16 * pb + r;
+    };
+    double ps[NRB][4];
+    if constexpr (V == V_FH) {
+#pragma unroll
+      for (int rb = 0; rb < NRB; ++rb)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) ps[rb][j] = *psi_at(rb, j);
+    }
 #pragma unroll
     for (int rb = 0; rb < NRB; ++rb) {
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const double y0 = yt.y(rb, pb, j);
+        const double y0 = V == V_FH ? ps[rb][j] : yt.y(rb, pb, j);
         if constexpr (V == V_FH) {
 #pragma unroll
-          for (int k = 0; k < 3; ++k) d[k] += y0 * yt.y(rb, (k + 1) * NPB + pb, j);
+          for (int k = 0; k < C::NT; ++k) d[k] += y0 * yt.y(rb, k * NPB + pb, j);
         } else {
+          if constexpr (HES) *psi_at(rb, j) = y0;
           d[0] += y0 * y0;
           d[NDF - 1] += 2.0 * fabs(y0) * (double)yt.bound[rb][pb][j];
           if constexpr (V == V_F4 || V == V_F5) {
@@ -224,7 +238,7 @@ __global__ __launch_bounds__(256, 2) void contract_mixed_kernel(const u4* __rest
                                                                 double* __restrict__ part, int nrt,
                                                                 int64_t nptile, const int32_t* __restrict__ klist,
                                                                 int64_t kstride, const int32_t* __restrict__ kcount,
-                                                                int kcount_all) {
+                                                                int kcount_all, double* __restrict__ psi) {
   using C = Cols<V, DT>;
   constexpr int TC = C::TC, NCB = C::NCB, NBC = C::NBC;
   constexpr int ND = mode_ndot(C::MODE);
@@ -268,7 +282,7 @@ __global__ __launch_bounds__(256, 2) void contract_mixed_kernel(const u4* __rest
   y_tile<TC, NBC, C::NF, C::TRI, C::FAC>(Pf, rows, mp, ld, i0, tid, bsrc, Bs, yt, C::TRI ? i0 / KB : 0, kl, kc,
                                           bcol, packed ? 4 * C::PTS : 0, packed ? (int64_t)NDER * 32 * C::PTS : 0);
   if constexpr (C::FAC) {
-    factor_epilogue<V, DT>(yt, &Bs[0][0][0][0], part, ld, rt, pbase, tid);
+    factor_epilogue<V, DT>(yt, &Bs[0][0][0][0], part, ld, rt, pbase, tid, psi);
     return;
   } else {
 
@@ -353,13 +367,13 @@ __global__ __launch_bounds__(256, 2) void contract_mixed_kernel(const u4* __rest
 template <int V, int DT = 0>
 void launch_var(const u4* A, int rows, int mp, const float* ao, int64_t ld, double* part, hipStream_t s,
                 const int32_t* klist = nullptr, int64_t kstride = 0, const int32_t* kcount = nullptr,
-                int kcount_all = 0) {
+                int kcount_all = 0, double* psi = nullptr) {
   using C = Cols<V, DT>;
   const int nrt = rows / TR;
   const int64_t nptile = ld / C::PTS;  // ld is a multiple of 256
   const dim3 block(256), grid((unsigned)(((nptile + 7) / 8) * 8 * nrt));
   hipLaunchKernelGGL((contract_mixed_kernel<V, DT>), grid, block, 0, s, A, rows, mp, ao, ld, part, nrt, nptile, klist,
-                     kstride, kcount, kcount_all);
+                     kstride, kcount, kcount_all, psi);
   CT_HIP_CHECK(hipGetLastError());
 }
 
@@ -411,15 +425,17 @@ int factor_tile_points(int mode) {
     case MODE_LAP:
       return Cols<V_F5, 0>::PTS;
     default:
-      static_assert(Cols<V_F4, MODE_HESS>::PTS == Cols<V_FH, 0>::PTS, "one tile size per mode");
+      static_assert(Cols<V_F4, MODE_HESS>::PTS == Cols<V_FH, 0>::PTS && Cols<V_FH, 0>::PTS == Cols<V_FH, 1>::PTS,
+                    "one tile size per mode");
       return Cols<V_F4, MODE_HESS>::PTS;
   }
 }
 
 void launch_contract_mixed_factor(int mode, const uint16_t* Lf, int rows_pad, int nbasis_pad, const float* ao,
                                   int64_t ld, double* part, const int32_t* klist, const int32_t* kcount,
-                                  hipStream_t s) {
+                                  hipStream_t s, double* psi) {
   CT_REQUIRE(klist != nullptr, "the factor route needs a k-step list");
+  CT_REQUIRE(mode != MODE_HESS || psi != nullptr, "the factor route's Hessian mode needs the psi buffer");
   // screened: per-tile lists of stride nk and their lengths; else one list
   // of all nk k-steps shared by every tile
   const int nk = nbasis_pad / KB;
@@ -438,9 +454,9 @@ void launch_contract_mixed_factor(int mode, const uint16_t* Lf, int rows_pad, in
       launch_var<V_F5>(L, rows_pad, nbasis_pad, ao, ld, part, s, klist, kstride, kcount, nk);
       break;
     default:
-      launch_var<V_F4, MODE_HESS>(L, rows_pad, nbasis_pad, ao, ld, part, s, klist, kstride, kcount, nk);
-      launch_var<V_FH, 0>(L, rows_pad, nbasis_pad, ao, ld, part, s, klist, kstride, kcount, nk);
-      launch_var<V_FH, 1>(L, rows_pad, nbasis_pad, ao, ld, part, s, klist, kstride, kcount, nk);
+      launch_var<V_F4, MODE_HESS>(L, rows_pad, nbasis_pad, ao, ld, part, s, klist, kstride, kcount, nk, psi);
+      launch_var<V_FH, 0>(L, rows_pad, nbasis_pad, ao, ld, part, s, klist, kstride, kcount, nk, psi);
+      launch_var<V_FH, 1>(L, rows_pad, nbasis_pad, ao, ld, part, s, klist, kstride, kcount, nk, psi);
       break;
   }
 }

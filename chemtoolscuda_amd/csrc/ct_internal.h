@@ -183,9 +183,11 @@ void launch_contract_mixed(int mode, const uint16_t* Pp, int nbasis_pad, const f
 // k-steps: screened, klist / kcount from launch_tile_activity at
 // factor_tile_points(mode), only the listed k-steps are contracted; else
 // kcount = nullptr and klist = 0 .. nbasis_pad / 32 - 1 for every tile.
+// psi (Hessian mode only): rows_pad x ld doubles, psi = L^T phi of the
+// first launch, read by the second-derivative launches.
 void launch_contract_mixed_factor(int mode, const uint16_t* Lf, int rows_pad, int nbasis_pad, const float* ao,
                                   int64_t ld, double* part, const int32_t* klist, const int32_t* kcount,
-                                  hipStream_t s);
+                                  hipStream_t s, double* psi = nullptr);
 // Pivoted Cholesky factorisation of the symmetric n x n matrix A (row stride
 // lda) on the host: A = L L^T with L (n x rank, column k at L[k * n ...]),
 // columns in pivot order, stopping when the largest remaining diagonal

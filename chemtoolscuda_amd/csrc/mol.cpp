@@ -1150,8 +1150,12 @@ void Mol::eval(const double* points, int64_t n, int layout, uint32_t q, int prec
   const int64_t slot_pt = 24 + in_pt + stage_width * 8 + 4;
   // AO block, dots (mixed: per row block, with the bound), the slots: the
   // pieces carved below, in that order
+  // the mixed factor route's Hessian mode: psi of its first launch
+  // (factor_rows_ doubles per point), read by the second-derivative launches
+  const int64_t psi_pt = mixed && use_factor && mode == MODE_HESS ? (int64_t)factor_rows_ * 8 : 0;
   const int64_t per_point =
-      (mixed ? ao_bytes_pt + (int64_t)nrt * (ndot + 1) * 8 : ao_bytes_pt + (int64_t)ndot * 8) + nslot * slot_pt;
+      (mixed ? ao_bytes_pt + (int64_t)nrt * (ndot + 1) * 8 + psi_pt : ao_bytes_pt + (int64_t)ndot * 8) +
+      nslot * slot_pt;
   // screened: keys (2 x 4 B), indices (2 x 4 B), sorted points (24 B) per
   // point; k-step list (2 nk B), count and mask (4 + 4 nwords B) per tile
   const int64_t per_point_s = screened ? 40 + (4 * nk + 4 + 4 * nwords + tp - 1) / tp : 0;
@@ -1208,6 +1212,8 @@ void Mol::eval(const double* points, int64_t n, int layout, uint32_t q, int prec
   cur += (size_t)Nc * ao_bytes_pt;
   double* dots = reinterpret_cast<double*>(cur);
   cur += (size_t)Nc * (mixed ? nrt * (ndot + 1) : ndot) * 8;
+  double* psi = psi_pt ? reinterpret_cast<double*>(cur) : nullptr;
+  cur += (size_t)Nc * psi_pt;
 
   // a pipeline slot: its staging and the chunk it holds
   struct Slot {
@@ -1437,7 +1443,7 @@ void Mol::eval(const double* points, int64_t n, int layout, uint32_t q, int prec
       if (use_factor)
         launch_contract_mixed_factor(mode, static_cast<const uint16_t*>(d_Lb_.get()), factor_rows_, Mp_, aoc, ld,
                                      dots, screened ? klist : static_cast<const int32_t*>(d_klist_all_.get()),
-                                     screened ? kcount : nullptr, stream_);
+                                     screened ? kcount : nullptr, stream_, psi);
       else
         launch_contract_mixed(mode, static_cast<const uint16_t*>(d_Pb_.get()), Mp_, aoc, ld, dots, stream_);
       timer_.end(stream_);
