@@ -418,13 +418,16 @@ __global__ __launch_bounds__(256) void reduce_row_blocks_kernel(const double* __
 // columns (phi^T P x = psi^T (L^T x)), reduced over the four lane groups.
 //   F1:  phi x 64 points (rho mode: rho);
 //   F4:  {phi, d_x phi, d_y phi, d_z phi} x 16 points: rho, g_k and t_kk (lap
-//        mode) or t_kl (Hessian mode);
-//   F2:  {phi, lap phi} x 32 points: psi . lap psi;
-//   FH0, FH1: {phi, d_kl phi for kl = xx, xy, xz | yy, yz, zz} x 16 points: h_kl.
+//        mode) or t_kl (Hessian mode); in the lap and Hessian modes it also
+//        stores psi (psi[row * ld + point]) for the launches below, which
+//        dot their columns with that psi instead of forming it again:
+//   L1:  lap phi x 64 points: psi . lap psi;
+//   H4:  {d_xx, d_xy, d_xz, d_yy phi} x 16 points: h_xx, h_xy, h_xz, h_yy;
+//   H2:  {d_yz, d_zz phi} x 32 points: h_yz, h_zz.
 // With gridDim.y > 1 (few points) each workgroup row y takes the 64 factor
 // columns of pass y only and writes its partial dots at dots + y * ndot * ld
 // (summed in order afterwards): more waves for the same work.
-enum VF64 : int { F64_F1, F64_F4, F64_F2, F64_FH0, F64_FH1 };
+enum VF64 : int { F64_F1, F64_F4, F64_L1, F64_H4, F64_H2 };
 
 // Three waves per SIMD (168 VGPRs) where that fits without spills; the
 // lap and Hessian F4 tiles keep two (their 3-wave builds spill).  Few-point
@@ -432,25 +435,27 @@ enum VF64 : int { F64_F1, F64_F4, F64_F2, F64_FH0, F64_FH1 };
 // bound: 1.14 -> 0.79 ms per step for 25 000 points.
 // kmask (optional, the screened layout): per tile of tp points, the 32-function
 // k-steps holding a shell within its cutoff radius (nwords 32-bit words per
-// tile); a wave contracts the union of its tiles' k-steps only (a wave's
-// points span at most two tiles).  kmask null: every k-step.
+// tile); a wave contracts the union of its tiles' k-steps only (the AO
+// kernel writes every point of its 64-point waves for that union).  kmask
+// null: every k-step.
 
 // SCR: the screened layout (kmask given), at two waves per SIMD (the masked
 // k-loop needs a few more registers than three waves leave); unscreened
 // launches keep the plain k-loop and three waves where the variant allows.
 template <int VF, int MODE, bool SCR>
-__global__ __launch_bounds__(256, SCR || (VF == F64_F4 && MODE != MODE_GRAD) ? 2 : 3) void contract_f64_factor_kernel(const double* __restrict__ L, int rows, int mp,
+__global__ __launch_bounds__(256, SCR || ((VF == F64_F4 || VF == F64_H4) && MODE != MODE_GRAD) ? 2 : 3) void contract_f64_factor_kernel(const double* __restrict__ L, int rows, int mp,
                                                                   const double* __restrict__ ao, int64_t ld,
                                                                   double* __restrict__ dots,
                                                                   const uint32_t* __restrict__ kmask, int nwords,
-                                                                  int tp) {
-  constexpr int NT = VF == F64_F1 ? 1 : VF == F64_F2 ? 2 : 4;
+                                                                  int tp, double* __restrict__ psi) {
+  constexpr int NT = VF == F64_F1 || VF == F64_L1 ? 1 : VF == F64_H2 ? 2 : 4;
+  constexpr bool PSI = VF == F64_L1 || VF == F64_H4 || VF == F64_H2;  // dots with the stored psi
   constexpr int PPW = 64 / NT;   // points per wave
   constexpr int NPB = PPW / 16;  // 16-point column blocks per type
   constexpr bool HES = MODE == MODE_HESS, LAP = MODE == MODE_LAP;
-  // dots per point: F1 [rho]; F2 [L0]; FH [h x 3]; F4 [rho, g_x, g_y, g_z,
-  // (lap: t_xx, t_yy, t_zz | Hessian: t_xx, t_xy, t_yy, t_xz, t_yz, t_zz)]
-  constexpr int NDV = VF == F64_F1 || VF == F64_F2 ? 1 : VF == F64_F4 ? (LAP ? 7 : HES ? 10 : 4) : 3;
+  // dots per point: F1 [rho]; L1 [L0]; H4, H2 [h x NT]; F4 [rho, g_x, g_y,
+  // g_z, (lap: t_xx, t_yy, t_zz | Hessian: t_xx, t_xy, t_yy, t_xz, t_yz, t_zz)]
+  constexpr int NDV = VF == F64_F1 || VF == F64_L1 ? 1 : VF == F64_F4 ? (LAP ? 7 : HES ? 10 : 4) : NT;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
   const int r = lane & 15, q = lane >> 4;
@@ -469,8 +474,8 @@ __global__ __launch_bounds__(256, SCR || (VF == F64_F4 && MODE != MODE_GRAD) ? 2
       const int w = k >> 5;
       uint32_t m = 0xFFFFFFFFu;
       if (kmask) {
-        m = kmask[t0 * nwords + w];
-        if (t1 != t0) m |= kmask[t1 * nwords + w];
+        m = 0;
+        for (int64_t t = t0; t <= t1; ++t) m |= kmask[t * nwords + w];
       }
       m >>= (k & 31);
       if (m) return k + __builtin_ctz(m);
@@ -482,10 +487,9 @@ __global__ __launch_bounds__(256, SCR || (VF == F64_F4 && MODE != MODE_GRAD) ? 2
   const int i_end = split ? i_begin + 64 : rows;
   dots += split ? (int64_t)blockIdx.y * mode_ndot(MODE) * ld : 0;
 
-  // AO array of type t (0 = phi, 1..3 = d phi, 4 = lap phi | 4..9 = d_kl phi)
-  auto arr = [](int t) {
-    return VF == F64_F2 ? (t == 0 ? 0 : 4) : VF == F64_FH0 ? (t == 0 ? 0 : 3 + t) : VF == F64_FH1 ? (t == 0 ? 0 : 6 + t) : t;
-  };
+  // AO array of type t (0 = phi, 1..3 = d phi, 4 = lap phi | 4..9 = d_xx,
+  // d_xy, d_xz, d_yy, d_yz, d_zz phi)
+  auto arr = [](int t) { return VF == F64_L1 ? 4 : VF == F64_H4 ? 4 + t : VF == F64_H2 ? 8 + t : t; };
   const double* bptr[4];
 #pragma unroll
   for (int cb = 0; cb < 4; ++cb) bptr[cb] = ao + arr(cb / NPB) * sd + (int64_t)q * ld + p0 + 16 * (cb % NPB) + r;
@@ -569,21 +573,28 @@ __global__ __launch_bounds__(256, SCR || (VF == F64_F4 && MODE != MODE_GRAD) ? 2
     }
     // lane (r, q) holds rows q + 4 j of every 16-row block rb, column r of
     // every column block: all column types of its point(s)
+    auto psi_at = [&](int rb, int j, int pb) {
+      return psi + (int64_t)(i0 + 16 * rb + q + 4 * j) * ld + p0 + 16 * pb + r;
+    };
 #pragma unroll
-    for (int rb = 0; rb < 4; ++rb) {
+    for (int pb = 0; pb < NPB; ++pb) {
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
+      for (int rb = 0; rb < 4; ++rb) {
+        double ps[4];  // the stored psi of the block's rows (PSI variants)
+        if constexpr (PSI) {
 #pragma unroll
-        for (int pb = 0; pb < NPB; ++pb) {
-          const double y0 = acc[rb][pb][j];
+          for (int j = 0; j < 4; ++j) ps[j] = *psi_at(rb, j, pb);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const double y0 = PSI ? ps[j] : acc[rb][pb][j];
           if constexpr (VF == F64_F1) {
             dacc[pb][0] += y0 * y0;
-          } else if constexpr (VF == F64_F2) {
-            dacc[pb][0] += y0 * acc[rb][NPB + pb][j];
-          } else if constexpr (VF == F64_FH0 || VF == F64_FH1) {
+          } else if constexpr (PSI) {
 #pragma unroll
-            for (int k = 0; k < 3; ++k) dacc[pb][k] += y0 * acc[rb][1 + k][j];
+            for (int k = 0; k < NT; ++k) dacc[pb][k] += y0 * acc[rb][k * NPB + pb][j];
           } else {
+            if constexpr (LAP || HES) *psi_at(rb, j, pb) = y0;
             const double yx = acc[rb][1][j], yy = acc[rb][2][j], yz = acc[rb][3][j];
             dacc[pb][0] += y0 * y0;
             dacc[pb][1] += y0 * yx;
@@ -622,12 +633,12 @@ __global__ __launch_bounds__(256, SCR || (VF == F64_F4 && MODE != MODE_GRAD) ? 2
 #pragma unroll
       for (int k = 0; k < NDV; ++k) {
         // slots of the mode's dot layout (mode_ndot)
-        const int slot = VF == F64_F2    ? 4
-                         : VF == F64_FH0 ? 4 + k
-                         : VF == F64_FH1 ? 7 + k
-                         : k < 4         ? k
-                         : HES           ? k + 6
-                                         : k + 1;
+        const int slot = VF == F64_L1   ? 4
+                         : VF == F64_H4 ? 4 + k
+                         : VF == F64_H2 ? 8 + k
+                         : k < 4        ? k
+                         : HES          ? k + 6
+                                        : k + 1;
         dots[(int64_t)slot * ld + p0 + 16 * pb + r] = dacc[pb][k];
       }
   }
@@ -635,15 +646,15 @@ __global__ __launch_bounds__(256, SCR || (VF == F64_F4 && MODE != MODE_GRAD) ? 2
 
 template <int VF, int MODE>
 void launch_factor_var(const double* L, int rows, int mp, const double* ao, int64_t ld, double* dots, hipStream_t s,
-                       bool split = false, const uint32_t* kmask = nullptr, int nwords = 0, int tp = 64) {
-  constexpr int PPW = VF == F64_F1 ? 64 : VF == F64_F2 ? 32 : 16;
+                       bool split, const uint32_t* kmask, int nwords, int tp, double* psi) {
+  constexpr int PPW = VF == F64_F1 || VF == F64_L1 ? 64 : VF == F64_H2 ? 32 : 16;
   const dim3 block(256), grid((unsigned)(ld / (4 * PPW)), split ? (unsigned)(rows / 64) : 1u);  // ld: multiple of 256
   if (kmask)
     hipLaunchKernelGGL((contract_f64_factor_kernel<VF, MODE, true>), grid, block, 0, s, L, rows, mp, ao, ld, dots,
-                       kmask, nwords, tp);
+                       kmask, nwords, tp, psi);
   else
     hipLaunchKernelGGL((contract_f64_factor_kernel<VF, MODE, false>), grid, block, 0, s, L, rows, mp, ao, ld, dots,
-                       kmask, nwords, tp);
+                       kmask, nwords, tp, psi);
   CT_HIP_CHECK(hipGetLastError());
 }
 
@@ -651,33 +662,35 @@ void launch_factor_var(const double* L, int rows, int mp, const double* ao, int6
 
 namespace {
 void factor_modes(int mode, const double* L, int rows_pad, int nbasis_pad, const double* ao, int64_t ld, double* dots,
-                  hipStream_t s, bool split, const uint32_t* kmask = nullptr, int nwords = 0, int tp = 64) {
+                  hipStream_t s, bool split, const uint32_t* kmask, int nwords, int tp, double* psi) {
   CT_REQUIRE(rows_pad % 64 == 0 && rows_pad > 0 && ld % 256 == 0, "factor padding / point stride");
   CT_REQUIRE(!kmask || (nwords == (nbasis_pad / 32 + 31) / 32 && tp >= 16 && ld % tp == 0),
              "activity masks do not match the FP64 factor contraction");
+  CT_REQUIRE(psi != nullptr || mode == MODE_RHO || mode == MODE_GRAD, "the lap and Hessian modes need the psi buffer");
   switch (mode) {
     case MODE_RHO:
-      launch_factor_var<F64_F1, MODE_RHO>(L, rows_pad, nbasis_pad, ao, ld, dots, s, split, kmask, nwords, tp);
+      launch_factor_var<F64_F1, MODE_RHO>(L, rows_pad, nbasis_pad, ao, ld, dots, s, split, kmask, nwords, tp, psi);
       break;
     case MODE_GRAD:
-      launch_factor_var<F64_F4, MODE_GRAD>(L, rows_pad, nbasis_pad, ao, ld, dots, s, split, kmask, nwords, tp);
+      launch_factor_var<F64_F4, MODE_GRAD>(L, rows_pad, nbasis_pad, ao, ld, dots, s, split, kmask, nwords, tp, psi);
       break;
     case MODE_LAP:
-      launch_factor_var<F64_F4, MODE_LAP>(L, rows_pad, nbasis_pad, ao, ld, dots, s, split, kmask, nwords, tp);
-      launch_factor_var<F64_F2, MODE_LAP>(L, rows_pad, nbasis_pad, ao, ld, dots, s, split, kmask, nwords, tp);
+      launch_factor_var<F64_F4, MODE_LAP>(L, rows_pad, nbasis_pad, ao, ld, dots, s, split, kmask, nwords, tp, psi);
+      launch_factor_var<F64_L1, MODE_LAP>(L, rows_pad, nbasis_pad, ao, ld, dots, s, split, kmask, nwords, tp, psi);
       break;
     default:
-      launch_factor_var<F64_F4, MODE_HESS>(L, rows_pad, nbasis_pad, ao, ld, dots, s, split, kmask, nwords, tp);
-      launch_factor_var<F64_FH0, MODE_HESS>(L, rows_pad, nbasis_pad, ao, ld, dots, s, split, kmask, nwords, tp);
-      launch_factor_var<F64_FH1, MODE_HESS>(L, rows_pad, nbasis_pad, ao, ld, dots, s, split, kmask, nwords, tp);
+      launch_factor_var<F64_F4, MODE_HESS>(L, rows_pad, nbasis_pad, ao, ld, dots, s, split, kmask, nwords, tp, psi);
+      launch_factor_var<F64_H4, MODE_HESS>(L, rows_pad, nbasis_pad, ao, ld, dots, s, split, kmask, nwords, tp, psi);
+      launch_factor_var<F64_H2, MODE_HESS>(L, rows_pad, nbasis_pad, ao, ld, dots, s, split, kmask, nwords, tp, psi);
       break;
   }
 }
 }  // namespace
 
 void launch_contract_f64_factor(int mode, const double* L, int rows_pad, int nbasis_pad, const double* ao,
-                                int64_t ld, double* dots, hipStream_t s, const uint32_t* kmask, int nwords, int tp) {
-  factor_modes(mode, L, rows_pad, nbasis_pad, ao, ld, dots, s, false, kmask, nwords, tp);
+                                int64_t ld, double* dots, hipStream_t s, const uint32_t* kmask, int nwords, int tp,
+                                double* psi) {
+  factor_modes(mode, L, rows_pad, nbasis_pad, ao, ld, dots, s, false, kmask, nwords, tp, psi);
 }
 
 int64_t contract_f64_factor_split_workspace(int mode, int rows_pad, int64_t ld) {
@@ -686,8 +699,8 @@ int64_t contract_f64_factor_split_workspace(int mode, int rows_pad, int64_t ld) 
 
 void launch_contract_f64_factor_split(int mode, const double* L, int rows_pad, int nbasis_pad, const double* ao,
                                       int64_t ld, double* part, double* dots, hipStream_t s, const uint32_t* kmask,
-                                      int nwords, int tp) {
-  factor_modes(mode, L, rows_pad, nbasis_pad, ao, ld, part, s, true, kmask, nwords, tp);
+                                      int nwords, int tp, double* psi) {
+  factor_modes(mode, L, rows_pad, nbasis_pad, ao, ld, part, s, true, kmask, nwords, tp, psi);
   const int64_t count = (int64_t)mode_ndot(mode) * ld;
   hipLaunchKernelGGL(reduce_row_blocks_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, part,
                      rows_pad / 64, count, dots);

@@ -145,17 +145,19 @@ void launch_finalize(const FinalizeArgs& a, hipStream_t s);
 // k-major, rows_pad a multiple of 64): rank(P) columns instead of nbasis.
 // kmask (optional, screened layout): per tile of
 // tile_points points the active 32-function k-steps (launch_tile_activity);
-// only those are contracted.
+// only those are contracted.  psi (lap and Hessian modes): rows_pad x ld
+// doubles, psi = L^T phi of the mode's first launch, read by the others.
 void launch_contract_f64_factor(int mode, const double* L, int rows_pad, int nbasis_pad, const double* ao,
                                 int64_t ld, double* dots, hipStream_t s, const uint32_t* kmask = nullptr,
-                                int nwords = 0, int tile_points = 64);
+                                int nwords = 0, int tile_points = 64, double* psi = nullptr);
 // The same with each 64-column pass of the factor in its own workgroups and
 // an ordered sum of the passes' partial dots (for small point counts); part
 // holds contract_f64_factor_split_workspace() doubles.
 int64_t contract_f64_factor_split_workspace(int mode, int rows_pad, int64_t ld);
 void launch_contract_f64_factor_split(int mode, const double* L, int rows_pad, int nbasis_pad, const double* ao,
                                       int64_t ld, double* part, double* dots, hipStream_t s,
-                                      const uint32_t* kmask = nullptr, int nwords = 0, int tile_points = 64);
+                                      const uint32_t* kmask = nullptr, int nwords = 0, int tile_points = 64,
+                                      double* psi = nullptr);
 // The same contraction with one 64-row block of P per wave and an ordered
 // sum of the blocks' partial dots (for small point counts); part holds
 // contract_f64_split_workspace() doubles.

@@ -977,14 +977,16 @@ int64_t Mol::chunk_points(int64_t per_point, int64_t n) const {
 }
 
 size_t Mol::reeval_bytes(int mode, int64_t count, bool factor) const {
-  // AO arrays, dots, gathered points and the split contraction's partials
-  // (an upper bound for every slice of at most count points); the screened
+  // AO arrays, dots, gathered points, the split contraction's partials and
+  // the factor route's psi (lap and Hessian modes) (an upper bound for every
+  // slice of at most count points); the screened
   // factor route: the active k-step counters, lists, counts and masks of the
   // slice's point tiles
   const int nder = mode_nder(mode), ndot = mode_ndot(mode);
   const int64_t ld = round_up(count, kPointAlign);
   const int blocks = factor ? factor_rows_ / 64 : Mp_ / 64;
-  size_t bytes = (size_t)ld * ((size_t)nder * Mp_ * 8 + (size_t)ndot * 8 + 24 + (size_t)blocks * ndot * 8);
+  const size_t psi = factor && (mode == MODE_LAP || mode == MODE_HESS) ? (size_t)factor_rows_ * 8 : 0;
+  size_t bytes = (size_t)ld * ((size_t)nder * Mp_ * 8 + (size_t)ndot * 8 + 24 + (size_t)blocks * ndot * 8 + psi);
   if (factor && screen_eps_ > 0.0) {
     const int nk = Mp_ / 32, nwords = (nk + 31) / 32;
     bytes += 128 * kScreenCounters + (size_t)(ld / factor_tile_points(mode)) * (4 * nk + 4 + 4 * nwords);
@@ -1030,6 +1032,8 @@ void Mol::reevaluate(int mode, const double* pts, int64_t col_stride, int64_t co
     double* dots = ao + (size_t)nder * Mp_ * ld;
     double* gp = dots + (size_t)ndot * ld;
     double* part = gp + 3 * (size_t)ld;
+    double* psi = part + (size_t)(factor ? factor_rows_ / 64 : Mp_ / 64) * ndot * ld;
+    const size_t psi_len = factor && (mode == MODE_LAP || mode == MODE_HESS) ? (size_t)factor_rows_ * ld : 0;
     launch_gather_points(pts, idx, c, gp, rs, col_stride);
     // the factor route in the order of its factor (the screened shell
     // order), screened as the FP64 path screens it: per tile of the slice's
@@ -1040,7 +1044,7 @@ void Mol::reevaluate(int mode, const double* pts, int64_t col_stride, int64_t co
     const int nk = Mp_ / 32, nwords = (nk + 31) / 32;
     uint32_t* kmask = nullptr;
     if (scr) {
-      char* sp = reinterpret_cast<char*>(part + (size_t)(factor_rows_ / 64) * ndot * ld);
+      char* sp = reinterpret_cast<char*>(psi + psi_len);
       auto* counters = reinterpret_cast<unsigned long long*>(sp);  // (unused here)
       const int64_t ntile = ld / tp;
       int32_t* klist = reinterpret_cast<int32_t*>(sp + 128 * kScreenCounters);
@@ -1056,10 +1060,10 @@ void Mol::reevaluate(int mode, const double* pts, int64_t col_stride, int64_t co
                   nwords, tp);
     if (factor && split)
       launch_contract_f64_factor_split(mode, static_cast<const double*>(d_L64s_.get()), factor_rows_, Mp_, ao, ld,
-                                       part, dots, rs, kmask, nwords, tp);
+                                       part, dots, rs, kmask, nwords, tp, psi_len ? psi : nullptr);
     else if (factor)
       launch_contract_f64_factor(mode, static_cast<const double*>(d_L64s_.get()), factor_rows_, Mp_, ao, ld, dots,
-                                 rs, kmask, nwords, tp);
+                                 rs, kmask, nwords, tp, psi_len ? psi : nullptr);
     else if (split && mode == MODE_RHO)
       launch_contract_f64_split(mode, static_cast<const double*>(d_P_.get()), Mp_, ao, ld, part, dots, rs);
     else
@@ -1150,11 +1154,13 @@ void Mol::eval(const double* points, int64_t n, int layout, uint32_t q, int prec
   const int64_t slot_pt = 24 + in_pt + stage_width * 8 + 4;
   // AO block, dots (mixed: per row block, with the bound), the slots: the
   // pieces carved below, in that order
-  // the mixed factor route's Hessian mode: psi of its first launch
-  // (factor_rows_ doubles per point), read by the second-derivative launches
-  const int64_t psi_pt = mixed && use_factor && mode == MODE_HESS ? (int64_t)factor_rows_ * 8 : 0;
+  // the factor route's psi of a mode's first launch (factor_rows_ doubles per
+  // point), read by its other launches: mixed Hessian mode, FP64 lap and
+  // Hessian modes
+  const bool psi_used = use_factor && (mode == MODE_HESS || (!mixed && mode == MODE_LAP));
+  const int64_t psi_pt = psi_used ? (int64_t)factor_rows_ * 8 : 0;
   const int64_t per_point =
-      (mixed ? ao_bytes_pt + (int64_t)nrt * (ndot + 1) * 8 + psi_pt : ao_bytes_pt + (int64_t)ndot * 8) +
+      (mixed ? ao_bytes_pt + (int64_t)nrt * (ndot + 1) * 8 : ao_bytes_pt + (int64_t)ndot * 8) + psi_pt +
       nslot * slot_pt;
   // screened: keys (2 x 4 B), indices (2 x 4 B), sorted points (24 B) per
   // point; k-step list (2 nk B), count and mask (4 + 4 nwords B) per tile
@@ -1425,7 +1431,7 @@ void Mol::eval(const double* points, int64_t n, int layout, uint32_t q, int prec
       if (use_factor)
         launch_contract_f64_factor(mode, static_cast<const double*>(d_L64s_.get()), factor_rows_, Mp_,
                                    static_cast<const double*>(ao), ld, dots, stream_, screened ? kmask : nullptr,
-                                   nwords, tp);
+                                   nwords, tp, psi);
       else
         launch_contract_f64(mode, static_cast<const double*>(d_P_.get()), Mp_, static_cast<const double*>(ao), ld,
                             dots, stream_);
