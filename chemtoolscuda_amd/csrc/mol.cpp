@@ -1311,7 +1311,7 @@ void Mol::eval(const double* points, int64_t n, int layout, uint32_t q, int prec
   }
   // the flag counters (bytes 0-3, 16-19, 32-35), the largest estimate (8-15)
   // and the active k-step counters start at zero
-  if (mixed) launch_fill_zero(reinterpret_cast<unsigned long long*>(ws), (int64_t)(head / 8), stream_);
+  if (mixed || screened) launch_fill_zero(reinterpret_cast<unsigned long long*>(ws), (int64_t)(head / 8), stream_);
 
   int64_t chunks = 0, reevaluated = 0;
   // Queues chunk [start, start + nc) in slot sl: upload, kernels, finaliser

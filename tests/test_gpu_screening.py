@@ -164,6 +164,10 @@ def test_screened_modes_and_cubic_grid(ct, organic):
     mol = ct.molecule_from_basis(w.basis, w.P, w.charges)
     f64 = mol.compute(region, ["rho", "grad", "hess"], precision="fp64")
     m = f64["rho"] >= 1e-8
+    # the FP64 path's active-block count starts at zero in every call
+    a64 = mol.last_report()["active_fraction"]
+    mol.compute(region, ["rho", "grad", "hess"], precision="fp64")
+    assert 0.0 < a64 < 1.0 and mol.last_report()["active_fraction"] == a64, a64
     for names, route in ((["rho"], "auto"), (["rho", "grad"], "auto"), (["rho", "grad", "hess"], "factor")):
         r = mol.compute(region, names, precision="mixed", route=route)
         rep = mol.last_report()
