@@ -83,3 +83,48 @@ def test_density_route_few_points(ct):
         assert np.max(np.abs(got["rho"] - ref["rho"]) / np.abs(ref["rho"])) < 1e-12, names
         if "hess" in names:
             assert _normwise(got["hess"], ref["hess"], 1e-10 * np.abs(ref["hess"]).max()) < 1e-12
+
+
+@pytest.mark.parametrize("names", [["rho", "grad", "lap", "tau_pd"], ["rho", "grad", "hess"]])
+def test_factor_route_psi_across_chunks(ct, names):
+    """The FP64 factor route's lap and Hessian modes store psi = L^T phi in
+    their first launch and read it in the later ones (slots of the chunk's
+    workspace): with the host-array call's own chunks and with a workspace
+    of a few hundred points per chunk, ragged last chunk included, every
+    point matches the density route to 1e-12 relative plus twice the
+    worst-case FP64 dot-product error M u of the quantity's condition scale
+    (lap and the Hessian cancel: their scale exceeds |value| by orders of
+    magnitude at some points); a psi read at a wrong point or row would be
+    off by O(1).  The mixed Hessian mode, whose re-evaluation slices hold
+    psi as well, stays within 1e-6 of FP64 on rho."""
+    from chemtoolscuda_amd import synthetic
+    from test_gpu_mixed import _factor_hess_scale, _factor_scales, _pivoted_cholesky
+
+    w = synthetic.organic60()
+    pts = _sample(w, 5003, 34)
+    mol = ct.molecule_from_basis(w.basis, w.P, w.charges)
+    dens = mol.compute(pts, names, precision="fp64", route="density")
+    one = mol.compute(pts, names, precision="fp64", route="factor")
+    rep = mol.last_report()
+    assert rep["factor_rank"] > 0
+    mol.set_workspace_limit(24 << 20)
+    many = mol.compute(pts, names, precision="fp64", route="factor")
+    assert mol.last_report()["n_chunks"] > rep["n_chunks"], (rep, mol.last_report())
+    L = _pivoted_cholesky(0.5 * (w.P + w.P.T))
+    flat = w.basis.flat()
+    sc = dict(zip(["rho", "grad", "lap", "tau_pd"], _factor_scales(flat, L, pts)))
+    if "hess" in names:
+        sc["hess"] = _factor_hess_scale(flat, L, pts)
+    mu = 2 * w.P.shape[0] * 2.0 ** -53
+    for got in (one, many):
+        for k in names:
+            a = np.asarray(got[k]).reshape(len(pts), -1)
+            b = np.asarray(dens[k]).reshape(len(pts), -1)
+            err = np.linalg.norm(a - b, axis=1)
+            lim = 1e-12 * np.linalg.norm(b, axis=1) + mu * sc[k]
+            assert np.all(err <= lim), (k, float(np.max(err / lim)))
+    if "hess" in names:
+        mixed = mol.compute(pts, names, precision="mixed", route="factor")
+        assert mol.last_report()["n_reevaluated"] > 0
+        m = dens["rho"] >= 1e-8
+        assert np.max(np.abs(mixed["rho"] - dens["rho"])[m] / dens["rho"][m]) <= 1e-6
