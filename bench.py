@@ -476,6 +476,9 @@ def main():
         extra["mixed_over_fp64_density_route"] = value / extra["fp64_density_route_points_per_s"]
         for q, v in _max_errors(torch, outs, dens, quantities).items():
             extra[f"max_rel_err_{q}"] = v
+        extra["max_rel_err_scales"] = ("over points with rho >= 1e-8, against the FP64 density route: rho and "
+                                       "tau_pd relative to themselves; grad to max(|grad|, sqrt(8 rho tau)); "
+                                       "lap to |lap - 4 tau| + 4 tau")
         extra["fp64_factor_vs_density_max_rel_err"] = max(_max_errors(torch, ref, dens, quantities).values())
         extra["n_reevaluated"] = rep.get("n_reevaluated") if rep else None
         del ref, dens
