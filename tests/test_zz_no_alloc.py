@@ -101,6 +101,43 @@ def test_noalloc_matches_ct_eval_on_two_calls(torch, case):
         assert repm["n_reevaluated"] >= 0 and repm["n_chunks"] >= 1
 
 
+def test_noalloc_hessian_mode_on_two_calls(torch):
+    """The Hessian mode in the scratch: the factor route's psi (FP64 path:
+    stored by its first launch and read by the two second-derivative
+    launches; mixed path: the same, and the FP64 re-evaluation's slices hold
+    their own) carved from the caller-sized region, two calls in a row, the
+    second smaller: FP64 within 1e-13 and mixed within 1e-6 of each
+    quantity's scale against ct_eval's FP64 (the Hessian against
+    max|H_kl| + 8 tau)."""
+    from chemtoolscuda_amd import capi, synthetic
+
+    w = synthetic.organic60()
+    rng = np.random.default_rng(43)
+    sel = rng.choice(w.points.shape[0], 30000, replace=False)
+    pts_all = torch.from_numpy(np.ascontiguousarray(w.points[sel])).cuda()
+    h = capi.Handle(w.basis.flat(), w.P, w.charges)
+    qs = ["rho", "grad", "hess", "tau_pd", "lap"]
+    for n in (pts_all.shape[0], pts_all.shape[0] // 3 + 5):
+        pts = pts_all[:n].contiguous()
+        ref64, _ = _run(torch, h, h.eval, pts, qs, "fp64", {"route": 2})
+        got64, rep = _run(torch, h, h.eval_noalloc, pts, qs, "fp64", {"route": 2})
+        assert rep["factor_rank"] > 0
+        gotm, repm = _run(torch, h, h.eval_noalloc, pts, qs, "mixed", {"route": 2})
+        assert repm["n_reevaluated"] > 0
+        mask = ref64["rho"][:, 0] >= 1e-8
+        tau = ref64["tau_pd"][:, 0].abs()
+        for q in ("rho", "grad", "hess"):
+            if q == "hess":
+                sc = ref64["hess"].abs().amax(dim=1) + 8 * tau
+            else:
+                sc = _scale(torch, q, ref64)
+            sc = sc.clamp(min=1e-300)
+            e64 = (torch.linalg.vector_norm(got64[q] - ref64[q], dim=1) / sc)[mask].max().item()
+            em = (torch.linalg.vector_norm(gotm[q] - ref64[q], dim=1) / sc)[mask].max().item()
+            print(f"hessian mode n={n} {q}: no-alloc FP64 {e64:.2e}, no-alloc mixed {em:.2e}")
+            assert e64 < 1e-13 and em < 1e-6, (n, q, e64, em)
+
+
 def test_noalloc_columns_and_rejections(torch):
     from chemtoolscuda_amd import capi, synthetic
 
