@@ -424,9 +424,11 @@ __global__ __launch_bounds__(256) void reduce_row_blocks_kernel(const double* __
 //   L1:  lap phi x 64 points: psi . lap psi;
 //   H4:  {d_xx, d_xy, d_xz, d_yy phi} x 16 points: h_xx, h_xy, h_xz, h_yy;
 //   H2:  {d_yz, d_zz phi} x 32 points: h_yz, h_zz.
-// With gridDim.y > 1 (few points) each workgroup row y takes the 64 factor
-// columns of pass y only and writes its partial dots at dots + y * ndot * ld
-// (summed in order afterwards): more waves for the same work.
+// Split (npass > 0: few points, or three or more passes of L^T) each
+// workgroup takes the 64 factor columns of one pass y only and writes its
+// partial dots at dots + y * ndot * ld (summed in order afterwards): more
+// waves for the same work, and the passes of a point group share its AO
+// columns through one XCD's L2.
 enum VF64 : int { F64_F1, F64_F4, F64_L1, F64_H4, F64_H2 };
 
 // Three waves per SIMD (168 VGPRs) where that fits without spills; the
@@ -447,7 +449,7 @@ __global__ __launch_bounds__(256, SCR || ((VF == F64_F4 || VF == F64_H4) && MODE
                                                                   const double* __restrict__ ao, int64_t ld,
                                                                   double* __restrict__ dots,
                                                                   const uint32_t* __restrict__ kmask, int nwords,
-                                                                  int tp, double* __restrict__ psi) {
+                                                                  int tp, double* __restrict__ psi, int npass) {
   constexpr int NT = VF == F64_F1 || VF == F64_L1 ? 1 : VF == F64_H2 ? 2 : 4;
   constexpr bool PSI = VF == F64_L1 || VF == F64_H4 || VF == F64_H2;  // dots with the stored psi
   constexpr int PPW = 64 / NT;   // points per wave
@@ -459,9 +461,21 @@ __global__ __launch_bounds__(256, SCR || ((VF == F64_F4 || VF == F64_H4) && MODE
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
   const int r = lane & 15, q = lane >> 4;
-  const int64_t p0 = ((int64_t)blockIdx.x * 4 + wave) * PPW;
+  // split (npass > 0): workgroup b runs on XCD b % 8 and takes pass
+  // j % npass of point group (j / npass) * 8 + b % 8, j = b / 8, so the
+  // passes of a group follow one another on one XCD and read its AO columns
+  // from that XCD's L2 instead of HBM
+  const bool split = npass > 0;
+  int64_t grp = blockIdx.x;
+  int pass = 0;
+  if (split) {
+    const int j = (int)(blockIdx.x >> 3);
+    pass = j % npass;
+    grp = (int64_t)(j / npass) * 8 + (blockIdx.x & 7);
+    if (grp >= ld / (4 * PPW)) return;  // (no barriers in this kernel)
+  }
+  const int64_t p0 = (grp * 4 + wave) * PPW;
   const int64_t sd = (int64_t)mp * ld;
-  const bool split = gridDim.y > 1;
   // the wave's active k-steps: the union of its (one or two) tiles' masks,
   // read word by word when the k-loop needs the next active k-step (once per
   // eight MFMA steps; wave-uniform, scalar loads)
@@ -483,9 +497,9 @@ __global__ __launch_bounds__(256, SCR || ((VF == F64_F4 || VF == F64_H4) && MODE
     }
     return nk;
   };
-  const int i_begin = split ? (int)blockIdx.y * 64 : 0;
+  const int i_begin = split ? pass * 64 : 0;
   const int i_end = split ? i_begin + 64 : rows;
-  dots += split ? (int64_t)blockIdx.y * mode_ndot(MODE) * ld : 0;
+  dots += split ? (int64_t)pass * mode_ndot(MODE) * ld : 0;
 
   // AO array of type t (0 = phi, 1..3 = d phi, 4 = lap phi | 4..9 = d_xx,
   // d_xy, d_xz, d_yy, d_yz, d_zz phi)
@@ -648,13 +662,17 @@ template <int VF, int MODE>
 void launch_factor_var(const double* L, int rows, int mp, const double* ao, int64_t ld, double* dots, hipStream_t s,
                        bool split, const uint32_t* kmask, int nwords, int tp, double* psi) {
   constexpr int PPW = VF == F64_F1 || VF == F64_L1 ? 64 : VF == F64_H2 ? 32 : 16;
-  const dim3 block(256), grid((unsigned)(ld / (4 * PPW)), split ? (unsigned)(rows / 64) : 1u);  // ld: multiple of 256
+  // ld: multiple of 256; split: rows / 64 passes per point group, the
+  // groups rounded up to a multiple of the 8 XCDs
+  const int64_t ngroups = ld / (4 * PPW);
+  const int npass = split ? rows / 64 : 0;
+  const dim3 block(256), grid(split ? (unsigned)((ngroups + 7) / 8 * 8 * npass) : (unsigned)ngroups);
   if (kmask)
     hipLaunchKernelGGL((contract_f64_factor_kernel<VF, MODE, true>), grid, block, 0, s, L, rows, mp, ao, ld, dots,
-                       kmask, nwords, tp, psi);
+                       kmask, nwords, tp, psi, npass);
   else
     hipLaunchKernelGGL((contract_f64_factor_kernel<VF, MODE, false>), grid, block, 0, s, L, rows, mp, ao, ld, dots,
-                       kmask, nwords, tp, psi);
+                       kmask, nwords, tp, psi, npass);
   CT_HIP_CHECK(hipGetLastError());
 }
 

@@ -1023,11 +1023,13 @@ void Mol::reevaluate(int mode, const double* pts, int64_t col_stride, int64_t co
     const int32_t* idx = index + s0;
     const int64_t ld = round_up(c, kPointAlign);
     // few points: one wave per (16 or 64 points, 64-row block of P, or
-    // 64-column pass of the factor) so the GPU fills.  Not for the density
+    // 64-column pass of the factor) so the GPU fills; the factor route with
+    // three or more passes at any size (the passes of a point group share
+    // its AO columns in one XCD's L2).  Not for the density
     // route's gradient, lap and Hessian modes: their point-set kernel has
     // four waves per point set already, and keeping it for every slice gives
     // a point the same bits whatever slice it falls in.
-    const bool split = ld < 65536;
+    const bool split = ld < 65536 || (factor && factor_rows_ / 64 >= 3);
     double* ao = reinterpret_cast<double*>(ws);
     double* dots = ao + (size_t)nder * Mp_ * ld;
     double* gp = dots + (size_t)ndot * ld;
@@ -1159,8 +1161,14 @@ void Mol::eval(const double* points, int64_t n, int layout, uint32_t q, int prec
   // Hessian modes
   const bool psi_used = use_factor && (mode == MODE_HESS || (!mixed && mode == MODE_LAP));
   const int64_t psi_pt = psi_used ? (int64_t)factor_rows_ * 8 : 0;
+  // the FP64 factor route with three or more 64-row passes of L^T runs one
+  // workgroup per (point group, pass), the passes of a group back to back on
+  // one XCD, sharing its AO columns in that L2 (one pass per wave streamed
+  // the AOs from HBM once per pass); partial dots per pass, summed in order
+  const bool pass_split = use_factor && !mixed && factor_rows_ / 64 >= 2;
+  const int64_t part_pt = pass_split ? (int64_t)(factor_rows_ / 64) * ndot * 8 : 0;
   const int64_t per_point =
-      (mixed ? ao_bytes_pt + (int64_t)nrt * (ndot + 1) * 8 : ao_bytes_pt + (int64_t)ndot * 8) + psi_pt +
+      (mixed ? ao_bytes_pt + (int64_t)nrt * (ndot + 1) * 8 : ao_bytes_pt + (int64_t)ndot * 8) + psi_pt + part_pt +
       nslot * slot_pt;
   // screened: keys (2 x 4 B), indices (2 x 4 B), sorted points (24 B) per
   // point; k-step list (2 nk B), count and mask (4 + 4 nwords B) per tile
@@ -1220,6 +1228,8 @@ void Mol::eval(const double* points, int64_t n, int layout, uint32_t q, int prec
   cur += (size_t)Nc * (mixed ? nrt * (ndot + 1) : ndot) * 8;
   double* psi = psi_pt ? reinterpret_cast<double*>(cur) : nullptr;
   cur += (size_t)Nc * psi_pt;
+  double* part = part_pt ? reinterpret_cast<double*>(cur) : nullptr;
+  cur += (size_t)Nc * part_pt;
 
   // a pipeline slot: its staging and the chunk it holds
   struct Slot {
@@ -1428,7 +1438,11 @@ void Mol::eval(const double* points, int64_t n, int layout, uint32_t q, int prec
                     stream_, screened ? kmask : nullptr, nwords, tp);
       timer_.end(stream_);
       timer_.begin(KernelTimer::CONTRACT, stream_);
-      if (use_factor)
+      if (use_factor && pass_split)
+        launch_contract_f64_factor_split(mode, static_cast<const double*>(d_L64s_.get()), factor_rows_, Mp_,
+                                         static_cast<const double*>(ao), ld, part, dots, stream_,
+                                         screened ? kmask : nullptr, nwords, tp, psi);
+      else if (use_factor)
         launch_contract_f64_factor(mode, static_cast<const double*>(d_L64s_.get()), factor_rows_, Mp_,
                                    static_cast<const double*>(ao), ld, dots, stream_, screened ? kmask : nullptr,
                                    nwords, tp, psi);
